@@ -1,0 +1,181 @@
+// ksw2_api.cpp -- host side of the C ABI declared in include/ksw2_hip.h.
+//
+// Responsibilities: normalise the scoring exactly as the reference does
+// (minimap2/ksw2_extd2_sse.c:80-110), reproduce its early returns, decide per
+// pair whether the unbanded fast kernel applies, stage sequences into HBM,
+// launch the DP and backtrack kernels, and hand the CIGAR back through
+// realloc/krealloc with the reference's m_cigar growth (minimap2/ksw2.h:144-154).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ksw2_hip.h"
+#include "ksw2_common.h"
+#include "ksw2_kernels.h"
+#include "ksw2_plan.h"
+
+using namespace kswhip;
+
+// kalloc's krealloc when the host program links it (AnchorWave does); with
+// km == NULL kalloc itself falls back to realloc (minimap2/kalloc.c:167-175).
+extern "C" void *krealloc(void *km, void *ptr, size_t size) __attribute__((weak));
+
+namespace {
+
+thread_local int t_last_error = KSW2_HIP_OK;
+thread_local ksw2_hip_progress_cb t_cb = nullptr;
+thread_local void *t_cb_args = nullptr;
+thread_local int t_cb_interval = 256;
+
+int set_err(int s)
+{
+    t_last_error = s;
+    return s;
+}
+
+void reset_extz(ksw_extz_t *ez)
+{
+    ez->max_q = ez->max_t = ez->mqe_t = ez->mte_q = -1;
+    ez->max = 0, ez->score = ez->mqe = ez->mte = KSW_NEG_INF;
+    ez->n_cigar = 0, ez->zdropped = 0, ez->reach_end = 0, ez->stopped = 0;
+}
+
+}  // namespace
+
+namespace kswhip {
+
+// Normalised scoring + the reference's early-return test.  Returns false when
+// the reference returns right after ksw_reset_extz (ksw2_extd2_sse.c:81,105).
+bool normalise_scoring(int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q2, int8_t e2,
+                       KswScore *sc, ScoreClass *cls)
+{
+    const int qe_orig = q + e;
+    if (q2 + e2 < q + e) std::swap(q, q2), std::swap(e, e2);
+    int max_sc = mat[0], min_sc = mat[1];
+    for (int t = 1; t < m * m; ++t) max_sc = std::max<int>(max_sc, mat[t]), min_sc = std::min<int>(min_sc, mat[t]);
+    if (-min_sc > 2 * (q + e)) return false;
+    int long_thres = e != e2 ? (q2 - q) / (e - e2) - 1 : 0;
+    if (q2 + e2 + long_thres * e2 > q + e + long_thres * e) ++long_thres;
+    const int long_diff = long_thres * (e - e2) - (q2 - q) - e2;
+    sc->q = q, sc->e = e, sc->q2 = q2, sc->e2 = e2;
+    sc->G1 = q + e, sc->G2 = q2 + e2;
+    sc->qe_orig = qe_orig;
+    sc->long_thres = long_thres, sc->long_diff = long_diff;
+    sc->mch = mat[0], sc->mis = mat[1];
+
+    // Score class.  generic_mm: with KSW_EZ_GENERIC_SC the matrix is constant
+    // on the diagonal and off it, so s = (t == q) ? mat[0] : mat[1].
+    bool diag_const = true;
+    for (int a = 0; a < m; ++a)
+        for (int b = 0; b < m; ++b)
+            if (mat[a * m + b] != (a == b ? mat[0] : mat[1])) diag_const = false;
+    cls->generic_mm = diag_const;
+    cls->m = m;
+    cls->sc_N = mat[m * m - 1] == 0 ? -e2 : mat[m * m - 1];
+    cls->sc_N = (int)(int8_t)(uint8_t)(cls->sc_N & 0xff);
+
+    // Exactness of the fast kernel's int32 arithmetic.  With e > e2 the
+    // boundary values (edge_diff) are exactly the two-piece gap cost
+    // min(q+l*e, q2+l*e2), so the recurrence is the exact DP; with e <= e2 the
+    // reference's boundary switches pieces at the wrong length and z can
+    // exceed mat[0].  Every in-band intermediate of the int8 recurrence then
+    // lies in [-128, 127] when the match score is 0, mismatch <= 0,
+    // penalties are non-negative and
+    //   2*G1 <= 128, G1+G2 <= 128, G1+e2 <= 128, q2-mis <= 128, 2*G1-q <= 128;
+    // the bounds follow from u,v in [-G1, G1], x in [-G1,-e], x2 in [-G2,-e2],
+    // z in [mis, 0] (tests/test_oracle.py checks them against the
+    // instrumented restatement).
+    const int G1 = sc->G1, G2 = sc->G2, mis = mat[1];
+    bool ok = q >= 0 && e > e2 && q2 >= 0 && e2 >= 0 && mat[0] == 0 && mis <= 0;
+    ok = ok && 2 * G1 <= 128 && G1 + G2 <= 128 && G1 + e2 <= 128 && q2 - mis <= 128 && 2 * G1 - q <= 128;
+    ok = ok && G2 <= 127 && long_diff >= -128 && long_diff <= 127;
+    cls->fast_ok = ok;
+    return true;
+}
+
+}  // namespace kswhip
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+int ksw2_hip_last_error(void) { return t_last_error; }
+
+const char *ksw2_hip_error_string(int status)
+{
+    switch (status) {
+    case KSW2_HIP_OK: return "ok";
+    case KSW2_HIP_EINVAL: return "invalid argument";
+    case KSW2_HIP_EUNSUPPORTED: return "parameters not supported by the device path";
+    case KSW2_HIP_ENOMEM: return "out of memory";
+    case KSW2_HIP_EDEVICE: return "HIP runtime error";
+    case KSW2_HIP_ENODEV: return "no usable gfx950 device";
+    default: return "unknown status";
+    }
+}
+
+int ksw2_hip_init(int device) { return set_err(kswhip::device_init(device)); }
+
+void ksw2_hip_set_progress_callback(ksw2_hip_progress_cb callback, void *arguments, int antidiagonal_interval)
+{
+    t_cb = callback, t_cb_args = arguments;
+    t_cb_interval = antidiagonal_interval > 0 ? antidiagonal_interval : 1;
+}
+
+void ksw2_hip_clear_progress_callback(void)
+{
+    t_cb = nullptr, t_cb_args = nullptr, t_cb_interval = 256;
+}
+
+int ksw_extd2_hip_batch(int n, const ksw2_pair_t *pairs, int8_t m, const int8_t *mat, int8_t q, int8_t e,
+                        int8_t q2, int8_t e2, ksw_extz_t *ez)
+{
+    if (n < 0 || (n > 0 && (!pairs || !mat || !ez))) return set_err(KSW2_HIP_EINVAL);
+    int st = kswhip::run_batch_host(n, pairs, m, mat, q, e, q2, e2, ez, nullptr);
+    return set_err(st);
+}
+
+void ksw_extd2_hip(void *km, int qlen, const uint8_t *query, int tlen, const uint8_t *target, int8_t m,
+                   const int8_t *mat, int8_t q, int8_t e, int8_t q2, int8_t e2, int w, int zdrop, int end_bonus,
+                   int flag, ksw_extz_t *ez)
+{
+    if (!ez) { set_err(KSW2_HIP_EINVAL); return; }
+    ksw2_pair_t p;
+    p.query = query, p.target = target, p.qlen = qlen, p.tlen = tlen;
+    p.w = w, p.zdrop = zdrop, p.end_bonus = end_bonus, p.flag = flag;
+    int st = kswhip::run_batch_host(1, &p, m, mat, q, e, q2, e2, ez, km);
+    set_err(st);
+}
+
+}  // extern "C"
+
+namespace kswhip {
+
+// Grows ez->cigar to hold n entries with the reference's policy (capacity
+// 4, then doubling from the caller's m_cigar) and copies them in.
+int deliver_cigar(ksw_extz_t *ez, const uint32_t *src, int n, void *km)
+{
+    int m = ez->m_cigar;
+    if (n > m) {
+        while (m < n) m = m ? m << 1 : 4;
+        void *p;
+        if (km && krealloc) p = krealloc(km, ez->cigar, (size_t)m << 2);
+        else if (km) return KSW2_HIP_EUNSUPPORTED;  // arena without kalloc linked
+        else p = realloc(ez->cigar, (size_t)m << 2);
+        if (!p) return KSW2_HIP_ENOMEM;
+        ez->cigar = (uint32_t *)p;
+        ez->m_cigar = m;
+    }
+    if (n > 0) memcpy(ez->cigar, src, (size_t)n * sizeof(uint32_t));
+    ez->n_cigar = n;
+    return KSW2_HIP_OK;
+}
+
+void reset_ez(ksw_extz_t *ez) { reset_extz(ez); }
+
+}  // namespace kswhip

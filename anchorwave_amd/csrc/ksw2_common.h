@@ -1,0 +1,96 @@
+// ksw2_common.h -- structures shared by the host library and the gfx950 kernels.
+//
+// Data layout in HBM (one batch):
+//   seq arena   query and target codes of every pair, each padded to 16 bytes
+//   dir arena   direction bytes of every pair in stripe layout (see dir_index())
+//   bnd arena   per pair, 2 dwords per query column: the bottom row of the
+//               previous stripe (x, x2 packed as int16 pair; v)
+//   cig arena   per pair, qlen+tlen+2 uint32 CIGAR slots
+//   jobs / outs one KswJob and one KswOut per pair
+#pragma once
+#include <stdint.h>
+
+#ifndef __HIPCC__
+#define KSW_HD
+#else
+#define KSW_HD __host__ __device__
+#endif
+
+namespace kswhip {
+
+constexpr int kWave = 64;
+
+// Scoring shared by a batch, already normalised the way the reference
+// normalises it (minimap2/ksw2_extd2_sse.c:83-110): pieces swapped so that
+// q+e <= q2+e2, long-gap threshold and boundary difference precomputed.
+struct KswScore {
+    int32_t q, e, q2, e2;      // after the swap
+    int32_t G1, G2;            // q+e, q2+e2
+    int32_t qe_orig;           // q+e before the swap (H at r == 0 uses it)
+    int32_t long_thres, long_diff;
+    int32_t mch, mis;          // class-1 scores: equal codes / unequal codes
+};
+
+struct KswJob {
+    uint64_t q_off, t_off;     // byte offsets in the seq arena
+    uint64_t dir_off;          // byte offset in the dir arena
+    uint64_t bnd_off;          // dword offset in the bnd arena
+    uint64_t cig_off;          // uint32 offset in the cig arena
+    int32_t qlen, tlen;
+    int32_t flag, end_bonus;
+};
+
+struct KswOut {
+    int32_t score, mqe, mqe_t, mte, mte_q;
+    int32_t max, max_t, max_q;
+    int32_t zdropped, stopped, reach_end, n_cigar;
+};
+
+// Boundary difference injected on the first row/column at anti-diagonal r
+// (minimap2/ksw2_extd2_sse.c:167,171), as the int8 the reference stores.
+KSW_HD inline int32_t edge_diff(const KswScore &sc, int r)
+{
+    int v = r == 0 ? -sc.G1 : r < sc.long_thres ? -sc.e : r == sc.long_thres ? sc.long_diff : -sc.e2;
+    return (int32_t)(int8_t)(uint8_t)(v & 0xff);
+}
+
+// Sum of edge_diff(0..n): with G1 - qe_orig added this is H on the
+// boundary row/column, H(-1, n) = H(n, -1).
+KSW_HD inline int64_t edge_prefix(const KswScore &sc, int n)
+{
+    if (n < 0) return 0;
+    int64_t s = edge_diff(sc, 0);
+    if (n == 0) return s;
+    // k in [1, n]: -e below long_thres, long_diff at it, -e2 above
+    int lt = sc.long_thres;
+    int64_t lo_end = (lt - 1 < n ? lt - 1 : n);          // last k with k < lt
+    if (lo_end >= 1) s += (int64_t)(lo_end) * edge_diff(sc, 1);
+    if (lt >= 1 && lt <= n) s += edge_diff(sc, lt);
+    int64_t hi_start = (lt + 1 > 1 ? lt + 1 : 1);        // first k with k > lt
+    if (hi_start <= n) s += (int64_t)(n - hi_start + 1) * edge_diff(sc, (int)hi_start);
+    return s;
+}
+
+// Rows (target positions) handled by one lane of the stripe kernel.
+constexpr int kRowsPerLane = 8;
+constexpr int kStripeRows = kWave * kRowsPerLane;
+
+// Direction-byte address of cell (i = target, j = query) in the stripe layout:
+// stripe s = i / (64R), lane k, row rho; the lane visits column j at step
+// c = j + k; each step the wave writes 64 lanes x R bytes contiguously.
+KSW_HD inline uint64_t dir_index(int i, int j, int qlen, int R)
+{
+    const int SR = kWave * R;
+    const int s = i / SR, rem = i - s * SR, k = rem / R, rho = rem - k * R;
+    const uint64_t nsteps = (uint64_t)qlen + kWave - 1;
+    return (((uint64_t)s * nsteps + (uint64_t)(j + k)) * kWave + (uint64_t)k) * (uint64_t)R + (uint64_t)rho;
+}
+
+KSW_HD inline uint64_t dir_bytes(int qlen, int tlen, int R)
+{
+    const int SR = kWave * R;
+    const uint64_t nstripes = (uint64_t)((tlen + SR - 1) / SR);
+    return nstripes * ((uint64_t)qlen + kWave - 1) * kWave * (uint64_t)R;
+}
+
+}  // namespace kswhip

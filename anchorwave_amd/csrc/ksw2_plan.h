@@ -1,0 +1,64 @@
+// ksw2_plan.h -- host-side batch machinery shared by the C ABI entry points.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../include/ksw2_hip.h"
+#include "ksw2_common.h"
+
+namespace kswhip {
+
+struct ScoreClass {
+    bool generic_mm;  // with GENERIC_SC: s = (t == q) ? mat[0] : mat[1]
+    bool fast_ok;     // scoring for which the fast kernel is exact
+    int m;
+    int sc_N;         // wildcard score of the match/mismatch mode
+};
+
+bool normalise_scoring(int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q2, int8_t e2, KswScore *sc,
+                       ScoreClass *cls);
+int device_init(int device);
+int run_batch_host(int n, const ksw2_pair_t *pairs, int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q2,
+                   int8_t e2, ksw_extz_t *ez, void *km);
+int deliver_cigar(ksw_extz_t *ez, const uint32_t *src, int n, void *km);
+void reset_ez(ksw_extz_t *ez);
+
+// Why a pair cannot take the fast (unbanded stripe) kernel; 0 when it can.
+enum FastReject : int {
+    kFastOK = 0,
+    kRejBand = 1,
+    kRejZdrop = 2,
+    kRejFlags = 4,
+    kRejScoring = 8,
+    kRejWildcard = 16,
+};
+int fast_path_reject(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls);
+
+// A device-resident batch: sequences uploaded, arenas sized, jobs built.
+// Created from host pairs, run any number of times, results read back.
+struct Plan {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    KswScore sc{};
+    int n = 0;                       // pairs in the plan (all fast-path)
+    int n_left = 0, n_right = 0, n_score = 0;  // job groups, in that order
+    std::vector<KswJob> jobs;        // host copy
+    std::vector<int> order;          // plan slot -> caller index
+    uint64_t seq_bytes = 0, dir_bytes = 0, bnd_dwords = 0, cig_words = 0;
+    uint8_t *d_seq = nullptr, *d_dir = nullptr;
+    int32_t *d_bnd = nullptr;
+    uint32_t *d_cig = nullptr;
+    KswJob *d_jobs = nullptr;
+    KswOut *d_out = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    float last_dp_ms = 0.f, last_bt_ms = 0.f;
+};
+
+int plan_create(Plan **out, int n, const ksw2_pair_t *pairs, const int *idx, int nidx, const KswScore &sc);
+int plan_run(Plan *p);
+int plan_fetch(Plan *p, ksw_extz_t *ez, void *km);
+void plan_destroy(Plan *p);
+
+}  // namespace kswhip

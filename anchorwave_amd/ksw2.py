@@ -1,0 +1,188 @@
+"""ctypes binding of ``lib/libksw2hip.so`` (C ABI in ``include/ksw2_hip.h``).
+
+Mirrors the reference call ``ksw_extd2_sse(km, qlen, query, tlen, target, m,
+mat, q, e, q2, e2, w, zdrop, end_bonus, flag, ez)`` (minimap2/ksw2.h:91-107):
+same argument meaning, same ``ksw_extz_t`` output, same error conventions
+(``ez.zdropped`` / ``score == KSW_NEG_INF``), plus a batch entry point.
+
+The library is the product path.  Loading fails loudly when it has not been
+built (run ``__graft_entry__.build()``); there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Iterable, List, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libksw2hip.so")
+
+KSW_NEG_INF = -0x40000000
+KSW_EZ_SCORE_ONLY = 0x01
+KSW_EZ_RIGHT = 0x02
+KSW_EZ_GENERIC_SC = 0x04
+KSW_EZ_APPROX_MAX = 0x08
+KSW_EZ_APPROX_DROP = 0x10
+KSW_EZ_EXTZ_ONLY = 0x40
+KSW_EZ_REV_CIGAR = 0x80
+KSW_EZ_SEMIGLOBAL_END = 0x800
+
+STATUS = {0: "ok", -1: "invalid argument", -2: "unsupported", -3: "out of memory",
+          -4: "HIP runtime error", -5: "no usable gfx950 device"}
+
+# Symbols include/ksw2_hip.h declares; tests check the library exports each.
+EXPORTS = (
+    "ksw_extd2_hip",
+    "ksw_extd2_hip_batch",
+    "ksw2_hip_last_error",
+    "ksw2_hip_error_string",
+    "ksw2_hip_init",
+    "ksw2_hip_set_progress_callback",
+    "ksw2_hip_clear_progress_callback",
+)
+
+
+class KswExtz(C.Structure):
+    """``ksw_extz_t``, layout of minimap2/ksw2.h:40-50."""
+
+    _fields_ = [
+        ("max", C.c_uint32, 31),
+        ("zdropped", C.c_uint32, 1),
+        ("max_q", C.c_int), ("max_t", C.c_int),
+        ("mqe", C.c_int), ("mqe_t", C.c_int),
+        ("mte", C.c_int), ("mte_q", C.c_int),
+        ("score", C.c_int),
+        ("m_cigar", C.c_int), ("n_cigar", C.c_int),
+        ("reach_end", C.c_int),
+        ("stopped", C.c_int),
+        ("cigar", C.POINTER(C.c_uint32)),
+    ]
+
+
+assert C.sizeof(KswExtz) == 56
+
+
+class KswPair(C.Structure):
+    """``ksw2_pair_t`` of include/ksw2_hip.h."""
+
+    _fields_ = [
+        ("query", C.c_void_p), ("target", C.c_void_p),
+        ("qlen", C.c_int32), ("tlen", C.c_int32),
+        ("w", C.c_int32), ("zdrop", C.c_int32), ("end_bonus", C.c_int32), ("flag", C.c_int32),
+    ]
+
+
+@dataclass(frozen=True)
+class Result:
+    """Every output field of ``ksw_extz_t`` plus the CIGAR as a tuple."""
+
+    max: int
+    zdropped: int
+    max_q: int
+    max_t: int
+    mqe: int
+    mqe_t: int
+    mte: int
+    mte_q: int
+    score: int
+    n_cigar: int
+    reach_end: int
+    stopped: int
+    cigar: tuple
+
+    @staticmethod
+    def from_ez(ez: KswExtz) -> "Result":
+        cig = tuple(ez.cigar[i] for i in range(ez.n_cigar)) if ez.n_cigar > 0 else ()
+        return Result(ez.max, ez.zdropped, ez.max_q, ez.max_t, ez.mqe, ez.mqe_t, ez.mte,
+                      ez.mte_q, ez.score, ez.n_cigar, ez.reach_end, ez.stopped, cig)
+
+
+class KswError(RuntimeError):
+    pass
+
+
+_lib = None
+_libc = C.CDLL(None)
+_libc.free.argtypes = [C.c_void_p]
+
+
+def lib():
+    """Loads the HIP library; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise KswError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.ksw_extd2_hip.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int8,
+                                    C.c_void_p, C.c_int8, C.c_int8, C.c_int8, C.c_int8, C.c_int,
+                                    C.c_int, C.c_int, C.c_int, C.POINTER(KswExtz)]
+        L.ksw_extd2_hip.restype = None
+        L.ksw_extd2_hip_batch.argtypes = [C.c_int, C.POINTER(KswPair), C.c_int8, C.c_void_p, C.c_int8,
+                                          C.c_int8, C.c_int8, C.c_int8, C.POINTER(KswExtz)]
+        L.ksw_extd2_hip_batch.restype = C.c_int
+        L.ksw2_hip_last_error.restype = C.c_int
+        L.ksw2_hip_error_string.restype = C.c_char_p
+        L.ksw2_hip_error_string.argtypes = [C.c_int]
+        L.ksw2_hip_init.argtypes = [C.c_int]
+        L.ksw2_hip_init.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def default_matrix(match: int = 0, mismatch: int = -6, m: int = 5) -> np.ndarray:
+    """AnchorWave's matrix (alignSlidingWindow.cpp:247-256): ``match`` on the
+    diagonal including N vs N, ``mismatch`` elsewhere."""
+    mat = np.full((m, m), mismatch, dtype=np.int8)
+    np.fill_diagonal(mat, match)
+    return mat.reshape(-1)
+
+
+def _u8(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint8))
+
+
+def extd2(query, target, mat=None, m=5, q=8, e=2, q2=75, e2=1, w=-1, zdrop=-1, end_bonus=0,
+          flag=0, check=True) -> Result:
+    """One call of ``ksw_extd2_hip`` (km = NULL)."""
+    L = lib()
+    qs, ts = _u8(query), _u8(target)
+    mat = np.ascontiguousarray(np.asarray(default_matrix() if mat is None else mat, dtype=np.int8))
+    ez = KswExtz()
+    L.ksw_extd2_hip(None, len(qs), qs.ctypes.data, len(ts), ts.ctypes.data, m, mat.ctypes.data,
+                    q, e, q2, e2, w, zdrop, end_bonus, flag, C.byref(ez))
+    st = L.ksw2_hip_last_error()
+    res = Result.from_ez(ez)
+    if ez.cigar:
+        _libc.free(C.cast(ez.cigar, C.c_void_p))
+    if check and st != 0:
+        raise KswError(f"ksw_extd2_hip: {STATUS.get(st, st)}")
+    return res
+
+
+def extd2_batch(pairs: Sequence[tuple], mat=None, m=5, q=8, e=2, q2=75, e2=1, w=-1, zdrop=-1,
+                end_bonus=0, flag=0, check=True) -> List[Result]:
+    """``ksw_extd2_hip_batch`` over ``pairs`` = [(query, target), ...] or
+    [(query, target, flag), ...]; band/zdrop/end_bonus shared here."""
+    L = lib()
+    n = len(pairs)
+    keep = []
+    arr = (KswPair * max(n, 1))()
+    for k, pr in enumerate(pairs):
+        qs, ts = _u8(pr[0]), _u8(pr[1])
+        keep.append((qs, ts))
+        f = pr[2] if len(pr) > 2 else flag
+        arr[k] = KswPair(qs.ctypes.data, ts.ctypes.data, len(qs), len(ts), w, zdrop, end_bonus, f)
+    mat = np.ascontiguousarray(np.asarray(default_matrix() if mat is None else mat, dtype=np.int8))
+    ezs = (KswExtz * max(n, 1))()
+    st = L.ksw_extd2_hip_batch(n, arr, m, mat.ctypes.data, q, e, q2, e2, ezs)
+    out = []
+    for k in range(n):
+        out.append(Result.from_ez(ezs[k]))
+        if ezs[k].cigar:
+            _libc.free(C.cast(ezs[k].cigar, C.c_void_p))
+    if check and st != 0:
+        raise KswError(f"ksw_extd2_hip_batch: {STATUS.get(st, st)}")
+    return out

@@ -1,0 +1,127 @@
+/*
+ * ksw2_hip.h -- C ABI of the MI355X (gfx950) implementation of KSW2's
+ * two-piece-affine difference-recurrence aligner, ksw_extd2.
+ *
+ * Drop-in boundary: ksw_extd2_hip() has exactly the signature, argument
+ * meaning, output struct and error conventions of
+ *     ksw_extd2_sse / ksw_extd2_avx2 / ksw_extd2_avx512
+ * declared in the reference at minimap2/ksw2.h:91-107 (called from
+ * src/myImportandFunction/alignSlidingWindow.cpp:290-296, 382-388, 763-783,
+ * 841-845).  The batch and serving entry points below have no counterpart in
+ * the reference; they carry many such calls per launch.
+ *
+ * Plain C types only: no HIP or torch types cross this boundary.
+ */
+#ifndef KSW2_HIP_H
+#define KSW2_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+/* ---- Types and flags shared with the reference header (minimap2/ksw2.h:18-50).
+ * When the reference ksw2.h was included first (it defines KSW2_H_) its
+ * definitions are used and these are skipped, so both headers can coexist. */
+#ifndef KSW2_H_
+#define KSW_NEG_INF -0x40000000
+
+#define KSW_EZ_SCORE_ONLY     0x01   /* no path/CIGAR */
+#define KSW_EZ_RIGHT          0x02   /* right-align gaps */
+#define KSW_EZ_GENERIC_SC     0x04   /* full m*m matrix; else match/mismatch + wildcard m-1 */
+#define KSW_EZ_APPROX_MAX     0x08   /* approximate max */
+#define KSW_EZ_APPROX_DROP    0x10   /* approximate z-drop */
+#define KSW_EZ_EXTZ_ONLY      0x40   /* extension only */
+#define KSW_EZ_REV_CIGAR      0x80   /* reversed CIGAR */
+#define KSW_EZ_SPLICE_FOR     0x100
+#define KSW_EZ_SPLICE_REV     0x200
+#define KSW_EZ_SPLICE_FLANK   0x400
+#define KSW_EZ_SEMIGLOBAL_END 0x800  /* traceback from the better sequence end */
+#define KSW_EZ_SINGLETRACK    0x1000 /* reference-internal two-track traceback */
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* Layout identical to minimap2/ksw2.h:40-50 (56 bytes on x86-64). */
+typedef struct {
+    uint32_t max:31, zdropped:1;
+    int max_q, max_t;   /* cell of the best score seen (extension) */
+    int mqe, mqe_t;     /* best score reaching the end of the query */
+    int mte, mte_q;     /* best score reaching the end of the target */
+    int score;          /* score reaching both ends; KSW_NEG_INF if not reached */
+    int m_cigar, n_cigar;
+    int reach_end;
+    int stopped;        /* the progress callback asked to stop */
+    uint32_t *cigar;    /* len<<4 | op, op 0=M 1=I 2=D */
+} ksw_extz_t;
+#ifdef __cplusplus
+}
+#endif
+#endif /* KSW2_H_ */
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes of the entry points that return one. */
+#define KSW2_HIP_OK               0
+#define KSW2_HIP_EINVAL          -1  /* bad argument (NULL pointer, size out of range) */
+#define KSW2_HIP_EUNSUPPORTED    -2  /* parameters outside what the device path implements */
+#define KSW2_HIP_ENOMEM          -3  /* device or host allocation failed */
+#define KSW2_HIP_EDEVICE         -4  /* a HIP runtime call failed */
+#define KSW2_HIP_ENODEV          -5  /* no usable gfx950 device */
+
+/*
+ * One alignment, same contract as ksw_extd2_sse (minimap2/ksw2.h:91-107,
+ * body minimap2/ksw2_extd2_sse.c:39-419):
+ *   - ez is caller-owned; every field except cigar/m_cigar is reset
+ *     (minimap2/ksw2.h:194-199);
+ *   - ez->cigar is grown with krealloc(km, ...) when km != NULL and the
+ *     process links kalloc (as AnchorWave does), else with realloc(); the
+ *     caller frees it with kfree(km, ...) / free();
+ *   - no return value: ez->score == KSW_NEG_INF when the end is not reached,
+ *     ez->zdropped = 1 on band exit / z-drop and also when the device path
+ *     cannot run the call (the AVX kernels' allocation-failure convention,
+ *     minimap2/ksw2_extd2_avx.c:138-141); ksw2_hip_last_error() then tells why.
+ */
+void ksw_extd2_hip(void *km, int qlen, const uint8_t *query, int tlen, const uint8_t *target,
+                   int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q2, int8_t e2,
+                   int w, int zdrop, int end_bonus, int flag, ksw_extz_t *ez);
+
+/* Status of the last entry point called on this host thread, and its text. */
+int ksw2_hip_last_error(void);
+const char *ksw2_hip_error_string(int status);
+
+/* Library / device set-up.  Optional: every entry point initialises lazily.
+ * Returns KSW2_HIP_OK or KSW2_HIP_ENODEV. */
+int ksw2_hip_init(int device);
+
+/* ---- Progress / deadline hook: same meaning as minimap2/ksw2.h:52-63 and
+ * minimap2/ksw2_progress.c:15-38.  The callback is polled on the host while
+ * a call runs; returning 0 raises a device stop flag, the call ends with
+ * ez->stopped = 1 and no CIGAR (callers discard stopped results:
+ * alignSlidingWindow.cpp:1890-1893).  Thread-local, like the reference. */
+typedef int (*ksw2_hip_progress_cb)(void *arguments, int antidiagonal, int total_antidiagonals);
+void ksw2_hip_set_progress_callback(ksw2_hip_progress_cb callback, void *arguments, int antidiagonal_interval);
+void ksw2_hip_clear_progress_callback(void);
+
+/* ---- Batch API (no reference counterpart: replaces many per-interval
+ * ksw_extd2 calls, AnchorTaskExecutor.cpp:406-470, with one launch).
+ * Scoring (m, mat, q, e, q2, e2) is shared by the batch; band, z-drop,
+ * end bonus and flags are per pair. */
+typedef struct {
+    const uint8_t *query;   /* host pointer, codes 0..m-1 */
+    const uint8_t *target;  /* host pointer, codes 0..m-1 */
+    int32_t qlen, tlen;
+    int32_t w, zdrop, end_bonus, flag;
+} ksw2_pair_t;
+
+/* Aligns n pairs; ez[i] receives pair i's result exactly as n calls to
+ * ksw_extd2_hip(NULL, ...) would (cigar grown with realloc).  Returns a
+ * status code; on error no ez[] is touched beyond ksw_reset_extz. */
+int ksw_extd2_hip_batch(int n, const ksw2_pair_t *pairs,
+                        int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q2, int8_t e2,
+                        ksw_extz_t *ez);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KSW2_HIP_H */
