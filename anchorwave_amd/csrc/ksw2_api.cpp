@@ -152,6 +152,65 @@ void ksw_extd2_hip(void *km, int qlen, const uint8_t *query, int tlen, const uin
     set_err(st);
 }
 
+struct ksw2_hip_plan {
+    kswhip::Plan *p;
+    double cells;
+};
+
+int ksw2_hip_plan_create(ksw2_hip_plan **plan, int n, const ksw2_pair_t *pairs, int8_t m, const int8_t *mat,
+                         int8_t q, int8_t e, int8_t q2, int8_t e2)
+{
+    if (!plan || n < 0 || (n > 0 && (!pairs || !mat))) return set_err(KSW2_HIP_EINVAL);
+    *plan = nullptr;
+    if (m <= 1) return set_err(KSW2_HIP_EUNSUPPORTED);
+    KswScore sc;
+    ScoreClass cls;
+    if (!normalise_scoring(m, mat, q, e, q2, e2, &sc, &cls)) return set_err(KSW2_HIP_EUNSUPPORTED);
+    std::vector<int> idx(n);
+    double cells = 0;
+    for (int k = 0; k < n; ++k) {
+        if (pairs[k].qlen <= 0 || pairs[k].tlen <= 0 || !pairs[k].query || !pairs[k].target)
+            return set_err(KSW2_HIP_EINVAL);
+        if (kswhip::fast_path_reject(pairs[k], m, cls) != kswhip::kFastOK) return set_err(KSW2_HIP_EUNSUPPORTED);
+        idx[k] = k;
+        cells += (double)pairs[k].qlen * pairs[k].tlen;
+    }
+    kswhip::Plan *p = nullptr;
+    int st = kswhip::plan_create(&p, n, pairs, idx.data(), n, sc);
+    if (st != KSW2_HIP_OK) return set_err(st);
+    *plan = new ksw2_hip_plan{p, cells};
+    return set_err(KSW2_HIP_OK);
+}
+
+int ksw2_hip_plan_run(ksw2_hip_plan *plan)
+{
+    if (!plan) return set_err(KSW2_HIP_EINVAL);
+    return set_err(kswhip::plan_run(plan->p));
+}
+
+int ksw2_hip_plan_fetch(ksw2_hip_plan *plan, ksw_extz_t *ez)
+{
+    if (!plan || (!ez && plan->p->n > 0)) return set_err(KSW2_HIP_EINVAL);
+    for (int k = 0; k < plan->p->n; ++k) kswhip::reset_ez(ez + k);
+    return set_err(kswhip::plan_fetch(plan->p, ez, nullptr));
+}
+
+int ksw2_hip_plan_timing(const ksw2_hip_plan *plan, float *dp_ms, float *backtrack_ms, double *cells)
+{
+    if (!plan) return set_err(KSW2_HIP_EINVAL);
+    if (dp_ms) *dp_ms = plan->p->last_dp_ms;
+    if (backtrack_ms) *backtrack_ms = plan->p->last_bt_ms;
+    if (cells) *cells = plan->cells;
+    return set_err(KSW2_HIP_OK);
+}
+
+void ksw2_hip_plan_destroy(ksw2_hip_plan *plan)
+{
+    if (!plan) return;
+    kswhip::plan_destroy(plan->p);
+    delete plan;
+}
+
 }  // extern "C"
 
 namespace kswhip {

@@ -42,6 +42,11 @@ EXPORTS = (
     "ksw2_hip_init",
     "ksw2_hip_set_progress_callback",
     "ksw2_hip_clear_progress_callback",
+    "ksw2_hip_plan_create",
+    "ksw2_hip_plan_run",
+    "ksw2_hip_plan_fetch",
+    "ksw2_hip_plan_timing",
+    "ksw2_hip_plan_destroy",
 )
 
 
@@ -128,6 +133,18 @@ def lib():
         L.ksw2_hip_error_string.argtypes = [C.c_int]
         L.ksw2_hip_init.argtypes = [C.c_int]
         L.ksw2_hip_init.restype = C.c_int
+        L.ksw2_hip_plan_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(KswPair), C.c_int8,
+                                           C.c_void_p, C.c_int8, C.c_int8, C.c_int8, C.c_int8]
+        L.ksw2_hip_plan_create.restype = C.c_int
+        L.ksw2_hip_plan_run.argtypes = [C.c_void_p]
+        L.ksw2_hip_plan_run.restype = C.c_int
+        L.ksw2_hip_plan_fetch.argtypes = [C.c_void_p, C.POINTER(KswExtz)]
+        L.ksw2_hip_plan_fetch.restype = C.c_int
+        L.ksw2_hip_plan_timing.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                           C.POINTER(C.c_double)]
+        L.ksw2_hip_plan_timing.restype = C.c_int
+        L.ksw2_hip_plan_destroy.argtypes = [C.c_void_p]
+        L.ksw2_hip_plan_destroy.restype = None
         _lib = L
     return _lib
 
@@ -186,3 +203,66 @@ def extd2_batch(pairs: Sequence[tuple], mat=None, m=5, q=8, e=2, q2=75, e2=1, w=
     if check and st != 0:
         raise KswError(f"ksw_extd2_hip_batch: {STATUS.get(st, st)}")
     return out
+
+
+def _pair_array(pairs, w=-1, zdrop=-1, end_bonus=0, flag=0):
+    n = len(pairs)
+    keep = []
+    arr = (KswPair * max(n, 1))()
+    for k, pr in enumerate(pairs):
+        qs, ts = _u8(pr[0]), _u8(pr[1])
+        keep.append((qs, ts))
+        f = pr[2] if len(pr) > 2 else flag
+        arr[k] = KswPair(qs.ctypes.data, ts.ctypes.data, len(qs), len(ts), w, zdrop, end_bonus, f)
+    return arr, keep
+
+
+class Plan:
+    """A device-resident batch (``ksw2_hip_plan_*``): upload once, run many
+    times, fetch results.  Used by the benchmark to time device work with the
+    inputs already in HBM."""
+
+    def __init__(self, pairs, mat=None, m=5, q=8, e=2, q2=75, e2=1, w=-1, zdrop=-1, end_bonus=0,
+                 flag=0):
+        L = lib()
+        self.n = len(pairs)
+        arr, keep = _pair_array(pairs, w, zdrop, end_bonus, flag)
+        mat = np.ascontiguousarray(np.asarray(default_matrix() if mat is None else mat, dtype=np.int8))
+        h = C.c_void_p()
+        st = L.ksw2_hip_plan_create(C.byref(h), self.n, arr, m, mat.ctypes.data, q, e, q2, e2)
+        if st != 0:
+            raise KswError(f"ksw2_hip_plan_create: {STATUS.get(st, st)}")
+        self.h = h
+
+    def run(self):
+        st = lib().ksw2_hip_plan_run(self.h)
+        if st != 0:
+            raise KswError(f"ksw2_hip_plan_run: {STATUS.get(st, st)}")
+
+    def timing(self):
+        dp, bt, cells = C.c_float(), C.c_float(), C.c_double()
+        lib().ksw2_hip_plan_timing(self.h, C.byref(dp), C.byref(bt), C.byref(cells))
+        return dp.value, bt.value, cells.value
+
+    def fetch(self) -> List[Result]:
+        ezs = (KswExtz * max(self.n, 1))()
+        st = lib().ksw2_hip_plan_fetch(self.h, ezs)
+        out = []
+        for k in range(self.n):
+            out.append(Result.from_ez(ezs[k]))
+            if ezs[k].cigar:
+                _libc.free(C.cast(ezs[k].cigar, C.c_void_p))
+        if st != 0:
+            raise KswError(f"ksw2_hip_plan_fetch: {STATUS.get(st, st)}")
+        return out
+
+    def close(self):
+        if self.h:
+            lib().ksw2_hip_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

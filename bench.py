@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark: GCUPS of batched ksw_extd2 (score + CIGAR, bit-exact) on one MI355X.
+
+Workload (one step): the fixed batch - 4,096 pairs, target 2,000 bp uniform
+ACGT, query with 5% substitutions, 1% geometric indels and 10% of pairs with
+one 50-300 bp indel; AnchorWave scoring (match 0, mismatch -6, q/e 8/2,
+q2/e2 75/1), global alignment, full CIGAR, all pairs in one launch.  Inputs are
+uploaded to HBM before the timed region; a step is the DP sweep plus the
+device backtrack that leaves every score and CIGAR in HBM.
+
+Prints one JSON line (rank 0).  Extra legs:
+  cpu_baseline  the reference ksw_extd2_avx512 (oracle/_ref, compiled from the
+                reference sources) over the same 4,096 pairs on all host cores
+                (or the C restatement when that build is absent);
+  exact         every GPU result compared field by field with those CPU results;
+  one_at_a_time the same kernels launched on single-pair batches.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from anchorwave_amd import ksw2, workloads as W  # noqa: E402
+
+METRIC = "GCUPS, batched ksw_extd2 score+CIGAR (bit-exact); p50/p99 task latency"
+# Algorithmic integer work per cell of the reference inner loop with
+# traceback (minimap2/ksw2_extd2_sse.c:237-283): 51 ops.
+OPS_PER_CELL = 51
+N_CU, LANES, CLK_GHZ = 256, 64, 2.4
+PEAK_PACKED16_TOPS = N_CU * LANES * CLK_GHZ * 2 / 1e3   # 78.6 T int16 lane-ops/s
+PEAK_INT32_TOPS = N_CU * LANES * CLK_GHZ / 1e3          # 39.3 T int32 lane-ops/s
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def read_traffic(name: str):
+    """HBM bytes per dispatch of the DP kernel, from the committed rocprofv3
+    PMC summary (line 'traffic_bytes_per_dispatch: N'), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            m = re.search(r"traffic_bytes_per_dispatch:\s*([0-9.eE+]+)", f.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def cpu_baseline(pairs, threads: int):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O  # checker / baseline only
+
+    if O.ref_available():
+        kind = "reference"
+        fn = lambda p: O.run_ref(p[0], p[1], mat=W_MAT, **SC)  # noqa: E731
+    else:
+        kind = "port"
+        fn = lambda p: O.run_oracle(p[0], p[1], mat=W_MAT, **SC)  # noqa: E731
+    # largest-cells-first, one pair per task
+    order = sorted(range(len(pairs)), key=lambda k: -len(pairs[k][0]) * len(pairs[k][1]))
+    res = [None] * len(pairs)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        for k, r in zip(order, ex.map(lambda k: fn(pairs[k]), order)):
+            res[k] = r
+    dt = time.perf_counter() - t0
+    return kind, dt, res
+
+
+SC = dict(m=5, q=8, e=2, q2=75, e2=1)
+W_MAT = ksw2.default_matrix(W.MATCH, W.MISMATCH)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--pairs", type=int, default=4096, help="pairs in the fixed batch")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / cross-check")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores (max 16)")
+    ap.add_argument("--single-sample", type=int, default=16, help="pairs timed one at a time")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: every pair is independent, so "
+                          "more GPUs would each take a share of the pairs; only --gpus 1 is measured"}))
+        return 2
+
+    log(f"[bench] generating fixed batch ({args.pairs} pairs x 2000 bp)")
+    pairs = W.fixed(seed=2, n=args.pairs)
+    cells = W.cells(pairs)
+    log("[bench] uploading plan")
+    plan = ksw2.Plan(pairs, mat=W_MAT, **SC)
+    for _ in range(args.warmup):
+        plan.run()
+    log(f"[bench] warmup done; timing {args.steps} steps")
+    dp_ms, bt_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.run()  # synchronous: DP + backtrack complete on return
+        d, b, _c = plan.timing()
+        dp_ms.append(d)
+        bt_ms.append(b)
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    gcups = cells / (ms_per_step * 1e-3) / 1e9
+    dp_avg = float(np.mean(dp_ms))
+    log(f"[bench] {ms_per_step:.2f} ms/step, {gcups:.1f} GCUPS (DP kernel {dp_avg:.2f} ms, "
+        f"backtrack {np.mean(bt_ms):.2f} ms)")
+    gpu_res = plan.fetch()
+    dir_bytes = sum(((len(t) + 511) // 512) * (len(q) + 63) * 512 for q, t in pairs)
+    plan.close()
+
+    # one pair per launch (inputs resident), the batching baseline
+    single = None
+    if args.single_sample > 0:
+        sp = pairs[: args.single_sample]
+        tot, cs = 0.0, 0
+        for q, t in sp:
+            p1 = ksw2.Plan([(q, t)], mat=W_MAT, **SC)
+            p1.run()
+            t1 = time.perf_counter()
+            p1.run()
+            tot += time.perf_counter() - t1
+            cs += len(q) * len(t)
+            p1.close()
+        single = cs / tot / 1e9
+        log(f"[bench] one-at-a-time: {single:.2f} GCUPS over {len(sp)} pairs")
+
+    achieved = cells * OPS_PER_CELL / (dp_avg * 1e-3) / 1e12
+    roofline = {
+        "bound": "valu",
+        "achieved": round(achieved, 3),
+        "peak": round(PEAK_PACKED16_TOPS, 1),
+        "unit": "Tops/s",
+        "frac": round(achieved / PEAK_PACKED16_TOPS, 4),
+        "traffic": read_traffic("stripe_kernel_fixed_4096x2000_pmc.md"),
+        "kernel": "ksw_stripe_kernel",
+        "ops_per_cell": OPS_PER_CELL,
+        "avg_launch_ms": round(dp_avg, 3),
+        "frac_of_int32_peak": round(achieved / PEAK_INT32_TOPS, 4),
+        "traceback_write_GBs": round(dir_bytes / (dp_avg * 1e-3) / 1e9, 1),
+        "traceback_write_frac_hbm": round(dir_bytes / (dp_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+    }
+    out = {
+        "metric": METRIC,
+        "value": round(gcups, 2),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"fixed batch: {args.pairs} pairs x 2000 bp, 5% subs, 1% indels, "
+                               "10% with a 50-300 bp indel; global, full CIGAR",
+                   "pairs": args.pairs, "cells": cells, "scoring": "0/-6, 8/2, 75/1",
+                   "device_ms_dp": round(dp_avg, 3), "device_ms_backtrack": round(float(np.mean(bt_ms)), 3)},
+        "roofline": roofline,
+        "one_at_a_time_gcups": None if single is None else round(single, 3),
+        "batched_over_one_at_a_time": None if not single else round(gcups / single, 1),
+    }
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        log(f"[bench] CPU baseline on {threads} threads")
+        kind, dt, cpu_res = cpu_baseline(pairs, threads)
+        exact = sum(1 for g, c in zip(gpu_res, cpu_res) if g.__dict__ == c.__dict__)
+        out["cpu_baseline"] = {"value": round(cells / dt / 1e9, 3), "unit": "GCUPS", "cores": threads,
+                               "kind": kind,
+                               "sample": f"the full fixed batch ({len(pairs)} pairs, {cells:.3g} cells), "
+                                         "ksw_extd2_avx512 one pair per task, largest first"}
+        out["exact"] = {"agree": exact, "of": len(pairs), "frac": exact / len(pairs)}
+        log(f"[bench] CPU {cells / dt / 1e9:.2f} GCUPS; exact {exact}/{len(pairs)}")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

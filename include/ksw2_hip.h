@@ -121,6 +121,23 @@ int ksw_extd2_hip_batch(int n, const ksw2_pair_t *pairs,
                         int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q2, int8_t e2,
                         ksw_extz_t *ez);
 
+/* ---- Device-resident batch ("plan"; no reference counterpart).  The
+ * pairs' codes are uploaded to HBM once; each run aligns all of them (DP
+ * sweep + device backtrack) with results left in HBM until fetched.  This
+ * is how a caller that owns many intervals keeps them resident and times
+ * the device work alone.  All pairs must be eligible for a device kernel,
+ * else KSW2_HIP_EUNSUPPORTED. */
+typedef struct ksw2_hip_plan ksw2_hip_plan;
+int ksw2_hip_plan_create(ksw2_hip_plan **plan, int n, const ksw2_pair_t *pairs,
+                         int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q2, int8_t e2);
+int ksw2_hip_plan_run(ksw2_hip_plan *plan);
+/* results of the last run into ez[0..n) (cigar grown with realloc) */
+int ksw2_hip_plan_fetch(ksw2_hip_plan *plan, ksw_extz_t *ez);
+/* device time of the last run: DP sweep kernels and backtrack kernel, from
+ * HIP events on the plan's stream; also the sum of qlen*tlen */
+int ksw2_hip_plan_timing(const ksw2_hip_plan *plan, float *dp_ms, float *backtrack_ms, double *cells);
+void ksw2_hip_plan_destroy(ksw2_hip_plan *plan);
+
 #ifdef __cplusplus
 }
 #endif
