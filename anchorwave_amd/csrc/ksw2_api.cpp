@@ -121,6 +121,20 @@ const char *ksw2_hip_error_string(int status)
 
 int ksw2_hip_init(int device) { return set_err(kswhip::device_init(device)); }
 
+int ksw2_hip_route(int qlen, const uint8_t *query, int tlen, const uint8_t *target, int8_t m, const int8_t *mat,
+                   int8_t q, int8_t e, int8_t q2, int8_t e2, int w, int zdrop, int flag)
+{
+    if (m <= 1 || qlen <= 0 || tlen <= 0 || !mat) return -1;
+    KswScore sc;
+    ScoreClass cls;
+    if (!normalise_scoring(m, mat, q, e, q2, e2, &sc, &cls)) return -1;
+    ksw2_pair_t p;
+    p.query = query, p.target = target, p.qlen = qlen, p.tlen = tlen;
+    p.w = w, p.zdrop = zdrop, p.end_bonus = 0, p.flag = flag;
+    if (!query || !target) return -1;
+    return kswhip::fast_path_reject(p, m, cls);
+}
+
 void ksw2_hip_set_progress_callback(ksw2_hip_progress_cb callback, void *arguments, int antidiagonal_interval)
 {
     t_cb = callback, t_cb_args = arguments;

@@ -40,6 +40,7 @@ EXPORTS = (
     "ksw2_hip_last_error",
     "ksw2_hip_error_string",
     "ksw2_hip_init",
+    "ksw2_hip_route",
     "ksw2_hip_set_progress_callback",
     "ksw2_hip_clear_progress_callback",
     "ksw2_hip_plan_create",
@@ -133,6 +134,9 @@ def lib():
         L.ksw2_hip_error_string.argtypes = [C.c_int]
         L.ksw2_hip_init.argtypes = [C.c_int]
         L.ksw2_hip_init.restype = C.c_int
+        L.ksw2_hip_route.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int8, C.c_void_p,
+                                     C.c_int8, C.c_int8, C.c_int8, C.c_int8, C.c_int, C.c_int, C.c_int]
+        L.ksw2_hip_route.restype = C.c_int
         L.ksw2_hip_plan_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(KswPair), C.c_int8,
                                            C.c_void_p, C.c_int8, C.c_int8, C.c_int8, C.c_int8]
         L.ksw2_hip_plan_create.restype = C.c_int
@@ -177,6 +181,15 @@ def extd2(query, target, mat=None, m=5, q=8, e=2, q2=75, e2=1, w=-1, zdrop=-1, e
     if check and st != 0:
         raise KswError(f"ksw_extd2_hip: {STATUS.get(st, st)}")
     return res
+
+
+def route(query, target, mat=None, m=5, q=8, e=2, q2=75, e2=1, w=-1, zdrop=-1, flag=0) -> int:
+    """``ksw2_hip_route``: 0 when the call takes the unbanded fast kernel,
+    else the reason mask; -1 when the reference returns before aligning."""
+    qs, ts = _u8(query), _u8(target)
+    mat = np.ascontiguousarray(np.asarray(default_matrix() if mat is None else mat, dtype=np.int8))
+    return lib().ksw2_hip_route(len(qs), qs.ctypes.data, len(ts), ts.ctypes.data, m, mat.ctypes.data,
+                                q, e, q2, e2, w, zdrop, flag)
 
 
 def extd2_batch(pairs: Sequence[tuple], mat=None, m=5, q=8, e=2, q2=75, e2=1, w=-1, zdrop=-1,

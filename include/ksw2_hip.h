@@ -94,6 +94,17 @@ const char *ksw2_hip_error_string(int status);
  * Returns KSW2_HIP_OK or KSW2_HIP_ENODEV. */
 int ksw2_hip_init(int device);
 
+/* Host-only (no GPU needed): which device kernel a ksw_extd2_hip call with
+ * these arguments takes.  0 = the unbanded fast kernel; otherwise a bit mask
+ * of why not: 1 band narrower than the matrix, 2 z-drop on, 4 APPROX_MAX or
+ * SINGLETRACK flag, 8 scoring outside the fast kernel's exactness domain,
+ * 16 wildcard code m-1 present in match/mismatch mode.  -1 when the
+ * reference returns before aligning (m <= 1, empty sequence, or
+ * -min(mat) > 2(q+e), minimap2/ksw2_extd2_sse.c:81,105). */
+int ksw2_hip_route(int qlen, const uint8_t *query, int tlen, const uint8_t *target,
+                   int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q2, int8_t e2,
+                   int w, int zdrop, int flag);
+
 /* ---- Progress / deadline hook: same meaning as minimap2/ksw2.h:52-63 and
  * minimap2/ksw2_progress.c:15-38.  The callback is polled on the host while
  * a call runs; returning 0 raises a device stop flag, the call ends with

@@ -28,6 +28,7 @@ void oracle_range_reset(void)
     int k;
     for (k = 0; k < 16; ++k) g_range.lo[k] = 0x7fffffff, g_range.hi[k] = -0x7fffffff;
     g_range.wraps = 0;
+    g_range.cap_binds = 0;
 }
 void oracle_range_enable(int on) { g_range_on = on; if (on) oracle_range_reset(); }
 void oracle_range_get(oracle_range_t *out) { *out = g_range; }
@@ -273,6 +274,7 @@ void oracle_ksw_extd2(int qlen, const uint8_t *query, int tlen, const uint8_t *t
                 d = z > a2 ? d : 3; z = max8(z, a2);
                 d = z > b2 ? d : 4; z = max8(z, b2);
             }
+            if (g_range_on && z > sc_mch && t >= st0 && t <= en0) ++g_range.cap_binds;
             z = min8(z, sc_mch);
             if (g_range_on) {
                 rng(4, z - vl, &wrapped), rng(5, z - ut, &wrapped);

@@ -34,11 +34,13 @@ void oracle_ksw_extd2(int qlen, const uint8_t *query, int tlen, const uint8_t *t
                       int isa, uint8_t *dir_ij, int stop_r);
 
 /* Range instrumentation: when enabled, every int8 intermediate of the cell
- * recurrence is also computed in int32 and the extremes are recorded, so a
- * test can show where (if anywhere) the reference's int8 lanes wrap. */
+ * recurrence is also computed in int32 and the extremes over in-band cells
+ * are recorded, so a test can show where (if anywhere) the reference's int8
+ * lanes wrap and whether the match-score cap ever changes z. */
 typedef struct {
     int32_t lo[16], hi[16];   /* per intermediate, see ksw2_oracle.c */
     int64_t wraps;            /* cells where some intermediate left [-128,127] */
+    int64_t cap_binds;        /* cells where z exceeded mat[0] before z = min(z, mat[0]) */
 } oracle_range_t;
 void oracle_range_enable(int on);
 void oracle_range_get(oracle_range_t *out);

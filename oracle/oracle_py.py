@@ -200,7 +200,8 @@ class RangeProbe:
         lib = oracle_lib()
 
         class R(C.Structure):
-            _fields_ = [("lo", C.c_int32 * 16), ("hi", C.c_int32 * 16), ("wraps", C.c_int64)]
+            _fields_ = [("lo", C.c_int32 * 16), ("hi", C.c_int32 * 16), ("wraps", C.c_int64),
+                        ("cap_binds", C.c_int64)]
 
         r = R()
         lib.oracle_range_get(C.byref(r))
@@ -208,6 +209,7 @@ class RangeProbe:
         self.lo = list(r.lo)
         self.hi = list(r.hi)
         self.wraps = r.wraps
+        self.cap_binds = r.cap_binds
         return False
 
 

@@ -1,0 +1,57 @@
+"""The C-ABI library (CPU only: loading, exports, host-side logic)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+
+from anchorwave_amd import ksw2
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ksw2_hip.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = set(re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(ksw\w+)\s*\(", src, flags=re.M))
+    return sorted(n for n in names if not n.endswith("_cb"))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ksw2.lib()  # fails loudly if the library was not built
+    decl = declared_functions()
+    assert "ksw_extd2_hip" in decl and len(decl) >= 10
+    nm = subprocess.run(["nm", "-D", "--defined-only", ksw2.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(line.split()[-1] for line in nm.splitlines() if " T " in line)
+    missing = [d for d in decl if d not in exported]
+    assert not missing, missing
+    for d in decl:
+        getattr(lib, d)
+    assert set(ksw2.EXPORTS) == set(decl)
+
+
+def test_struct_layout():
+    assert C.sizeof(ksw2.KswExtz) == 56
+    assert ksw2.KswExtz.max_q.offset == 4 and ksw2.KswExtz.cigar.offset == 48
+    assert C.sizeof(ksw2.KswPair) == 40
+
+
+def test_error_strings_and_no_device_path():
+    lib = ksw2.lib()
+    for st in (0, -1, -2, -3, -4, -5):
+        assert lib.ksw2_hip_error_string(st)
+    if not os.path.exists("/dev/kfd"):
+        assert lib.ksw2_hip_init(0) == -5  # no GPU: ENODEV, no crash
+
+
+def test_early_returns_need_no_device():
+    """Calls the reference answers before aligning return the reset ez
+    without touching a GPU (minimap2/ksw2_extd2_sse.c:81,105)."""
+    t = np.zeros(10, np.uint8)
+    r = ksw2.extd2(t[:0], t)  # qlen == 0
+    assert r.score == ksw2.KSW_NEG_INF and r.n_cigar == 0 and r.zdropped == 0
+    r = ksw2.extd2(t, t, mat=ksw2.default_matrix(0, -30))  # -min_sc > 2(q+e)
+    assert r.score == ksw2.KSW_NEG_INF and r.n_cigar == 0 and r.mqe == ksw2.KSW_NEG_INF
+    assert r.max_t == -1 and r.max_q == -1 and r.mte_q == -1 and r.mqe_t == -1
