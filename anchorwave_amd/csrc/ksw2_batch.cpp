@@ -74,6 +74,46 @@ int fast_path_reject(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls)
     return why;
 }
 
+// Queue order for the stripe kernel.  A pair with S stripes finishes no
+// earlier than (qlen + 63) + (S - 1) * lag steps, lag being the distance a
+// stripe trails its predecessor.  Pairs are ranked by that critical path
+// (longest first); stripe s of the pair at rank p gets key p + s * L, where L
+// is about the number of tasks the workers start during one lag, so the
+// stripes of a pair enter the queue roughly when their predecessor has
+// published enough columns.  Keys grow with s within a pair, which keeps
+// every stripe behind its predecessor in the queue.
+static void order_tasks(const std::vector<KswJob> &jobs, const std::vector<int> &members, int R, int workers,
+                        std::vector<StripeTask> &out)
+{
+    const int SR = kWave * R;
+    const double lag = 3.0 * kWave;  // steps a stripe trails its predecessor
+    std::vector<int> rank(members);
+    std::stable_sort(rank.begin(), rank.end(), [&](int a, int b) {
+        const int Sa = (jobs[a].tlen + SR - 1) / SR, Sb = (jobs[b].tlen + SR - 1) / SR;
+        const double ta = jobs[a].qlen + 63.0 + (Sa - 1) * lag, tb = jobs[b].qlen + 63.0 + (Sb - 1) * lag;
+        return ta > tb;
+    });
+    double steps = 0, ntask = 0;
+    for (int id : members) {
+        const int S = (jobs[id].tlen + SR - 1) / SR;
+        steps += (double)S * (jobs[id].qlen + 63.0), ntask += S;
+    }
+    const double mean_steps = ntask > 0 ? steps / ntask : 1.0;
+    const double L = std::max(1.0, workers * lag / mean_steps);
+    std::vector<std::pair<double, StripeTask>> keyed;
+    keyed.reserve((size_t)ntask);
+    for (size_t p = 0; p < rank.size(); ++p) {
+        const int id = rank[p];
+        const int S = (jobs[id].tlen + SR - 1) / SR;
+        for (int st = 0; st < S; ++st) keyed.push_back({(double)p + st * L, StripeTask{id, st}});
+    }
+    std::stable_sort(keyed.begin(), keyed.end(),
+                     [](const std::pair<double, StripeTask> &a, const std::pair<double, StripeTask> &b) {
+                         return a.first < b.first;
+                     });
+    for (auto &k : keyed) out.push_back(k.second);
+}
+
 int plan_create(Plan **out, int n, const ksw2_pair_t *pairs, const int *idx, int nidx, const KswScore &sc)
 {
     (void)n;
@@ -82,6 +122,8 @@ int plan_create(Plan **out, int n, const ksw2_pair_t *pairs, const int *idx, int
     Plan *p = new Plan();
     p->device = g_device;
     p->sc = sc;
+    p->R = kRowsPerLane;
+    const int R = p->R, SR = kWave * R;
     // job groups: left-aligned traceback, right-aligned, score-only
     std::vector<int> g[3];
     for (int k = 0; k < nidx; ++k) {
@@ -89,22 +131,39 @@ int plan_create(Plan **out, int n, const ksw2_pair_t *pairs, const int *idx, int
         const int grp = (pp.flag & KSW_EZ_SCORE_ONLY) ? 2 : (pp.flag & KSW_EZ_RIGHT) ? 1 : 0;
         g[grp].push_back(idx[k]);
     }
-    p->n_left = (int)g[0].size(), p->n_right = (int)g[1].size(), p->n_score = (int)g[2].size();
+    std::vector<int> grp_of;
     for (int grp = 0; grp < 3; ++grp)
-        for (int id : g[grp]) p->order.push_back(id);
+        for (int id : g[grp]) p->order.push_back(id), grp_of.push_back(grp);
     p->n = (int)p->order.size();
     p->jobs.resize(p->n);
     uint64_t so = 0, dof = 0, bo = 0, co = 0;
+    int stripes = 0;
     for (int k = 0; k < p->n; ++k) {
         const ksw2_pair_t &pp = pairs[p->order[k]];
         KswJob &j = p->jobs[k];
         j.qlen = pp.qlen, j.tlen = pp.tlen, j.flag = pp.flag, j.end_bonus = pp.end_bonus;
+        j.R = R;
         j.q_off = so, so = round_up(so + pp.qlen, 16);
         j.t_off = so, so = round_up(so + pp.tlen, 16);
         j.dir_off = dof;
-        if (!(pp.flag & KSW_EZ_SCORE_ONLY)) dof = round_up(dof + dir_bytes(pp.qlen, pp.tlen, kRowsPerLane), 256);
+        if (!(pp.flag & KSW_EZ_SCORE_ONLY)) dof = round_up(dof + dir_bytes(pp.qlen, pp.tlen, R), 256);
         j.bnd_off = bo, bo += round_up(2ull * pp.qlen, 64);
         j.cig_off = co, co += round_up((uint64_t)pp.qlen + pp.tlen + 2, 16);
+        j.stripe_off = stripes, stripes += (pp.tlen + SR - 1) / SR;
+    }
+    p->nstripes_total = stripes;
+    // task queues, one per group, concatenated
+    std::vector<StripeTask> tasks;
+    for (int grp = 0; grp < 3; ++grp) {
+        std::vector<int> members;
+        for (int k = 0; k < p->n; ++k)
+            if (grp_of[k] == grp) members.push_back(k);
+        p->task_off[grp] = (int)tasks.size();
+        p->workers[grp] = members.empty() ? 0 : stripe_resident_waves(R, grp == 1, grp == 2);
+        if (const char *ev = getenv("KSW2_HIP_WORKERS"))  // tuning override
+            if (atoi(ev) > 0) p->workers[grp] = std::min(p->workers[grp], atoi(ev));
+        order_tasks(p->jobs, members, R, std::max(p->workers[grp], 1), tasks);
+        p->ntasks[grp] = (int)tasks.size() - p->task_off[grp];
     }
     p->seq_bytes = std::max<uint64_t>(so, 16), p->dir_bytes = std::max<uint64_t>(dof, 256);
     p->bnd_dwords = std::max<uint64_t>(bo, 64), p->cig_words = std::max<uint64_t>(co, 16);
@@ -121,6 +180,10 @@ int plan_create(Plan **out, int n, const ksw2_pair_t *pairs, const int *idx, int
     if (hipMalloc(&p->d_cig, p->cig_words * 4) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
     if (hipMalloc(&p->d_jobs, sizeof(KswJob) * std::max(p->n, 1)) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
     if (hipMalloc(&p->d_out, sizeof(KswOut) * std::max(p->n, 1)) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
+    if (hipMalloc(&p->d_tasks, sizeof(StripeTask) * std::max<size_t>(tasks.size(), 1)) != hipSuccess)
+        return fail(KSW2_HIP_ENOMEM);
+    if (hipMalloc(&p->d_sync, sizeof(int32_t) * (4 + (size_t)stripes)) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
+    if (hipMalloc(&p->d_recs, sizeof(StripeRec) * std::max(stripes, 1)) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
     // stage sequences through pinned memory
     uint8_t *h_seq = nullptr;
     if (hipHostMalloc((void **)&h_seq, p->seq_bytes, hipHostMallocDefault) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
@@ -132,9 +195,12 @@ int plan_create(Plan **out, int n, const ksw2_pair_t *pairs, const int *idx, int
     }
     hipError_t e1 = hipMemcpyAsync(p->d_seq, h_seq, p->seq_bytes, hipMemcpyHostToDevice, p->stream);
     hipError_t e2 = hipMemcpyAsync(p->d_jobs, p->jobs.data(), sizeof(KswJob) * p->n, hipMemcpyHostToDevice, p->stream);
+    hipError_t e4 = tasks.empty() ? hipSuccess
+                                  : hipMemcpyAsync(p->d_tasks, tasks.data(), sizeof(StripeTask) * tasks.size(),
+                                                   hipMemcpyHostToDevice, p->stream);
     hipError_t e3 = hipStreamSynchronize(p->stream);
     (void)hipHostFree(h_seq);
-    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) return fail(KSW2_HIP_EDEVICE);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) return fail(KSW2_HIP_EDEVICE);
     *out = p;
     return KSW2_HIP_OK;
 }
@@ -142,20 +208,20 @@ int plan_create(Plan **out, int n, const ksw2_pair_t *pairs, const int *idx, int
 int plan_run(Plan *p)
 {
     if (p->n == 0) return KSW2_HIP_OK;
+    KSW_HIP_TRY(hipEventRecord(p->ev[0], p->stream));
+    KSW_HIP_TRY(launch_zero(p->d_sync, 4 + (size_t)p->nstripes_total, p->stream));
     StripeArgs sa;
     sa.seq = p->d_seq, sa.dir = p->d_dir, sa.bnd = p->d_bnd, sa.sc = p->sc;
-    KSW_HIP_TRY(hipEventRecord(p->ev[0], p->stream));
-    int base = 0;
-    const int counts[3] = {p->n_left, p->n_right, p->n_score};
+    sa.jobs = p->d_jobs, sa.out = p->d_out, sa.progress = p->d_sync + 4, sa.recs = p->d_recs;
     for (int grp = 0; grp < 3; ++grp) {
-        sa.jobs = p->d_jobs + base, sa.out = p->d_out + base, sa.njobs = counts[grp];
-        KSW_HIP_TRY(launch_stripe(sa, grp == 1, grp == 2, p->stream));
-        base += counts[grp];
+        sa.tasks = p->d_tasks + p->task_off[grp], sa.ntasks = p->ntasks[grp];
+        sa.queue_head = p->d_sync + grp;
+        KSW_HIP_TRY(launch_stripe(sa, p->R, grp == 1, grp == 2, p->workers[grp], p->stream));
     }
     KSW_HIP_TRY(hipEventRecord(p->ev[1], p->stream));
     BacktrackArgs ba;
-    ba.dir = p->d_dir, ba.cig = p->d_cig, ba.jobs = p->d_jobs, ba.out = p->d_out, ba.njobs = p->n;
-    ba.R = kRowsPerLane;
+    ba.dir = p->d_dir, ba.cig = p->d_cig, ba.jobs = p->d_jobs, ba.out = p->d_out, ba.recs = p->d_recs;
+    ba.njobs = p->n, ba.sc = p->sc;
     KSW_HIP_TRY(launch_backtrack(ba, p->stream));
     KSW_HIP_TRY(hipEventRecord(p->ev[2], p->stream));
     KSW_HIP_TRY(hipStreamSynchronize(p->stream));
@@ -188,7 +254,7 @@ void plan_destroy(Plan *p)
 {
     if (!p) return;
     for (void *ptr : {(void *)p->d_seq, (void *)p->d_dir, (void *)p->d_bnd, (void *)p->d_cig, (void *)p->d_jobs,
-                      (void *)p->d_out})
+                      (void *)p->d_out, (void *)p->d_tasks, (void *)p->d_sync, (void *)p->d_recs})
         if (ptr) (void)hipFree(ptr);
     for (auto &e : p->ev)
         if (e) (void)hipEventDestroy(e);

@@ -19,6 +19,7 @@
 namespace kswhip {
 
 constexpr int kWave = 64;
+constexpr int32_t KSW_NEG_INF_I = -0x40000000;
 
 // Scoring shared by a batch, already normalised the way the reference
 // normalises it (minimap2/ksw2_extd2_sse.c:83-110): pieces swapped so that
@@ -38,6 +39,22 @@ struct KswJob {
     uint64_t cig_off;          // uint32 offset in the cig arena
     int32_t qlen, tlen;
     int32_t flag, end_bonus;
+    int32_t stripe_off;        // first slot of this pair in the per-stripe arrays
+    int32_t R;                 // rows per lane of the stripe kernel that ran it
+};
+
+// One unit of the stripe kernel's work queue: stripe `stripe` of pair `job`.
+// Stripes of a pair appear in increasing order, so when a wave takes stripe
+// s the wave that took s-1 is already running.
+struct StripeTask {
+    int32_t job, stripe;
+};
+
+// Per-stripe partial of the mqe bookkeeping (column qlen-1): the sum of
+// -8u over the stripe's rows, and the stripe-local minimum of the running
+// sum with the row that attains it (ties to the larger row).
+struct StripeRec {
+    int32_t total, best, best_i, pad;
 };
 
 struct KswOut {
@@ -71,9 +88,9 @@ KSW_HD inline int64_t edge_prefix(const KswScore &sc, int n)
     return s;
 }
 
-// Rows (target positions) handled by one lane of the stripe kernel.
+// Rows (target positions) handled by one lane of the stripe kernel; the
+// kernel is instantiated for these values and the planner picks one per pair.
 constexpr int kRowsPerLane = 8;
-constexpr int kStripeRows = kWave * kRowsPerLane;
 
 // Direction-byte address of cell (i = target, j = query) in the stripe layout:
 // stripe s = i / (64R), lane k, row rho; the lane visits column j at step

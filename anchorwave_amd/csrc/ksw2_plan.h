@@ -43,15 +43,23 @@ struct Plan {
     hipStream_t stream = nullptr;
     KswScore sc{};
     int n = 0;                       // pairs in the plan (all fast-path)
-    int n_left = 0, n_right = 0, n_score = 0;  // job groups, in that order
+    int R = kRowsPerLane;            // rows per lane of the stripe kernel
     std::vector<KswJob> jobs;        // host copy
     std::vector<int> order;          // plan slot -> caller index
+    // job groups: 0 left-aligned traceback, 1 right-aligned, 2 score-only
+    int ntasks[3] = {0, 0, 0};
+    int task_off[3] = {0, 0, 0};
+    int workers[3] = {0, 0, 0};
+    int nstripes_total = 0;
     uint64_t seq_bytes = 0, dir_bytes = 0, bnd_dwords = 0, cig_words = 0;
     uint8_t *d_seq = nullptr, *d_dir = nullptr;
     int32_t *d_bnd = nullptr;
     uint32_t *d_cig = nullptr;
     KswJob *d_jobs = nullptr;
     KswOut *d_out = nullptr;
+    StripeTask *d_tasks = nullptr;
+    int32_t *d_sync = nullptr;       // 4 queue heads, then one progress counter per stripe
+    StripeRec *d_recs = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     float last_dp_ms = 0.f, last_bt_ms = 0.f;
 };

@@ -5,12 +5,17 @@
 // score-only), evaluated for every cell (i = target, j = query).  This file
 // evaluates it in a different order and representation than the reference:
 //
-//  * Stripes.  The target is cut into stripes of 64*R rows.  Lane k of the
+//  * Stripes.  The target is cut into stripes of 64*R rows.  Lane k of a
 //    wavefront owns R consecutive rows and sweeps the query; at step c it is
 //    at column j = c - k, so the cell above (i-1, j) was produced by lane k-1
 //    one step earlier and arrives through a DPP wave_shr:1.  Lane 0 takes the
 //    bottom row of the previous stripe from a per-pair column buffer; the
 //    left neighbour (i, j-1) stays in the lane's registers.
+//  * Persistent workers.  Waves pull (pair, stripe) tasks from a device
+//    queue.  Consecutive stripes of one pair run concurrently on different
+//    waves: stripe s waits on a per-stripe progress counter before it reads
+//    the columns stripe s-1 has published.  A stripe is queued after its
+//    predecessor, so the predecessor is always held by a running wave.
 //  * Scaled, negated state.  Every difference value w is kept as -8*w plus a
 //    3-bit tag, so one v_min3 pair yields both the maximum of the five
 //    candidates (s, a, b, a2, b2) and the index the reference's strict '>'
@@ -21,7 +26,6 @@
 //    (then int32 arithmetic gives the same values), the match score is 0 and
 //    no mismatch scores above it (then z = min(z, mat[0]) never binds and H
 //    never exceeds 0, so ez->max stays 0), there is no band and no z-drop.
-//    Everything else goes to the general kernel.
 //
 // Outputs per pair: the direction byte of every cell (bits as in
 // minimap2/ksw2.h:156-159), and the H bookkeeping of
@@ -127,11 +131,6 @@ __device__ __forceinline__ int cell(const Consts &k, int tb, int qb, int &Xa, in
     return d;
 }
 
-template <int R> struct DirVec;
-template <> struct DirVec<4> { using T = uint32_t; };
-template <> struct DirVec<8> { using T = uint2; };
-template <> struct DirVec<16> { using T = uint4; };
-
 template <int R>
 __device__ __forceinline__ void store_dirs(uint8_t *p, const int (&d)[R])
 {
@@ -149,178 +148,212 @@ __device__ __forceinline__ void store_dirs(uint8_t *p, const int (&d)[R])
     }
 }
 
-__device__ __forceinline__ int bnd_load(const int32_t *p)
+__device__ __forceinline__ int ld_agent(const int32_t *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-}  // namespace
-
-// One wavefront per pair.  Grid: one 64-thread block per job.
-template <int R, bool RIGHT, bool SCORE_ONLY>
-__global__ void __launch_bounds__(64) ksw_stripe_kernel(StripeArgs a)
+// Spins until stripe `prev`'s published column count reaches `need`.  Every
+// lane polls the same counter and the exit test uses lane 0's value, so the
+// loop is uniform.  The producer always ends at qlen, so `need` <= qlen
+// guarantees termination.
+__device__ __forceinline__ int wait_columns(const int32_t *prog, int need, int known)
 {
-    const int job_id = blockIdx.x;
-    if (job_id >= a.njobs) return;
+    if (known >= need) return known;
+    int seen = __builtin_amdgcn_readfirstlane(__hip_atomic_load(prog, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+    while (seen < need) {
+        __builtin_amdgcn_s_sleep(1);
+        seen = __builtin_amdgcn_readfirstlane(__hip_atomic_load(prog, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    return seen;
+}
+
+// One stripe of one pair, computed by the calling wave.
+template <int R, bool RIGHT, bool SCORE_ONLY>
+__device__ __attribute__((noinline)) void run_stripe(const StripeArgs &a, const Consts &kc, int job_id, int s)
+{
     const KswJob jb = a.jobs[job_id];
-    const KswScore sc = a.sc;
-    const Consts kc = make_consts<RIGHT>(sc);
-    const int lane = threadIdx.x;
-    const int qlen = jb.qlen, tlen = jb.tlen;
+    const KswScore &sc = a.sc;
+    const int lane = threadIdx.x % kWave;
+    const int qlen = __builtin_amdgcn_readfirstlane(jb.qlen), tlen = __builtin_amdgcn_readfirstlane(jb.tlen);
     const uint8_t *qs = a.seq + jb.q_off;
     const uint8_t *ts = a.seq + jb.t_off;
     uint8_t *dir = a.dir + jb.dir_off;
     int32_t *bnd = a.bnd + jb.bnd_off;
     constexpr int SR = kWave * R;
+    s = __builtin_amdgcn_readfirstlane(s);
     const int nstripes = (tlen + SR - 1) / SR;
     const int nsteps = qlen + kWave - 1;
-    const int s_last = (tlen - 1) / SR;
-    const int lane_last = ((tlen - 1) % SR) / R;
+    const bool last_stripe = s == nstripes - 1;
     const int rho_last = (tlen - 1) % R;
-
-    // H on the boundary: H(-1,n) = H(n,-1) = G1 - qe_orig + sum edge_diff(0..n)
-    const int64_t h_anchor = (int64_t)sc.G1 - sc.qe_orig;
-    const int h_top_last_col = (int)(h_anchor + edge_prefix(sc, qlen - 1));  // H(-1, qlen-1)
-    const int h_left_last_row = (int)(h_anchor + edge_prefix(sc, tlen - 1)); // H(tlen-1, -1)
-
-    // mqe bookkeeping in the -8 domain: Hn8(i) = -8 * (H(i,qlen-1) - H(-1,qlen-1))
-    int mqe_best = 0x7fffffff, mqe_i = -1;
-    int col_carry = 0;  // Hn8 of the last row of the previous stripe
-    // mte / score for row tlen-1: Hr8(j) = -8 * (H(tlen-1,j) - H(tlen-1,-1))
-    int row_acc = 0, mte_best = 0x7fffffff, mte_j = -1;
+    int32_t *prog_prev = a.progress + jb.stripe_off + s - 1;
+    int32_t *prog_mine = a.progress + jb.stripe_off + s;
+    const bool has_next = s + 1 < nstripes;
 
     int U[R], Yh[R], Y2h[R], TB[R];
-    for (int s = 0; s < nstripes; ++s) {
-        const int i0 = s * SR + lane * R;
+    const int i0 = s * SR + lane * R;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int i = i0 + r;
-            TB[r] = i < tlen ? (int)ts[i] : 0;
-            U[r] = -8 * edge_diff(sc, i);
-            Yh[r] = 0;
-            Y2h[r] = 0;
-        }
-        const bool last_stripe = s == s_last;
-        int Xo = 0, X2o = 0, Vo = 0;        // this lane's bottom-row outputs
-        int bin_xx = 0, bin_v = 0;          // lane-0 boundary queue (rotating)
-        int bnx_xx = 0, bnx_v = 0;          // next block of the queue
-        int out_xx = 0, out_v = 0;          // lane-63 output queue (rotating)
-        if (s > 0) {
-            __threadfence();
-            const int c0 = lane, c1 = kWave + lane;
-            bin_xx = c0 < qlen ? bnd_load(bnd + 2 * c0) : 0;
-            bin_v = c0 < qlen ? bnd_load(bnd + 2 * c0 + 1) : 0;
-            bnx_xx = c1 < qlen ? bnd_load(bnd + 2 * c1) : 0;
-            bnx_v = c1 < qlen ? bnd_load(bnd + 2 * c1 + 1) : 0;
-        }
-        int qb_next = lane == 0 ? (int)qs[0] : 0;
-        for (int c = 0; c < nsteps; ++c) {
-            const int j = c - lane;
-            // value for lane 0: the cell above the stripe in column c
-            int bx, bx2, bv;
-            if (s == 0) {
-                bx = 0, bx2 = 0, bv = -8 * edge_diff(sc, c);
-            } else {
-                bx = (int)(int16_t)(bin_xx & 0xffff);
-                bx2 = bin_xx >> 16;
-                bv = bin_v;
-            }
-            int Xa = dpp_shr1(bx, Xo), X2a = dpp_shr1(bx2, X2o), Va = dpp_shr1(bv, Vo);
-            if (s > 0) {
-                // rotate the queue so lane 0 sees column c+1 next step
-                if ((c & (kWave - 1)) == kWave - 1) {
-                    bin_xx = bnx_xx, bin_v = bnx_v;
-                    const int cn = c + 1 + kWave + lane;
-                    bnx_xx = cn < qlen ? bnd_load(bnd + 2 * cn) : 0;
-                    bnx_v = cn < qlen ? bnd_load(bnd + 2 * cn + 1) : 0;
-                } else {
-                    bin_xx = dpp_shl1(0, bin_xx);
-                    bin_v = dpp_shl1(0, bin_v);
-                }
-            }
-            const int qb = qb_next;
-            qb_next = (j + 1 >= 0 && j + 1 < qlen) ? (int)qs[j + 1] : 0;
-            if (j >= 0 && j < qlen) {
-                int d[R];
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    d[r] = cell<RIGHT, SCORE_ONLY>(kc, TB[r], qb, Xa, X2a, Va, U[r], Yh[r], Y2h[r]);
-                    if (last_stripe && r == rho_last) {
-                        row_acc += Va;  // Va now holds this cell's V
-                        if (row_acc <= mte_best) mte_best = row_acc, mte_j = j;
-                    }
-                }
-                Xo = Xa, X2o = X2a, Vo = Va;
-                if (!SCORE_ONLY)
-                    store_dirs<R>(dir + (((uint64_t)s * nsteps + c) * kWave + lane) * R, d);
-            }
-            if (s + 1 < nstripes) {
-                // lane 63 appends its bottom-row output (column c-63) to the
-                // queue; every 64 steps the queue is written out
-                const int packed = (Xo & 0xffff) | (X2o << 16);
-                out_xx = dpp_shl1(packed, out_xx);
-                out_v = dpp_shl1(Vo, out_v);
-                if ((c & (kWave - 1)) == kWave - 1 || c == nsteps - 1) {
-                    const int col = c - 2 * (kWave - 1) + lane;
-                    if (col >= 0 && col < qlen) {
-                        bnd[2 * col] = out_xx;
-                        bnd[2 * col + 1] = out_v;
-                    }
-                }
-            }
-        }
-        // mqe: every lane holds U = -8u(i, qlen-1) of its rows (frozen after
-        // its last active step); H(i,qlen-1) = H(-1,qlen-1) + sum_{i'<=i} u.
-        int lsum = 0;
-#pragma unroll
-        for (int r = 0; r < R; ++r) lsum += U[r];
-        int incl = lsum;
-#pragma unroll
-        for (int off = 1; off < kWave; off <<= 1) {
-            const int o = __shfl_up(incl, off, kWave);
-            if (lane >= off) incl += o;
-        }
-        int run = col_carry + incl - lsum;
-        int best = 0x7fffffff, best_i = -1;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            run += U[r];
-            const int i = i0 + r;
-            if (i < tlen && run <= best) best = run, best_i = i;
-        }
-        // wave reduction: smallest Hn8 (largest H), ties to the larger row
-#pragma unroll
-        for (int off = 1; off < kWave; off <<= 1) {
-            const int ob = __shfl_xor(best, off, kWave);
-            const int oi = __shfl_xor(best_i, off, kWave);
-            if (ob < best || (ob == best && oi > best_i)) best = ob, best_i = oi;
-        }
-        if (best_i >= 0 && best <= mqe_best) mqe_best = best, mqe_i = best_i;
-        col_carry += __shfl(incl, kWave - 1, kWave);
+    for (int r = 0; r < R; ++r) {
+        const int i = i0 + r;
+        TB[r] = i < tlen ? (int)ts[i] : 0;
+        U[r] = -8 * edge_diff(sc, i);
+        Yh[r] = 0;
+        Y2h[r] = 0;
     }
-    // publish from the lane that owns row tlen-1
-    const int sc_row = __shfl(row_acc, lane_last, kWave);
-    const int mte_b = __shfl(mte_best, lane_last, kWave);
-    const int mte_jj = __shfl(mte_j, lane_last, kWave);
+    int row_acc = 0, mte_best = 0x7fffffff, mte_j = -1;  // row tlen-1 in the -8 domain
+    int Xo = 0, X2o = 0, Vo = 0;        // this lane's bottom-row outputs
+    int bin_xx = 0, bin_v = 0;          // lane-0 boundary queue (rotating)
+    int bnx_xx = 0, bnx_v = 0;          // next block of the queue
+    int out_xx = 0, out_v = 0;          // lane-63 output queue (rotating)
+    int known = 0;                      // columns of stripe s-1 known published
+    if (s > 0) {
+        known = wait_columns(prog_prev, min(2 * kWave, qlen), known);
+        const int c0 = lane, c1 = kWave + lane;
+        bin_xx = c0 < qlen ? ld_agent(bnd + 2 * c0) : 0;
+        bin_v = c0 < qlen ? ld_agent(bnd + 2 * c0 + 1) : 0;
+        bnx_xx = c1 < qlen ? ld_agent(bnd + 2 * c1) : 0;
+        bnx_v = c1 < qlen ? ld_agent(bnd + 2 * c1 + 1) : 0;
+    }
+    int qb_next = lane == 0 ? (int)qs[0] : 0;
+    for (int c = 0; c < nsteps; ++c) {
+        const int j = c - lane;
+        // value for lane 0: the cell above the stripe in column c
+        const int top_v = -8 * edge_diff(sc, c);  // row -1 (first stripe)
+        const int bx = s == 0 ? 0 : (int)(int16_t)(bin_xx & 0xffff);
+        const int bx2 = s == 0 ? 0 : bin_xx >> 16;
+        const int bv = s == 0 ? top_v : bin_v;
+        int Xa = dpp_shr1(bx, Xo), X2a = dpp_shr1(bx2, X2o), Va = dpp_shr1(bv, Vo);
+        if (s > 0) {
+            // rotate the queue so lane 0 sees column c+1 next step
+            if ((c & (kWave - 1)) == kWave - 1) {
+                bin_xx = bnx_xx, bin_v = bnx_v;
+                known = wait_columns(prog_prev, min(c + 1 + 2 * kWave, qlen), known);
+                const int cn = c + 1 + kWave + lane;
+                bnx_xx = cn < qlen ? ld_agent(bnd + 2 * cn) : 0;
+                bnx_v = cn < qlen ? ld_agent(bnd + 2 * cn + 1) : 0;
+            } else {
+                bin_xx = dpp_shl1(0, bin_xx);
+                bin_v = dpp_shl1(0, bin_v);
+            }
+        }
+        const int qb = qb_next;
+        qb_next = (j + 1 >= 0 && j + 1 < qlen) ? (int)qs[j + 1] : 0;
+        if (j >= 0 && j < qlen) {
+            int d[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                d[r] = cell<RIGHT, SCORE_ONLY>(kc, TB[r], qb, Xa, X2a, Va, U[r], Yh[r], Y2h[r]);
+                if (last_stripe && r == rho_last) {
+                    row_acc += Va;  // Va now holds this cell's V
+                    if (row_acc <= mte_best) mte_best = row_acc, mte_j = j;
+                }
+            }
+            Xo = Xa, X2o = X2a, Vo = Va;
+            if (!SCORE_ONLY)
+                store_dirs<R>(dir + (((uint64_t)s * nsteps + c) * kWave + lane) * R, d);
+        }
+        if (has_next) {
+            // lane 63 appends its bottom-row output (column c-63) to the
+            // queue; every 64 steps the queue is written out and published
+            const int packed = (Xo & 0xffff) | (X2o << 16);
+            out_xx = dpp_shl1(packed, out_xx);
+            out_v = dpp_shl1(Vo, out_v);
+            if ((c & (kWave - 1)) == kWave - 1 || c == nsteps - 1) {
+                const int col = c - 2 * (kWave - 1) + lane;
+                if (col >= 0 && col < qlen) {
+                    bnd[2 * col] = out_xx;
+                    bnd[2 * col + 1] = out_v;
+                }
+                __threadfence();
+                if (lane == 0)
+                    __hip_atomic_store(prog_mine, min(c - (kWave - 1) + 1, qlen), __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+    // mqe partials: every lane holds U = -8u(i, qlen-1) of its rows (frozen
+    // after its last active step); H(i,qlen-1) = H(-1,qlen-1) + sum_{i'<=i} u.
+    int lsum = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) lsum += U[r];
+    int incl = lsum;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int o = __shfl_up(incl, off, kWave);
+        if (lane >= off) incl += o;
+    }
+    int run = incl - lsum;
+    int best = 0x7fffffff, best_i = -1;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        run += U[r];
+        const int i = i0 + r;
+        if (i < tlen && run <= best) best = run, best_i = i;
+    }
+    // wave reduction: smallest value (largest H), ties to the larger row
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int ob = __shfl_xor(best, off, kWave);
+        const int oi = __shfl_xor(best_i, off, kWave);
+        if (ob < best || (ob == best && oi > best_i)) best = ob, best_i = oi;
+    }
+    const int total = __builtin_amdgcn_readlane(incl, kWave - 1);
     if (lane == 0) {
-        KswOut o;
-        o.score = h_left_last_row - (sc_row >> 3);
-        o.mte = h_left_last_row - (mte_b >> 3);
-        o.mte_q = mte_jj;
-        o.mqe = h_top_last_col - (mqe_best >> 3);
-        o.mqe_t = mqe_i;
-        o.max = 0, o.max_t = -1, o.max_q = -1;
-        o.zdropped = 0, o.stopped = 0, o.reach_end = 0, o.n_cigar = 0;
-        a.out[job_id] = o;
+        StripeRec rec;
+        rec.total = total, rec.best = best, rec.best_i = best_i, rec.pad = 0;
+        a.recs[jb.stripe_off + s] = rec;
+    }
+    if (last_stripe) {
+        // publish score / mte from the lane that owns row tlen-1
+        const int lane_last = ((tlen - 1) % SR) / R;
+        const int sc_row = __builtin_amdgcn_readlane(row_acc, lane_last);
+        const int mte_b = __builtin_amdgcn_readlane(mte_best, lane_last);
+        const int mte_jj = __builtin_amdgcn_readlane(mte_j, lane_last);
+        if (lane == 0) {
+            const int64_t h_anchor = (int64_t)sc.G1 - sc.qe_orig;
+            const int h_left_last_row = (int)(h_anchor + edge_prefix(sc, tlen - 1));  // H(tlen-1, -1)
+            KswOut o;
+            o.score = h_left_last_row - (sc_row >> 3);
+            o.mte = h_left_last_row - (mte_b >> 3);
+            o.mte_q = mte_jj;
+            o.mqe = KSW_NEG_INF_I, o.mqe_t = -1;  // finalised by the backtrack kernel
+            o.max = 0, o.max_t = -1, o.max_q = -1;
+            o.zdropped = 0, o.stopped = 0, o.reach_end = 0, o.n_cigar = 0;
+            a.out[job_id] = o;
+        }
     }
 }
 
-// Device backtrack: one thread per pair walks the direction bytes from the
-// start cell chosen as in minimap2/ksw2_extd2_sse.c:398-418 and runs the
+}  // namespace
+
+// Persistent workers: each wave repeatedly takes the next (pair, stripe)
+// task; the loop ends for every wave once the queue is drained.
+template <int R, bool RIGHT, bool SCORE_ONLY>
+__global__ void __launch_bounds__(64) ksw_stripe_kernel(StripeArgs a)
+{
+    const Consts kc = make_consts<RIGHT>(a.sc);
+    const int lane = threadIdx.x % kWave;
+    for (;;) {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(a.queue_head, 1);
+        // the task index is wave-uniform: keep it (and everything derived
+        // from it) in scalar registers so the stripe loop stays uniform
+        t = __builtin_amdgcn_readfirstlane(t);
+        if (t >= a.ntasks) break;
+        const int job = __builtin_amdgcn_readfirstlane(a.tasks[t].job);
+        const int stripe = __builtin_amdgcn_readfirstlane(a.tasks[t].stripe);
+        run_stripe<R, RIGHT, SCORE_ONLY>(a, kc, job, stripe);
+    }
+}
+
+// Device backtrack: one thread per pair.  First folds the per-stripe mqe
+// partials in row order (ties to the later row, the '>=' of
+// minimap2/ksw2_extd2_sse.c:371), then walks the direction bytes from the
+// start cell chosen as in minimap2/ksw2_extd2_sse.c:398-418, running the
 // state machine of ksw_backtrack (minimap2/ksw2.h:160-192) for an unbanded
-// matrix (no cell of the path can leave the band there, so the
-// off/off_end forcing never triggers).  CIGAR ops are pushed run-length
-// encoded and reversed at the end unless KSW_EZ_REV_CIGAR.
+// matrix (no cell of the path can leave the band there, so the off/off_end
+// forcing never triggers).  CIGAR ops are pushed run-length encoded and
+// reversed at the end unless KSW_EZ_REV_CIGAR.
 __global__ void __launch_bounds__(64) ksw_backtrack_kernel(BacktrackArgs a)
 {
     const int job_id = blockIdx.x * blockDim.x + threadIdx.x;
@@ -329,6 +362,20 @@ __global__ void __launch_bounds__(64) ksw_backtrack_kernel(BacktrackArgs a)
     KswOut o = a.out[job_id];
     const int flag = jb.flag;
     const int qlen = jb.qlen, tlen = jb.tlen;
+    {
+        const int SR = kWave * jb.R;
+        const int nstripes = (tlen + SR - 1) / SR;
+        int carry = 0, best = 0x7fffffff, best_i = -1;
+        for (int s = 0; s < nstripes; ++s) {
+            const StripeRec rec = a.recs[jb.stripe_off + s];
+            if (rec.best_i >= 0 && carry + rec.best <= best) best = carry + rec.best, best_i = rec.best_i;
+            carry += rec.total;
+        }
+        const int64_t h_anchor = (int64_t)a.sc.G1 - a.sc.qe_orig;
+        const int h_top_last_col = (int)(h_anchor + edge_prefix(a.sc, qlen - 1));  // H(-1, qlen-1)
+        o.mqe = h_top_last_col - (best >> 3);
+        o.mqe_t = best_i;
+    }
     int i0 = -1, j0 = -1;
     if (!(flag & 0x01)) {  // KSW_EZ_SCORE_ONLY: no path
         if (!o.stopped && !o.zdropped && (flag & 0x800)) {          // SEMIGLOBAL_END
@@ -353,7 +400,7 @@ __global__ void __launch_bounds__(64) ksw_backtrack_kernel(BacktrackArgs a)
             else cig[n++] = (uint32_t)len << 4 | op, last = op;
         };
         while (i >= 0 && j >= 0) {
-            const int tmp = dir[dir_index(i, j, qlen, a.R)];
+            const int tmp = dir[dir_index(i, j, qlen, jb.R)];
             if (state == 0) state = tmp & 7;
             else if (!(tmp >> (state + 2) & 1)) state = 0;
             if (state == 0) state = tmp & 7;
@@ -374,25 +421,82 @@ __global__ void __launch_bounds__(64) ksw_backtrack_kernel(BacktrackArgs a)
     a.out[job_id] = o;
 }
 
-template <int R, bool RIGHT, bool SCORE_ONLY>
-static hipError_t launch_stripe_t(const StripeArgs &a, hipStream_t st)
+__global__ void ksw_zero_kernel(int32_t *p, size_t n)
 {
-    hipLaunchKernelGGL((ksw_stripe_kernel<R, RIGHT, SCORE_ONLY>), dim3(a.njobs), dim3(kWave), 0, st, a);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+template <int R, bool RIGHT, bool SCORE_ONLY>
+static hipError_t launch_stripe_t(const StripeArgs &a, int workers, hipStream_t st)
+{
+    hipLaunchKernelGGL((ksw_stripe_kernel<R, RIGHT, SCORE_ONLY>), dim3(workers), dim3(kWave), 0, st, a);
     return hipGetLastError();
 }
 
-hipError_t launch_stripe(const StripeArgs &a, bool right, bool score_only, hipStream_t st)
+template <int R>
+static hipError_t launch_stripe_r(const StripeArgs &a, bool right, bool score_only, int workers, hipStream_t st)
 {
-    if (a.njobs <= 0) return hipSuccess;
-    if (score_only) return launch_stripe_t<kRowsPerLane, false, true>(a, st);
-    if (right) return launch_stripe_t<kRowsPerLane, true, false>(a, st);
-    return launch_stripe_t<kRowsPerLane, false, false>(a, st);
+    if (score_only) return launch_stripe_t<R, false, true>(a, workers, st);
+    if (right) return launch_stripe_t<R, true, false>(a, workers, st);
+    return launch_stripe_t<R, false, false>(a, workers, st);
+}
+
+hipError_t launch_stripe(const StripeArgs &a, int R, bool right, bool score_only, int workers, hipStream_t st)
+{
+    if (a.ntasks <= 0) return hipSuccess;
+    if (workers > a.ntasks) workers = a.ntasks;
+    switch (R) {
+    case 4: return launch_stripe_r<4>(a, right, score_only, workers, st);
+    case 8: return launch_stripe_r<8>(a, right, score_only, workers, st);
+    case 16: return launch_stripe_r<16>(a, right, score_only, workers, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int R, bool RIGHT, bool SCORE_ONLY>
+static int resident_t()
+{
+    int per_cu = 0, dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 1024;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ksw_stripe_kernel<R, RIGHT, SCORE_ONLY>, kWave, 0) !=
+            hipSuccess ||
+        per_cu <= 0)
+        return prop.multiProcessorCount * 4;
+    return per_cu * prop.multiProcessorCount;
+}
+
+template <int R>
+static int resident_r(bool right, bool score_only)
+{
+    if (score_only) return resident_t<R, false, true>();
+    if (right) return resident_t<R, true, false>();
+    return resident_t<R, false, false>();
+}
+
+int stripe_resident_waves(int R, bool right, bool score_only)
+{
+    switch (R) {
+    case 4: return resident_r<4>(right, score_only);
+    case 8: return resident_r<8>(right, score_only);
+    case 16: return resident_r<16>(right, score_only);
+    default: return 1024;
+    }
 }
 
 hipError_t launch_backtrack(const BacktrackArgs &a, hipStream_t st)
 {
     if (a.njobs <= 0) return hipSuccess;
     hipLaunchKernelGGL(ksw_backtrack_kernel, dim3((a.njobs + 63) / 64), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(int32_t *p, size_t n, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const size_t blocks = (n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024;
+    hipLaunchKernelGGL(ksw_zero_kernel, dim3((unsigned)blocks), dim3(256), 0, st, p, n);
     return hipGetLastError();
 }
 

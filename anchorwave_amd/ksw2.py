@@ -48,6 +48,7 @@ EXPORTS = (
     "ksw2_hip_plan_fetch",
     "ksw2_hip_plan_timing",
     "ksw2_hip_plan_destroy",
+    "ksw2_hip_plan_dirs",
 )
 
 
@@ -147,6 +148,8 @@ def lib():
         L.ksw2_hip_plan_timing.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                            C.POINTER(C.c_double)]
         L.ksw2_hip_plan_timing.restype = C.c_int
+        L.ksw2_hip_plan_dirs.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_int)]
+        L.ksw2_hip_plan_dirs.restype = C.c_int
         L.ksw2_hip_plan_destroy.argtypes = [C.c_void_p]
         L.ksw2_hip_plan_destroy.restype = None
         _lib = L
@@ -268,6 +271,28 @@ class Plan:
         if st != 0:
             raise KswError(f"ksw2_hip_plan_fetch: {STATUS.get(st, st)}")
         return out
+
+    def dirs(self, index: int, qlen: int, tlen: int, raw: bool = False) -> np.ndarray:
+        """Direction bytes of pair ``index`` as a (tlen, qlen) matrix (or the
+        raw stripe-layout buffer when ``raw``)."""
+        R = 8
+        SR = 64 * R
+        nbytes = ((tlen + SR - 1) // SR) * (qlen + 63) * 64 * 16 * 2
+        buf = np.zeros(nbytes, dtype=np.uint8)
+        r = C.c_int()
+        st = lib().ksw2_hip_plan_dirs(self.h, index, buf.ctypes.data, nbytes, C.byref(r))
+        if st != 0:
+            raise KswError(f"ksw2_hip_plan_dirs: {STATUS.get(st, st)}")
+        R = r.value
+        SR = 64 * R
+        if raw:
+            return buf[: ((tlen + SR - 1) // SR) * (qlen + 63) * 64 * R]
+        i = np.arange(tlen)[:, None]
+        j = np.arange(qlen)[None, :]
+        s, rem = i // SR, i % SR
+        k, rho = rem // R, rem % R
+        idx = ((s * (qlen + 63) + j + k) * 64 + k) * R + rho
+        return buf[idx]
 
     def close(self):
         if self.h:

@@ -148,6 +148,11 @@ int ksw2_hip_plan_fetch(ksw2_hip_plan *plan, ksw_extz_t *ez);
  * HIP events on the plan's stream; also the sum of qlen*tlen */
 int ksw2_hip_plan_timing(const ksw2_hip_plan *plan, float *dp_ms, float *backtrack_ms, double *cells);
 void ksw2_hip_plan_destroy(ksw2_hip_plan *plan);
+/* Diagnostics: copies pair `index`'s raw direction bytes (stripe layout:
+ * cell (i, j) at ((s*(qlen+63) + j+k)*64 + k)*R + rho with s = i/(64R),
+ * k = (i%(64R))/R, rho = i%R) into `out` and reports R; `bytes` must be at
+ * least (ceil(tlen/(64R))*(qlen+63)*64*R).  Returns a status code. */
+int ksw2_hip_plan_dirs(const ksw2_hip_plan *plan, int index, uint8_t *out, size_t bytes, int *R);
 
 #ifdef __cplusplus
 }

@@ -51,3 +51,72 @@ def test_single_config_pair():
 
 def test_fixed_config_sample():
     check_batch(W.fixed(seed=2, n=48))
+
+
+def _check_against_cpu(pairs, **kw):
+    """Large cases: compare with the compiled reference when it is present
+    (oracle/_ref, built from the reference sources), else the oracle."""
+    from concurrent.futures import ThreadPoolExecutor
+    fn = (lambda p: O.run_ref(p[0], p[1], **kw)) if O.ref_available() else (lambda p: O.run_oracle(p[0], p[1], **kw))
+    got = ksw2.extd2_batch(pairs, **kw)
+    with ThreadPoolExecutor(16) as ex:
+        exp = list(ex.map(fn, pairs))
+    bad = [f"pair {k} qlen={len(q)} tlen={len(t)}: {diff(g, r)}"
+           for k, ((q, t), g, r) in enumerate(zip(pairs, got, exp)) if not same(g, r)]
+    assert not bad, "\n".join(bad[:5]) + f"\n({len(bad)} of {len(pairs)} differ)"
+
+
+def test_uneven_sample_multistripe():
+    """Uneven inter-anchor intervals: many stripes per pair run concurrently
+    on different waves (progress-counter hand-off)."""
+    pairs = [p for p in W.uneven(seed=3, n=400) if len(p[0]) * len(p[1]) <= 1.5e8][:120]
+    assert max(len(t) for _, t in pairs) > 8 * 512
+    _check_against_cpu(pairs)
+
+
+def test_uneven_sample_flags():
+    pairs = [p for p in W.uneven(seed=31, n=200) if len(p[0]) * len(p[1]) <= 6e7][:60]
+    for flag in (ksw2.KSW_EZ_RIGHT, ksw2.KSW_EZ_SEMIGLOBAL_END, ksw2.KSW_EZ_SCORE_ONLY):
+        _check_against_cpu(pairs, flag=flag)
+
+
+def test_mixed_flags_one_batch():
+    """Pairs with different flags in one batch land in different kernel
+    groups of one plan; each result must match its own flag."""
+    pairs = random_pairs(77, 90, 1, 1500)
+    flags = [FLAGS[k % len(FLAGS)] for k in range(len(pairs))]
+    got = ksw2.extd2_batch([(q, t, f) for (q, t), f in zip(pairs, flags)])
+    for k, ((q, t), f, g) in enumerate(zip(pairs, flags, got)):
+        assert same(g, O.run_oracle(q, t, flag=f)), (k, f)
+
+
+def test_direction_bytes_every_cell():
+    """Every direction byte the device writes (not only those on the
+    traceback path) equals the reference's byte for that cell."""
+    rng = np.random.default_rng(12)
+    pairs = random_pairs(91, 6, 300, 1400) + [W.fixed(seed=2, n=1)[0]]
+    for flag in (0, ksw2.KSW_EZ_RIGHT):
+        plan = ksw2.Plan(pairs, flag=flag)
+        plan.run()
+        got = plan.fetch()
+        for k, (q, t) in enumerate(pairs):
+            ref, rd = O.run_oracle(q, t, flag=flag, want_dirs=True)
+            gd = plan.dirs(k, len(q), len(t))
+            nbad = int((gd != rd).sum())
+            assert nbad == 0, f"pair {k} flag {flag}: {nbad} cells differ, first {np.argwhere(gd != rd)[:4].tolist()}"
+            assert same(got[k], ref)
+        plan.close()
+
+
+def test_repeat_runs_identical():
+    """A plan run repeatedly (stripes of a pair on concurrent waves) gives
+    identical, exact results every time."""
+    pairs = random_pairs(100, 60, 1, 700) + [p for p in W.uneven(seed=9, n=60) if len(p[0]) * len(p[1]) < 4e7][:20]
+    refs = [O.run_oracle(q, t) for q, t in pairs]
+    plan = ksw2.Plan(pairs)
+    for _ in range(4):
+        plan.run()
+        got = plan.fetch()
+        bad = [k for k in range(len(pairs)) if not same(got[k], refs[k])]
+        assert not bad, bad
+    plan.close()
