@@ -74,45 +74,24 @@ int fast_path_reject(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls)
     return why;
 }
 
-// Queue order for the stripe kernel.  A pair with S stripes finishes no
-// earlier than (qlen + 63) + (S - 1) * lag steps, lag being the distance a
-// stripe trails its predecessor.  Pairs are ranked by that critical path
-// (longest first); stripe s of the pair at rank p gets key p + s * L, where L
-// is about the number of tasks the workers start during one lag, so the
-// stripes of a pair enter the queue roughly when their predecessor has
-// published enough columns.  Keys grow with s within a pair, which keeps
-// every stripe behind its predecessor in the queue.
-static void order_tasks(const std::vector<KswJob> &jobs, const std::vector<int> &members, int R, int workers,
-                        std::vector<StripeTask> &out)
+// Team size for one pair.  With T the batch's step count per resident wave
+// (the time the batch needs when every wave is busy), a pair whose stripes
+// take S * (qlen + 63) steps on one wave gets the smallest power-of-two team
+// that brings it under T, capped by its stripe count and kMaxTeam: small
+// pairs run on one wave with no synchronisation, long pairs do not outlast
+// the batch.  KSW2_HIP_TEAM forces a size (tuning and tests).
+static int choose_team(const KswJob &j, int R, double T, int forced)
 {
     const int SR = kWave * R;
-    const double lag = 3.0 * kWave;  // steps a stripe trails its predecessor
-    std::vector<int> rank(members);
-    std::stable_sort(rank.begin(), rank.end(), [&](int a, int b) {
-        const int Sa = (jobs[a].tlen + SR - 1) / SR, Sb = (jobs[b].tlen + SR - 1) / SR;
-        const double ta = jobs[a].qlen + 63.0 + (Sa - 1) * lag, tb = jobs[b].qlen + 63.0 + (Sb - 1) * lag;
-        return ta > tb;
-    });
-    double steps = 0, ntask = 0;
-    for (int id : members) {
-        const int S = (jobs[id].tlen + SR - 1) / SR;
-        steps += (double)S * (jobs[id].qlen + 63.0), ntask += S;
-    }
-    const double mean_steps = ntask > 0 ? steps / ntask : 1.0;
-    const double L = std::max(1.0, workers * lag / mean_steps);
-    std::vector<std::pair<double, StripeTask>> keyed;
-    keyed.reserve((size_t)ntask);
-    for (size_t p = 0; p < rank.size(); ++p) {
-        const int id = rank[p];
-        const int S = (jobs[id].tlen + SR - 1) / SR;
-        for (int st = 0; st < S; ++st) keyed.push_back({(double)p + st * L, StripeTask{id, st}});
-    }
-    std::stable_sort(keyed.begin(), keyed.end(),
-                     [](const std::pair<double, StripeTask> &a, const std::pair<double, StripeTask> &b) {
-                         return a.first < b.first;
-                     });
-    for (auto &k : keyed) out.push_back(k.second);
+    const int S = (j.tlen + SR - 1) / SR;
+    if (forced > 0) return std::max(1, std::min(std::min(forced, kMaxTeam), S));
+    const double steps = (double)S * (j.qlen + kWave - 1);
+    int team = 1;
+    while (team < kMaxTeam && team < S && steps / team > T) team <<= 1;
+    return std::min(team, S);
 }
+
+static int lane_of_team(int team) { return team >= 8 ? 0 : team >= 4 ? 1 : team >= 2 ? 2 : 3; }
 
 int plan_create(Plan **out, int n, const ksw2_pair_t *pairs, const int *idx, int nidx, const KswScore &sc)
 {
@@ -152,18 +131,29 @@ int plan_create(Plan **out, int n, const ksw2_pair_t *pairs, const int *idx, int
         j.stripe_off = stripes, stripes += (pp.tlen + SR - 1) / SR;
     }
     p->nstripes_total = stripes;
-    // task queues, one per group, concatenated
-    std::vector<StripeTask> tasks;
-    for (int grp = 0; grp < 3; ++grp) {
-        std::vector<int> members;
-        for (int k = 0; k < p->n; ++k)
-            if (grp_of[k] == grp) members.push_back(k);
-        p->task_off[grp] = (int)tasks.size();
-        p->workers[grp] = members.empty() ? 0 : stripe_resident_waves(R, grp == 1, grp == 2);
-        if (const char *ev = getenv("KSW2_HIP_WORKERS"))  // tuning override
-            if (atoi(ev) > 0) p->workers[grp] = std::min(p->workers[grp], atoi(ev));
-        order_tasks(p->jobs, members, R, std::max(p->workers[grp], 1), tasks);
-        p->ntasks[grp] = (int)tasks.size() - p->task_off[grp];
+    // launches: per group, per team size, longest pairs first
+    std::vector<int32_t> order;
+    {
+        double total_steps = 0;
+        for (const KswJob &j : p->jobs) total_steps += (double)((j.tlen + SR - 1) / SR) * (j.qlen + kWave - 1);
+        const double T = std::max(1.0, total_steps / std::max(1, stripe_resident_waves(R, false, false)));
+        int forced = 0;
+        if (const char *ev = getenv("KSW2_HIP_TEAM")) forced = atoi(ev);
+        std::vector<int> team(p->n);
+        for (int k = 0; k < p->n; ++k) team[k] = choose_team(p->jobs[k], R, T, forced);
+        for (int grp = 0; grp < 3; ++grp)
+            for (int t = kMaxTeam; t >= 1; --t) {
+                std::vector<int> members;
+                for (int k = 0; k < p->n; ++k)
+                    if (grp_of[k] == grp && team[k] == t) members.push_back(k);
+                if (members.empty()) continue;
+                std::stable_sort(members.begin(), members.end(), [&](int x, int y) {
+                    const KswJob &a = p->jobs[x], &b = p->jobs[y];
+                    return (double)a.tlen * (a.qlen + kWave - 1) > (double)b.tlen * (b.qlen + kWave - 1);
+                });
+                p->launches.push_back({grp, t, (int)order.size(), (int)members.size(), lane_of_team(t)});
+                order.insert(order.end(), members.begin(), members.end());
+            }
     }
     p->seq_bytes = std::max<uint64_t>(so, 16), p->dir_bytes = std::max<uint64_t>(dof, 256);
     p->bnd_dwords = std::max<uint64_t>(bo, 64), p->cig_words = std::max<uint64_t>(co, 16);
@@ -174,15 +164,20 @@ int plan_create(Plan **out, int n, const ksw2_pair_t *pairs, const int *idx, int
     if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) return fail(KSW2_HIP_EDEVICE);
     for (auto &e : p->ev)
         if (hipEventCreate(&e) != hipSuccess) return fail(KSW2_HIP_EDEVICE);
+    for (int l = 0; l < Plan::kLanes; ++l) {
+        if (hipStreamCreateWithFlags(&p->lane_stream[l], hipStreamNonBlocking) != hipSuccess)
+            return fail(KSW2_HIP_EDEVICE);
+        if (hipEventCreateWithFlags(&p->lane_done[l], hipEventDisableTiming) != hipSuccess)
+            return fail(KSW2_HIP_EDEVICE);
+    }
     if (hipMalloc(&p->d_seq, p->seq_bytes) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
     if (hipMalloc(&p->d_dir, p->dir_bytes) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
     if (hipMalloc(&p->d_bnd, p->bnd_dwords * 4) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
     if (hipMalloc(&p->d_cig, p->cig_words * 4) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
     if (hipMalloc(&p->d_jobs, sizeof(KswJob) * std::max(p->n, 1)) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
     if (hipMalloc(&p->d_out, sizeof(KswOut) * std::max(p->n, 1)) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
-    if (hipMalloc(&p->d_tasks, sizeof(StripeTask) * std::max<size_t>(tasks.size(), 1)) != hipSuccess)
+    if (hipMalloc(&p->d_order, sizeof(int32_t) * std::max<size_t>(order.size(), 1)) != hipSuccess)
         return fail(KSW2_HIP_ENOMEM);
-    if (hipMalloc(&p->d_sync, sizeof(int32_t) * (4 + (size_t)stripes)) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
     if (hipMalloc(&p->d_recs, sizeof(StripeRec) * std::max(stripes, 1)) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
     // stage sequences through pinned memory
     uint8_t *h_seq = nullptr;
@@ -195,8 +190,8 @@ int plan_create(Plan **out, int n, const ksw2_pair_t *pairs, const int *idx, int
     }
     hipError_t e1 = hipMemcpyAsync(p->d_seq, h_seq, p->seq_bytes, hipMemcpyHostToDevice, p->stream);
     hipError_t e2 = hipMemcpyAsync(p->d_jobs, p->jobs.data(), sizeof(KswJob) * p->n, hipMemcpyHostToDevice, p->stream);
-    hipError_t e4 = tasks.empty() ? hipSuccess
-                                  : hipMemcpyAsync(p->d_tasks, tasks.data(), sizeof(StripeTask) * tasks.size(),
+    hipError_t e4 = order.empty() ? hipSuccess
+                                  : hipMemcpyAsync(p->d_order, order.data(), sizeof(int32_t) * order.size(),
                                                    hipMemcpyHostToDevice, p->stream);
     hipError_t e3 = hipStreamSynchronize(p->stream);
     (void)hipHostFree(h_seq);
@@ -209,15 +204,24 @@ int plan_run(Plan *p)
 {
     if (p->n == 0) return KSW2_HIP_OK;
     KSW_HIP_TRY(hipEventRecord(p->ev[0], p->stream));
-    KSW_HIP_TRY(launch_zero(p->d_sync, 4 + (size_t)p->nstripes_total, p->stream));
     StripeArgs sa;
     sa.seq = p->d_seq, sa.dir = p->d_dir, sa.bnd = p->d_bnd, sa.sc = p->sc;
-    sa.jobs = p->d_jobs, sa.out = p->d_out, sa.progress = p->d_sync + 4, sa.recs = p->d_recs;
-    for (int grp = 0; grp < 3; ++grp) {
-        sa.tasks = p->d_tasks + p->task_off[grp], sa.ntasks = p->ntasks[grp];
-        sa.queue_head = p->d_sync + grp;
-        KSW_HIP_TRY(launch_stripe(sa, p->R, grp == 1, grp == 2, p->workers[grp], p->stream));
+    sa.jobs = p->d_jobs, sa.out = p->d_out, sa.recs = p->d_recs;
+    bool used[Plan::kLanes] = {false, false, false, false};
+    for (const Plan::Launch &l : p->launches) {
+        hipStream_t st = p->lane_stream[l.lane];
+        if (!used[l.lane]) {
+            KSW_HIP_TRY(hipStreamWaitEvent(st, p->ev[0], 0));
+            used[l.lane] = true;
+        }
+        sa.order = p->d_order + l.off, sa.njobs = l.count;
+        KSW_HIP_TRY(launch_stripe(sa, p->R, l.grp == 1, l.grp == 2, l.team, st));
     }
+    for (int k = 0; k < Plan::kLanes; ++k)
+        if (used[k]) {
+            KSW_HIP_TRY(hipEventRecord(p->lane_done[k], p->lane_stream[k]));
+            KSW_HIP_TRY(hipStreamWaitEvent(p->stream, p->lane_done[k], 0));
+        }
     KSW_HIP_TRY(hipEventRecord(p->ev[1], p->stream));
     BacktrackArgs ba;
     ba.dir = p->d_dir, ba.cig = p->d_cig, ba.jobs = p->d_jobs, ba.out = p->d_out, ba.recs = p->d_recs;
@@ -254,10 +258,14 @@ void plan_destroy(Plan *p)
 {
     if (!p) return;
     for (void *ptr : {(void *)p->d_seq, (void *)p->d_dir, (void *)p->d_bnd, (void *)p->d_cig, (void *)p->d_jobs,
-                      (void *)p->d_out, (void *)p->d_tasks, (void *)p->d_sync, (void *)p->d_recs})
+                      (void *)p->d_out, (void *)p->d_order, (void *)p->d_recs})
         if (ptr) (void)hipFree(ptr);
     for (auto &e : p->ev)
         if (e) (void)hipEventDestroy(e);
+    for (int l = 0; l < Plan::kLanes; ++l) {
+        if (p->lane_done[l]) (void)hipEventDestroy(p->lane_done[l]);
+        if (p->lane_stream[l]) (void)hipStreamDestroy(p->lane_stream[l]);
+    }
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }

@@ -19,6 +19,7 @@
 namespace kswhip {
 
 constexpr int kWave = 64;
+constexpr int kMaxTeam = 16;   // waves cooperating on one pair (one workgroup)
 constexpr int32_t KSW_NEG_INF_I = -0x40000000;
 
 // Scoring shared by a batch, already normalised the way the reference
@@ -41,13 +42,6 @@ struct KswJob {
     int32_t flag, end_bonus;
     int32_t stripe_off;        // first slot of this pair in the per-stripe arrays
     int32_t R;                 // rows per lane of the stripe kernel that ran it
-};
-
-// One unit of the stripe kernel's work queue: stripe `stripe` of pair `job`.
-// Stripes of a pair appear in increasing order, so when a wave takes stripe
-// s the wave that took s-1 is already running.
-struct StripeTask {
-    int32_t job, stripe;
 };
 
 // Per-stripe partial of the mqe bookkeeping (column qlen-1): the sum of

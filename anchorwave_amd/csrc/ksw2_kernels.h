@@ -12,10 +12,8 @@ struct StripeArgs {
     int32_t *bnd;
     const KswJob *jobs;
     KswOut *out;
-    const StripeTask *tasks;   // this launch's queue
-    int ntasks;
-    int32_t *queue_head;       // next task index (zeroed before the launch)
-    int32_t *progress;         // per stripe: bottom-row columns published
+    const int32_t *order;      // this launch's jobs, one workgroup each
+    int njobs;
     StripeRec *recs;           // per stripe: mqe partials
     KswScore sc;
 };
@@ -30,12 +28,9 @@ struct BacktrackArgs {
     KswScore sc;
 };
 
-// Persistent stripe kernel: `workers` waves drain the task queue.
-hipError_t launch_stripe(const StripeArgs &a, int R, bool right, bool score_only, int workers, hipStream_t st);
+// Stripe kernel: one workgroup of `team` waves (1..kMaxTeam) per job.
+hipError_t launch_stripe(const StripeArgs &a, int R, bool right, bool score_only, int team, hipStream_t st);
 hipError_t launch_backtrack(const BacktrackArgs &a, hipStream_t st);
-// Zeroes n int32 (queue heads / progress counters) with a kernel, so the
-// sequence stays capturable in a HIP graph.
-hipError_t launch_zero(int32_t *p, size_t n, hipStream_t st);
 // Waves of the stripe kernel that fit on the device at once.
 int stripe_resident_waves(int R, bool right, bool score_only);
 

@@ -46,10 +46,12 @@ struct Plan {
     int R = kRowsPerLane;            // rows per lane of the stripe kernel
     std::vector<KswJob> jobs;        // host copy
     std::vector<int> order;          // plan slot -> caller index
-    // job groups: 0 left-aligned traceback, 1 right-aligned, 2 score-only
-    int ntasks[3] = {0, 0, 0};
-    int task_off[3] = {0, 0, 0};
-    int workers[3] = {0, 0, 0};
+    // One launch per (job group, team size).  Groups: 0 left-aligned
+    // traceback, 1 right-aligned, 2 score-only.  `off`/`count` index d_order.
+    struct Launch {
+        int grp, team, off, count, lane;
+    };
+    std::vector<Launch> launches;
     int nstripes_total = 0;
     uint64_t seq_bytes = 0, dir_bytes = 0, bnd_dwords = 0, cig_words = 0;
     uint8_t *d_seq = nullptr, *d_dir = nullptr;
@@ -57,10 +59,13 @@ struct Plan {
     uint32_t *d_cig = nullptr;
     KswJob *d_jobs = nullptr;
     KswOut *d_out = nullptr;
-    StripeTask *d_tasks = nullptr;
-    int32_t *d_sync = nullptr;       // 4 queue heads, then one progress counter per stripe
+    int32_t *d_order = nullptr;      // job indices, grouped by launch, longest first
     StripeRec *d_recs = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // launches of different team sizes overlap on their own streams
+    static constexpr int kLanes = 4;
+    hipStream_t lane_stream[kLanes] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t lane_done[kLanes] = {nullptr, nullptr, nullptr, nullptr};
     float last_dp_ms = 0.f, last_bt_ms = 0.f;
 };
 

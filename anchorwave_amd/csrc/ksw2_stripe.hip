@@ -11,11 +11,14 @@
 //    one step earlier and arrives through a DPP wave_shr:1.  Lane 0 takes the
 //    bottom row of the previous stripe from a per-pair column buffer; the
 //    left neighbour (i, j-1) stays in the lane's registers.
-//  * Persistent workers.  Waves pull (pair, stripe) tasks from a device
-//    queue.  Consecutive stripes of one pair run concurrently on different
-//    waves: stripe s waits on a per-stripe progress counter before it reads
-//    the columns stripe s-1 has published.  A stripe is queued after its
-//    predecessor, so the predecessor is always held by a running wave.
+//  * Teams.  One workgroup of W waves computes one pair; wave w takes
+//    stripes w, w+W, w+2W, ...  Stripe s trails stripe s-1 by at least two
+//    64-column blocks: the producer publishes its bottom-row columns to the
+//    pair's column buffer and then a column count in LDS (workgroup-scope
+//    release), the consumer polls that count (workgroup-scope acquire) before
+//    it reads.  All waves of a team share one CU, so no cache maintenance
+//    beyond the CU is involved.  W = 1 (no waiting at all) for pairs that
+//    are small against the batch; long pairs get up to 16 waves.
 //  * Scaled, negated state.  Every difference value w is kept as -8*w plus a
 //    3-bit tag, so one v_min3 pair yields both the maximum of the five
 //    candidates (s, a, b, a2, b2) and the index the reference's strict '>'
@@ -148,47 +151,54 @@ __device__ __forceinline__ void store_dirs(uint8_t *p, const int (&d)[R])
     }
 }
 
-__device__ __forceinline__ int ld_agent(const int32_t *p)
+// Team progress: wave w publishes, for its k-th stripe, k*(qlen+1) + the
+// number of bottom-row columns written so far (monotonic per wave).
+__device__ __forceinline__ void publish(int32_t *slot, int value)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // every lane stores the same value (no divergent branch); the release
+    // orders the preceding column-buffer stores before the count
+    __hip_atomic_store(slot, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Spins until stripe `prev`'s published column count reaches `need`.  Every
-// lane polls the same counter and the exit test uses lane 0's value, so the
-// loop is uniform.  The producer always ends at qlen, so `need` <= qlen
-// guarantees termination.
-__device__ __forceinline__ int wait_columns(const int32_t *prog, int need, int known)
+// Spins until the producer's published count reaches `need`.  The exit test
+// uses lane 0's value, so the loop is uniform.  A producer always ends its
+// stripe at k*(qlen+1) + qlen, so `need` <= that bound terminates.
+__device__ __forceinline__ int wait_count(const int32_t *slot, int need, int known)
 {
     if (known >= need) return known;
-    int seen = __builtin_amdgcn_readfirstlane(__hip_atomic_load(prog, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+    int seen = __builtin_amdgcn_readfirstlane(__hip_atomic_load(slot, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
     while (seen < need) {
         __builtin_amdgcn_s_sleep(1);
-        seen = __builtin_amdgcn_readfirstlane(__hip_atomic_load(prog, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+        seen = __builtin_amdgcn_readfirstlane(__hip_atomic_load(slot, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
     }
     return seen;
 }
 
-// One stripe of one pair, computed by the calling wave.
+// One stripe of one pair, computed by the calling wave.  Roles (uniform
+// flags): the first stripe (s == 0) takes row -1 as its upper boundary
+// instead of the column buffer; the last stripe owns row tlen-1 (score / mte)
+// and publishes nothing; every other stripe publishes its bottom row.
 template <int R, bool RIGHT, bool SCORE_ONLY>
-__device__ __attribute__((noinline)) void run_stripe(const StripeArgs &a, const Consts &kc, int job_id, int s)
+__device__ __forceinline__ void stripe_body(const StripeArgs &a, const Consts &kc, int job_id, int s,
+                                            const KswJob &jb, int qlen, int tlen, int32_t *prog, int W)
 {
-    const KswJob jb = a.jobs[job_id];
+    constexpr int SR_ = kWave * R;
+    const bool first = s == 0;
+    const bool last = s == (tlen + SR_ - 1) / SR_ - 1;
     const KswScore &sc = a.sc;
     const int lane = threadIdx.x % kWave;
-    const int qlen = __builtin_amdgcn_readfirstlane(jb.qlen), tlen = __builtin_amdgcn_readfirstlane(jb.tlen);
     const uint8_t *qs = a.seq + jb.q_off;
     const uint8_t *ts = a.seq + jb.t_off;
     uint8_t *dir = a.dir + jb.dir_off;
     int32_t *bnd = a.bnd + jb.bnd_off;
     constexpr int SR = kWave * R;
-    s = __builtin_amdgcn_readfirstlane(s);
-    const int nstripes = (tlen + SR - 1) / SR;
     const int nsteps = qlen + kWave - 1;
-    const bool last_stripe = s == nstripes - 1;
-    const int rho_last = (tlen - 1) % R;
-    int32_t *prog_prev = a.progress + jb.stripe_off + s - 1;
-    int32_t *prog_mine = a.progress + jb.stripe_off + s;
-    const bool has_next = s + 1 < nstripes;
+    const int rho_last = last ? (tlen - 1) % R : -1;
+    // progress slots: stripe s-1 is wave (s-1) % W's ((s-1)/W)-th stripe
+    const int32_t *prog_prev = prog + (s + W - 1) % W;
+    const int base_prev = first ? 0 : (s - 1) / W * (qlen + 1);
+    int32_t *prog_mine = prog + s % W;
+    const int base_mine = s / W * (qlen + 1);
 
     int U[R], Yh[R], Y2h[R], TB[R];
     const int i0 = s * SR + lane * R;
@@ -206,31 +216,35 @@ __device__ __attribute__((noinline)) void run_stripe(const StripeArgs &a, const 
     int bnx_xx = 0, bnx_v = 0;          // next block of the queue
     int out_xx = 0, out_v = 0;          // lane-63 output queue (rotating)
     int known = 0;                      // columns of stripe s-1 known published
-    if (s > 0) {
-        known = wait_columns(prog_prev, min(2 * kWave, qlen), known);
+    if (!first) {
+        known = wait_count(prog_prev, base_prev + min(2 * kWave, qlen), known);
         const int c0 = lane, c1 = kWave + lane;
-        bin_xx = c0 < qlen ? ld_agent(bnd + 2 * c0) : 0;
-        bin_v = c0 < qlen ? ld_agent(bnd + 2 * c0 + 1) : 0;
-        bnx_xx = c1 < qlen ? ld_agent(bnd + 2 * c1) : 0;
-        bnx_v = c1 < qlen ? ld_agent(bnd + 2 * c1 + 1) : 0;
+        bin_xx = c0 < qlen ? bnd[2 * c0] : 0;
+        bin_v = c0 < qlen ? bnd[2 * c0 + 1] : 0;
+        bnx_xx = c1 < qlen ? bnd[2 * c1] : 0;
+        bnx_v = c1 < qlen ? bnd[2 * c1 + 1] : 0;
     }
     int qb_next = lane == 0 ? (int)qs[0] : 0;
     for (int c = 0; c < nsteps; ++c) {
         const int j = c - lane;
         // value for lane 0: the cell above the stripe in column c
-        const int top_v = -8 * edge_diff(sc, c);  // row -1 (first stripe)
-        const int bx = s == 0 ? 0 : (int)(int16_t)(bin_xx & 0xffff);
-        const int bx2 = s == 0 ? 0 : bin_xx >> 16;
-        const int bv = s == 0 ? top_v : bin_v;
+        int bx, bx2, bv;
+        if (first) {
+            bx = 0, bx2 = 0, bv = -8 * edge_diff(sc, c);
+        } else {
+            bx = (int)(int16_t)(bin_xx & 0xffff);
+            bx2 = bin_xx >> 16;
+            bv = bin_v;
+        }
         int Xa = dpp_shr1(bx, Xo), X2a = dpp_shr1(bx2, X2o), Va = dpp_shr1(bv, Vo);
-        if (s > 0) {
+        if (!first) {
             // rotate the queue so lane 0 sees column c+1 next step
             if ((c & (kWave - 1)) == kWave - 1) {
                 bin_xx = bnx_xx, bin_v = bnx_v;
-                known = wait_columns(prog_prev, min(c + 1 + 2 * kWave, qlen), known);
+                known = wait_count(prog_prev, base_prev + min(c + 1 + 2 * kWave, qlen), known);
                 const int cn = c + 1 + kWave + lane;
-                bnx_xx = cn < qlen ? ld_agent(bnd + 2 * cn) : 0;
-                bnx_v = cn < qlen ? ld_agent(bnd + 2 * cn + 1) : 0;
+                bnx_xx = cn < qlen ? bnd[2 * cn] : 0;
+                bnx_v = cn < qlen ? bnd[2 * cn + 1] : 0;
             } else {
                 bin_xx = dpp_shl1(0, bin_xx);
                 bin_v = dpp_shl1(0, bin_v);
@@ -240,19 +254,21 @@ __device__ __attribute__((noinline)) void run_stripe(const StripeArgs &a, const 
         qb_next = (j + 1 >= 0 && j + 1 < qlen) ? (int)qs[j + 1] : 0;
         if (j >= 0 && j < qlen) {
             int d[R];
+            int vl = 0;  // V of the cell in row tlen-1 (last stripe)
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 d[r] = cell<RIGHT, SCORE_ONLY>(kc, TB[r], qb, Xa, X2a, Va, U[r], Yh[r], Y2h[r]);
-                if (last_stripe && r == rho_last) {
-                    row_acc += Va;  // Va now holds this cell's V
-                    if (row_acc <= mte_best) mte_best = row_acc, mte_j = j;
-                }
+                vl = r == rho_last ? Va : vl;
+            }
+            if (last) {
+                row_acc += vl;
+                if (row_acc <= mte_best) mte_best = row_acc, mte_j = j;
             }
             Xo = Xa, X2o = X2a, Vo = Va;
             if (!SCORE_ONLY)
                 store_dirs<R>(dir + (((uint64_t)s * nsteps + c) * kWave + lane) * R, d);
         }
-        if (has_next) {
+        if (!last) {
             // lane 63 appends its bottom-row output (column c-63) to the
             // queue; every 64 steps the queue is written out and published
             const int packed = (Xo & 0xffff) | (X2o << 16);
@@ -264,10 +280,7 @@ __device__ __attribute__((noinline)) void run_stripe(const StripeArgs &a, const 
                     bnd[2 * col] = out_xx;
                     bnd[2 * col + 1] = out_v;
                 }
-                __threadfence();
-                if (lane == 0)
-                    __hip_atomic_store(prog_mine, min(c - (kWave - 1) + 1, qlen), __ATOMIC_RELEASE,
-                                       __HIP_MEMORY_SCOPE_AGENT);
+                publish(prog_mine, base_mine + min(c - (kWave - 1) + 1, qlen));
             }
         }
     }
@@ -303,7 +316,7 @@ __device__ __attribute__((noinline)) void run_stripe(const StripeArgs &a, const 
         rec.total = total, rec.best = best, rec.best_i = best_i, rec.pad = 0;
         a.recs[jb.stripe_off + s] = rec;
     }
-    if (last_stripe) {
+    if (last) {
         // publish score / mte from the lane that owns row tlen-1
         const int lane_last = ((tlen - 1) % SR) / R;
         const int sc_row = __builtin_amdgcn_readlane(row_acc, lane_last);
@@ -326,24 +339,23 @@ __device__ __attribute__((noinline)) void run_stripe(const StripeArgs &a, const 
 
 }  // namespace
 
-// Persistent workers: each wave repeatedly takes the next (pair, stripe)
-// task; the loop ends for every wave once the queue is drained.
+// One workgroup (a team of blockDim.x / 64 waves) per pair of this launch's
+// job list; wave w computes stripes w, w+W, ...
 template <int R, bool RIGHT, bool SCORE_ONLY>
-__global__ void __launch_bounds__(64) ksw_stripe_kernel(StripeArgs a)
+__global__ void __launch_bounds__(kWave * kMaxTeam) ksw_stripe_kernel(StripeArgs a)
 {
+    __shared__ int32_t prog[kMaxTeam];
+    const int W = __builtin_amdgcn_readfirstlane(blockDim.x / kWave);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int job = __builtin_amdgcn_readfirstlane(a.order[blockIdx.x]);
+    prog[w] = 0;  // every lane of wave w writes the same zero
+    __syncthreads();
     const Consts kc = make_consts<RIGHT>(a.sc);
-    const int lane = threadIdx.x % kWave;
-    for (;;) {
-        int t = 0;
-        if (lane == 0) t = atomicAdd(a.queue_head, 1);
-        // the task index is wave-uniform: keep it (and everything derived
-        // from it) in scalar registers so the stripe loop stays uniform
-        t = __builtin_amdgcn_readfirstlane(t);
-        if (t >= a.ntasks) break;
-        const int job = __builtin_amdgcn_readfirstlane(a.tasks[t].job);
-        const int stripe = __builtin_amdgcn_readfirstlane(a.tasks[t].stripe);
-        run_stripe<R, RIGHT, SCORE_ONLY>(a, kc, job, stripe);
-    }
+    const KswJob jb = a.jobs[job];
+    const int qlen = __builtin_amdgcn_readfirstlane(jb.qlen), tlen = __builtin_amdgcn_readfirstlane(jb.tlen);
+    constexpr int SR = kWave * R;
+    const int nstripes = (tlen + SR - 1) / SR;
+    for (int s = w; s < nstripes; s += W) stripe_body<R, RIGHT, SCORE_ONLY>(a, kc, job, s, jb, qlen, tlen, prog, W);
 }
 
 // Device backtrack: one thread per pair.  First folds the per-stripe mqe
@@ -421,35 +433,29 @@ __global__ void __launch_bounds__(64) ksw_backtrack_kernel(BacktrackArgs a)
     a.out[job_id] = o;
 }
 
-__global__ void ksw_zero_kernel(int32_t *p, size_t n)
-{
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        p[i] = 0;
-}
-
 template <int R, bool RIGHT, bool SCORE_ONLY>
-static hipError_t launch_stripe_t(const StripeArgs &a, int workers, hipStream_t st)
+static hipError_t launch_stripe_t(const StripeArgs &a, int team, hipStream_t st)
 {
-    hipLaunchKernelGGL((ksw_stripe_kernel<R, RIGHT, SCORE_ONLY>), dim3(workers), dim3(kWave), 0, st, a);
+    hipLaunchKernelGGL((ksw_stripe_kernel<R, RIGHT, SCORE_ONLY>), dim3(a.njobs), dim3(kWave * team), 0, st, a);
     return hipGetLastError();
 }
 
 template <int R>
-static hipError_t launch_stripe_r(const StripeArgs &a, bool right, bool score_only, int workers, hipStream_t st)
+static hipError_t launch_stripe_r(const StripeArgs &a, bool right, bool score_only, int team, hipStream_t st)
 {
-    if (score_only) return launch_stripe_t<R, false, true>(a, workers, st);
-    if (right) return launch_stripe_t<R, true, false>(a, workers, st);
-    return launch_stripe_t<R, false, false>(a, workers, st);
+    if (score_only) return launch_stripe_t<R, false, true>(a, team, st);
+    if (right) return launch_stripe_t<R, true, false>(a, team, st);
+    return launch_stripe_t<R, false, false>(a, team, st);
 }
 
-hipError_t launch_stripe(const StripeArgs &a, int R, bool right, bool score_only, int workers, hipStream_t st)
+hipError_t launch_stripe(const StripeArgs &a, int R, bool right, bool score_only, int team, hipStream_t st)
 {
-    if (a.ntasks <= 0) return hipSuccess;
-    if (workers > a.ntasks) workers = a.ntasks;
+    if (a.njobs <= 0) return hipSuccess;
+    if (team < 1 || team > kMaxTeam) return hipErrorInvalidValue;
     switch (R) {
-    case 4: return launch_stripe_r<4>(a, right, score_only, workers, st);
-    case 8: return launch_stripe_r<8>(a, right, score_only, workers, st);
-    case 16: return launch_stripe_r<16>(a, right, score_only, workers, st);
+    case 4: return launch_stripe_r<4>(a, right, score_only, team, st);
+    case 8: return launch_stripe_r<8>(a, right, score_only, team, st);
+    case 16: return launch_stripe_r<16>(a, right, score_only, team, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -489,14 +495,6 @@ hipError_t launch_backtrack(const BacktrackArgs &a, hipStream_t st)
 {
     if (a.njobs <= 0) return hipSuccess;
     hipLaunchKernelGGL(ksw_backtrack_kernel, dim3((a.njobs + 63) / 64), dim3(64), 0, st, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_zero(int32_t *p, size_t n, hipStream_t st)
-{
-    if (n == 0) return hipSuccess;
-    const size_t blocks = (n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024;
-    hipLaunchKernelGGL(ksw_zero_kernel, dim3((unsigned)blocks), dim3(256), 0, st, p, n);
     return hipGetLastError();
 }
 

@@ -79,6 +79,22 @@ def cpu_baseline(pairs, threads: int):
     return kind, dt, res
 
 
+def copy_calibration(nbytes: int) -> None:
+    """One hipMemcpy device-to-device of nbytes: a plain copy kernel whose
+    FETCH_SIZE / WRITE_SIZE readings scale the counters to bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"[bench] copy calibration: {nbytes} bytes device-to-device")
+
+
 SC = dict(m=5, q=8, e=2, q2=75, e2=1)
 W_MAT = ksw2.default_matrix(W.MATCH, W.MISMATCH)
 
@@ -92,12 +108,16 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / cross-check")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores (max 16)")
     ap.add_argument("--single-sample", type=int, default=16, help="pairs timed one at a time")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (PMC byte-counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: every pair is independent, so "
                           "more GPUs would each take a share of the pairs; only --gpus 1 is measured"}))
         return 2
 
+    if args.copy_calib > 0:
+        copy_calibration(args.copy_calib << 20)
     log(f"[bench] generating fixed batch ({args.pairs} pairs x 2000 bp)")
     pairs = W.fixed(seed=2, n=args.pairs)
     cells = W.cells(pairs)

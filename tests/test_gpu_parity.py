@@ -120,3 +120,30 @@ def test_repeat_runs_identical():
         bad = [k for k in range(len(pairs)) if not same(got[k], refs[k])]
         assert not bad, bad
     plan.close()
+
+
+@pytest.mark.parametrize("team", [1, 2, 3, 16])
+def test_forced_team_sizes(team, monkeypatch):
+    """Every team size gives the same exact results: the stripes of a pair
+    handed between the waves of one workgroup (team > 1), or run in turn on
+    one wave (team 1)."""
+    monkeypatch.setenv("KSW2_HIP_TEAM", str(team))
+    pairs = random_pairs(300 + team, 12, 600, 5000) + \
+        [p for p in W.uneven(seed=13, n=80) if len(p[0]) * len(p[1]) < 6e7][:12]
+    for flag in (0, ksw2.KSW_EZ_RIGHT, ksw2.KSW_EZ_SCORE_ONLY):
+        _check_against_cpu(pairs, flag=flag)
+
+
+def test_direction_bytes_team_of_three(monkeypatch):
+    """Direction bytes of every cell with stripes spread over three waves."""
+    monkeypatch.setenv("KSW2_HIP_TEAM", "3")
+    pairs = random_pairs(92, 3, 1600, 2600)
+    plan = ksw2.Plan(pairs)
+    plan.run()
+    got = plan.fetch()
+    for k, (q, t) in enumerate(pairs):
+        ref, rd = O.run_oracle(q, t, want_dirs=True)
+        gd = plan.dirs(k, len(q), len(t))
+        assert int((gd != rd).sum()) == 0, k
+        assert same(got[k], ref)
+    plan.close()
