@@ -176,21 +176,23 @@ int ksw2_hip_plan_create(ksw2_hip_plan **plan, int n, const ksw2_pair_t *pairs, 
 {
     if (!plan || n < 0 || (n > 0 && (!pairs || !mat))) return set_err(KSW2_HIP_EINVAL);
     *plan = nullptr;
-    if (m <= 1) return set_err(KSW2_HIP_EUNSUPPORTED);
+    if (m <= 1 || m > kMaxM) return set_err(KSW2_HIP_EUNSUPPORTED);
     KswScore sc;
     ScoreClass cls;
     if (!normalise_scoring(m, mat, q, e, q2, e2, &sc, &cls)) return set_err(KSW2_HIP_EUNSUPPORTED);
+    GenScore gsc;
+    kswhip::make_gen_score(m, mat, sc, cls, &gsc);
     std::vector<int> idx(n);
     double cells = 0;
     for (int k = 0; k < n; ++k) {
         if (pairs[k].qlen <= 0 || pairs[k].tlen <= 0 || !pairs[k].query || !pairs[k].target)
             return set_err(KSW2_HIP_EINVAL);
-        if (kswhip::fast_path_reject(pairs[k], m, cls) != kswhip::kFastOK) return set_err(KSW2_HIP_EUNSUPPORTED);
+        if (pairs[k].qlen > 0x7fffffff - pairs[k].tlen + 1) return set_err(KSW2_HIP_EINVAL);
         idx[k] = k;
         cells += (double)pairs[k].qlen * pairs[k].tlen;
     }
     kswhip::Plan *p = nullptr;
-    int st = kswhip::plan_create(&p, n, pairs, idx.data(), n, sc);
+    int st = kswhip::plan_create(&p, pairs, idx.data(), n, m, cls, gsc);
     if (st != KSW2_HIP_OK) return set_err(st);
     *plan = new ksw2_hip_plan{p, cells};
     return set_err(KSW2_HIP_OK);
@@ -225,6 +227,7 @@ int ksw2_hip_plan_dirs(const ksw2_hip_plan *plan, int index, uint8_t *out, size_
     int k = -1;
     for (int x = 0; x < p->n; ++x)
         if (p->order[x] == index) k = x;
+    if (k < 0 || k >= p->n_fast) return set_err(KSW2_HIP_EINVAL);  // general-kernel pair: other layout
     const kswhip::KswJob &j = p->jobs[k];
     const uint64_t need = kswhip::dir_bytes(j.qlen, j.tlen, j.R);
     if (bytes < need || (j.flag & KSW_EZ_SCORE_ONLY)) return set_err(KSW2_HIP_EINVAL);

@@ -77,7 +77,7 @@ int fast_path_reject(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls)
 // Team size for one pair.  With T the batch's step count per resident wave
 // (the time the batch needs when every wave is busy), a pair whose stripes
 // take S * (qlen + 63) steps on one wave gets the smallest power-of-two team
-// that brings it under T, capped by its stripe count and kMaxTeam: small
+// that brings it under 2T, capped by its stripe count and kMaxTeam: small
 // pairs run on one wave with no synchronisation, long pairs do not outlast
 // the batch.  KSW2_HIP_TEAM forces a size (tuning and tests).
 static int choose_team(const KswJob &j, int R, double T, int forced)
@@ -87,37 +87,91 @@ static int choose_team(const KswJob &j, int R, double T, int forced)
     if (forced > 0) return std::max(1, std::min(std::min(forced, kMaxTeam), S));
     const double steps = (double)S * (j.qlen + kWave - 1);
     int team = 1;
-    while (team < kMaxTeam && team < S && steps / team > T) team <<= 1;
+    while (team < kMaxTeam && team < S && steps / team > 2 * T) team <<= 1;
     return std::min(team, S);
 }
 
 static int lane_of_team(int team) { return team >= 8 ? 0 : team >= 4 ? 1 : team >= 2 ? 2 : 3; }
 
-int plan_create(Plan **out, int n, const ksw2_pair_t *pairs, const int *idx, int nidx, const KswScore &sc)
+bool takes_general(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls)
 {
-    (void)n;
+    const char *ev = getenv("KSW2_HIP_FORCE_GENERAL");
+    return (ev && atoi(ev) > 0) || fast_path_reject(p, m, cls) != kFastOK;
+}
+
+static int norm_w(const ksw2_pair_t &p) { return p.w < 0 ? std::max(p.qlen, p.tlen) : p.w; }
+
+uint64_t pair_footprint(const ksw2_pair_t &p, bool general)
+{
+    const uint64_t cig = 4ull * ((uint64_t)p.qlen + p.tlen + 2) + 64, seq = (uint64_t)p.qlen + p.tlen + 32;
+    const bool cigar = !(p.flag & KSW_EZ_SCORE_ONLY);
+    if (!general)
+        return (cigar ? dir_bytes(p.qlen, p.tlen, kRowsPerLane) : 0) + 8ull * p.qlen + cig + seq + 512;
+    const int tlenB = (p.tlen + 63) / 64 * 64, qlenB = (p.qlen + 63) / 64 * 64;
+    const uint64_t diags = (uint64_t)p.qlen + p.tlen - 1;
+    const uint64_t rows = cigar ? diags * (uint64_t)gen_ncolB(p.qlen, p.tlen, norm_w(p)) + 256 : 0;
+    return gen_layout(tlenB, qlenB).bytes + rows + (cigar ? 8 * diags : 0) + cig + seq + 512;
+}
+
+void make_gen_score(int8_t m, const int8_t *mat, const KswScore &sc, const ScoreClass &cls, GenScore *g)
+{
+    memset(g, 0, sizeof(*g));
+    g->k = sc;
+    g->m = m;
+    g->sc_N = cls.sc_N;
+    for (int k = 0; k < m * m && k < kMaxM * kMaxM; ++k) g->mat[k] = mat[k];
+}
+
+// Threads of the general kernel's workgroup for one pair: about four cells
+// of the widest anti-diagonal per thread, a power of two in [64, 1024].
+// KSW2_HIP_GEN_THREADS forces a size (tuning and tests).
+static int gen_threads(const ksw2_pair_t &p)
+{
+    const char *ev = getenv("KSW2_HIP_GEN_THREADS");
+    const int forced = ev ? atoi(ev) : 0;
+    if (forced >= 64 && forced <= 1024 && forced % 64 == 0) return forced;
+    const int window = std::min(std::min(p.qlen, p.tlen), norm_w(p) + 1) + 128;
+    int t = 64;
+    while (t < 1024 && 2 * t <= window / 4) t <<= 1;
+    return t;
+}
+
+int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, int8_t m, const ScoreClass &cls,
+                const GenScore &gsc)
+{
     int st = device_init(-1);
     if (st != KSW2_HIP_OK) return st;
     Plan *p = new Plan();
     p->device = g_device;
-    p->sc = sc;
+    p->sc = gsc.k;
+    p->gsc = gsc;
     p->R = kRowsPerLane;
     const int R = p->R, SR = kWave * R;
-    // job groups: left-aligned traceback, right-aligned, score-only
-    std::vector<int> g[3];
+    // slots: stripe-kernel pairs by group (left, right, score-only), then
+    // general-kernel pairs
+    std::vector<int> g[3], gen;
     for (int k = 0; k < nidx; ++k) {
         const ksw2_pair_t &pp = pairs[idx[k]];
+        if (takes_general(pp, m, cls)) {
+            gen.push_back(idx[k]);
+            continue;
+        }
         const int grp = (pp.flag & KSW_EZ_SCORE_ONLY) ? 2 : (pp.flag & KSW_EZ_RIGHT) ? 1 : 0;
         g[grp].push_back(idx[k]);
     }
     std::vector<int> grp_of;
     for (int grp = 0; grp < 3; ++grp)
         for (int id : g[grp]) p->order.push_back(id), grp_of.push_back(grp);
+    p->n_fast = (int)p->order.size();
+    for (int id : gen) p->order.push_back(id);
+    p->n_gen = (int)gen.size();
     p->n = (int)p->order.size();
-    p->jobs.resize(p->n);
-    uint64_t so = 0, dof = 0, bo = 0, co = 0;
+    p->jobs.resize(p->n_fast);
+    p->gjobs.resize(p->n_gen);
+    p->cig_off.resize(p->n);
+    uint64_t so = 0, dof = 0, bo = 0, co = 0, sto = 0, oo = 0;
     int stripes = 0;
-    for (int k = 0; k < p->n; ++k) {
+    for (int k = 0; k < p->n_fast; ++k) {
         const ksw2_pair_t &pp = pairs[p->order[k]];
         KswJob &j = p->jobs[k];
         j.qlen = pp.qlen, j.tlen = pp.tlen, j.flag = pp.flag, j.end_bonus = pp.end_bonus;
@@ -129,22 +183,45 @@ int plan_create(Plan **out, int n, const ksw2_pair_t *pairs, const int *idx, int
         j.bnd_off = bo, bo += round_up(2ull * pp.qlen, 64);
         j.cig_off = co, co += round_up((uint64_t)pp.qlen + pp.tlen + 2, 16);
         j.stripe_off = stripes, stripes += (pp.tlen + SR - 1) / SR;
+        p->cig_off[k] = j.cig_off;
     }
     p->nstripes_total = stripes;
-    // launches: per group, per team size, longest pairs first
+    for (int gk = 0; gk < p->n_gen; ++gk) {
+        const int slot = p->n_fast + gk;
+        const ksw2_pair_t &pp = pairs[p->order[slot]];
+        GenJob &j = p->gjobs[gk];
+        j.qlen = pp.qlen, j.tlen = pp.tlen, j.w = norm_w(pp), j.zdrop = pp.zdrop;
+        j.end_bonus = pp.end_bonus, j.flag = pp.flag;
+        j.tlenB = (pp.tlen + 63) / 64 * 64, j.qlenB = (pp.qlen + 63) / 64 * 64;
+        j.ncolB = gen_ncolB(pp.qlen, pp.tlen, j.w);
+        j.out = slot;
+        j.q_off = so, so = round_up(so + pp.qlen, 16);
+        j.t_off = so, so = round_up(so + pp.tlen, 16);
+        j.st_off = sto, sto += gen_layout(j.tlenB, j.qlenB).bytes;
+        const uint64_t diags = (uint64_t)pp.qlen + pp.tlen - 1;
+        j.dir_off = dof;
+        j.off_off = oo;
+        if (!(pp.flag & KSW_EZ_SCORE_ONLY)) {
+            dof = round_up(dof + diags * (uint64_t)j.ncolB, 256);
+            oo += round_up(2 * diags, 16);
+        }
+        j.cig_off = co, co += round_up((uint64_t)pp.qlen + pp.tlen + 2, 16);
+        p->cig_off[slot] = j.cig_off;
+    }
+    // stripe-kernel launches: per group, per team size, longest pairs first
     std::vector<int32_t> order;
-    {
+    if (p->n_fast > 0) {
         double total_steps = 0;
         for (const KswJob &j : p->jobs) total_steps += (double)((j.tlen + SR - 1) / SR) * (j.qlen + kWave - 1);
         const double T = std::max(1.0, total_steps / std::max(1, stripe_resident_waves(R, false, false)));
         int forced = 0;
         if (const char *ev = getenv("KSW2_HIP_TEAM")) forced = atoi(ev);
-        std::vector<int> team(p->n);
-        for (int k = 0; k < p->n; ++k) team[k] = choose_team(p->jobs[k], R, T, forced);
+        std::vector<int> team(p->n_fast);
+        for (int k = 0; k < p->n_fast; ++k) team[k] = choose_team(p->jobs[k], R, T, forced);
         for (int grp = 0; grp < 3; ++grp)
             for (int t = kMaxTeam; t >= 1; --t) {
                 std::vector<int> members;
-                for (int k = 0; k < p->n; ++k)
+                for (int k = 0; k < p->n_fast; ++k)
                     if (grp_of[k] == grp && team[k] == t) members.push_back(k);
                 if (members.empty()) continue;
                 std::stable_sort(members.begin(), members.end(), [&](int x, int y) {
@@ -155,8 +232,27 @@ int plan_create(Plan **out, int n, const ksw2_pair_t *pairs, const int *idx, int
                 order.insert(order.end(), members.begin(), members.end());
             }
     }
+    // general-kernel launches: per block size, most anti-diagonal cells first
+    std::vector<int32_t> gorder;
+    {
+        std::vector<int> thr(p->n_gen);
+        for (int gk = 0; gk < p->n_gen; ++gk) thr[gk] = gen_threads(pairs[p->order[p->n_fast + gk]]);
+        for (int t = 1024; t >= 64; t >>= 1) {
+            std::vector<int> members;
+            for (int gk = 0; gk < p->n_gen; ++gk)
+                if (thr[gk] == t) members.push_back(gk);
+            if (members.empty()) continue;
+            std::stable_sort(members.begin(), members.end(), [&](int x, int y) {
+                const GenJob &a = p->gjobs[x], &b = p->gjobs[y];
+                return (double)(a.qlen + a.tlen) * a.ncolB > (double)(b.qlen + b.tlen) * b.ncolB;
+            });
+            p->glaunches.push_back({t, (int)gorder.size(), (int)members.size(), t >= 512 ? 0 : t >= 256 ? 1 : 2});
+            gorder.insert(gorder.end(), members.begin(), members.end());
+        }
+    }
     p->seq_bytes = std::max<uint64_t>(so, 16), p->dir_bytes = std::max<uint64_t>(dof, 256);
     p->bnd_dwords = std::max<uint64_t>(bo, 64), p->cig_words = std::max<uint64_t>(co, 16);
+    p->state_bytes = std::max<uint64_t>(sto, 256), p->offs_ints = std::max<uint64_t>(oo, 16);
     auto fail = [&](int code) {
         plan_destroy(p);
         return code;
@@ -174,28 +270,52 @@ int plan_create(Plan **out, int n, const ksw2_pair_t *pairs, const int *idx, int
     if (hipMalloc(&p->d_dir, p->dir_bytes) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
     if (hipMalloc(&p->d_bnd, p->bnd_dwords * 4) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
     if (hipMalloc(&p->d_cig, p->cig_words * 4) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
-    if (hipMalloc(&p->d_jobs, sizeof(KswJob) * std::max(p->n, 1)) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
+    if (hipMalloc(&p->d_jobs, sizeof(KswJob) * std::max(p->n_fast, 1)) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
     if (hipMalloc(&p->d_out, sizeof(KswOut) * std::max(p->n, 1)) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
     if (hipMalloc(&p->d_order, sizeof(int32_t) * std::max<size_t>(order.size(), 1)) != hipSuccess)
         return fail(KSW2_HIP_ENOMEM);
     if (hipMalloc(&p->d_recs, sizeof(StripeRec) * std::max(stripes, 1)) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
+    if (p->n_gen > 0) {
+        if (hipMalloc(&p->d_state, p->state_bytes) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
+        if (hipMalloc(&p->d_offs, p->offs_ints * 4) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
+        if (hipMalloc(&p->d_gjobs, sizeof(GenJob) * p->n_gen) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
+        if (hipMalloc(&p->d_gorder, sizeof(int32_t) * gorder.size()) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
+        if (hipHostMalloc((void **)&p->h_stop, sizeof(int32_t), hipHostMallocMapped) != hipSuccess)
+            return fail(KSW2_HIP_ENOMEM);
+        *p->h_stop = 0;
+        if (hipHostGetDevicePointer((void **)&p->d_stop, p->h_stop, 0) != hipSuccess) return fail(KSW2_HIP_EDEVICE);
+    }
     // stage sequences through pinned memory
     uint8_t *h_seq = nullptr;
     if (hipHostMalloc((void **)&h_seq, p->seq_bytes, hipHostMallocDefault) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
     memset(h_seq, 0, p->seq_bytes);
-    for (int k = 0; k < p->n; ++k) {
+    for (int k = 0; k < p->n_fast; ++k) {
         const ksw2_pair_t &pp = pairs[p->order[k]];
         memcpy(h_seq + p->jobs[k].q_off, pp.query, (size_t)pp.qlen);
         memcpy(h_seq + p->jobs[k].t_off, pp.target, (size_t)pp.tlen);
     }
-    hipError_t e1 = hipMemcpyAsync(p->d_seq, h_seq, p->seq_bytes, hipMemcpyHostToDevice, p->stream);
-    hipError_t e2 = hipMemcpyAsync(p->d_jobs, p->jobs.data(), sizeof(KswJob) * p->n, hipMemcpyHostToDevice, p->stream);
-    hipError_t e4 = order.empty() ? hipSuccess
-                                  : hipMemcpyAsync(p->d_order, order.data(), sizeof(int32_t) * order.size(),
-                                                   hipMemcpyHostToDevice, p->stream);
-    hipError_t e3 = hipStreamSynchronize(p->stream);
+    for (int gk = 0; gk < p->n_gen; ++gk) {
+        const ksw2_pair_t &pp = pairs[p->order[p->n_fast + gk]];
+        memcpy(h_seq + p->gjobs[gk].q_off, pp.query, (size_t)pp.qlen);
+        memcpy(h_seq + p->gjobs[gk].t_off, pp.target, (size_t)pp.tlen);
+    }
+    hipError_t e[6] = {hipSuccess, hipSuccess, hipSuccess, hipSuccess, hipSuccess, hipSuccess};
+    e[0] = hipMemcpyAsync(p->d_seq, h_seq, p->seq_bytes, hipMemcpyHostToDevice, p->stream);
+    if (p->n_fast > 0)
+        e[1] = hipMemcpyAsync(p->d_jobs, p->jobs.data(), sizeof(KswJob) * p->n_fast, hipMemcpyHostToDevice, p->stream);
+    if (!order.empty())
+        e[2] = hipMemcpyAsync(p->d_order, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice,
+                              p->stream);
+    if (p->n_gen > 0) {
+        e[3] = hipMemcpyAsync(p->d_gjobs, p->gjobs.data(), sizeof(GenJob) * p->n_gen, hipMemcpyHostToDevice,
+                              p->stream);
+        e[4] = hipMemcpyAsync(p->d_gorder, gorder.data(), sizeof(int32_t) * gorder.size(), hipMemcpyHostToDevice,
+                              p->stream);
+    }
+    e[5] = hipStreamSynchronize(p->stream);
     (void)hipHostFree(h_seq);
-    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) return fail(KSW2_HIP_EDEVICE);
+    for (hipError_t x : e)
+        if (x != hipSuccess) return fail(KSW2_HIP_EDEVICE);
     *out = p;
     return KSW2_HIP_OK;
 }
@@ -203,17 +323,32 @@ int plan_create(Plan **out, int n, const ksw2_pair_t *pairs, const int *idx, int
 int plan_run(Plan *p)
 {
     if (p->n == 0) return KSW2_HIP_OK;
+    if (p->h_stop) *p->h_stop = 0;
     KSW_HIP_TRY(hipEventRecord(p->ev[0], p->stream));
+    bool used[Plan::kLanes] = {false, false, false, false};
+    auto lane = [&](int l) -> hipStream_t {
+        if (!used[l]) {
+            if (hipStreamWaitEvent(p->lane_stream[l], p->ev[0], 0) != hipSuccess) return nullptr;
+            used[l] = true;
+        }
+        return p->lane_stream[l];
+    };
+    // general pairs first: they are the long, banded ones
+    GenArgs ga;
+    ga.seq = p->d_seq, ga.state = p->d_state, ga.dir = p->d_dir, ga.offs = p->d_offs, ga.jobs = p->d_gjobs;
+    ga.out = p->d_out, ga.stop = p->d_stop, ga.sc = p->gsc;
+    for (const Plan::GLaunch &l : p->glaunches) {
+        hipStream_t st = lane(l.lane);
+        if (!st) return KSW2_HIP_EDEVICE;
+        ga.order = p->d_gorder + l.off, ga.njobs = l.count;
+        KSW_HIP_TRY(launch_general(ga, l.threads, st));
+    }
     StripeArgs sa;
     sa.seq = p->d_seq, sa.dir = p->d_dir, sa.bnd = p->d_bnd, sa.sc = p->sc;
     sa.jobs = p->d_jobs, sa.out = p->d_out, sa.recs = p->d_recs;
-    bool used[Plan::kLanes] = {false, false, false, false};
     for (const Plan::Launch &l : p->launches) {
-        hipStream_t st = p->lane_stream[l.lane];
-        if (!used[l.lane]) {
-            KSW_HIP_TRY(hipStreamWaitEvent(st, p->ev[0], 0));
-            used[l.lane] = true;
-        }
+        hipStream_t st = lane(l.lane);
+        if (!st) return KSW2_HIP_EDEVICE;
         sa.order = p->d_order + l.off, sa.njobs = l.count;
         KSW_HIP_TRY(launch_stripe(sa, p->R, l.grp == 1, l.grp == 2, l.team, st));
     }
@@ -223,10 +358,18 @@ int plan_run(Plan *p)
             KSW_HIP_TRY(hipStreamWaitEvent(p->stream, p->lane_done[k], 0));
         }
     KSW_HIP_TRY(hipEventRecord(p->ev[1], p->stream));
-    BacktrackArgs ba;
-    ba.dir = p->d_dir, ba.cig = p->d_cig, ba.jobs = p->d_jobs, ba.out = p->d_out, ba.recs = p->d_recs;
-    ba.njobs = p->n, ba.sc = p->sc;
-    KSW_HIP_TRY(launch_backtrack(ba, p->stream));
+    if (p->n_fast > 0) {
+        BacktrackArgs ba;
+        ba.dir = p->d_dir, ba.cig = p->d_cig, ba.jobs = p->d_jobs, ba.out = p->d_out, ba.recs = p->d_recs;
+        ba.njobs = p->n_fast, ba.sc = p->sc;
+        KSW_HIP_TRY(launch_backtrack(ba, p->stream));
+    }
+    if (p->n_gen > 0) {
+        GenBacktrackArgs gb;
+        gb.dir = p->d_dir, gb.offs = p->d_offs, gb.cig = p->d_cig, gb.jobs = p->d_gjobs, gb.out = p->d_out;
+        gb.njobs = p->n_gen;
+        KSW_HIP_TRY(launch_general_backtrack(gb, p->stream));
+    }
     KSW_HIP_TRY(hipEventRecord(p->ev[2], p->stream));
     KSW_HIP_TRY(hipStreamSynchronize(p->stream));
     KSW_HIP_TRY(hipEventElapsedTime(&p->last_dp_ms, p->ev[0], p->ev[1]));
@@ -248,7 +391,7 @@ int plan_fetch(Plan *p, ksw_extz_t *ez, void *km)
         z->max_q = o.max_q, z->max_t = o.max_t;
         z->mqe = o.mqe, z->mqe_t = o.mqe_t, z->mte = o.mte, z->mte_q = o.mte_q;
         z->score = o.score, z->reach_end = o.reach_end, z->stopped = o.stopped;
-        const int st = deliver_cigar(z, cig.data() + p->jobs[k].cig_off, o.n_cigar, km);
+        const int st = deliver_cigar(z, cig.data() + p->cig_off[k], o.n_cigar, km);
         if (st != KSW2_HIP_OK) return st;
     }
     return KSW2_HIP_OK;
@@ -258,8 +401,10 @@ void plan_destroy(Plan *p)
 {
     if (!p) return;
     for (void *ptr : {(void *)p->d_seq, (void *)p->d_dir, (void *)p->d_bnd, (void *)p->d_cig, (void *)p->d_jobs,
-                      (void *)p->d_out, (void *)p->d_order, (void *)p->d_recs})
+                      (void *)p->d_out, (void *)p->d_order, (void *)p->d_recs, (void *)p->d_state, (void *)p->d_offs,
+                      (void *)p->d_gjobs, (void *)p->d_gorder})
         if (ptr) (void)hipFree(ptr);
+    if (p->h_stop) (void)hipHostFree(p->h_stop);
     for (auto &e : p->ev)
         if (e) (void)hipEventDestroy(e);
     for (int l = 0; l < Plan::kLanes; ++l) {
@@ -278,8 +423,13 @@ int run_batch_host(int n, const ksw2_pair_t *pairs, int8_t m, const int8_t *mat,
     KswScore sc;
     ScoreClass cls;
     if (!normalise_scoring(m, mat, q, e, q2, e2, &sc, &cls)) return KSW2_HIP_OK;  // reference returns early
-    std::vector<int> fast;
-    int status = KSW2_HIP_OK;
+    if (m > kMaxM) {
+        for (int k = 0; k < n; ++k) ez[k].zdropped = 1;
+        return KSW2_HIP_EUNSUPPORTED;
+    }
+    GenScore gsc;
+    make_gen_score(m, mat, sc, cls, &gsc);
+    std::vector<int> todo;
     for (int k = 0; k < n; ++k) {
         const ksw2_pair_t &p = pairs[k];
         if (p.qlen <= 0 || p.tlen <= 0) continue;            // reference returns early
@@ -288,29 +438,23 @@ int run_batch_host(int n, const ksw2_pair_t *pairs, int8_t m, const int8_t *mat,
             ez[k].zdropped = 1;
             continue;
         }
-        if (fast_path_reject(p, m, cls) == kFastOK) {
-            fast.push_back(k);
-        } else {
-            ez[k].zdropped = 1;  // not yet served by a device kernel
-            status = KSW2_HIP_EUNSUPPORTED;
-        }
+        todo.push_back(k);
     }
     // chunk by the device memory budget
     const uint64_t budget = mem_budget();
     size_t pos = 0;
-    while (pos < fast.size()) {
+    while (pos < todo.size()) {
         uint64_t used = 0;
         size_t end = pos;
-        while (end < fast.size()) {
-            const ksw2_pair_t &p = pairs[fast[end]];
-            const uint64_t need = dir_bytes(p.qlen, p.tlen, kRowsPerLane) + 8ull * p.qlen + 4ull * (p.qlen + p.tlen) +
-                                  (uint64_t)p.qlen + p.tlen + 512;
+        while (end < todo.size()) {
+            const ksw2_pair_t &p = pairs[todo[end]];
+            const uint64_t need = pair_footprint(p, takes_general(p, m, cls));
             if (end > pos && used + need > budget) break;
             used += need;
             ++end;
         }
         Plan *plan = nullptr;
-        int st = plan_create(&plan, n, pairs, fast.data() + pos, (int)(end - pos), sc);
+        int st = plan_create(&plan, pairs, todo.data() + pos, (int)(end - pos), m, cls, gsc);
         if (st != KSW2_HIP_OK) return st;
         st = plan_run(plan);
         if (st == KSW2_HIP_OK) st = plan_fetch(plan, ez, km);
@@ -318,7 +462,7 @@ int run_batch_host(int n, const ksw2_pair_t *pairs, int8_t m, const int8_t *mat,
         if (st != KSW2_HIP_OK) return st;
         pos = end;
     }
-    return status;
+    return KSW2_HIP_OK;
 }
 
 }  // namespace kswhip

@@ -51,6 +51,57 @@ struct StripeRec {
     int32_t total, best, best_i, pad;
 };
 
+// Scoring of the general kernel: the normalised pieces plus the full matrix
+// (KSW_EZ_GENERIC_SC) and the wildcard score of match/mismatch mode.
+constexpr int kMaxM = 16;
+struct GenScore {
+    KswScore k;
+    int32_t m, sc_N;
+    int8_t mat[kMaxM * kMaxM];
+};
+
+// One pair of the general (anti-diagonal) kernel.  Its state block in the
+// state arena holds the reference's per-target-position int8 arrays, the
+// target copy and the reversed query (laid out so that the reference's
+// over-reads see the same bytes), and H; see gen_layout().
+struct GenJob {
+    uint64_t q_off, t_off;     // byte offsets in the seq arena
+    uint64_t st_off;           // byte offset of the state block
+    uint64_t dir_off;          // byte offset of row 0 of the direction rows
+    uint64_t off_off;          // int32 offset of off[] (off_end[] follows)
+    uint64_t cig_off;          // uint32 offset in the cig arena
+    int32_t qlen, tlen, w, zdrop, end_bonus, flag;
+    int32_t ncolB;             // bytes per direction row: n_col_ * 64
+    int32_t tlenB, qlenB;      // lengths rounded up to 64
+    int32_t out;               // slot in the output array
+};
+
+// State block of one general pair: ten int8 arrays of tlenB (u, y, y2, s and
+// two parity copies of x, v, x2), the target copy sf (zero-padded to tlenB),
+// the reversed query qr followed by a zero tail, then int32 H[tlenB].
+struct GenLayout {
+    uint64_t u, y, y2, s, x0, x1, v0, v1, xx0, xx1, sf, qr, H, bytes;
+};
+KSW_HD inline GenLayout gen_layout(int tlenB, int qlenB)
+{
+    GenLayout L;
+    const uint64_t T = (uint64_t)tlenB;
+    L.u = 0, L.y = T, L.y2 = 2 * T, L.s = 3 * T, L.x0 = 4 * T, L.x1 = 5 * T, L.v0 = 6 * T, L.v1 = 7 * T;
+    L.xx0 = 8 * T, L.xx1 = 9 * T, L.sf = 10 * T, L.qr = 11 * T;
+    L.H = (L.qr + (uint64_t)qlenB + 128 + 255) / 256 * 256;
+    L.bytes = (L.H + 4 * T + 255) / 256 * 256;
+    return L;
+}
+
+// The reference's direction-row width for 64-byte blocks
+// (minimap2/ksw2_extd2_avx.c:198-206): n_col_ = ceil(min(qlen, tlen, w+1)/64) + 1.
+KSW_HD inline int gen_ncolB(int qlen, int tlen, int w)
+{
+    int n = qlen < tlen ? qlen : tlen;
+    n = n < w + 1 ? n : w + 1;
+    return ((n + 63) / 64 + 1) * 64;
+}
+
 struct KswOut {
     int32_t score, mqe, mqe_t, mte, mte_q;
     int32_t max, max_t, max_q;

@@ -28,6 +28,32 @@ struct BacktrackArgs {
     KswScore sc;
 };
 
+struct GenArgs {
+    const uint8_t *seq;
+    uint8_t *state;            // state blocks (gen_layout)
+    uint8_t *dir;              // direction rows
+    int32_t *offs;             // off[] / off_end[] per pair
+    const GenJob *jobs;
+    const int32_t *order;      // this launch's jobs, one workgroup each
+    int njobs;
+    KswOut *out;
+    const int32_t *stop;       // host-raised stop flag (pinned), or null
+    GenScore sc;
+};
+
+struct GenBacktrackArgs {
+    const uint8_t *dir;
+    const int32_t *offs;
+    uint32_t *cig;
+    const GenJob *jobs;
+    KswOut *out;
+    int njobs;
+};
+
+// General kernel: one workgroup of `threads` (64..1024) per job.
+hipError_t launch_general(const GenArgs &a, int threads, hipStream_t st);
+hipError_t launch_general_backtrack(const GenBacktrackArgs &a, hipStream_t st);
+
 // Stripe kernel: one workgroup of `team` waves (1..kMaxTeam) per job.
 hipError_t launch_stripe(const StripeArgs &a, int R, bool right, bool score_only, int team, hipStream_t st);
 hipError_t launch_backtrack(const BacktrackArgs &a, hipStream_t st);

@@ -20,6 +20,8 @@ struct ScoreClass {
 bool normalise_scoring(int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q2, int8_t e2, KswScore *sc,
                        ScoreClass *cls);
 int device_init(int device);
+// GenScore of a normalised scoring (matrix copied; m <= kMaxM checked by caller).
+void make_gen_score(int8_t m, const int8_t *mat, const KswScore &sc, const ScoreClass &cls, GenScore *g);
 int run_batch_host(int n, const ksw2_pair_t *pairs, int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q2,
                    int8_t e2, ksw_extz_t *ez, void *km);
 int deliver_cigar(ksw_extz_t *ez, const uint32_t *src, int n, void *km);
@@ -35,6 +37,11 @@ enum FastReject : int {
     kRejWildcard = 16,
 };
 int fast_path_reject(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls);
+// True when the pair runs on the general kernel (not fast-path eligible, or
+// KSW2_HIP_FORCE_GENERAL set).
+bool takes_general(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls);
+// Device bytes a pair needs while its plan is alive.
+uint64_t pair_footprint(const ksw2_pair_t &p, bool general);
 
 // A device-resident batch: sequences uploaded, arenas sized, jobs built.
 // Created from host pairs, run any number of times, results read back.
@@ -42,16 +49,24 @@ struct Plan {
     int device = 0;
     hipStream_t stream = nullptr;
     KswScore sc{};
-    int n = 0;                       // pairs in the plan (all fast-path)
+    GenScore gsc{};
+    int n = 0;                       // pairs in the plan
+    int n_fast = 0, n_gen = 0;       // slots [0, n_fast) stripe kernel, then general kernel
     int R = kRowsPerLane;            // rows per lane of the stripe kernel
-    std::vector<KswJob> jobs;        // host copy
+    std::vector<KswJob> jobs;        // host copy, stripe-kernel slots
+    std::vector<GenJob> gjobs;       // host copy, general-kernel slots
     std::vector<int> order;          // plan slot -> caller index
+    std::vector<uint64_t> cig_off;   // plan slot -> uint32 offset in the cig arena
     // One launch per (job group, team size).  Groups: 0 left-aligned
     // traceback, 1 right-aligned, 2 score-only.  `off`/`count` index d_order.
     struct Launch {
         int grp, team, off, count, lane;
     };
     std::vector<Launch> launches;
+    struct GLaunch {
+        int threads, off, count, lane;
+    };
+    std::vector<GLaunch> glaunches;  // general kernel, per block size
     int nstripes_total = 0;
     uint64_t seq_bytes = 0, dir_bytes = 0, bnd_dwords = 0, cig_words = 0;
     uint8_t *d_seq = nullptr, *d_dir = nullptr;
@@ -60,6 +75,13 @@ struct Plan {
     KswJob *d_jobs = nullptr;
     KswOut *d_out = nullptr;
     int32_t *d_order = nullptr;      // job indices, grouped by launch, longest first
+    uint64_t state_bytes = 0, offs_ints = 0;
+    uint8_t *d_state = nullptr;      // general kernel state blocks
+    int32_t *d_offs = nullptr;       // general kernel off[]/off_end[]
+    GenJob *d_gjobs = nullptr;
+    int32_t *d_gorder = nullptr;
+    int32_t *h_stop = nullptr;       // pinned stop flag the general kernel polls
+    int32_t *d_stop = nullptr;
     StripeRec *d_recs = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     // launches of different team sizes overlap on their own streams
@@ -69,7 +91,8 @@ struct Plan {
     float last_dp_ms = 0.f, last_bt_ms = 0.f;
 };
 
-int plan_create(Plan **out, int n, const ksw2_pair_t *pairs, const int *idx, int nidx, const KswScore &sc);
+int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, int8_t m, const ScoreClass &cls,
+                const GenScore &gsc);
 int plan_run(Plan *p);
 int plan_fetch(Plan *p, ksw_extz_t *ez, void *km);
 void plan_destroy(Plan *p);

@@ -197,17 +197,12 @@ def route(query, target, mat=None, m=5, q=8, e=2, q2=75, e2=1, w=-1, zdrop=-1, f
 
 def extd2_batch(pairs: Sequence[tuple], mat=None, m=5, q=8, e=2, q2=75, e2=1, w=-1, zdrop=-1,
                 end_bonus=0, flag=0, check=True) -> List[Result]:
-    """``ksw_extd2_hip_batch`` over ``pairs`` = [(query, target), ...] or
-    [(query, target, flag), ...]; band/zdrop/end_bonus shared here."""
+    """``ksw_extd2_hip_batch`` over ``pairs``: tuples (query, target) or
+    (query, target, flag[, w[, zdrop[, end_bonus]]]); the keyword values are
+    the defaults for fields a tuple leaves out.  Scoring is shared."""
     L = lib()
     n = len(pairs)
-    keep = []
-    arr = (KswPair * max(n, 1))()
-    for k, pr in enumerate(pairs):
-        qs, ts = _u8(pr[0]), _u8(pr[1])
-        keep.append((qs, ts))
-        f = pr[2] if len(pr) > 2 else flag
-        arr[k] = KswPair(qs.ctypes.data, ts.ctypes.data, len(qs), len(ts), w, zdrop, end_bonus, f)
+    arr, keep = _pair_array(pairs, w=w, zdrop=zdrop, end_bonus=end_bonus, flag=flag)
     mat = np.ascontiguousarray(np.asarray(default_matrix() if mat is None else mat, dtype=np.int8))
     ezs = (KswExtz * max(n, 1))()
     st = L.ksw_extd2_hip_batch(n, arr, m, mat.ctypes.data, q, e, q2, e2, ezs)
@@ -216,6 +211,7 @@ def extd2_batch(pairs: Sequence[tuple], mat=None, m=5, q=8, e=2, q2=75, e2=1, w=
         out.append(Result.from_ez(ezs[k]))
         if ezs[k].cigar:
             _libc.free(C.cast(ezs[k].cigar, C.c_void_p))
+    del keep
     if check and st != 0:
         raise KswError(f"ksw_extd2_hip_batch: {STATUS.get(st, st)}")
     return out
@@ -229,7 +225,10 @@ def _pair_array(pairs, w=-1, zdrop=-1, end_bonus=0, flag=0):
         qs, ts = _u8(pr[0]), _u8(pr[1])
         keep.append((qs, ts))
         f = pr[2] if len(pr) > 2 else flag
-        arr[k] = KswPair(qs.ctypes.data, ts.ctypes.data, len(qs), len(ts), w, zdrop, end_bonus, f)
+        pw = pr[3] if len(pr) > 3 else w
+        pz = pr[4] if len(pr) > 4 else zdrop
+        pb = pr[5] if len(pr) > 5 else end_bonus
+        arr[k] = KswPair(qs.ctypes.data, ts.ctypes.data, len(qs), len(ts), pw, pz, pb, f)
     return arr, keep
 
 
