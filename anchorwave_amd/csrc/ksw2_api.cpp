@@ -33,6 +33,13 @@ thread_local ksw2_hip_progress_cb t_cb = nullptr;
 thread_local void *t_cb_args = nullptr;
 thread_local int t_cb_interval = 256;
 
+kswhip::Progress current_progress()
+{
+    kswhip::Progress p;
+    p.cb = t_cb, p.args = t_cb_args, p.interval = t_cb_interval;
+    return p;
+}
+
 int set_err(int s)
 {
     t_last_error = s;
@@ -150,7 +157,8 @@ int ksw_extd2_hip_batch(int n, const ksw2_pair_t *pairs, int8_t m, const int8_t 
                         int8_t q2, int8_t e2, ksw_extz_t *ez)
 {
     if (n < 0 || (n > 0 && (!pairs || !mat || !ez))) return set_err(KSW2_HIP_EINVAL);
-    int st = kswhip::run_batch_host(n, pairs, m, mat, q, e, q2, e2, ez, nullptr);
+    const kswhip::Progress prog = current_progress();
+    int st = kswhip::run_batch_host(n, pairs, m, mat, q, e, q2, e2, ez, nullptr, &prog);
     return set_err(st);
 }
 
@@ -162,7 +170,8 @@ void ksw_extd2_hip(void *km, int qlen, const uint8_t *query, int tlen, const uin
     ksw2_pair_t p;
     p.query = query, p.target = target, p.qlen = qlen, p.tlen = tlen;
     p.w = w, p.zdrop = zdrop, p.end_bonus = end_bonus, p.flag = flag;
-    int st = kswhip::run_batch_host(1, &p, m, mat, q, e, q2, e2, ez, km);
+    const kswhip::Progress prog = current_progress();
+    int st = kswhip::run_batch_host(1, &p, m, mat, q, e, q2, e2, ez, km, &prog);
     set_err(st);
 }
 
@@ -188,6 +197,7 @@ int ksw2_hip_plan_create(ksw2_hip_plan **plan, int n, const ksw2_pair_t *pairs, 
         if (pairs[k].qlen <= 0 || pairs[k].tlen <= 0 || !pairs[k].query || !pairs[k].target)
             return set_err(KSW2_HIP_EINVAL);
         if (pairs[k].qlen > 0x7fffffff - pairs[k].tlen + 1) return set_err(KSW2_HIP_EINVAL);
+        if (pairs[k].flag & KSW_EZ_SINGLETRACK) return set_err(KSW2_HIP_EUNSUPPORTED);
         idx[k] = k;
         cells += (double)pairs[k].qlen * pairs[k].tlen;
     }

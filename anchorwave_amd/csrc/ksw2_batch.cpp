@@ -6,7 +6,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "../../include/ksw2_hip.h"
@@ -166,6 +168,8 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
     for (int id : gen) p->order.push_back(id);
     p->n_gen = (int)gen.size();
     p->n = (int)p->order.size();
+    for (int k = 0; k < p->n; ++k)
+        p->max_diags = std::max(p->max_diags, pairs[p->order[k]].qlen + pairs[p->order[k]].tlen - 1);
     p->jobs.resize(p->n_fast);
     p->gjobs.resize(p->n_gen);
     p->cig_off.resize(p->n);
@@ -320,10 +324,31 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
     return KSW2_HIP_OK;
 }
 
-int plan_run(Plan *p)
+// Calls the progress callback the way ksw_progress_continue does
+// (minimap2/ksw2_progress.c:30-38): at anti-diagonals 0, interval, 2*interval,
+// ... of `total`; here the k-th call is made on the k-th host poll.
+static bool progress_continue(const Progress *prog, int &calls, int total)
+{
+    if (!prog || !prog->cb) return true;
+    const int64_t r = std::min<int64_t>((int64_t)calls * prog->interval, std::max(total - 1, 0));
+    ++calls;
+    return prog->cb(prog->args, (int)r, total) != 0;
+}
+
+int plan_run(Plan *p, const Progress *prog)
 {
     if (p->n == 0) return KSW2_HIP_OK;
-    if (p->h_stop) *p->h_stop = 0;
+    p->stop_requested = false;
+    p->started = false;
+    int calls = 0;
+    if (!progress_continue(prog, calls, p->max_diags)) {
+        // stopped before anti-diagonal 0: nothing runs (ksw2_extd2_avx.c:249-252)
+        p->stop_requested = true;
+        p->last_dp_ms = p->last_bt_ms = 0.f;
+        return KSW2_HIP_OK;
+    }
+    if (p->h_stop) *(volatile int32_t *)p->h_stop = 0;
+    p->started = true;
     KSW_HIP_TRY(hipEventRecord(p->ev[0], p->stream));
     bool used[Plan::kLanes] = {false, false, false, false};
     auto lane = [&](int l) -> hipStream_t {
@@ -371,6 +396,19 @@ int plan_run(Plan *p)
         KSW_HIP_TRY(launch_general_backtrack(gb, p->stream));
     }
     KSW_HIP_TRY(hipEventRecord(p->ev[2], p->stream));
+    if (prog && prog->cb) {
+        // poll the callback while the device works; a stop raises the flag
+        // the general kernel samples every 256 anti-diagonals
+        hipError_t q;
+        while ((q = hipStreamQuery(p->stream)) == hipErrorNotReady) {
+            if (!p->stop_requested && !progress_continue(prog, calls, p->max_diags)) {
+                p->stop_requested = true;
+                if (p->h_stop) __atomic_store_n(p->h_stop, 1, __ATOMIC_SEQ_CST);
+            }
+            std::this_thread::sleep_for(std::chrono::microseconds(200));
+        }
+        if (q != hipSuccess) return KSW2_HIP_EDEVICE;
+    }
     KSW_HIP_TRY(hipStreamSynchronize(p->stream));
     KSW_HIP_TRY(hipEventElapsedTime(&p->last_dp_ms, p->ev[0], p->ev[1]));
     KSW_HIP_TRY(hipEventElapsedTime(&p->last_bt_ms, p->ev[1], p->ev[2]));
@@ -380,6 +418,10 @@ int plan_run(Plan *p)
 int plan_fetch(Plan *p, ksw_extz_t *ez, void *km)
 {
     if (p->n == 0) return KSW2_HIP_OK;
+    if (p->stop_requested && !p->started) {
+        for (int k = 0; k < p->n; ++k) ez[p->order[k]].stopped = 1;  // stopped at r = 0
+        return KSW2_HIP_OK;
+    }
     std::vector<KswOut> outs(p->n);
     std::vector<uint32_t> cig(p->cig_words);
     KSW_HIP_TRY(hipMemcpy(outs.data(), p->d_out, sizeof(KswOut) * p->n, hipMemcpyDeviceToHost));
@@ -391,6 +433,14 @@ int plan_fetch(Plan *p, ksw_extz_t *ez, void *km)
         z->max_q = o.max_q, z->max_t = o.max_t;
         z->mqe = o.mqe, z->mqe_t = o.mqe_t, z->mte = o.mte, z->mte_q = o.mte_q;
         z->score = o.score, z->reach_end = o.reach_end, z->stopped = o.stopped;
+        if (p->stop_requested && k < p->n_fast) {
+            // the stripe kernel has no stop point: a pair whose deadline passed
+            // while it ran is reported stopped with no path (callers discard
+            // stopped results, alignSlidingWindow.cpp:957-963)
+            z->stopped = 1, z->score = KSW_NEG_INF;
+            z->n_cigar = 0;
+            continue;
+        }
         const int st = deliver_cigar(z, cig.data() + p->cig_off[k], o.n_cigar, km);
         if (st != KSW2_HIP_OK) return st;
     }
@@ -416,7 +466,7 @@ void plan_destroy(Plan *p)
 }
 
 int run_batch_host(int n, const ksw2_pair_t *pairs, int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q2,
-                   int8_t e2, ksw_extz_t *ez, void *km)
+                   int8_t e2, ksw_extz_t *ez, void *km, const Progress *prog)
 {
     for (int k = 0; k < n; ++k) reset_ez(ez + k);
     if (n == 0 || m <= 1) return KSW2_HIP_OK;
@@ -429,6 +479,7 @@ int run_batch_host(int n, const ksw2_pair_t *pairs, int8_t m, const int8_t *mat,
     }
     GenScore gsc;
     make_gen_score(m, mat, sc, cls, &gsc);
+    int status = KSW2_HIP_OK;
     std::vector<int> todo;
     for (int k = 0; k < n; ++k) {
         const ksw2_pair_t &p = pairs[k];
@@ -436,6 +487,11 @@ int run_batch_host(int n, const ksw2_pair_t *pairs, int8_t m, const int8_t *mat,
         if (!p.query || !p.target) return KSW2_HIP_EINVAL;
         if (p.qlen > 0x7fffffff - p.tlen + 1) {               // ksw2_extd2_avx.c:138-141
             ez[k].zdropped = 1;
+            continue;
+        }
+        if (p.flag & KSW_EZ_SINGLETRACK) {  // two-track traceback: not implemented
+            ez[k].zdropped = 1;
+            status = KSW2_HIP_EUNSUPPORTED;
             continue;
         }
         todo.push_back(k);
@@ -456,13 +512,13 @@ int run_batch_host(int n, const ksw2_pair_t *pairs, int8_t m, const int8_t *mat,
         Plan *plan = nullptr;
         int st = plan_create(&plan, pairs, todo.data() + pos, (int)(end - pos), m, cls, gsc);
         if (st != KSW2_HIP_OK) return st;
-        st = plan_run(plan);
+        st = plan_run(plan, prog);
         if (st == KSW2_HIP_OK) st = plan_fetch(plan, ez, km);
         plan_destroy(plan);
         if (st != KSW2_HIP_OK) return st;
         pos = end;
     }
-    return KSW2_HIP_OK;
+    return status;
 }
 
 }  // namespace kswhip

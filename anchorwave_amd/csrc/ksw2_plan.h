@@ -22,8 +22,16 @@ bool normalise_scoring(int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q
 int device_init(int device);
 // GenScore of a normalised scoring (matrix copied; m <= kMaxM checked by caller).
 void make_gen_score(int8_t m, const int8_t *mat, const KswScore &sc, const ScoreClass &cls, GenScore *g);
+// The calling thread's progress callback (ksw2_hip_set_progress_callback):
+// polled on the host while a plan runs; a zero return raises the plan's stop
+// flag.  cb == nullptr: none.
+struct Progress {
+    ksw2_hip_progress_cb cb = nullptr;
+    void *args = nullptr;
+    int interval = 256;
+};
 int run_batch_host(int n, const ksw2_pair_t *pairs, int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q2,
-                   int8_t e2, ksw_extz_t *ez, void *km);
+                   int8_t e2, ksw_extz_t *ez, void *km, const Progress *prog = nullptr);
 int deliver_cigar(ksw_extz_t *ez, const uint32_t *src, int n, void *km);
 void reset_ez(ksw_extz_t *ez);
 
@@ -89,11 +97,14 @@ struct Plan {
     hipStream_t lane_stream[kLanes] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t lane_done[kLanes] = {nullptr, nullptr, nullptr, nullptr};
     float last_dp_ms = 0.f, last_bt_ms = 0.f;
+    bool stop_requested = false;     // the progress callback stopped the last run
+    bool started = false;            // the last run launched (not stopped at r = 0)
+    int max_diags = 0;               // largest qlen + tlen - 1 in the plan
 };
 
 int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, int8_t m, const ScoreClass &cls,
                 const GenScore &gsc);
-int plan_run(Plan *p);
+int plan_run(Plan *p, const Progress *prog = nullptr);
 int plan_fetch(Plan *p, ksw_extz_t *ez, void *km);
 void plan_destroy(Plan *p);
 

@@ -152,8 +152,39 @@ def lib():
         L.ksw2_hip_plan_dirs.restype = C.c_int
         L.ksw2_hip_plan_destroy.argtypes = [C.c_void_p]
         L.ksw2_hip_plan_destroy.restype = None
+        L.ksw2_hip_set_progress_callback.argtypes = [PROGRESS_CB, C.c_void_p, C.c_int]
+        L.ksw2_hip_set_progress_callback.restype = None
+        L.ksw2_hip_clear_progress_callback.argtypes = []
+        L.ksw2_hip_clear_progress_callback.restype = None
         _lib = L
     return _lib
+
+
+# int (*)(void *arguments, int antidiagonal, int total_antidiagonals)
+PROGRESS_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_int)
+
+
+class progress_callback:
+    """Context manager installing ``fn(antidiagonal, total) -> bool`` as the
+    calling thread's progress callback (``ksw2_hip_set_progress_callback``,
+    same meaning as minimap2's ``ksw_set_progress_callback``): a False
+    return stops the running call, which then reports ``stopped = 1``."""
+
+    def __init__(self, fn, interval: int = 256):
+        self.fn, self.interval, self.calls = fn, interval, 0
+
+    def _call(self, _args, r, total):
+        self.calls += 1
+        return 1 if self.fn(r, total) else 0
+
+    def __enter__(self):
+        self._cb = PROGRESS_CB(self._call)
+        lib().ksw2_hip_set_progress_callback(self._cb, None, self.interval)
+        return self
+
+    def __exit__(self, *exc):
+        lib().ksw2_hip_clear_progress_callback()
+        return False
 
 
 def default_matrix(match: int = 0, mismatch: int = -6, m: int = 5) -> np.ndarray:

@@ -79,6 +79,45 @@ def cpu_baseline(pairs, threads: int):
     return kind, dt, res
 
 
+def uneven_leg(args) -> dict:
+    """The uneven inter-anchor batch through ksw_extd2_hip_batch (host
+    buffers in and out, chunked by the device memory budget), against the
+    same kind of call made one pair at a time on a seeded sample, and checked
+    bit-exact against the CPU reference on every pair."""
+    log(f"[bench] uneven batch: generating {args.uneven} pairs")
+    pu = W.uneven(seed=3, n=args.uneven)
+    cu = W.cells(pu)
+    ksw2.extd2(pu[0][0][:64], pu[0][1][:64], mat=W_MAT, **SC)  # device initialised
+    t0 = time.perf_counter()
+    gres = ksw2.extd2_batch(pu, mat=W_MAT, **SC)
+    dt = time.perf_counter() - t0
+    log(f"[bench] uneven batch: {cu:.3g} cells in {dt:.2f} s = {cu / dt / 1e9:.1f} GCUPS (host API)")
+    rng = np.random.default_rng(7)
+    idx = rng.choice(len(pu), size=min(args.single_sample or 24, len(pu)), replace=False)
+    tot, cs = 0.0, 0
+    for k in idx:
+        q, t = pu[k]
+        t1 = time.perf_counter()
+        ksw2.extd2(q, t, mat=W_MAT, **SC)
+        tot += time.perf_counter() - t1
+        cs += len(q) * len(t)
+    single = cs / tot / 1e9
+    log(f"[bench] uneven one-at-a-time: {single:.2f} GCUPS over {len(idx)} sampled pairs")
+    res = {"pairs": len(pu), "cells": cu, "seconds": round(dt, 3), "gcups": round(cu / dt / 1e9, 2),
+           "one_at_a_time_gcups": round(single, 3), "one_at_a_time_sample": int(len(idx)),
+           "batched_over_one_at_a_time": round(cu / dt / 1e9 / single, 1),
+           "timing": "wall clock of ksw_extd2_hip_batch: upload, DP, backtrack, CIGAR download"}
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        log(f"[bench] uneven batch: CPU reference on {threads} threads (exactness check)")
+        kind, cdt, cres = cpu_baseline(pu, threads)
+        exact = sum(1 for g, c in zip(gres, cres) if g.__dict__ == c.__dict__)
+        res["exact"] = {"agree": exact, "of": len(pu), "frac": exact / len(pu)}
+        res["cpu"] = {"kind": kind, "gcups": round(cu / cdt / 1e9, 3), "cores": threads}
+        log(f"[bench] uneven CPU {cu / cdt / 1e9:.2f} GCUPS; exact {exact}/{len(pu)}")
+    return res
+
+
 def copy_calibration(nbytes: int) -> None:
     """One hipMemcpy device-to-device of nbytes: a plain copy kernel whose
     FETCH_SIZE / WRITE_SIZE readings scale the counters to bytes."""
@@ -108,6 +147,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / cross-check")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores (max 16)")
     ap.add_argument("--single-sample", type=int, default=16, help="pairs timed one at a time")
+    ap.add_argument("--uneven", type=int, default=20000,
+                    help="pairs in the uneven-batch leg (0: skip it)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (PMC byte-counter scale)")
     args = ap.parse_args()
@@ -195,6 +236,8 @@ def main() -> int:
         "one_at_a_time_gcups": None if single is None else round(single, 3),
         "batched_over_one_at_a_time": None if not single else round(gcups / single, 1),
     }
+    if args.uneven > 0:
+        out["uneven"] = uneven_leg(args)
     if not args.no_cpu:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         log(f"[bench] CPU baseline on {threads} threads")
