@@ -17,6 +17,7 @@
 
 #include "../../include/ksw2_hip.h"
 #include "ksw2_common.h"
+#include "ksw2_engine.h"
 #include "ksw2_kernels.h"
 #include "ksw2_plan.h"
 
@@ -239,7 +240,7 @@ int ksw2_hip_plan_dirs(const ksw2_hip_plan *plan, int index, uint8_t *out, size_
         if (p->order[x] == index) k = x;
     if (k < 0 || k >= p->n_fast) return set_err(KSW2_HIP_EINVAL);  // general-kernel pair: other layout
     const kswhip::KswJob &j = p->jobs[k];
-    const uint64_t need = kswhip::dir_bytes(j.qlen, j.tlen, j.R);
+    const uint64_t need = kswhip::dir_bytes(j.qlen, j.tlen);
     if (bytes < need || (j.flag & KSW_EZ_SCORE_ONLY)) return set_err(KSW2_HIP_EINVAL);
     if (hipMemcpy(out, p->d_dir + j.dir_off, need, hipMemcpyDeviceToHost) != hipSuccess)
         return set_err(KSW2_HIP_EDEVICE);
@@ -252,6 +253,59 @@ void ksw2_hip_plan_destroy(ksw2_hip_plan *plan)
     if (!plan) return;
     kswhip::plan_destroy(plan->p);
     delete plan;
+}
+
+struct ksw2_hip_server {
+    kswhip::Engine *e;
+};
+
+int ksw2_hip_server_create(ksw2_hip_server **server, int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q2,
+                           int8_t e2, size_t device_bytes)
+{
+    if (!server || !mat) return set_err(KSW2_HIP_EINVAL);
+    *server = nullptr;
+    if (m <= 1 || m > kMaxM) return set_err(KSW2_HIP_EUNSUPPORTED);
+    KswScore sc;
+    ScoreClass cls;
+    if (!normalise_scoring(m, mat, q, e, q2, e2, &sc, &cls)) return set_err(KSW2_HIP_EUNSUPPORTED);
+    int st = kswhip::device_init(-1);
+    if (st != KSW2_HIP_OK) return set_err(st);
+    uint64_t budget = device_bytes;
+    if (budget == 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) return set_err(KSW2_HIP_EDEVICE);
+        budget = (uint64_t)(fr * 0.85);
+    }
+    auto *eng = new kswhip::Engine(m, mat, sc, cls, budget);
+    st = eng->start();
+    if (st != KSW2_HIP_OK) {
+        delete eng;
+        return set_err(st);
+    }
+    *server = new ksw2_hip_server{eng};
+    return set_err(KSW2_HIP_OK);
+}
+
+int ksw2_hip_server_submit(ksw2_hip_server *server, const ksw2_pair_t *pair, ksw_extz_t *ez, uint64_t *ticket)
+{
+    if (!server || !pair || !ez) return set_err(KSW2_HIP_EINVAL);
+    return set_err(server->e->submit(*pair, ez, true, ticket));
+}
+
+int ksw2_hip_server_poll(ksw2_hip_server *server, uint64_t *tickets, double *latency_s, int max_tickets,
+                         int wait_ms)
+{
+    if (!server || max_tickets < 0 || (max_tickets > 0 && !tickets)) return set_err(KSW2_HIP_EINVAL);
+    const int n = server->e->poll(tickets, latency_s, max_tickets, wait_ms);
+    set_err(KSW2_HIP_OK);
+    return n;
+}
+
+void ksw2_hip_server_destroy(ksw2_hip_server *server)
+{
+    if (!server) return;
+    delete server->e;
+    delete server;
 }
 
 }  // extern "C"

@@ -13,6 +13,7 @@
 
 #include "../../include/ksw2_hip.h"
 #include "ksw2_common.h"
+#include "ksw2_engine.h"
 #include "ksw2_kernels.h"
 #include "ksw2_plan.h"
 
@@ -77,20 +78,21 @@ int fast_path_reject(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls)
 }
 
 // Team size for one pair.  With T the batch's step count per resident wave
-// (the time the batch needs when every wave is busy), a pair whose stripes
-// take S * (qlen + 63) steps on one wave gets the smallest power-of-two team
-// that brings it under 2T, capped by its stripe count and kMaxTeam: small
+// (the time the batch needs when every wave is busy), a pair whose passes
+// take P * (qlen + 127) steps on one wave gets the smallest power-of-two team
+// that brings it under 2T, capped by its pass count and kMaxTeam: small
 // pairs run on one wave with no synchronisation, long pairs do not outlast
 // the batch.  KSW2_HIP_TEAM forces a size (tuning and tests).
-static int choose_team(const KswJob &j, int R, double T, int forced)
+static double pair_steps(const KswJob &j) { return (double)pass_count(j.tlen) * (double)pass_steps(j.qlen); }
+
+static int choose_team(const KswJob &j, double T, int forced)
 {
-    const int SR = kWave * R;
-    const int S = (j.tlen + SR - 1) / SR;
-    if (forced > 0) return std::max(1, std::min(std::min(forced, kMaxTeam), S));
-    const double steps = (double)S * (j.qlen + kWave - 1);
+    const int P = pass_count(j.tlen);
+    if (forced > 0) return std::max(1, std::min(std::min(forced, kMaxTeam), P));
+    const double steps = pair_steps(j);
     int team = 1;
-    while (team < kMaxTeam && team < S && steps / team > 2 * T) team <<= 1;
-    return std::min(team, S);
+    while (team < kMaxTeam && team < P && steps / team > 2 * T) team <<= 1;
+    return std::min(team, P);
 }
 
 static int lane_of_team(int team) { return team >= 8 ? 0 : team >= 4 ? 1 : team >= 2 ? 2 : 3; }
@@ -108,7 +110,7 @@ uint64_t pair_footprint(const ksw2_pair_t &p, bool general)
     const uint64_t cig = 4ull * ((uint64_t)p.qlen + p.tlen + 2) + 64, seq = (uint64_t)p.qlen + p.tlen + 32;
     const bool cigar = !(p.flag & KSW_EZ_SCORE_ONLY);
     if (!general)
-        return (cigar ? dir_bytes(p.qlen, p.tlen, kRowsPerLane) : 0) + 8ull * p.qlen + cig + seq + 512;
+        return (cigar ? dir_bytes(p.qlen, p.tlen) : 0) + 8ull * p.qlen + cig + seq + 16ull * stripe_count(p.tlen) + 512;
     const int tlenB = (p.tlen + 63) / 64 * 64, qlenB = (p.qlen + 63) / 64 * 64;
     const uint64_t diags = (uint64_t)p.qlen + p.tlen - 1;
     const uint64_t rows = cigar ? diags * (uint64_t)gen_ncolB(p.qlen, p.tlen, norm_w(p)) + 256 : 0;
@@ -139,7 +141,7 @@ static int gen_threads(const ksw2_pair_t &p)
 }
 
 int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, int8_t m, const ScoreClass &cls,
-                const GenScore &gsc)
+                const GenScore &gsc, const PlanEnv *env)
 {
     int st = device_init(-1);
     if (st != KSW2_HIP_OK) return st;
@@ -183,7 +185,7 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         j.q_off = so, so = round_up(so + pp.qlen, 16);
         j.t_off = so, so = round_up(so + pp.tlen, 16);
         j.dir_off = dof;
-        if (!(pp.flag & KSW_EZ_SCORE_ONLY)) dof = round_up(dof + dir_bytes(pp.qlen, pp.tlen, R), 256);
+        if (!(pp.flag & KSW_EZ_SCORE_ONLY)) dof = round_up(dof + dir_bytes(pp.qlen, pp.tlen), 256);
         j.bnd_off = bo, bo += round_up(2ull * pp.qlen, 64);
         j.cig_off = co, co += round_up((uint64_t)pp.qlen + pp.tlen + 2, 16);
         j.stripe_off = stripes, stripes += (pp.tlen + SR - 1) / SR;
@@ -216,12 +218,12 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
     std::vector<int32_t> order;
     if (p->n_fast > 0) {
         double total_steps = 0;
-        for (const KswJob &j : p->jobs) total_steps += (double)((j.tlen + SR - 1) / SR) * (j.qlen + kWave - 1);
-        const double T = std::max(1.0, total_steps / std::max(1, stripe_resident_waves(R, false, false)));
+        for (const KswJob &j : p->jobs) total_steps += pair_steps(j);
+        const double T = std::max(1.0, total_steps / std::max(1, stripe_resident_waves(false, false)));
         int forced = 0;
         if (const char *ev = getenv("KSW2_HIP_TEAM")) forced = atoi(ev);
         std::vector<int> team(p->n_fast);
-        for (int k = 0; k < p->n_fast; ++k) team[k] = choose_team(p->jobs[k], R, T, forced);
+        for (int k = 0; k < p->n_fast; ++k) team[k] = choose_team(p->jobs[k], T, forced);
         for (int grp = 0; grp < 3; ++grp)
             for (int t = kMaxTeam; t >= 1; --t) {
                 std::vector<int> members;
@@ -230,7 +232,7 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
                 if (members.empty()) continue;
                 std::stable_sort(members.begin(), members.end(), [&](int x, int y) {
                     const KswJob &a = p->jobs[x], &b = p->jobs[y];
-                    return (double)a.tlen * (a.qlen + kWave - 1) > (double)b.tlen * (b.qlen + kWave - 1);
+                    return pair_steps(a) > pair_steps(b);
                 });
                 p->launches.push_back({grp, t, (int)order.size(), (int)members.size(), lane_of_team(t)});
                 order.insert(order.end(), members.begin(), members.end());
@@ -261,37 +263,63 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         plan_destroy(p);
         return code;
     };
-    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) return fail(KSW2_HIP_EDEVICE);
+    if (env && env->stream) {
+        p->stream = env->stream;  // caller's stream: every launch goes there
+    } else {
+        p->own_streams = true;
+        if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) return fail(KSW2_HIP_EDEVICE);
+        for (int l = 0; l < Plan::kLanes; ++l) {
+            if (hipStreamCreateWithFlags(&p->lane_stream[l], hipStreamNonBlocking) != hipSuccess)
+                return fail(KSW2_HIP_EDEVICE);
+            if (hipEventCreateWithFlags(&p->lane_done[l], hipEventDisableTiming) != hipSuccess)
+                return fail(KSW2_HIP_EDEVICE);
+        }
+    }
     for (auto &e : p->ev)
         if (hipEventCreate(&e) != hipSuccess) return fail(KSW2_HIP_EDEVICE);
-    for (int l = 0; l < Plan::kLanes; ++l) {
-        if (hipStreamCreateWithFlags(&p->lane_stream[l], hipStreamNonBlocking) != hipSuccess)
-            return fail(KSW2_HIP_EDEVICE);
-        if (hipEventCreateWithFlags(&p->lane_done[l], hipEventDisableTiming) != hipSuccess)
-            return fail(KSW2_HIP_EDEVICE);
-    }
-    if (hipMalloc(&p->d_seq, p->seq_bytes) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
-    if (hipMalloc(&p->d_dir, p->dir_bytes) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
-    if (hipMalloc(&p->d_bnd, p->bnd_dwords * 4) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
-    if (hipMalloc(&p->d_cig, p->cig_words * 4) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
-    if (hipMalloc(&p->d_jobs, sizeof(KswJob) * std::max(p->n_fast, 1)) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
-    if (hipMalloc(&p->d_out, sizeof(KswOut) * std::max(p->n, 1)) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
-    if (hipMalloc(&p->d_order, sizeof(int32_t) * std::max<size_t>(order.size(), 1)) != hipSuccess)
-        return fail(KSW2_HIP_ENOMEM);
-    if (hipMalloc(&p->d_recs, sizeof(StripeRec) * std::max(stripes, 1)) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
+    // one device block holds every arena
+    uint64_t at = 0;
+    auto take = [&](uint64_t bytes) {
+        const uint64_t o = at;
+        at = round_up(at + std::max<uint64_t>(bytes, 1), 256);
+        return o;
+    };
+    const uint64_t o_seq = take(p->seq_bytes), o_dir = take(p->dir_bytes), o_bnd = take(p->bnd_dwords * 4);
+    const uint64_t o_cig = take(p->cig_words * 4), o_jobs = take(sizeof(KswJob) * std::max(p->n_fast, 1));
+    const uint64_t o_out = take(sizeof(KswOut) * std::max(p->n, 1));
+    const uint64_t o_order = take(sizeof(int32_t) * std::max<size_t>(order.size(), 1));
+    const uint64_t o_recs = take(sizeof(StripeRec) * std::max(stripes, 1));
+    uint64_t o_state = 0, o_offs = 0, o_gjobs = 0, o_gorder = 0;
     if (p->n_gen > 0) {
-        if (hipMalloc(&p->d_state, p->state_bytes) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
-        if (hipMalloc(&p->d_offs, p->offs_ints * 4) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
-        if (hipMalloc(&p->d_gjobs, sizeof(GenJob) * p->n_gen) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
-        if (hipMalloc(&p->d_gorder, sizeof(int32_t) * gorder.size()) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
+        o_state = take(p->state_bytes), o_offs = take(p->offs_ints * 4);
+        o_gjobs = take(sizeof(GenJob) * p->n_gen), o_gorder = take(sizeof(int32_t) * gorder.size());
+    }
+    p->block_bytes = at;
+    uint8_t *base = nullptr;
+    if (env && env->pool) {
+        if (!env->pool->alloc(p->block_bytes, &p->pool_off)) return fail(KSW2_HIP_ENOMEM);
+        p->pool = env->pool;
+        base = env->pool->base + p->pool_off;
+    } else {
+        if (hipMalloc(&p->block, p->block_bytes) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
+        base = p->block;
+    }
+    p->d_seq = base + o_seq, p->d_dir = base + o_dir, p->d_bnd = (int32_t *)(base + o_bnd);
+    p->d_cig = (uint32_t *)(base + o_cig), p->d_jobs = (KswJob *)(base + o_jobs), p->d_out = (KswOut *)(base + o_out);
+    p->d_order = (int32_t *)(base + o_order), p->d_recs = (StripeRec *)(base + o_recs);
+    if (p->n_gen > 0) {
+        p->d_state = base + o_state, p->d_offs = (int32_t *)(base + o_offs);
+        p->d_gjobs = (GenJob *)(base + o_gjobs), p->d_gorder = (int32_t *)(base + o_gorder);
         if (hipHostMalloc((void **)&p->h_stop, sizeof(int32_t), hipHostMallocMapped) != hipSuccess)
             return fail(KSW2_HIP_ENOMEM);
         *p->h_stop = 0;
         if (hipHostGetDevicePointer((void **)&p->d_stop, p->h_stop, 0) != hipSuccess) return fail(KSW2_HIP_EDEVICE);
     }
-    // stage sequences through pinned memory
+    // stage sequences through pinned memory (kept until the plan is
+    // destroyed when the copy is left in flight on the caller's stream)
     uint8_t *h_seq = nullptr;
     if (hipHostMalloc((void **)&h_seq, p->seq_bytes, hipHostMallocDefault) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
+    p->h_seq = h_seq;
     memset(h_seq, 0, p->seq_bytes);
     for (int k = 0; k < p->n_fast; ++k) {
         const ksw2_pair_t &pp = pairs[p->order[k]];
@@ -316,8 +344,11 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         e[4] = hipMemcpyAsync(p->d_gorder, gorder.data(), sizeof(int32_t) * gorder.size(), hipMemcpyHostToDevice,
                               p->stream);
     }
-    e[5] = hipStreamSynchronize(p->stream);
-    (void)hipHostFree(h_seq);
+    if (p->own_streams) {
+        e[5] = hipStreamSynchronize(p->stream);
+        (void)hipHostFree(h_seq);
+        p->h_seq = nullptr;
+    }
     for (hipError_t x : e)
         if (x != hipSuccess) return fail(KSW2_HIP_EDEVICE);
     *out = p;
@@ -327,7 +358,7 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
 // Calls the progress callback the way ksw_progress_continue does
 // (minimap2/ksw2_progress.c:30-38): at anti-diagonals 0, interval, 2*interval,
 // ... of `total`; here the k-th call is made on the k-th host poll.
-static bool progress_continue(const Progress *prog, int &calls, int total)
+bool progress_continue(const Progress *prog, int &calls, int total)
 {
     if (!prog || !prog->cb) return true;
     const int64_t r = std::min<int64_t>((int64_t)calls * prog->interval, std::max(total - 1, 0));
@@ -335,13 +366,13 @@ static bool progress_continue(const Progress *prog, int &calls, int total)
     return prog->cb(prog->args, (int)r, total) != 0;
 }
 
-int plan_run(Plan *p, const Progress *prog)
+int plan_launch(Plan *p, const Progress *prog)
 {
-    if (p->n == 0) return KSW2_HIP_OK;
     p->stop_requested = false;
     p->started = false;
-    int calls = 0;
-    if (!progress_continue(prog, calls, p->max_diags)) {
+    p->progress_calls = 0;
+    if (p->n == 0) return KSW2_HIP_OK;
+    if (!progress_continue(prog, p->progress_calls, p->max_diags)) {
         // stopped before anti-diagonal 0: nothing runs (ksw2_extd2_avx.c:249-252)
         p->stop_requested = true;
         p->last_dp_ms = p->last_bt_ms = 0.f;
@@ -352,6 +383,7 @@ int plan_run(Plan *p, const Progress *prog)
     KSW_HIP_TRY(hipEventRecord(p->ev[0], p->stream));
     bool used[Plan::kLanes] = {false, false, false, false};
     auto lane = [&](int l) -> hipStream_t {
+        if (!p->own_streams) return p->stream;
         if (!used[l]) {
             if (hipStreamWaitEvent(p->lane_stream[l], p->ev[0], 0) != hipSuccess) return nullptr;
             used[l] = true;
@@ -375,7 +407,7 @@ int plan_run(Plan *p, const Progress *prog)
         hipStream_t st = lane(l.lane);
         if (!st) return KSW2_HIP_EDEVICE;
         sa.order = p->d_order + l.off, sa.njobs = l.count;
-        KSW_HIP_TRY(launch_stripe(sa, p->R, l.grp == 1, l.grp == 2, l.team, st));
+        KSW_HIP_TRY(launch_stripe(sa, l.grp == 1, l.grp == 2, l.team, st));
     }
     for (int k = 0; k < Plan::kLanes; ++k)
         if (used[k]) {
@@ -396,12 +428,24 @@ int plan_run(Plan *p, const Progress *prog)
         KSW_HIP_TRY(launch_general_backtrack(gb, p->stream));
     }
     KSW_HIP_TRY(hipEventRecord(p->ev[2], p->stream));
+    return KSW2_HIP_OK;
+}
+
+bool plan_done(Plan *p)
+{
+    if (!p->started) return true;
+    return hipEventQuery(p->ev[2]) != hipErrorNotReady;
+}
+
+int plan_wait(Plan *p, const Progress *prog)
+{
+    if (!p->started) return KSW2_HIP_OK;
     if (prog && prog->cb) {
         // poll the callback while the device works; a stop raises the flag
         // the general kernel samples every 256 anti-diagonals
         hipError_t q;
-        while ((q = hipStreamQuery(p->stream)) == hipErrorNotReady) {
-            if (!p->stop_requested && !progress_continue(prog, calls, p->max_diags)) {
+        while ((q = hipEventQuery(p->ev[2])) == hipErrorNotReady) {
+            if (!p->stop_requested && !progress_continue(prog, p->progress_calls, p->max_diags)) {
                 p->stop_requested = true;
                 if (p->h_stop) __atomic_store_n(p->h_stop, 1, __ATOMIC_SEQ_CST);
             }
@@ -409,10 +453,17 @@ int plan_run(Plan *p, const Progress *prog)
         }
         if (q != hipSuccess) return KSW2_HIP_EDEVICE;
     }
-    KSW_HIP_TRY(hipStreamSynchronize(p->stream));
+    KSW_HIP_TRY(hipEventSynchronize(p->ev[2]));
     KSW_HIP_TRY(hipEventElapsedTime(&p->last_dp_ms, p->ev[0], p->ev[1]));
     KSW_HIP_TRY(hipEventElapsedTime(&p->last_bt_ms, p->ev[1], p->ev[2]));
     return KSW2_HIP_OK;
+}
+
+int plan_run(Plan *p, const Progress *prog)
+{
+    const int st = plan_launch(p, prog);
+    if (st != KSW2_HIP_OK) return st;
+    return plan_wait(p, prog);
 }
 
 int plan_fetch(Plan *p, ksw_extz_t *ez, void *km)
@@ -424,8 +475,9 @@ int plan_fetch(Plan *p, ksw_extz_t *ez, void *km)
     }
     std::vector<KswOut> outs(p->n);
     std::vector<uint32_t> cig(p->cig_words);
-    KSW_HIP_TRY(hipMemcpy(outs.data(), p->d_out, sizeof(KswOut) * p->n, hipMemcpyDeviceToHost));
-    KSW_HIP_TRY(hipMemcpy(cig.data(), p->d_cig, p->cig_words * 4, hipMemcpyDeviceToHost));
+    KSW_HIP_TRY(hipMemcpyAsync(outs.data(), p->d_out, sizeof(KswOut) * p->n, hipMemcpyDeviceToHost, p->stream));
+    KSW_HIP_TRY(hipMemcpyAsync(cig.data(), p->d_cig, p->cig_words * 4, hipMemcpyDeviceToHost, p->stream));
+    KSW_HIP_TRY(hipStreamSynchronize(p->stream));
     for (int k = 0; k < p->n; ++k) {
         ksw_extz_t *z = ez + p->order[k];
         const KswOut &o = outs[k];
@@ -450,18 +502,19 @@ int plan_fetch(Plan *p, ksw_extz_t *ez, void *km)
 void plan_destroy(Plan *p)
 {
     if (!p) return;
-    for (void *ptr : {(void *)p->d_seq, (void *)p->d_dir, (void *)p->d_bnd, (void *)p->d_cig, (void *)p->d_jobs,
-                      (void *)p->d_out, (void *)p->d_order, (void *)p->d_recs, (void *)p->d_state, (void *)p->d_offs,
-                      (void *)p->d_gjobs, (void *)p->d_gorder})
-        if (ptr) (void)hipFree(ptr);
+    if (p->pool) p->pool->free(p->pool_off);
+    if (p->block) (void)hipFree(p->block);
     if (p->h_stop) (void)hipHostFree(p->h_stop);
+    if (p->h_seq) (void)hipHostFree(p->h_seq);
     for (auto &e : p->ev)
         if (e) (void)hipEventDestroy(e);
-    for (int l = 0; l < Plan::kLanes; ++l) {
-        if (p->lane_done[l]) (void)hipEventDestroy(p->lane_done[l]);
-        if (p->lane_stream[l]) (void)hipStreamDestroy(p->lane_stream[l]);
+    if (p->own_streams) {
+        for (int l = 0; l < Plan::kLanes; ++l) {
+            if (p->lane_done[l]) (void)hipEventDestroy(p->lane_done[l]);
+            if (p->lane_stream[l]) (void)hipStreamDestroy(p->lane_stream[l]);
+        }
+        if (p->stream) (void)hipStreamDestroy(p->stream);
     }
-    if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }
 
@@ -496,28 +549,37 @@ int run_batch_host(int n, const ksw2_pair_t *pairs, int8_t m, const int8_t *mat,
         }
         todo.push_back(k);
     }
-    // chunk by the device memory budget
     const uint64_t budget = mem_budget();
-    size_t pos = 0;
-    while (pos < todo.size()) {
-        uint64_t used = 0;
-        size_t end = pos;
-        while (end < todo.size()) {
-            const ksw2_pair_t &p = pairs[todo[end]];
-            const uint64_t need = pair_footprint(p, takes_general(p, m, cls));
-            if (end > pos && used + need > budget) break;
-            used += need;
-            ++end;
-        }
+    uint64_t total = Engine::kPlanSlack;
+    for (int k : todo) total += pair_footprint(pairs[k], takes_general(pairs[k], m, cls));
+    if (total <= budget || todo.size() <= 1) {
+        // one plan holds the whole batch
+        if (todo.empty()) return status;
         Plan *plan = nullptr;
-        int st = plan_create(&plan, pairs, todo.data() + pos, (int)(end - pos), m, cls, gsc);
-        if (st != KSW2_HIP_OK) return st;
-        st = plan_run(plan, prog);
+        int st = plan_create(&plan, pairs, todo.data(), (int)todo.size(), m, cls, gsc);
+        if (st == KSW2_HIP_OK) st = plan_run(plan, prog);
         if (st == KSW2_HIP_OK) st = plan_fetch(plan, ez, km);
         plan_destroy(plan);
-        if (st != KSW2_HIP_OK) return st;
-        pos = end;
+        if (st != KSW2_HIP_OK) {
+            for (int k : todo) ez[k].zdropped = 1;  // the device path did not run
+            return st;
+        }
+        return status;
     }
+    // larger than the budget: stream it through the engine, longest first
+    // (CIGARs grown with realloc, as with km == NULL)
+    Engine eng(m, mat, sc, cls, budget);
+    int st = eng.start();
+    if (st != KSW2_HIP_OK) {
+        for (int k : todo) ez[k].zdropped = 1;
+        return st;
+    }
+    eng.hold(true);
+    for (int k : todo) eng.submit(pairs[k], ez + k, false, nullptr);
+    eng.order_longest_first();
+    eng.hold(false);
+    st = eng.drain(prog);
+    if (st != KSW2_HIP_OK) status = st;
     return status;
 }
 

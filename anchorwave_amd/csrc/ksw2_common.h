@@ -133,26 +133,32 @@ KSW_HD inline int64_t edge_prefix(const KswScore &sc, int n)
     return s;
 }
 
-// Rows (target positions) handled by one lane of the stripe kernel; the
-// kernel is instantiated for these values and the planner picks one per pair.
+// Rows (target positions) handled by one lane of the stripe kernel; a
+// stripe is 64 lanes x kRowsPerLane rows, a pass is two stripes computed by
+// one wave (the second in the high halves, kWave steps behind).
 constexpr int kRowsPerLane = 8;
+constexpr int kStripeRows = kWave * kRowsPerLane;
 
-// Direction-byte address of cell (i = target, j = query) in the stripe layout:
-// stripe s = i / (64R), lane k, row rho; the lane visits column j at step
-// c = j + k; each step the wave writes 64 lanes x R bytes contiguously.
-KSW_HD inline uint64_t dir_index(int i, int j, int qlen, int R)
+KSW_HD inline int stripe_count(int tlen) { return (tlen + kStripeRows - 1) / kStripeRows; }
+KSW_HD inline int pass_count(int tlen) { return (stripe_count(tlen) + 1) / 2; }
+KSW_HD inline uint64_t pass_steps(int qlen) { return (uint64_t)qlen + 2 * kWave - 1; }
+
+// Direction-byte address of cell (i = target, j = query) in the pass layout:
+// stripe s = i / 512 is half h = s & 1 of pass p = s >> 1; lane k, row rho;
+// the lane visits column j of half h at step c = j + k + 64h; each step the
+// wave writes 64 lanes x 16 bytes, byte 4*(rho/2) + 2h + (rho&1) of a lane's
+// 16 being row rho of half h.
+KSW_HD inline uint64_t dir_index(int i, int j, int qlen)
 {
-    const int SR = kWave * R;
-    const int s = i / SR, rem = i - s * SR, k = rem / R, rho = rem - k * R;
-    const uint64_t nsteps = (uint64_t)qlen + kWave - 1;
-    return (((uint64_t)s * nsteps + (uint64_t)(j + k)) * kWave + (uint64_t)k) * (uint64_t)R + (uint64_t)rho;
+    const int s = i / kStripeRows, rem = i - s * kStripeRows, k = rem / kRowsPerLane, rho = rem % kRowsPerLane;
+    const int p = s >> 1, h = s & 1;
+    const uint64_t c = (uint64_t)j + k + (uint64_t)kWave * h;
+    return (((uint64_t)p * pass_steps(qlen) + c) * kWave + (uint64_t)k) * 16 + 4 * (rho >> 1) + 2 * h + (rho & 1);
 }
 
-KSW_HD inline uint64_t dir_bytes(int qlen, int tlen, int R)
+KSW_HD inline uint64_t dir_bytes(int qlen, int tlen)
 {
-    const int SR = kWave * R;
-    const uint64_t nstripes = (uint64_t)((tlen + SR - 1) / SR);
-    return nstripes * ((uint64_t)qlen + kWave - 1) * kWave * (uint64_t)R;
+    return (uint64_t)pass_count(tlen) * pass_steps(qlen) * kWave * 16;
 }
 
 }  // namespace kswhip

@@ -1,0 +1,455 @@
+// ksw2_engine.cpp -- continuous batching on one device.
+//
+// Tasks (pairs) wait in a host queue and are admitted by their device
+// footprint (direction bytes, state, CIGAR slots) into a pool carved out of
+// one hipMalloc.  Admitted tasks are grouped into plans ("flights") of one
+// size class that run concurrently on a few streams; a flight's results are
+// delivered and its memory returned as soon as it completes, and the freed
+// memory admits the next tasks.  When the task at the head of the queue does
+// not fit, smaller tasks from the back of the queue are admitted instead
+// (backfill), so the device keeps working while memory drains.
+//
+// Two users: ksw_extd2_hip_batch for batches larger than the memory budget
+// (queue ordered longest first), and the serving API (first come, first
+// served).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/ksw2_hip.h"
+#include "ksw2_common.h"
+#include "ksw2_engine.h"
+#include "ksw2_kernels.h"
+#include "ksw2_plan.h"
+
+namespace kswhip {
+
+namespace {
+
+uint64_t round_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// First-fit allocator over one device allocation; blocks are 256-aligned and
+// coalesced on release.
+class FirstFitPool : public DevicePool {
+public:
+    int init(uint64_t n)
+    {
+        bytes = round_up(n, 256);
+        if (hipMalloc(&base, bytes) != hipSuccess) {
+            base = nullptr;
+            return KSW2_HIP_ENOMEM;
+        }
+        free_[0] = bytes;
+        return KSW2_HIP_OK;
+    }
+    ~FirstFitPool() override
+    {
+        if (base) (void)hipFree(base);
+    }
+    bool alloc(uint64_t n, uint64_t *off) override
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        n = round_up(std::max<uint64_t>(n, 1), 256);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            *off = it->first;
+            const uint64_t rest = it->second - n, at = it->first + n;
+            free_.erase(it);
+            if (rest) free_[at] = rest;
+            used_[*off] = n;
+            return true;
+        }
+        return false;
+    }
+    void free(uint64_t off) override
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        auto u = used_.find(off);
+        if (u == used_.end()) return;
+        uint64_t start = off, len = u->second;
+        used_.erase(u);
+        auto next = free_.lower_bound(start);
+        if (next != free_.end() && next->first == start + len) len += next->second, next = free_.erase(next);
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == start) start = prev->first, len += prev->second, free_.erase(prev);
+        }
+        free_[start] = len;
+    }
+    uint64_t largest_free()
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        uint64_t m = 0;
+        for (auto &f : free_) m = std::max(m, f.second);
+        return m;
+    }
+
+private:
+    std::mutex mu_;
+    std::map<uint64_t, uint64_t> free_;
+    std::unordered_map<uint64_t, uint64_t> used_;
+};
+
+}  // namespace
+
+// Size classes keep a flight's members of similar duration, so short tasks
+// are not held back by long ones that share their plan.
+static int size_class(double cells) { return cells < 2e7 ? 0 : cells < 5e8 ? 1 : 2; }
+
+struct Engine::Task {
+    uint64_t ticket = 0;
+    ksw2_pair_t pair{};
+    std::vector<uint8_t> qcopy, tcopy;  // owned copies (serving) or empty
+    ksw_extz_t *ez = nullptr;           // caller's output
+    uint64_t footprint = 0;
+    double cells = 0;
+    int waves = 1, cls = 0;
+    bool general = false;
+    double t_submit = 0, t_start = 0, t_done = 0;
+};
+
+struct Engine::Flight {
+    Plan *plan = nullptr;
+    std::vector<Task *> tasks;
+    std::vector<ksw2_pair_t> pairs;
+    int stream = 0;
+};
+
+Engine::Engine(int8_t m, const int8_t *mat, const KswScore &sc, const ScoreClass &cls, uint64_t budget)
+    : m_(m), sc_(sc), cls_(cls), budget_(budget)
+{
+    make_gen_score(m, mat, sc, cls, &gsc_);
+}
+
+int Engine::start()
+{
+    int st = device_init(-1);
+    if (st != KSW2_HIP_OK) return st;
+    if (hipGetDevice(&device_) != hipSuccess) return KSW2_HIP_EDEVICE;
+    auto *pool = new FirstFitPool();
+    pool_.reset(pool);
+    st = pool->init(budget_);
+    if (st != KSW2_HIP_OK) return st;
+    for (auto &s : streams_)
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return KSW2_HIP_EDEVICE;
+    resident_waves_ = std::max(1, stripe_resident_waves(false, false));
+    worker_ = std::thread([this] { loop(); });
+    return KSW2_HIP_OK;
+}
+
+Engine::~Engine()
+{
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        quit_ = true;
+    }
+    cv_work_.notify_all();
+    if (worker_.joinable()) worker_.join();
+    for (auto &f : flights_) {
+        (void)plan_wait(f->plan);
+        plan_destroy(f->plan);
+    }
+    for (auto &s : streams_)
+        if (s) (void)hipStreamDestroy(s);
+    for (auto &kv : tasks_) delete kv.second;
+}
+
+int Engine::submit(const ksw2_pair_t &p, ksw_extz_t *ez, bool copy, uint64_t *ticket)
+{
+    auto *t = new Task();
+    t->pair = p;
+    t->ez = ez;
+    t->t_submit = now_s();
+    reset_ez(ez);
+    // calls that finish at submission, as the reference finishes them
+    bool finished = true;
+    int why = KSW2_HIP_OK;
+    if (p.qlen <= 0 || p.tlen <= 0) {
+        // reference returns early
+    } else if (!p.query || !p.target) {
+        ez->zdropped = 1, why = KSW2_HIP_EINVAL;
+    } else if (p.qlen > 0x7fffffff - p.tlen + 1) {
+        ez->zdropped = 1;                                   // ksw2_extd2_avx.c:138-141
+    } else if (p.flag & KSW_EZ_SINGLETRACK) {
+        ez->zdropped = 1, why = KSW2_HIP_EUNSUPPORTED;      // two-track traceback: not implemented
+    } else {
+        t->general = takes_general(p, m_, cls_);
+        t->footprint = pair_footprint(p, t->general);
+        if (t->footprint + kPlanSlack > budget_) ez->zdropped = 1, why = KSW2_HIP_ENOMEM;  // never fits
+        else finished = false;
+    }
+    if (!finished) {
+        if (copy) {
+            t->qcopy.assign(p.query, p.query + p.qlen);
+            t->tcopy.assign(p.target, p.target + p.tlen);
+            t->pair.query = t->qcopy.data(), t->pair.target = t->tcopy.data();
+        }
+        t->cells = (double)p.qlen * p.tlen;
+        t->cls = size_class(t->cells);
+        t->waves = t->cls == 2 ? std::min(kMaxTeam, pass_count(p.tlen)) : 1;
+    }
+    std::lock_guard<std::mutex> lk(mu_);
+    t->ticket = ++next_ticket_;
+    tasks_[t->ticket] = t;
+    if (finished) {
+        t->t_done = t->t_submit;
+        done_.push_back(t->ticket);
+        if (why != KSW2_HIP_OK && status_ == KSW2_HIP_OK) status_ = why;
+        cv_done_.notify_all();
+    } else {
+        queue_.push_back(t);
+        ++pending_;
+    }
+    if (ticket) *ticket = t->ticket;
+    cv_work_.notify_all();
+    return KSW2_HIP_OK;
+}
+
+void Engine::hold(bool on)
+{
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        hold_ = on;
+    }
+    cv_work_.notify_all();
+}
+
+void Engine::order_longest_first()
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    std::stable_sort(queue_.begin(), queue_.end(), [](const Task *a, const Task *b) { return a->cells > b->cells; });
+}
+
+int Engine::poll(uint64_t *tickets, double *latency, int max, int wait_ms)
+{
+    std::unique_lock<std::mutex> lk(mu_);
+    if (done_.empty() && wait_ms != 0) {
+        auto pred = [this] { return !done_.empty(); };
+        if (wait_ms < 0) cv_done_.wait(lk, pred);
+        else cv_done_.wait_for(lk, std::chrono::milliseconds(wait_ms), pred);
+    }
+    int n = 0;
+    while (n < max && !done_.empty()) {
+        const uint64_t tk = done_.front();
+        done_.pop_front();
+        if (tickets) tickets[n] = tk;
+        ++n;
+        auto it = tasks_.find(tk);
+        if (it != tasks_.end()) {
+            if (latency) latency[n - 1] = it->second->t_done - it->second->t_submit;
+            delete it->second;
+            tasks_.erase(it);
+        } else if (latency) {
+            latency[n - 1] = 0;
+        }
+    }
+    return n;
+}
+
+int Engine::drain(const Progress *prog)
+{
+    int calls = 0;
+    bool stopped = false;
+    std::unique_lock<std::mutex> lk(mu_);
+    while (pending_ > 0) {
+        if (prog && prog->cb && !stopped) {
+            lk.unlock();
+            const bool go = progress_continue(prog, calls, 1);
+            lk.lock();
+            if (!go) {
+                stopped = true;
+                stop_all_locked();
+            }
+        }
+        cv_done_.wait_for(lk, std::chrono::microseconds(500));
+    }
+    // forget completed tickets (batch mode has no poller)
+    for (uint64_t tk : done_) {
+        auto it = tasks_.find(tk);
+        if (it != tasks_.end()) delete it->second, tasks_.erase(it);
+    }
+    done_.clear();
+    return status_;
+}
+
+// The progress callback refused: queued tasks end as the reference's stop at
+// anti-diagonal 0; running plans get their stop flag raised.
+void Engine::stop_all_locked()
+{
+    for (Task *t : queue_) {
+        t->ez->stopped = 1;
+        t->t_done = now_s();
+        done_.push_back(t->ticket);
+        --pending_;
+    }
+    queue_.clear();
+    for (auto &f : flights_) {
+        f->plan->stop_requested = true;
+        if (f->plan->h_stop) __atomic_store_n(f->plan->h_stop, 1, __ATOMIC_SEQ_CST);
+    }
+    cv_done_.notify_all();
+}
+
+void Engine::loop()
+{
+    (void)hipSetDevice(device_);
+    std::unique_lock<std::mutex> lk(mu_);
+    for (;;) {
+        if (quit_ && queue_.empty() && flights_.empty()) break;
+        const bool reaped = reap_locked(lk);
+        const bool admitted = admit_locked(lk);
+        if (reaped || admitted) continue;
+        if (flights_.empty()) {
+            if (quit_) break;
+            cv_work_.wait_for(lk, std::chrono::milliseconds(5));
+        } else {
+            lk.unlock();
+            std::this_thread::sleep_for(std::chrono::microseconds(100));
+            lk.lock();
+        }
+    }
+}
+
+// Completed flights: results to the callers' ez, memory back to the pool.
+bool Engine::reap_locked(std::unique_lock<std::mutex> &lk)
+{
+    bool any = false;
+    for (size_t i = 0; i < flights_.size();) {
+        Flight *f = flights_[i].get();
+        if (!plan_done(f->plan)) {
+            ++i;
+            continue;
+        }
+        std::unique_ptr<Flight> own = std::move(flights_[i]);
+        flights_.erase(flights_.begin() + (long)i);
+        busy_[f->stream] = false;
+        lk.unlock();
+        int st = plan_wait(f->plan);
+        std::vector<ksw_extz_t> ez(f->tasks.size());
+        for (size_t k = 0; k < f->tasks.size(); ++k) ez[k] = *f->tasks[k]->ez;
+        if (st == KSW2_HIP_OK) st = plan_fetch(f->plan, ez.data(), nullptr);
+        plan_destroy(f->plan);
+        const double t = now_s();
+        lk.lock();
+        for (size_t k = 0; k < f->tasks.size(); ++k) {
+            Task *tk = f->tasks[k];
+            if (st == KSW2_HIP_OK) {
+                *tk->ez = ez[k];
+            } else {
+                // keep the (possibly reallocated) CIGAR buffer with the caller
+                tk->ez->cigar = ez[k].cigar, tk->ez->m_cigar = ez[k].m_cigar;
+                tk->ez->zdropped = 1, status_ = st;
+            }
+            tk->t_done = t;
+            done_.push_back(tk->ticket);
+            --pending_;
+        }
+        cv_done_.notify_all();
+        any = true;
+    }
+    return any;
+}
+
+// Forms and launches flights while streams and memory allow.
+bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
+{
+    bool any = false;
+    if (hold_) return false;
+    while (!queue_.empty()) {
+        int s = -1;
+        for (int k = 0; k < kStreams; ++k)
+            if (!busy_[k]) {
+                s = k;
+                break;
+            }
+        if (s < 0) break;
+        auto *pool = static_cast<FirstFitPool *>(pool_.get());
+        const uint64_t avail = pool->largest_free();
+        std::vector<Task *> pick;
+        uint64_t bytes = kPlanSlack;
+        int waves = 0;
+        const int wave_cap = resident_waves_;
+        // from the head: tasks of the head's size class, in queue order
+        const int cls = queue_.front()->cls;
+        for (auto it = queue_.begin(); it != queue_.end();) {
+            Task *t = *it;
+            if (t->cls != cls) {
+                ++it;
+                continue;
+            }
+            if (bytes + t->footprint > avail || (!pick.empty() && waves + t->waves > wave_cap)) break;
+            pick.push_back(t);
+            bytes += t->footprint, waves += t->waves;
+            it = queue_.erase(it);
+            if ((int)pick.size() >= kMaxFlightPairs) break;
+        }
+        if (pick.empty()) {
+            // backfill from the back (the smallest tasks when longest-first)
+            while (!queue_.empty()) {
+                Task *t = queue_.back();
+                if (bytes + t->footprint > avail || waves + t->waves > wave_cap) break;
+                pick.push_back(t);
+                bytes += t->footprint, waves += t->waves;
+                queue_.pop_back();
+                if ((int)pick.size() >= kMaxFlightPairs) break;
+            }
+        }
+        if (pick.empty()) break;  // wait for memory
+        auto f = std::make_unique<Flight>();
+        f->tasks = pick;
+        f->stream = s;
+        for (Task *t : pick) f->pairs.push_back(t->pair);
+        busy_[s] = true;
+        lk.unlock();
+        std::vector<int> idx(f->pairs.size());
+        for (size_t k = 0; k < idx.size(); ++k) idx[k] = (int)k;
+        PlanEnv env;
+        env.pool = pool_.get();
+        env.stream = streams_[s];
+        int st = plan_create(&f->plan, f->pairs.data(), idx.data(), (int)idx.size(), m_, cls_, gsc_, &env);
+        if (st == KSW2_HIP_OK) st = plan_launch(f->plan);
+        const double t0 = now_s();
+        lk.lock();
+        if (st != KSW2_HIP_OK) {
+            busy_[s] = false;
+            if (f->plan) plan_destroy(f->plan);
+            if (st == KSW2_HIP_ENOMEM && !flights_.empty()) {
+                // the estimate was short of the pool's fragmentation: requeue
+                for (auto it = pick.rbegin(); it != pick.rend(); ++it) queue_.push_front(*it);
+                break;
+            }
+            for (Task *t : pick) {
+                t->ez->zdropped = 1;
+                t->t_done = now_s();
+                done_.push_back(t->ticket);
+                --pending_;
+            }
+            status_ = st;
+            cv_done_.notify_all();
+            continue;
+        }
+        for (Task *t : pick) t->t_start = t0;
+        flights_.push_back(std::move(f));
+        any = true;
+    }
+    return any;
+}
+
+}  // namespace kswhip

@@ -1,0 +1,75 @@
+// ksw2_engine.h -- continuous-batching engine (see ksw2_engine.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/ksw2_hip.h"
+#include "ksw2_common.h"
+#include "ksw2_plan.h"
+
+namespace kswhip {
+
+class Engine {
+public:
+    static constexpr int kStreams = 4;             // flights in flight at once
+    static constexpr int kMaxFlightPairs = 8192;
+    static constexpr uint64_t kPlanSlack = 1 << 16;
+
+    Engine(int8_t m, const int8_t *mat, const KswScore &sc, const ScoreClass &cls, uint64_t budget);
+    ~Engine();
+    int start();
+    // Queues one pair; ez receives the result (reset now, filled when done).
+    // copy: keep private copies of the sequences (the caller may free them).
+    int submit(const ksw2_pair_t &p, ksw_extz_t *ez, bool copy, uint64_t *ticket);
+    // Batch mode: hold admission while a batch is queued, then sort the
+    // queue longest first and release it.
+    void hold(bool on);
+    void order_longest_first();
+    // Completed tickets (up to max) and their submit-to-done latencies
+    // (latency may be null); waits up to wait_ms (<0: until one).
+    int poll(uint64_t *tickets, double *latency, int max, int wait_ms);
+    // Waits until every submitted task is done (batch mode); polls the
+    // progress callback meanwhile.  Returns the first error seen, else OK.
+    int drain(const Progress *prog);
+
+private:
+    struct Task;
+    struct Flight;
+    void loop();
+    bool reap_locked(std::unique_lock<std::mutex> &lk);
+    bool admit_locked(std::unique_lock<std::mutex> &lk);
+    void stop_all_locked();
+
+    int8_t m_;
+    KswScore sc_;
+    ScoreClass cls_;
+    GenScore gsc_{};
+    uint64_t budget_;
+    int device_ = 0;
+    int resident_waves_ = 4096;
+    std::unique_ptr<DevicePool> pool_;
+    hipStream_t streams_[kStreams] = {};
+    bool busy_[kStreams] = {};
+    std::thread worker_;
+    std::mutex mu_;
+    std::condition_variable cv_work_, cv_done_;
+    std::deque<Task *> queue_;
+    std::vector<std::unique_ptr<Flight>> flights_;
+    std::deque<uint64_t> done_;
+    std::unordered_map<uint64_t, Task *> tasks_;
+    uint64_t next_ticket_ = 0;
+    int64_t pending_ = 0;
+    int status_ = 0;
+    bool quit_ = false;
+    bool hold_ = false;
+};
+
+}  // namespace kswhip

@@ -55,9 +55,9 @@ hipError_t launch_general(const GenArgs &a, int threads, hipStream_t st);
 hipError_t launch_general_backtrack(const GenBacktrackArgs &a, hipStream_t st);
 
 // Stripe kernel: one workgroup of `team` waves (1..kMaxTeam) per job.
-hipError_t launch_stripe(const StripeArgs &a, int R, bool right, bool score_only, int team, hipStream_t st);
+hipError_t launch_stripe(const StripeArgs &a, bool right, bool score_only, int team, hipStream_t st);
 hipError_t launch_backtrack(const BacktrackArgs &a, hipStream_t st);
 // Waves of the stripe kernel that fit on the device at once.
-int stripe_resident_waves(int R, bool right, bool score_only);
+int stripe_resident_waves(bool right, bool score_only);
 
 }  // namespace kswhip

@@ -51,6 +51,22 @@ bool takes_general(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls);
 // Device bytes a pair needs while its plan is alive.
 uint64_t pair_footprint(const ksw2_pair_t &p, bool general);
 
+// Device memory pool of a serving engine: one hipMalloc carved first-fit.
+struct DevicePool {
+    uint8_t *base = nullptr;
+    uint64_t bytes = 0;
+    virtual bool alloc(uint64_t n, uint64_t *off) = 0;
+    virtual void free(uint64_t off) = 0;
+    virtual ~DevicePool() {}
+};
+
+// Where a plan gets its memory and stream: defaults are its own hipMalloc'd
+// block and its own streams (one per team-size lane).
+struct PlanEnv {
+    DevicePool *pool = nullptr;
+    hipStream_t stream = nullptr;
+};
+
 // A device-resident batch: sequences uploaded, arenas sized, jobs built.
 // Created from host pairs, run any number of times, results read back.
 struct Plan {
@@ -77,6 +93,12 @@ struct Plan {
     std::vector<GLaunch> glaunches;  // general kernel, per block size
     int nstripes_total = 0;
     uint64_t seq_bytes = 0, dir_bytes = 0, bnd_dwords = 0, cig_words = 0;
+    uint8_t *block = nullptr;        // own device block (no pool)
+    DevicePool *pool = nullptr;      // or a block of this pool
+    uint64_t pool_off = 0, block_bytes = 0;
+    bool own_streams = false;
+    uint8_t *h_seq = nullptr;        // pinned staging kept while an upload may be in flight
+    int progress_calls = 0;
     uint8_t *d_seq = nullptr, *d_dir = nullptr;
     int32_t *d_bnd = nullptr;
     uint32_t *d_cig = nullptr;
@@ -103,8 +125,13 @@ struct Plan {
 };
 
 int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, int8_t m, const ScoreClass &cls,
-                const GenScore &gsc);
+                const GenScore &gsc, const PlanEnv *env = nullptr);
+// launch (asynchronous) / completed? / wait (polls the progress callback) / both
+int plan_launch(Plan *p, const Progress *prog = nullptr);
+bool plan_done(Plan *p);
+int plan_wait(Plan *p, const Progress *prog = nullptr);
 int plan_run(Plan *p, const Progress *prog = nullptr);
+bool progress_continue(const Progress *prog, int &calls, int total);
 int plan_fetch(Plan *p, ksw_extz_t *ez, void *km);
 void plan_destroy(Plan *p);
 

@@ -5,33 +5,42 @@
 // score-only), evaluated for every cell (i = target, j = query).  This file
 // evaluates it in a different order and representation than the reference:
 //
-//  * Stripes.  The target is cut into stripes of 64*R rows.  Lane k of a
-//    wavefront owns R consecutive rows and sweeps the query; at step c it is
-//    at column j = c - k, so the cell above (i-1, j) was produced by lane k-1
-//    one step earlier and arrives through a DPP wave_shr:1.  Lane 0 takes the
-//    bottom row of the previous stripe from a per-pair column buffer; the
-//    left neighbour (i, j-1) stays in the lane's registers.
-//  * Teams.  One workgroup of W waves computes one pair; wave w takes
-//    stripes w, w+W, w+2W, ...  Stripe s trails stripe s-1 by at least two
-//    64-column blocks: the producer publishes its bottom-row columns to the
-//    pair's column buffer and then a column count in LDS (workgroup-scope
-//    release), the consumer polls that count (workgroup-scope acquire) before
-//    it reads.  All waves of a team share one CU, so no cache maintenance
-//    beyond the CU is involved.  W = 1 (no waiting at all) for pairs that
-//    are small against the batch; long pairs get up to 16 waves.
+//  * Stripes and passes.  The target is cut into stripes of 512 rows.  Lane
+//    k of a wavefront owns 8 consecutive rows of a stripe and sweeps the
+//    query; at step c it is at column j = c - k, so the cell above (i-1, j)
+//    was produced by lane k-1 one step earlier and arrives through a DPP
+//    wave rotate.  A wave computes two stripes at once (a "pass"): stripe 2p
+//    in the low 16 bits of every register and stripe 2p+1 in the high 16
+//    bits, 64 steps behind, so every VALU instruction of the cell is a
+//    packed 16-bit operation on two cells.  The high stripe's lane 0 takes
+//    the low stripe's lane-63 output of the previous step (the rotate wraps
+//    it around); the low stripe's lane 0 takes the previous pass's bottom row
+//    from a per-pair column buffer; the left neighbour (i, j-1) stays in
+//    registers.
+//  * Teams.  One workgroup of W waves computes one pair; wave w takes passes
+//    w, w+W, ...  Pass p trails pass p-1 by at least two 64-column blocks:
+//    the producer writes its bottom-row columns to the column buffer and then
+//    publishes a column count in LDS (workgroup-scope release); the consumer
+//    polls that count (workgroup-scope acquire) before it reads.  All waves
+//    of a team share one CU, so no cache maintenance beyond the CU is
+//    involved.  W = 1 (no waiting at all) for pairs that are small against
+//    the batch; long pairs get up to 16 waves.
 //  * Scaled, negated state.  Every difference value w is kept as -8*w plus a
-//    3-bit tag, so one v_min3 pair yields both the maximum of the five
-//    candidates (s, a, b, a2, b2) and the index the reference's strict '>'
-//    (or '>=' for right alignment) chain would pick: ties resolve to the
-//    smallest (left) or largest (right) index through the tag.
+//    3-bit tag, so a chain of v_pk_min_i16 yields both the maximum of the
+//    five candidates (s, a, b, a2, b2) and the index the reference's strict
+//    '>' (or '>=' for right alignment) chain would pick: ties resolve to the
+//    smallest (left) or largest (right) index through the tag.  With
+//    T = (x+v) + 8e - 8z (tag-free), the new x is min(T, 0) and the
+//    continuation flag is the sign of T (left); right alignment keeps x
+//    biased by -8 so that the sign of T is its '>= 0' flag.
 //  * Exactness.  The reference computes on int8 lanes.  The host only routes
 //    a call here when every in-band intermediate provably stays inside int8
-//    (then int32 arithmetic gives the same values), the match score is 0 and
-//    no mismatch scores above it (then z = min(z, mat[0]) never binds and H
-//    never exceeds 0, so ez->max stays 0), there is no band and no z-drop.
+//    (then 16-bit arithmetic gives the same values), the match score is 0
+//    and no mismatch scores above it (then z = min(z, mat[0]) never binds and
+//    H never exceeds 0, so ez->max stays 0), there is no band and no z-drop.
 //
 // Outputs per pair: the direction byte of every cell (bits as in
-// minimap2/ksw2.h:156-159), and the H bookkeeping of
+// minimap2/ksw2.h:156-159, layout dir_index()), and the H bookkeeping of
 // minimap2/ksw2_extd2_sse.c:369-375 (score, mqe/mqe_t, mte/mte_q).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -42,19 +51,37 @@ namespace kswhip {
 
 namespace {
 
-constexpr int kDppWaveShr1 = 0x138;
 constexpr int kDppWaveShl1 = 0x130;
+constexpr int kDppWaveRor1 = 0x13C;
+constexpr int R = kRowsPerLane;  // rows per lane
+constexpr int SR = kWave * R;    // rows per stripe
+constexpr int kLag = kWave;      // steps the high stripe trails the low one
 
-__device__ __forceinline__ int dpp_shr1(int old, int src)
-{
-    return __builtin_amdgcn_update_dpp(old, src, kDppWaveShr1, 0xF, 0xF, false);
-}
+typedef short v2s __attribute__((ext_vector_type(2)));
+typedef unsigned short v2u __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ int bits(v2s v) { return __builtin_bit_cast(int, v); }
+__device__ __forceinline__ v2s vec(int x) { return __builtin_bit_cast(v2s, x); }
+__device__ __forceinline__ v2s splat(int x) { return (v2s){(short)x, (short)x}; }
+__device__ __forceinline__ v2s vmin(v2s a, v2s b) { return __builtin_elementwise_min(a, b); }
+__device__ __forceinline__ int lo16(int x) { return (int)(int16_t)(x & 0xffff); }
+__device__ __forceinline__ int hi16(int x) { return x >> 16; }
+
 __device__ __forceinline__ int dpp_shl1(int old, int src)
 {
     return __builtin_amdgcn_update_dpp(old, src, kDppWaveShl1, 0xF, 0xF, false);
 }
-__device__ __forceinline__ int min3i(int a, int b, int c) { return min(min(a, b), c); }
-__device__ __forceinline__ int med3i(int a, int lo, int hi) { return max(min(a, hi), lo); }
+
+// The value from the cell above for every lane: lane k takes lane k-1's
+// (both halves); lane 0 takes lane 63's low half of the previous step into
+// its high half, and the low stripe's boundary value (the high half of b_hi)
+// into its low half.
+__device__ __forceinline__ int carry(int y, int b_hi, bool lane0)
+{
+    const int rot = __builtin_amdgcn_update_dpp(0, y, kDppWaveRor1, 0xF, 0xF, false);
+    const int fix = (int)__builtin_amdgcn_alignbit((unsigned)rot, (unsigned)b_hi, 16);
+    return lane0 ? fix : rot;
+}
 
 // Tags: state k = 0 (diagonal s), 1 (a), 2 (b), 3 (a2), 4 (b2).  Under
 // v_min the smallest tag wins a tie, so left alignment (first strict maximum)
@@ -64,94 +91,74 @@ template <bool RIGHT> struct Tags {
     static constexpr int A2 = RIGHT ? 4 : 3, B2 = RIGHT ? 3 : 4;
 };
 
-// Per-kernel constants of the scaled representation:
-//   stored  U = -8u, V = -8v, Xh = -8(x+G1), X2h = -8(x2+G2), Yh, Y2h likewise
-//   cands   A = Xh+V+CA = -8a+tagA, ... , S = -8s+tagS
-//   after   A' = A+nZ+KA = -8(a-z+q), Xh' = min(A',0)
+// Constants of the scaled representation (both halves):
+//   stored  U = -8u, V = -8v, X = -8(x+G1) - BIAS, X2 = -8(x2+G2) - BIAS,
+//           Y, Y2 likewise; BIAS = 0 (left) or 8 (right)
+//   cands   A = (X+V) + CA = -8a + tagA, ..., S = -8s + tagS
+//   update  T = (X+V) + 8e - (Z & ~7) = -8(a-z+q) - BIAS, X' = min(T, -BIAS)
 struct Consts {
-    int CA, CB, CA2, CB2, KA, KB, KA2, KB2, SM, SX;
+    v2s CA, CB, CA2, CB2, E1, E2, LIM, SM, SD;
 };
 
 template <bool RIGHT>
 __device__ __forceinline__ Consts make_consts(const KswScore &sc)
 {
     using T = Tags<RIGHT>;
+    const int bias = RIGHT ? 8 : 0;
     Consts k;
-    k.CA = 8 * sc.G1 + T::A;
-    k.CB = 8 * sc.G1 + T::B;
-    k.CA2 = 8 * sc.G2 + T::A2;
-    k.CB2 = 8 * sc.G2 + T::B2;
-    k.KA = -(8 * sc.q + T::A);
-    k.KB = -(8 * sc.q + T::B);
-    k.KA2 = -(8 * sc.q2 + T::A2);
-    k.KB2 = -(8 * sc.q2 + T::B2);
-    k.SM = -8 * sc.mch + T::S;
-    k.SX = -8 * sc.mis + T::S;
+    k.CA = splat(8 * sc.G1 + T::A + bias);
+    k.CB = splat(8 * sc.G1 + T::B + bias);
+    k.CA2 = splat(8 * sc.G2 + T::A2 + bias);
+    k.CB2 = splat(8 * sc.G2 + T::B2 + bias);
+    k.E1 = splat(8 * sc.e);
+    k.E2 = splat(8 * sc.e2);
+    k.LIM = splat(-bias);
+    k.SM = splat(-8 * sc.mch + T::S);
+    k.SD = splat(-8 * (sc.mis - sc.mch));
     return k;
 }
 
-// One cell.  In: Xa/X2a/Va from the cell above (replaced by this cell's
-// outputs for the cell below), U/Yh/Y2h from the cell to the left (replaced
-// by this cell's outputs for the cell to the right).  Returns the direction
-// byte (0 when SCORE_ONLY).
+// Two cells (low and high half).  In: X/X2/V from the cells above (replaced
+// by these cells' outputs for the cells below), U/Y/Y2 from the cells to the
+// left (replaced for the cells to the right), tb/qb the target and query
+// codes.  Returns the two direction bytes in the low byte of each half (0
+// when SCORE_ONLY).
 template <bool RIGHT, bool SCORE_ONLY>
-__device__ __forceinline__ int cell(const Consts &k, int tb, int qb, int &Xa, int &X2a, int &Va,
-                                    int &U, int &Yh, int &Y2h)
+__device__ __forceinline__ int cell2(const Consts &k, v2u tb, v2u qb, v2s &X, v2s &X2, v2s &V, v2s &U,
+                                     v2s &Y, v2s &Y2)
 {
-    const int S = tb == qb ? k.SM : k.SX;
-    const int A = Xa + Va + k.CA;
-    const int B = Yh + U + k.CB;
-    const int A2 = X2a + Va + k.CA2;
-    const int B2 = Y2h + U + k.CB2;
-    const int Z = min3i(min3i(S, A, B), A2, B2);
-    const int Z8 = Z & ~7;
-    const int nZ = -Z8;
-    const int Un = Z8 - Va;
-    const int Vn = Z8 - U;
-    const int Ap = A + nZ + k.KA;
-    const int Bp = B + nZ + k.KB;
-    const int A2p = A2 + nZ + k.KA2;
-    const int B2p = B2 + nZ + k.KB2;
-    const int Xn = min(Ap, 0), Yn = min(Bp, 0), X2n = min(A2p, 0), Y2n = min(B2p, 0);
-    int d = 0;
-    if (!SCORE_ONLY) {
-        if (!RIGHT) {
-            // continuation flags: a' > 0  <=>  A' < 0  <=>  max(Xh', -8) == -8
-            d = Z & 7;
-            d |= max(Xn, -8) & 0x08;
-            d |= max(Yn, -8) & 0x10;
-            d |= max(X2n, -8) & 0x20;
-            d |= max(Y2n, -8) & 0x40;
-        } else {
-            // a' >= 0  <=>  A' <= 0: med3(A',0,8) is 0 when set, 8 when clear
-            int t = med3i(Ap, 0, 8) | (med3i(Bp, 0, 8) << 1) | (med3i(A2p, 0, 8) << 2) |
-                    (med3i(B2p, 0, 8) << 3);
-            d = ((Z & 7) | t) ^ 0x7F;
-        }
-    }
-    Xa = Xn, X2a = X2n, Va = Vn;
-    U = Un, Yh = Yn, Y2h = Y2n;
+    v2u mm = __builtin_elementwise_min(tb ^ qb, (v2u){1, 1});  // 0 match, 1 mismatch
+    // keep the multiply-add form (one v_pk_mad) rather than a per-half
+    // compare-and-select the optimiser would otherwise derive from mm <= 1
+    asm volatile("" : "+v"(mm));
+    const v2s S = k.SM + __builtin_bit_cast(v2s, mm) * k.SD;
+    const v2s XV = X + V, X2V = X2 + V, YU = Y + U, Y2U = Y2 + U;
+    const v2s A = XV + k.CA, B = YU + k.CB, A2 = X2V + k.CA2, B2 = Y2U + k.CB2;
+    const v2s Z = vmin(vmin(vmin(vmin(S, A), B), A2), B2);
+    const v2s Z8 = vec(bits(Z) & (int)0xFFF8FFF8u);
+    const v2s M1 = k.E1 - Z8, M2 = k.E2 - Z8;
+    const v2s Ta = XV + M1, Tb = YU + M1, Ta2 = X2V + M2, Tb2 = Y2U + M2;
+    const v2s Un = Z8 - V, Vn = Z8 - U;
+    X = vmin(Ta, k.LIM), Y = vmin(Tb, k.LIM), X2 = vmin(Ta2, k.LIM), Y2 = vmin(Tb2, k.LIM);
+    U = Un, V = Vn;
+    if (SCORE_ONLY) return 0;
+    // state index (tags) and the four continuation flags (sign bits of T)
+    int d = RIGHT ? (~bits(Z) & 0x00070007) : (bits(Z) & 0x00070007);
+    d |= bits(__builtin_bit_cast(v2s, __builtin_bit_cast(v2u, Ta) >> (v2u){12, 12})) & 0x00080008;
+    d |= bits(__builtin_bit_cast(v2s, __builtin_bit_cast(v2u, Tb) >> (v2u){11, 11})) & 0x00100010;
+    d |= bits(__builtin_bit_cast(v2s, __builtin_bit_cast(v2u, Ta2) >> (v2u){10, 10})) & 0x00200020;
+    d |= bits(__builtin_bit_cast(v2s, __builtin_bit_cast(v2u, Tb2) >> (v2u){9, 9})) & 0x00400040;
     return d;
 }
 
-template <int R>
-__device__ __forceinline__ void store_dirs(uint8_t *p, const int (&d)[R])
+// Sets one half of a packed register.
+__device__ __forceinline__ v2s set_half(v2s v, int value, bool hi)
 {
-    uint32_t w[R / 4];
-#pragma unroll
-    for (int g = 0; g < R / 4; ++g)
-        w[g] = (uint32_t)d[4 * g] | ((uint32_t)d[4 * g + 1] << 8) | ((uint32_t)d[4 * g + 2] << 16) |
-               ((uint32_t)d[4 * g + 3] << 24);
-    if constexpr (R == 4) {
-        *(uint32_t *)p = w[0];
-    } else if constexpr (R == 8) {
-        *(uint2 *)p = make_uint2(w[0], w[1]);
-    } else {
-        *(uint4 *)p = make_uint4(w[0], w[1], w[2], w[3]);
-    }
+    const int b = bits(v);
+    return vec(hi ? (b & 0xffff) | (value << 16) : (int)((unsigned)b & 0xffff0000u) | (value & 0xffff));
 }
 
-// Team progress: wave w publishes, for its k-th stripe, k*(qlen+1) + the
+// Team progress: wave w publishes, for its k-th pass, k*(qlen+1) + the
 // number of bottom-row columns written so far (monotonic per wave).
 __device__ __forceinline__ void publish(int32_t *slot, int value)
 {
@@ -162,7 +169,7 @@ __device__ __forceinline__ void publish(int32_t *slot, int value)
 
 // Spins until the producer's published count reaches `need`.  The exit test
 // uses lane 0's value, so the loop is uniform.  A producer always ends its
-// stripe at k*(qlen+1) + qlen, so `need` <= that bound terminates.
+// pass at k*(qlen+1) + qlen, so `need` <= that bound terminates.
 __device__ __forceinline__ int wait_count(const int32_t *slot, int need, int known)
 {
     if (known >= need) return known;
@@ -174,121 +181,15 @@ __device__ __forceinline__ int wait_count(const int32_t *slot, int need, int kno
     return seen;
 }
 
-// One stripe of one pair, computed by the calling wave.  Roles (uniform
-// flags): the first stripe (s == 0) takes row -1 as its upper boundary
-// instead of the column buffer; the last stripe owns row tlen-1 (score / mte)
-// and publishes nothing; every other stripe publishes its bottom row.
-template <int R, bool RIGHT, bool SCORE_ONLY>
-__device__ __forceinline__ void stripe_body(const StripeArgs &a, const Consts &kc, int job_id, int s,
-                                            const KswJob &jb, int qlen, int tlen, int32_t *prog, int W)
+// mqe partial of one stripe from the snapshot of U at column qlen-1 (one
+// half of Us): H(i,qlen-1) = H(-1,qlen-1) + sum_{i'<=i} u(i'), kept in the
+// -8 domain; the smallest running sum (largest H) with ties to the larger
+// row, and the stripe total.
+__device__ __forceinline__ void stripe_rec(const v2s (&Us)[R], bool hi, int i0, int tlen, int lane, StripeRec *out)
 {
-    constexpr int SR_ = kWave * R;
-    const bool first = s == 0;
-    const bool last = s == (tlen + SR_ - 1) / SR_ - 1;
-    const KswScore &sc = a.sc;
-    const int lane = threadIdx.x % kWave;
-    const uint8_t *qs = a.seq + jb.q_off;
-    const uint8_t *ts = a.seq + jb.t_off;
-    uint8_t *dir = a.dir + jb.dir_off;
-    int32_t *bnd = a.bnd + jb.bnd_off;
-    constexpr int SR = kWave * R;
-    const int nsteps = qlen + kWave - 1;
-    const int rho_last = last ? (tlen - 1) % R : -1;
-    // progress slots: stripe s-1 is wave (s-1) % W's ((s-1)/W)-th stripe
-    const int32_t *prog_prev = prog + (s + W - 1) % W;
-    const int base_prev = first ? 0 : (s - 1) / W * (qlen + 1);
-    int32_t *prog_mine = prog + s % W;
-    const int base_mine = s / W * (qlen + 1);
-
-    int U[R], Yh[R], Y2h[R], TB[R];
-    const int i0 = s * SR + lane * R;
+    int u[R], lsum = 0;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int i = i0 + r;
-        TB[r] = i < tlen ? (int)ts[i] : 0;
-        U[r] = -8 * edge_diff(sc, i);
-        Yh[r] = 0;
-        Y2h[r] = 0;
-    }
-    int row_acc = 0, mte_best = 0x7fffffff, mte_j = -1;  // row tlen-1 in the -8 domain
-    int Xo = 0, X2o = 0, Vo = 0;        // this lane's bottom-row outputs
-    int bin_xx = 0, bin_v = 0;          // lane-0 boundary queue (rotating)
-    int bnx_xx = 0, bnx_v = 0;          // next block of the queue
-    int out_xx = 0, out_v = 0;          // lane-63 output queue (rotating)
-    int known = 0;                      // columns of stripe s-1 known published
-    if (!first) {
-        known = wait_count(prog_prev, base_prev + min(2 * kWave, qlen), known);
-        const int c0 = lane, c1 = kWave + lane;
-        bin_xx = c0 < qlen ? bnd[2 * c0] : 0;
-        bin_v = c0 < qlen ? bnd[2 * c0 + 1] : 0;
-        bnx_xx = c1 < qlen ? bnd[2 * c1] : 0;
-        bnx_v = c1 < qlen ? bnd[2 * c1 + 1] : 0;
-    }
-    int qb_next = lane == 0 ? (int)qs[0] : 0;
-    for (int c = 0; c < nsteps; ++c) {
-        const int j = c - lane;
-        // value for lane 0: the cell above the stripe in column c
-        int bx, bx2, bv;
-        if (first) {
-            bx = 0, bx2 = 0, bv = -8 * edge_diff(sc, c);
-        } else {
-            bx = (int)(int16_t)(bin_xx & 0xffff);
-            bx2 = bin_xx >> 16;
-            bv = bin_v;
-        }
-        int Xa = dpp_shr1(bx, Xo), X2a = dpp_shr1(bx2, X2o), Va = dpp_shr1(bv, Vo);
-        if (!first) {
-            // rotate the queue so lane 0 sees column c+1 next step
-            if ((c & (kWave - 1)) == kWave - 1) {
-                bin_xx = bnx_xx, bin_v = bnx_v;
-                known = wait_count(prog_prev, base_prev + min(c + 1 + 2 * kWave, qlen), known);
-                const int cn = c + 1 + kWave + lane;
-                bnx_xx = cn < qlen ? bnd[2 * cn] : 0;
-                bnx_v = cn < qlen ? bnd[2 * cn + 1] : 0;
-            } else {
-                bin_xx = dpp_shl1(0, bin_xx);
-                bin_v = dpp_shl1(0, bin_v);
-            }
-        }
-        const int qb = qb_next;
-        qb_next = (j + 1 >= 0 && j + 1 < qlen) ? (int)qs[j + 1] : 0;
-        if (j >= 0 && j < qlen) {
-            int d[R];
-            int vl = 0;  // V of the cell in row tlen-1 (last stripe)
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                d[r] = cell<RIGHT, SCORE_ONLY>(kc, TB[r], qb, Xa, X2a, Va, U[r], Yh[r], Y2h[r]);
-                vl = r == rho_last ? Va : vl;
-            }
-            if (last) {
-                row_acc += vl;
-                if (row_acc <= mte_best) mte_best = row_acc, mte_j = j;
-            }
-            Xo = Xa, X2o = X2a, Vo = Va;
-            if (!SCORE_ONLY)
-                store_dirs<R>(dir + (((uint64_t)s * nsteps + c) * kWave + lane) * R, d);
-        }
-        if (!last) {
-            // lane 63 appends its bottom-row output (column c-63) to the
-            // queue; every 64 steps the queue is written out and published
-            const int packed = (Xo & 0xffff) | (X2o << 16);
-            out_xx = dpp_shl1(packed, out_xx);
-            out_v = dpp_shl1(Vo, out_v);
-            if ((c & (kWave - 1)) == kWave - 1 || c == nsteps - 1) {
-                const int col = c - 2 * (kWave - 1) + lane;
-                if (col >= 0 && col < qlen) {
-                    bnd[2 * col] = out_xx;
-                    bnd[2 * col + 1] = out_v;
-                }
-                publish(prog_mine, base_mine + min(c - (kWave - 1) + 1, qlen));
-            }
-        }
-    }
-    // mqe partials: every lane holds U = -8u(i, qlen-1) of its rows (frozen
-    // after its last active step); H(i,qlen-1) = H(-1,qlen-1) + sum_{i'<=i} u.
-    int lsum = 0;
-#pragma unroll
-    for (int r = 0; r < R; ++r) lsum += U[r];
+    for (int r = 0; r < R; ++r) u[r] = hi ? hi16(bits(Us[r])) : lo16(bits(Us[r])), lsum += u[r];
     int incl = lsum;
 #pragma unroll
     for (int off = 1; off < kWave; off <<= 1) {
@@ -299,11 +200,10 @@ __device__ __forceinline__ void stripe_body(const StripeArgs &a, const Consts &k
     int best = 0x7fffffff, best_i = -1;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        run += U[r];
+        run += u[r];
         const int i = i0 + r;
         if (i < tlen && run <= best) best = run, best_i = i;
     }
-    // wave reduction: smallest value (largest H), ties to the larger row
 #pragma unroll
     for (int off = 1; off < kWave; off <<= 1) {
         const int ob = __shfl_xor(best, off, kWave);
@@ -314,11 +214,156 @@ __device__ __forceinline__ void stripe_body(const StripeArgs &a, const Consts &k
     if (lane == 0) {
         StripeRec rec;
         rec.total = total, rec.best = best, rec.best_i = best_i, rec.pad = 0;
-        a.recs[jb.stripe_off + s] = rec;
+        *out = rec;
     }
+}
+
+// One pass (stripes 2p and 2p+1) of one pair, computed by the calling wave.
+// Roles (uniform): the first pass takes row -1 as the low stripe's upper
+// boundary instead of the column buffer; the last pass holds row tlen-1
+// (score / mte) and publishes nothing; every other pass publishes the high
+// stripe's bottom row.  Lanes compute every step; each half's left state is
+// (re)initialised at its first column and U is snapshot at its last one, so
+// the cells a half computes outside its columns never reach a result.
+template <bool RIGHT, bool SCORE_ONLY>
+__device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc, int job_id, int p, const KswJob &jb,
+                                          int qlen, int tlen, int32_t *prog, int W)
+{
+    const KswScore &sc = a.sc;
+    const int nstripes = (tlen + SR - 1) / SR, npass = (nstripes + 1) / 2;
+    const bool first = p == 0, last = p == npass - 1, has_hi = 2 * p + 1 < nstripes;
+    const int lane = threadIdx.x % kWave;
+    const bool lane0 = lane == 0;
+    const uint8_t *qs = a.seq + jb.q_off;
+    const uint8_t *ts = a.seq + jb.t_off;
+    uint8_t *dir = a.dir + jb.dir_off;
+    int32_t *bnd = a.bnd + jb.bnd_off;
+    const int nsteps = qlen + kWave - 1 + kLag;
+    // row tlen-1 (score, mte): stripe s_last, half h_last of the last pass
+    const int h_last = ((tlen - 1) / SR) & 1;
+    const int rho_last = last ? (tlen - 1) % R : -1;
+    const int lane_last = ((tlen - 1) % SR) / R;
+    // progress slots: pass p-1 is wave (p-1) % W's ((p-1)/W)-th pass
+    const int32_t *prog_prev = prog + (p + W - 1) % W;
+    const int base_prev = first ? 0 : (p - 1) / W * (qlen + 1);
+    int32_t *prog_mine = prog + p % W;
+    const int base_mine = p / W * (qlen + 1);
+
+    const int ilo = 2 * p * SR + lane * R, ihi = ilo + SR;  // this lane's first rows
+    v2u TB[R];
+    v2s U[R], Y[R], Y2[R], Us[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int tl = ilo + r < tlen ? (int)ts[ilo + r] : 0, th = ihi + r < tlen ? (int)ts[ihi + r] : 0;
+        TB[r] = (v2u){(unsigned short)tl, (unsigned short)th};
+        U[r] = (v2s){(short)(-8 * edge_diff(sc, ilo + r)), (short)(-8 * edge_diff(sc, ihi + r))};
+        Y[r] = kc.LIM, Y2[r] = kc.LIM, Us[r] = U[r];
+    }
+    int row_acc = 0, mte_best = 0x7fffffff, mte_j = -1;  // row tlen-1 in the -8 domain
+    int Xo = bits(kc.LIM), X2o = bits(kc.LIM), Vo = 0, Qo = 0;  // this lane's bottom-row outputs
+    int bin0 = 0, bin1 = 0, bnx0 = 0, bnx1 = 0;  // lane-0 boundary queue (rotating), next block
+    int out0 = 0, out1 = 0;                      // lane-63 output queue (rotating)
+    int known = 0;                               // columns of pass p-1 known published
+    if (!first) {
+        known = wait_count(prog_prev, base_prev + min(2 * kWave, qlen), known);
+        const int c0 = lane, c1 = kWave + lane;
+        bin0 = c0 < qlen ? bnd[2 * c0] : 0, bin1 = c0 < qlen ? bnd[2 * c0 + 1] : 0;
+        bnx0 = c1 < qlen ? bnd[2 * c1] : 0, bnx1 = c1 < qlen ? bnd[2 * c1 + 1] : 0;
+    }
+    const int lim_hi = bits(kc.LIM) << 16;
+    uint32_t qword = 0;
+    for (int c = 0; c < nsteps; ++c) {
+        // lane 0's low-stripe inputs for column c, each in a high half
+        int bxh, bx2h, bvh;
+        if (first) {
+            bxh = lim_hi, bx2h = lim_hi, bvh = (-8 * edge_diff(sc, c)) << 16;
+        } else {
+            bxh = bin0, bx2h = bin0 << 16, bvh = bin1;
+        }
+        if ((c & 3) == 0) qword = c < qlen ? *(const uint32_t *)(qs + c) : 0u;
+        const int qh = (int)((qword >> (8 * (c & 3))) & 0xffu) << 16;
+        v2s X = vec(carry(Xo, bxh, lane0)), X2 = vec(carry(X2o, bx2h, lane0)), V = vec(carry(Vo, bvh, lane0));
+        const int Qw = carry(Qo, qh, lane0);
+        if (!first) {
+            // rotate the queue so lane 0 sees column c+1 next step
+            if ((c & (kWave - 1)) == kWave - 1) {
+                bin0 = bnx0, bin1 = bnx1;
+                known = wait_count(prog_prev, base_prev + min(c + 1 + 2 * kWave, qlen), known);
+                const int cn = c + 1 + kWave + lane;
+                bnx0 = cn < qlen ? bnd[2 * cn] : 0, bnx1 = cn < qlen ? bnd[2 * cn + 1] : 0;
+            } else {
+                bin0 = dpp_shl1(0, bin0), bin1 = dpp_shl1(0, bin1);
+            }
+        }
+        // a half starts at its first column: initial left state (only the
+        // first 128 steps can hold a start; the guard is wave-uniform)
+        const int jl = c - lane;
+        if (c < kWave + kLag) {
+            if (jl == 0 || jl == kLag) {
+                const bool hi = jl == kLag;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    U[r] = set_half(U[r], -8 * edge_diff(sc, (hi ? ihi : ilo) + r), hi);
+                    Y[r] = set_half(Y[r], -(RIGHT ? 8 : 0), hi), Y2[r] = set_half(Y2[r], -(RIGHT ? 8 : 0), hi);
+                }
+            }
+        }
+        const v2u qb = __builtin_bit_cast(v2u, Qw);
+        int d[R];
+        if (!last) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) d[r] = cell2<RIGHT, SCORE_ONLY>(kc, TB[r], qb, X, X2, V, U[r], Y[r], Y2[r]);
+        } else {
+            // the last pass also follows V along row tlen-1
+            v2s vl = V;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                d[r] = cell2<RIGHT, SCORE_ONLY>(kc, TB[r], qb, X, X2, V, U[r], Y[r], Y2[r]);
+                if (r == rho_last) vl = V;
+            }
+            const int jh = jl - kLag * h_last;
+            const int v = h_last ? hi16(bits(vl)) : lo16(bits(vl));
+            if (lane == lane_last && jh >= 0 && jh < qlen) {
+                row_acc += v;
+                if (row_acc <= mte_best) mte_best = row_acc, mte_j = jh;
+            }
+        }
+        // a half ends at column qlen-1: keep U for the mqe partials
+        if (c >= qlen - 1 && c < qlen - 1 + kWave + kLag) {
+            if (jl == qlen - 1 || jl == qlen - 1 + kLag) {
+                const bool hi = jl != qlen - 1;
+#pragma unroll
+                for (int r = 0; r < R; ++r) Us[r] = set_half(Us[r], hi ? hi16(bits(U[r])) : lo16(bits(U[r])), hi);
+            }
+        }
+        Xo = bits(X), X2o = bits(X2), Vo = bits(V), Qo = Qw;
+        if (!SCORE_ONLY) {
+            // 16 bytes per lane: byte 4g + 2h + (rho & 1) is row 2g + (rho & 1)
+            // of half h
+            uint4 w;
+            w.x = __builtin_amdgcn_perm((unsigned)d[1], (unsigned)d[0], 0x06020400u);
+            w.y = __builtin_amdgcn_perm((unsigned)d[3], (unsigned)d[2], 0x06020400u);
+            w.z = __builtin_amdgcn_perm((unsigned)d[5], (unsigned)d[4], 0x06020400u);
+            w.w = __builtin_amdgcn_perm((unsigned)d[7], (unsigned)d[6], 0x06020400u);
+            *(uint4 *)(dir + (((uint64_t)p * nsteps + c) * kWave + lane) * 16) = w;
+        }
+        if (!last) {
+            // lane 63 appends the high stripe's bottom-row output (column
+            // c-127: x in the high half, x2 in the low half; v in the high
+            // half) to the queue; every 64 steps the queue is written out
+            const int w0 = (int)__builtin_amdgcn_perm((unsigned)Xo, (unsigned)X2o, 0x07060302u);
+            out0 = dpp_shl1(w0, out0), out1 = dpp_shl1(Vo, out1);
+            if ((c & (kWave - 1)) == kWave - 1 || c == nsteps - 1) {
+                const int col = c - (kWave - 1) - kLag - (kWave - 1) + lane;
+                if (col >= 0 && col < qlen) bnd[2 * col] = out0, bnd[2 * col + 1] = out1;
+                publish(prog_mine, base_mine + max(0, min(c - (kWave - 1) - kLag + 1, qlen)));
+            }
+        }
+    }
+    stripe_rec(Us, false, ilo, tlen, lane, a.recs + jb.stripe_off + 2 * p);
+    if (has_hi) stripe_rec(Us, true, ihi, tlen, lane, a.recs + jb.stripe_off + 2 * p + 1);
     if (last) {
-        // publish score / mte from the lane that owns row tlen-1
-        const int lane_last = ((tlen - 1) % SR) / R;
+        // score / mte from the lane that owns row tlen-1
         const int sc_row = __builtin_amdgcn_readlane(row_acc, lane_last);
         const int mte_b = __builtin_amdgcn_readlane(mte_best, lane_last);
         const int mte_jj = __builtin_amdgcn_readlane(mte_j, lane_last);
@@ -340,8 +385,8 @@ __device__ __forceinline__ void stripe_body(const StripeArgs &a, const Consts &k
 }  // namespace
 
 // One workgroup (a team of blockDim.x / 64 waves) per pair of this launch's
-// job list; wave w computes stripes w, w+W, ...
-template <int R, bool RIGHT, bool SCORE_ONLY>
+// job list; wave w computes passes w, w+W, ...
+template <bool RIGHT, bool SCORE_ONLY>
 __global__ void __launch_bounds__(kWave * kMaxTeam) ksw_stripe_kernel(StripeArgs a)
 {
     __shared__ int32_t prog[kMaxTeam];
@@ -353,35 +398,38 @@ __global__ void __launch_bounds__(kWave * kMaxTeam) ksw_stripe_kernel(StripeArgs
     const Consts kc = make_consts<RIGHT>(a.sc);
     const KswJob jb = a.jobs[job];
     const int qlen = __builtin_amdgcn_readfirstlane(jb.qlen), tlen = __builtin_amdgcn_readfirstlane(jb.tlen);
-    constexpr int SR = kWave * R;
-    const int nstripes = (tlen + SR - 1) / SR;
-    for (int s = w; s < nstripes; s += W) stripe_body<R, RIGHT, SCORE_ONLY>(a, kc, job, s, jb, qlen, tlen, prog, W);
+    const int npass = ((tlen + SR - 1) / SR + 1) / 2;
+    for (int p = w; p < npass; p += W) pass_body<RIGHT, SCORE_ONLY>(a, kc, job, p, jb, qlen, tlen, prog, W);
 }
 
-// Device backtrack: one thread per pair.  First folds the per-stripe mqe
-// partials in row order (ties to the later row, the '>=' of
+// Device backtrack: one wave per pair.  The wave first folds the
+// per-stripe mqe partials in row order (ties to the later row, the '>=' of
 // minimap2/ksw2_extd2_sse.c:371), then walks the direction bytes from the
 // start cell chosen as in minimap2/ksw2_extd2_sse.c:398-418, running the
 // state machine of ksw_backtrack (minimap2/ksw2.h:160-192) for an unbanded
 // matrix (no cell of the path can leave the band there, so the off/off_end
-// forcing never triggers).  CIGAR ops are pushed run-length encoded and
-// reversed at the end unless KSW_EZ_REV_CIGAR.
+// forcing never triggers).  The walk reads a tile of 8 lanes x 64 columns
+// (up to 64 rows x 64 columns of one stripe) staged in LDS; when the path
+// leaves the tile the wave stages the next one, so HBM latency is paid once
+// per tile rather than once per step.  Every lane runs the same walk
+// (uniform control flow, LDS broadcast reads); lane 0 writes the run-length
+// encoded CIGAR, which the wave reverses at the end unless KSW_EZ_REV_CIGAR.
 __global__ void __launch_bounds__(64) ksw_backtrack_kernel(BacktrackArgs a)
 {
-    const int job_id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (job_id >= a.njobs) return;
+    __shared__ uint64_t tile[8][64];
+    const int job_id = blockIdx.x;
+    const int lane = threadIdx.x;
     const KswJob jb = a.jobs[job_id];
     KswOut o = a.out[job_id];
     const int flag = jb.flag;
     const int qlen = jb.qlen, tlen = jb.tlen;
     {
-        const int SR = kWave * jb.R;
         const int nstripes = (tlen + SR - 1) / SR;
-        int carry = 0, best = 0x7fffffff, best_i = -1;
+        int carry_ = 0, best = 0x7fffffff, best_i = -1;
         for (int s = 0; s < nstripes; ++s) {
             const StripeRec rec = a.recs[jb.stripe_off + s];
-            if (rec.best_i >= 0 && carry + rec.best <= best) best = carry + rec.best, best_i = rec.best_i;
-            carry += rec.total;
+            if (rec.best_i >= 0 && carry_ + rec.best <= best) best = carry_ + rec.best, best_i = rec.best_i;
+            carry_ += rec.total;
         }
         const int64_t h_anchor = (int64_t)a.sc.G1 - a.sc.qe_orig;
         const int h_top_last_col = (int)(h_anchor + edge_prefix(a.sc, qlen - 1));  // H(-1, qlen-1)
@@ -405,14 +453,46 @@ __global__ void __launch_bounds__(64) ksw_backtrack_kernel(BacktrackArgs a)
     if (i0 >= 0 || j0 >= 0) {
         const uint8_t *dir = a.dir + jb.dir_off;
         uint32_t *cig = a.cig + jb.cig_off;
+        const uint64_t nsteps = (uint64_t)qlen + kWave - 1 + kLag;
         int i = i0, j = j0, state = 0;
-        uint32_t last = 0xffffffffu;
-        auto push = [&](uint32_t op, int len) {
-            if (n > 0 && op == last) cig[n - 1] += (uint32_t)len << 4;
-            else cig[n++] = (uint32_t)len << 4 | op, last = op;
+        int t_i_lo = 1, t_i_hi = 0, t_j_lo = 1, t_j_hi = 0;  // staged tile (empty)
+        int run_op = -1, run_len = 0;
+        auto push = [&](int op, int len) {
+            if (op == run_op) {
+                run_len += len;
+            } else {
+                if (run_len > 0 && lane == 0) cig[n] = (uint32_t)run_len << 4 | (uint32_t)run_op;
+                if (run_len > 0) ++n;
+                run_op = op, run_len = len;
+            }
         };
         while (i >= 0 && j >= 0) {
-            const int tmp = dir[dir_index(i, j, qlen, jb.R)];
+            if (i < t_i_lo || i > t_i_hi || j < t_j_lo || j > t_j_hi) {
+                // stage lanes kb..k of stripe s (rows up to i) x columns j-63..j
+                const int s = i / SR, k = (i - s * SR) / R, kb = k >= 7 ? k - 7 : 0;
+                const int pp = s >> 1, h = s & 1;
+                t_i_lo = s * SR + kb * R, t_i_hi = s * SR + k * R + R - 1;
+                t_j_hi = j, t_j_lo = j >= 63 ? j - 63 : 0;
+                const int c_lo = t_j_lo + kb + kLag * h, c_hi = t_j_hi + k + kLag * h;
+                __syncthreads();  // the previous tile is no longer read
+                for (int c = c_lo + lane; c <= c_hi; c += kWave) {
+                    const uint4 *src = (const uint4 *)(dir + (((uint64_t)pp * nsteps + (uint64_t)c) * kWave + kb) * 16);
+                    for (int kr = 0; kr <= k - kb; ++kr) {
+                        const int jj = c - kLag * h - (kb + kr);
+                        if (jj >= t_j_lo && jj <= t_j_hi) {
+                            const uint4 w = src[kr];
+                            // the 8 row bytes of half h: bytes 2h, 2h+1 of each dword
+                            const unsigned sel = h ? 0x07060302u : 0x05040100u;
+                            const uint32_t lo = __builtin_amdgcn_perm(w.y, w.x, sel);
+                            const uint32_t hi = __builtin_amdgcn_perm(w.w, w.z, sel);
+                            tile[kr][jj - t_j_lo] = (uint64_t)hi << 32 | lo;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+            const int rel = i - t_i_lo;
+            const int tmp = (int)(tile[rel / R][j - t_j_lo] >> (8 * (rel % R))) & 0xff;
             if (state == 0) state = tmp & 7;
             else if (!(tmp >> (state + 2) & 1)) state = 0;
             if (state == 0) state = tmp & 7;
@@ -422,79 +502,59 @@ __global__ void __launch_bounds__(64) ksw_backtrack_kernel(BacktrackArgs a)
         }
         if (i >= 0) push(2, i + 1);
         if (j >= 0) push(1, j + 1);
+        push(-2, 0);  // flush the last run
+        __syncthreads();
         if (!(flag & 0x80))
-            for (int x = 0; x < (n >> 1); ++x) {
+            for (int x = lane; x < (n >> 1); x += kWave) {
                 const uint32_t t = cig[x];
                 cig[x] = cig[n - 1 - x];
                 cig[n - 1 - x] = t;
             }
     }
     o.n_cigar = n;
-    a.out[job_id] = o;
+    if (lane == 0) a.out[job_id] = o;
 }
 
-template <int R, bool RIGHT, bool SCORE_ONLY>
+template <bool RIGHT, bool SCORE_ONLY>
 static hipError_t launch_stripe_t(const StripeArgs &a, int team, hipStream_t st)
 {
-    hipLaunchKernelGGL((ksw_stripe_kernel<R, RIGHT, SCORE_ONLY>), dim3(a.njobs), dim3(kWave * team), 0, st, a);
+    hipLaunchKernelGGL((ksw_stripe_kernel<RIGHT, SCORE_ONLY>), dim3(a.njobs), dim3(kWave * team), 0, st, a);
     return hipGetLastError();
 }
 
-template <int R>
-static hipError_t launch_stripe_r(const StripeArgs &a, bool right, bool score_only, int team, hipStream_t st)
-{
-    if (score_only) return launch_stripe_t<R, false, true>(a, team, st);
-    if (right) return launch_stripe_t<R, true, false>(a, team, st);
-    return launch_stripe_t<R, false, false>(a, team, st);
-}
-
-hipError_t launch_stripe(const StripeArgs &a, int R, bool right, bool score_only, int team, hipStream_t st)
+hipError_t launch_stripe(const StripeArgs &a, bool right, bool score_only, int team, hipStream_t st)
 {
     if (a.njobs <= 0) return hipSuccess;
     if (team < 1 || team > kMaxTeam) return hipErrorInvalidValue;
-    switch (R) {
-    case 4: return launch_stripe_r<4>(a, right, score_only, team, st);
-    case 8: return launch_stripe_r<8>(a, right, score_only, team, st);
-    case 16: return launch_stripe_r<16>(a, right, score_only, team, st);
-    default: return hipErrorInvalidValue;
-    }
+    if (score_only) return launch_stripe_t<false, true>(a, team, st);
+    if (right) return launch_stripe_t<true, false>(a, team, st);
+    return launch_stripe_t<false, false>(a, team, st);
 }
 
-template <int R, bool RIGHT, bool SCORE_ONLY>
+template <bool RIGHT, bool SCORE_ONLY>
 static int resident_t()
 {
     int per_cu = 0, dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 1024;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ksw_stripe_kernel<R, RIGHT, SCORE_ONLY>, kWave, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ksw_stripe_kernel<RIGHT, SCORE_ONLY>, kWave, 0) !=
             hipSuccess ||
         per_cu <= 0)
         return prop.multiProcessorCount * 4;
     return per_cu * prop.multiProcessorCount;
 }
 
-template <int R>
-static int resident_r(bool right, bool score_only)
+int stripe_resident_waves(bool right, bool score_only)
 {
-    if (score_only) return resident_t<R, false, true>();
-    if (right) return resident_t<R, true, false>();
-    return resident_t<R, false, false>();
-}
-
-int stripe_resident_waves(int R, bool right, bool score_only)
-{
-    switch (R) {
-    case 4: return resident_r<4>(right, score_only);
-    case 8: return resident_r<8>(right, score_only);
-    case 16: return resident_r<16>(right, score_only);
-    default: return 1024;
-    }
+    if (score_only) return resident_t<false, true>();
+    if (right) return resident_t<true, false>();
+    return resident_t<false, false>();
 }
 
 hipError_t launch_backtrack(const BacktrackArgs &a, hipStream_t st)
 {
     if (a.njobs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(ksw_backtrack_kernel, dim3((a.njobs + 63) / 64), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(ksw_backtrack_kernel, dim3(a.njobs), dim3(kWave), 0, st, a);
     return hipGetLastError();
 }
 

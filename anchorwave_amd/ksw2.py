@@ -43,6 +43,10 @@ EXPORTS = (
     "ksw2_hip_route",
     "ksw2_hip_set_progress_callback",
     "ksw2_hip_clear_progress_callback",
+    "ksw2_hip_server_create",
+    "ksw2_hip_server_submit",
+    "ksw2_hip_server_poll",
+    "ksw2_hip_server_destroy",
     "ksw2_hip_plan_create",
     "ksw2_hip_plan_run",
     "ksw2_hip_plan_fetch",
@@ -156,6 +160,17 @@ def lib():
         L.ksw2_hip_set_progress_callback.restype = None
         L.ksw2_hip_clear_progress_callback.argtypes = []
         L.ksw2_hip_clear_progress_callback.restype = None
+        L.ksw2_hip_server_create.argtypes = [C.POINTER(C.c_void_p), C.c_int8, C.c_void_p, C.c_int8, C.c_int8,
+                                             C.c_int8, C.c_int8, C.c_size_t]
+        L.ksw2_hip_server_create.restype = C.c_int
+        L.ksw2_hip_server_submit.argtypes = [C.c_void_p, C.POINTER(KswPair), C.POINTER(KswExtz),
+                                             C.POINTER(C.c_uint64)]
+        L.ksw2_hip_server_submit.restype = C.c_int
+        L.ksw2_hip_server_poll.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.c_int,
+                                           C.c_int]
+        L.ksw2_hip_server_poll.restype = C.c_int
+        L.ksw2_hip_server_destroy.argtypes = [C.c_void_p]
+        L.ksw2_hip_server_destroy.restype = None
         _lib = L
     return _lib
 
@@ -304,30 +319,94 @@ class Plan:
 
     def dirs(self, index: int, qlen: int, tlen: int, raw: bool = False) -> np.ndarray:
         """Direction bytes of pair ``index`` as a (tlen, qlen) matrix (or the
-        raw stripe-layout buffer when ``raw``)."""
-        R = 8
-        SR = 64 * R
-        nbytes = ((tlen + SR - 1) // SR) * (qlen + 63) * 64 * 16 * 2
+        raw pass-layout buffer when ``raw``).  Layout: stripe s = i // 512 is
+        half h = s & 1 of pass p = s >> 1; lane k = (i % 512) // 8, row
+        rho = i % 8; column j of that half is written at step c = j + k + 64h;
+        byte ((p * (qlen + 127) + c) * 64 + k) * 16 + 4 * (rho // 2) + 2h +
+        rho % 2."""
+        npass = ((tlen + 511) // 512 + 1) // 2
+        nsteps = qlen + 127
+        nbytes = npass * nsteps * 64 * 16
         buf = np.zeros(nbytes, dtype=np.uint8)
         r = C.c_int()
         st = lib().ksw2_hip_plan_dirs(self.h, index, buf.ctypes.data, nbytes, C.byref(r))
         if st != 0:
             raise KswError(f"ksw2_hip_plan_dirs: {STATUS.get(st, st)}")
-        R = r.value
-        SR = 64 * R
         if raw:
-            return buf[: ((tlen + SR - 1) // SR) * (qlen + 63) * 64 * R]
+            return buf
         i = np.arange(tlen)[:, None]
         j = np.arange(qlen)[None, :]
-        s, rem = i // SR, i % SR
-        k, rho = rem // R, rem % R
-        idx = ((s * (qlen + 63) + j + k) * 64 + k) * R + rho
+        s, rem = i // 512, i % 512
+        p, h = s // 2, s % 2
+        k, rho = rem // 8, rem % 8
+        c = j + k + 64 * h
+        idx = ((p * nsteps + c) * 64 + k) * 16 + 4 * (rho // 2) + 2 * h + rho % 2
         return buf[idx]
 
     def close(self):
         if self.h:
             lib().ksw2_hip_plan_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The serving API (``ksw2_hip_server_*``): submit pairs as they arrive,
+    collect results as they complete.  ``submit`` returns a ticket;
+    ``poll`` returns ``[(ticket, Result), ...]`` for completed pairs."""
+
+    def __init__(self, mat=None, m=5, q=8, e=2, q2=75, e2=1, device_bytes=0):
+        self._L = lib()
+        self._mat = np.ascontiguousarray(np.asarray(default_matrix() if mat is None else mat, dtype=np.int8))
+        h = C.c_void_p()
+        st = self._L.ksw2_hip_server_create(C.byref(h), m, self._mat.ctypes.data, q, e, q2, e2, device_bytes)
+        if st != 0:
+            raise KswError(f"ksw2_hip_server_create: {STATUS.get(st, st)}")
+        self._h = h
+        self._ez = {}   # ticket -> KswExtz (kept alive until completion)
+
+    def submit(self, query, target, w=-1, zdrop=-1, end_bonus=0, flag=0) -> int:
+        qs, ts = _u8(query), _u8(target)
+        pr = KswPair(qs.ctypes.data, ts.ctypes.data, len(qs), len(ts), w, zdrop, end_bonus, flag)
+        ez = KswExtz()
+        tk = C.c_uint64()
+        st = self._L.ksw2_hip_server_submit(self._h, C.byref(pr), C.byref(ez), C.byref(tk))
+        if st != 0:
+            raise KswError(f"ksw2_hip_server_submit: {STATUS.get(st, st)}")
+        self._ez[tk.value] = ez
+        return tk.value
+
+    def poll(self, max_n: int = 1024, wait_ms: int = 0, results: bool = True):
+        """Completed pairs: [(ticket, Result or None, latency_s), ...]."""
+        buf = (C.c_uint64 * max_n)()
+        lat = (C.c_double * max_n)()
+        n = self._L.ksw2_hip_server_poll(self._h, buf, lat, max_n, wait_ms)
+        if n < 0:
+            raise KswError(f"ksw2_hip_server_poll: {STATUS.get(n, n)}")
+        out = []
+        for k in range(n):
+            ez = self._ez.pop(buf[k])
+            out.append((buf[k], Result.from_ez(ez) if results else None, lat[k]))
+            if ez.cigar:
+                _libc.free(C.cast(ez.cigar, C.c_void_p))
+        return out
+
+    def pending(self) -> int:
+        return len(self._ez)
+
+    def close(self):
+        if self._h:
+            self._L.ksw2_hip_server_destroy(self._h)
+            self._h = None
+            for ez in self._ez.values():
+                if ez.cigar:
+                    _libc.free(C.cast(ez.cigar, C.c_void_p))
+            self._ez.clear()
 
     def __del__(self):
         try:

@@ -118,6 +118,71 @@ def uneven_leg(args) -> dict:
     return res
 
 
+def serving_leg(args) -> dict:
+    """Continuous batching under Poisson arrivals (ksw2_hip_server_*): task
+    sizes drawn 50/40/10 from the single-pair, fixed-batch and uneven-batch
+    distributions.  Capacity is measured with every task submitted at once;
+    then each offered load (a fraction of capacity) replays the tasks with
+    exponential inter-arrival gaps.  Latency is submission to completion,
+    timed inside the library."""
+    log(f"[bench] serving: generating {args.serve} tasks")
+    tasks = W.serving(seed=5, n=args.serve)
+    cells = np.array([len(t.query) * len(t.target) for t in tasks], dtype=np.float64)
+    srv = ksw2.Server(mat=W_MAT, **SC)
+
+    def run(arrivals):
+        n = len(tasks)
+        lat = np.zeros(n)
+        t0 = time.perf_counter()
+        i = done = 0
+        while done < n:
+            now = time.perf_counter() - t0
+            while i < n and (arrivals is None or arrivals[i] <= now):
+                srv.submit(tasks[i].query, tasks[i].target)
+                i += 1
+            for tk, _, lt in srv.poll(max_n=4096, wait_ms=0 if i < n else 1, results=False):
+                lat[done] = lt
+                done += 1
+            if i < n and arrivals is not None:
+                gap = arrivals[i] - (time.perf_counter() - t0)
+                if gap > 2e-4:
+                    time.sleep(min(gap, 1e-3))
+        return time.perf_counter() - t0, lat
+
+    run(None)  # warm-up: device buffers, code objects
+    wall, lat = run(None)
+    cap = len(tasks) / wall
+    log(f"[bench] serving capacity: {cap:.0f} tasks/s, {cells.sum() / wall / 1e9:.1f} GCUPS")
+    levels = []
+    rng = np.random.default_rng(11)
+    for load in [float(x) for x in args.loads.split(",") if x]:
+        arr = np.cumsum(rng.exponential(1.0 / (load * cap), len(tasks)))
+        wall, lat = run(arr)
+        levels.append({"load": load, "offered_tasks_per_s": round(load * cap, 1),
+                       "achieved_tasks_per_s": round(len(tasks) / wall, 1),
+                       "gcups": round(cells.sum() / wall / 1e9, 2),
+                       "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                       "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3)})
+        log(f"[bench] serving load {load:.2f}: {levels[-1]}")
+    srv.close()
+    # the same tasks one call at a time (seeded sample per kind)
+    single = {}
+    for kind in ("single", "fixed", "uneven"):
+        idx = [k for k, t in enumerate(tasks) if t.kind == kind][:12]
+        ts = []
+        for k in idx:
+            t1 = time.perf_counter()
+            ksw2.extd2(tasks[k].query, tasks[k].target, mat=W_MAT, **SC)
+            ts.append(time.perf_counter() - t1)
+        single[kind] = {"p50_ms": round(float(np.median(ts)) * 1e3, 3),
+                        "gcups": round(float(cells[idx].sum() / sum(ts) / 1e9), 3), "sample": len(idx)}
+    log(f"[bench] serving one-at-a-time: {single}")
+    return {"tasks_per_level": len(tasks), "mix": "50% 5 kbp pair, 40% 2 kbp pair, 10% uneven-batch pair",
+            "capacity_tasks_per_s": round(cap, 1), "capacity_gcups": round(cells.sum() * cap / len(tasks) / 1e9, 2),
+            "levels": levels, "one_at_a_time": single,
+            "latency": "submission to completion inside the library (queueing + device + CIGAR delivery)"}
+
+
 def copy_calibration(nbytes: int) -> None:
     """One hipMemcpy device-to-device of nbytes: a plain copy kernel whose
     FETCH_SIZE / WRITE_SIZE readings scale the counters to bytes."""
@@ -149,6 +214,10 @@ def main() -> int:
     ap.add_argument("--single-sample", type=int, default=16, help="pairs timed one at a time")
     ap.add_argument("--uneven", type=int, default=20000,
                     help="pairs in the uneven-batch leg (0: skip it)")
+    ap.add_argument("--serve", type=int, default=20000,
+                    help="tasks per offered-load level in the serving leg (0: skip it)")
+    ap.add_argument("--loads", default="0.2,0.5,0.8,1.0,1.2",
+                    help="offered loads of the serving leg, as fractions of measured capacity")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (PMC byte-counter scale)")
     args = ap.parse_args()
@@ -181,7 +250,7 @@ def main() -> int:
     log(f"[bench] {ms_per_step:.2f} ms/step, {gcups:.1f} GCUPS (DP kernel {dp_avg:.2f} ms, "
         f"backtrack {np.mean(bt_ms):.2f} ms)")
     gpu_res = plan.fetch()
-    dir_bytes = sum(((len(t) + 511) // 512) * (len(q) + 63) * 512 for q, t in pairs)
+    dir_bytes = sum(((len(t) + 511) // 512 + 1) // 2 * (len(q) + 127) * 1024 for q, t in pairs)
     plan.close()
 
     # one pair per launch (inputs resident), the batching baseline
@@ -238,6 +307,8 @@ def main() -> int:
     }
     if args.uneven > 0:
         out["uneven"] = uneven_leg(args)
+    if args.serve > 0:
+        out["serving"] = serving_leg(args)
     if not args.no_cpu:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         log(f"[bench] CPU baseline on {threads} threads")

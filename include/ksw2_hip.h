@@ -148,10 +148,39 @@ int ksw2_hip_plan_fetch(ksw2_hip_plan *plan, ksw_extz_t *ez);
  * HIP events on the plan's stream; also the sum of qlen*tlen */
 int ksw2_hip_plan_timing(const ksw2_hip_plan *plan, float *dp_ms, float *backtrack_ms, double *cells);
 void ksw2_hip_plan_destroy(ksw2_hip_plan *plan);
-/* Diagnostics: copies pair `index`'s raw direction bytes (stripe layout:
- * cell (i, j) at ((s*(qlen+63) + j+k)*64 + k)*R + rho with s = i/(64R),
- * k = (i%(64R))/R, rho = i%R) into `out` and reports R; `bytes` must be at
- * least (ceil(tlen/(64R))*(qlen+63)*64*R).  Returns a status code. */
+/* ---- Serving (no reference counterpart: the request loop that keeps the
+ * device busy while interval tasks arrive, replacing AnchorWave's
+ * per-thread ksw_extd2 calls under AnchorTaskExecutor).  A server owns a
+ * device memory budget and a worker thread.  Submitted pairs are admitted
+ * by their device footprint (direction bytes + state + CIGAR slots) as
+ * memory frees up, grouped with other waiting pairs of similar size into
+ * launches that overlap on the device (continuous batching), and complete
+ * independently.  Each result is exactly what ksw_extd2_hip(NULL, ...)
+ * returns for the same pair, alone or in any mix. */
+typedef struct ksw2_hip_server ksw2_hip_server;
+/* device_bytes == 0: 85% of the device's free memory. */
+int ksw2_hip_server_create(ksw2_hip_server **server, int8_t m, const int8_t *mat, int8_t q, int8_t e,
+                           int8_t q2, int8_t e2, size_t device_bytes);
+/* Queues one pair (its sequences are copied).  *ez must stay valid until the
+ * ticket is returned by ksw2_hip_server_poll; ez->cigar is grown with realloc
+ * (release it with free()).  Does not wait for the device. */
+int ksw2_hip_server_submit(ksw2_hip_server *server, const ksw2_pair_t *pair, ksw_extz_t *ez, uint64_t *ticket);
+/* Writes up to max_tickets completed tickets, waiting up to wait_ms
+ * milliseconds for the first (-1: until one completes; 0: no wait).  When
+ * latency_s is not NULL it receives each ticket's time from submission to
+ * completion, in seconds.  Returns how many were written, or a negative
+ * status. */
+int ksw2_hip_server_poll(ksw2_hip_server *server, uint64_t *tickets, double *latency_s, int max_tickets,
+                         int wait_ms);
+/* Finishes every submitted pair, then releases the server. */
+void ksw2_hip_server_destroy(ksw2_hip_server *server);
+
+/* Diagnostics: copies pair `index`'s raw direction bytes (stripe-kernel
+ * pairs only; pass layout: stripe s = i/512 is half h = s&1 of pass p = s>>1,
+ * lane k = (i%512)/8, row rho = i%8, and cell (i, j) is byte
+ * ((p*(qlen+127) + j+k+64h)*64 + k)*16 + 4*(rho/2) + 2h + rho%2) into `out`
+ * and reports the rows per lane (8); `bytes` must be at least
+ * ceil(ceil(tlen/512)/2)*(qlen+127)*1024.  Returns a status code. */
 int ksw2_hip_plan_dirs(const ksw2_hip_plan *plan, int index, uint8_t *out, size_t bytes, int *R);
 
 #ifdef __cplusplus

@@ -1,0 +1,101 @@
+"""Continuous batching: the serving API and batches larger than the device
+memory budget.  Every result must be exactly what the same pair gives alone
+(ksw_extd2_hip) and what the oracle gives."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+from anchorwave_amd import ksw2, workloads as W
+from helpers import diff, random_pairs, same
+
+pytestmark = pytest.mark.gpu
+
+
+def _mix(seed, n):
+    """Pairs of mixed size, flags, bands and wildcard content."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k, (q, t) in enumerate(random_pairs(seed, n, 1, 3000)):
+        flag = [0, ksw2.KSW_EZ_RIGHT, ksw2.KSW_EZ_SCORE_ONLY, ksw2.KSW_EZ_SEMIGLOBAL_END, ksw2.KSW_EZ_EXTZ_ONLY][k % 5]
+        w = -1 if k % 3 else int(rng.integers(20, 400))
+        zdrop = -1 if k % 4 else int(rng.integers(30, 200))
+        if k % 7 == 0:
+            t = t.copy()
+            t[rng.integers(0, len(t), max(1, len(t) // 40))] = 4
+        out.append((q, t, flag, w, zdrop, int(rng.integers(0, 20))))
+    return out
+
+
+def _oracle(p):
+    q, t, flag, w, zdrop, eb = p
+    return O.run_oracle(q, t, flag=flag, w=w, zdrop=zdrop, end_bonus=eb)
+
+
+def test_server_results_alone_and_in_the_mix():
+    pairs = _mix(81, 60) + [(q, t, 0, -1, -1, 0) for q, t in W.fixed(seed=2, n=8)] + \
+        [(q, t, 0, -1, -1, 0) for q, t in W.single(seed=1)]
+    srv = ksw2.Server()
+    tickets = {}
+    for k, p in enumerate(pairs):
+        tickets[srv.submit(p[0], p[1], w=p[3], zdrop=p[4], end_bonus=p[5], flag=p[2])] = k
+    got = {}
+    while len(got) < len(pairs):
+        for tk, res, lat in srv.poll(wait_ms=2000):
+            got[tickets[tk]] = res
+            assert lat >= 0
+    srv.close()
+    for k, p in enumerate(pairs):
+        alone = ksw2.extd2(p[0], p[1], flag=p[2], w=p[3], zdrop=p[4], end_bonus=p[5])
+        exp = _oracle(p)
+        assert same(got[k], alone), f"pair {k}: mix vs alone: {diff(got[k], alone)}"
+        assert same(got[k], exp), f"pair {k}: vs oracle: {diff(got[k], exp)}"
+
+
+def test_server_submissions_while_running():
+    """Pairs submitted while earlier ones are on the device are admitted as
+    they arrive and complete independently."""
+    srv = ksw2.Server()
+    batches = [random_pairs(90 + b, 25, 200, 2500) for b in range(4)]
+    expect, got = {}, {}
+    for b, pairs in enumerate(batches):
+        for q, t in pairs:
+            expect[srv.submit(q, t)] = (q, t)
+        for tk, res, _ in srv.poll(wait_ms=0):
+            got[tk] = res
+    while len(got) < len(expect):
+        for tk, res, _ in srv.poll(wait_ms=2000):
+            got[tk] = res
+    srv.close()
+    for tk, (q, t) in expect.items():
+        assert same(got[tk], O.run_oracle(q, t))
+
+
+def test_server_budget_too_small_for_a_pair():
+    """A pair whose footprint exceeds the server's whole budget completes at
+    once with zdropped = 1 (the AVX kernels' allocation-failure convention);
+    the others are unaffected."""
+    srv = ksw2.Server(device_bytes=8 << 20)
+    rng = np.random.default_rng(3)
+    big_t = W.random_seq(rng, 6000)
+    small = random_pairs(93, 5, 100, 400)
+    tk_big = srv.submit(W.mutate(rng, big_t, 0.05, 0.01), big_t)
+    tks = {srv.submit(q, t): (q, t) for q, t in small}
+    got = {}
+    while len(got) < 6:
+        for tk, res, _ in srv.poll(wait_ms=2000):
+            got[tk] = res
+    srv.close()
+    assert got[tk_big].zdropped == 1 and got[tk_big].n_cigar == 0
+    for tk, (q, t) in tks.items():
+        assert same(got[tk], O.run_oracle(q, t))
+
+
+def test_batch_larger_than_budget_streams_through_the_engine(monkeypatch):
+    """ksw_extd2_hip_batch over a batch several times the memory budget:
+    admitted in flights by footprint, longest first, with backfill."""
+    monkeypatch.setenv("KSW2_HIP_MEM_BUDGET", str(48 << 20))
+    pairs = _mix(95, 80) + [(q, t, 0, -1, -1, 0) for q, t in W.fixed(seed=2, n=12)]
+    got = ksw2.extd2_batch(pairs)
+    for k, (p, g) in enumerate(zip(pairs, got)):
+        exp = _oracle(p)
+        assert same(g, exp), f"pair {k}: {diff(g, exp)}"

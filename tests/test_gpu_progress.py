@@ -37,10 +37,13 @@ def test_deadline_already_passed_stops_at_antidiagonal_zero():
 def test_stop_midway_banded_pair_backtracks_from_its_maximum():
     """A long banded pair (general kernel) stopped after its first poll: the
     reference's stopped result backtracks from (max_t, max_q), so the CIGAR
-    spans exactly that prefix and rescores to ez->max."""
+    spans exactly that prefix and rescores to ez->max.  A positive match
+    score makes H grow along the diagonal, so the maximum is a real cell
+    (with match 0, H never exceeds 0 and no maximum cell is recorded)."""
     rng = np.random.default_rng(73)
     t = W.random_seq(rng, 40000)
     q = W.mutate(rng, t, 0.05, 0.01)
+    mat = O.default_matrix(2, -4)
     state = {"n": 0}
 
     def cb(r, total):
@@ -48,12 +51,12 @@ def test_stop_midway_banded_pair_backtracks_from_its_maximum():
         return state["n"] <= 1   # allow anti-diagonal 0, stop at the next poll
 
     with ksw2.progress_callback(cb):
-        g = ksw2.extd2(q, t, w=1500, check=False)
+        g = ksw2.extd2(q, t, mat=mat, w=1500, check=False)
     assert g.stopped == 1 and g.zdropped == 0
     assert 0 <= g.max_t < len(t) and 0 <= g.max_q < len(q)
     ql, tl = O.cigar_lengths(g.cigar)
     assert (ql, tl) == (g.max_q + 1, g.max_t + 1)
-    assert O.cigar_score(g.cigar, q[: g.max_q + 1], t[: g.max_t + 1]) == g.max
+    assert O.cigar_score(g.cigar, q[: g.max_q + 1], t[: g.max_t + 1], mat=mat) == g.max
 
 
 def test_stop_midway_fast_pair_reports_stopped():
