@@ -310,15 +310,24 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
     if (p->n_gen > 0) {
         p->d_state = base + o_state, p->d_offs = (int32_t *)(base + o_offs);
         p->d_gjobs = (GenJob *)(base + o_gjobs), p->d_gorder = (int32_t *)(base + o_gorder);
-        if (hipHostMalloc((void **)&p->h_stop, sizeof(int32_t), hipHostMallocMapped) != hipSuccess)
-            return fail(KSW2_HIP_ENOMEM);
-        *p->h_stop = 0;
-        if (hipHostGetDevicePointer((void **)&p->d_stop, p->h_stop, 0) != hipSuccess) return fail(KSW2_HIP_EDEVICE);
+        if (env && env->h_stop) {
+            p->h_stop = env->h_stop, p->d_stop = env->d_stop, p->own_stop = false;
+        } else {
+            if (hipHostMalloc((void **)&p->h_stop, sizeof(int32_t), hipHostMallocMapped) != hipSuccess)
+                return fail(KSW2_HIP_ENOMEM);
+            *p->h_stop = 0;
+            if (hipHostGetDevicePointer((void **)&p->d_stop, p->h_stop, 0) != hipSuccess)
+                return fail(KSW2_HIP_EDEVICE);
+        }
     }
     // stage sequences through pinned memory (kept until the plan is
     // destroyed when the copy is left in flight on the caller's stream)
     uint8_t *h_seq = nullptr;
-    if (hipHostMalloc((void **)&h_seq, p->seq_bytes, hipHostMallocDefault) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
+    if (env && env->staging && env->staging_bytes >= p->seq_bytes) {
+        h_seq = env->staging, p->own_seq = false;
+    } else if (hipHostMalloc((void **)&h_seq, p->seq_bytes, hipHostMallocDefault) != hipSuccess) {
+        return fail(KSW2_HIP_ENOMEM);
+    }
     p->h_seq = h_seq;
     memset(h_seq, 0, p->seq_bytes);
     for (int k = 0; k < p->n_fast; ++k) {
@@ -346,7 +355,7 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
     }
     if (p->own_streams) {
         e[5] = hipStreamSynchronize(p->stream);
-        (void)hipHostFree(h_seq);
+        if (p->own_seq) (void)hipHostFree(h_seq);
         p->h_seq = nullptr;
     }
     for (hipError_t x : e)
@@ -378,7 +387,7 @@ int plan_launch(Plan *p, const Progress *prog)
         p->last_dp_ms = p->last_bt_ms = 0.f;
         return KSW2_HIP_OK;
     }
-    if (p->h_stop) *(volatile int32_t *)p->h_stop = 0;
+    if (p->h_stop && p->own_stop) *(volatile int32_t *)p->h_stop = 0;
     p->started = true;
     KSW_HIP_TRY(hipEventRecord(p->ev[0], p->stream));
     bool used[Plan::kLanes] = {false, false, false, false};
@@ -504,8 +513,8 @@ void plan_destroy(Plan *p)
     if (!p) return;
     if (p->pool) p->pool->free(p->pool_off);
     if (p->block) (void)hipFree(p->block);
-    if (p->h_stop) (void)hipHostFree(p->h_stop);
-    if (p->h_seq) (void)hipHostFree(p->h_seq);
+    if (p->h_stop && p->own_stop) (void)hipHostFree(p->h_stop);
+    if (p->h_seq && p->own_seq) (void)hipHostFree(p->h_seq);
     for (auto &e : p->ev)
         if (e) (void)hipEventDestroy(e);
     if (p->own_streams) {

@@ -14,6 +14,7 @@
 // served).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -136,6 +137,8 @@ Engine::Engine(int8_t m, const int8_t *mat, const KswScore &sc, const ScoreClass
     : m_(m), sc_(sc), cls_(cls), budget_(budget)
 {
     make_gen_score(m, mat, sc, cls, &gsc_);
+    const char *ev = getenv("KSW2_HIP_ENGINE_STATS");
+    stats_ = ev && atoi(ev) > 0;
 }
 
 int Engine::start()
@@ -150,6 +153,9 @@ int Engine::start()
     for (auto &s : streams_)
         if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return KSW2_HIP_EDEVICE;
     resident_waves_ = std::max(1, stripe_resident_waves(false, false));
+    if (hipHostMalloc((void **)&h_stop_, sizeof(int32_t), hipHostMallocMapped) != hipSuccess) return KSW2_HIP_ENOMEM;
+    *h_stop_ = 0;
+    if (hipHostGetDevicePointer((void **)&d_stop_, h_stop_, 0) != hipSuccess) return KSW2_HIP_EDEVICE;
     worker_ = std::thread([this] { loop(); });
     return KSW2_HIP_OK;
 }
@@ -168,7 +174,22 @@ Engine::~Engine()
     }
     for (auto &s : streams_)
         if (s) (void)hipStreamDestroy(s);
+    for (auto *b : staging_)
+        if (b) (void)hipHostFree(b);
+    if (h_stop_) (void)hipHostFree(h_stop_);
     for (auto &kv : tasks_) delete kv.second;
+    if (stats_)
+        fprintf(stderr,
+                "[ksw2 engine] flights %lld tasks %lld | host s: create %.3f launch %.3f fetch+deliver %.3f | "
+                "device s (sum over flights): %.3f\n",
+                (long long)st_flights_, (long long)st_tasks_, st_create_, st_launch_, st_fetch_, st_device_);
+}
+
+bool Engine::queues_empty() const
+{
+    for (const auto &q : queue_)
+        if (!q.empty()) return false;
+    return true;
 }
 
 int Engine::submit(const ksw2_pair_t &p, ksw_extz_t *ez, bool copy, uint64_t *ticket)
@@ -214,7 +235,7 @@ int Engine::submit(const ksw2_pair_t &p, ksw_extz_t *ez, bool copy, uint64_t *ti
         if (why != KSW2_HIP_OK && status_ == KSW2_HIP_OK) status_ = why;
         cv_done_.notify_all();
     } else {
-        queue_.push_back(t);
+        queue_[t->cls].push_back(t);
         ++pending_;
     }
     if (ticket) *ticket = t->ticket;
@@ -234,7 +255,8 @@ void Engine::hold(bool on)
 void Engine::order_longest_first()
 {
     std::lock_guard<std::mutex> lk(mu_);
-    std::stable_sort(queue_.begin(), queue_.end(), [](const Task *a, const Task *b) { return a->cells > b->cells; });
+    for (auto &q : queue_)
+        std::stable_sort(q.begin(), q.end(), [](const Task *a, const Task *b) { return a->cells > b->cells; });
 }
 
 int Engine::poll(uint64_t *tickets, double *latency, int max, int wait_ms)
@@ -293,17 +315,17 @@ int Engine::drain(const Progress *prog)
 // anti-diagonal 0; running plans get their stop flag raised.
 void Engine::stop_all_locked()
 {
-    for (Task *t : queue_) {
-        t->ez->stopped = 1;
-        t->t_done = now_s();
-        done_.push_back(t->ticket);
-        --pending_;
+    for (auto &q : queue_) {
+        for (Task *t : q) {
+            t->ez->stopped = 1;
+            t->t_done = now_s();
+            done_.push_back(t->ticket);
+            --pending_;
+        }
+        q.clear();
     }
-    queue_.clear();
-    for (auto &f : flights_) {
-        f->plan->stop_requested = true;
-        if (f->plan->h_stop) __atomic_store_n(f->plan->h_stop, 1, __ATOMIC_SEQ_CST);
-    }
+    for (auto &f : flights_) f->plan->stop_requested = true;
+    if (h_stop_) __atomic_store_n(h_stop_, 1, __ATOMIC_SEQ_CST);
     cv_done_.notify_all();
 }
 
@@ -312,7 +334,7 @@ void Engine::loop()
     (void)hipSetDevice(device_);
     std::unique_lock<std::mutex> lk(mu_);
     for (;;) {
-        if (quit_ && queue_.empty() && flights_.empty()) break;
+        if (quit_ && queues_empty() && flights_.empty()) break;
         const bool reaped = reap_locked(lk);
         const bool admitted = admit_locked(lk);
         if (reaped || admitted) continue;
@@ -341,13 +363,16 @@ bool Engine::reap_locked(std::unique_lock<std::mutex> &lk)
         flights_.erase(flights_.begin() + (long)i);
         busy_[f->stream] = false;
         lk.unlock();
+        const double t_r0 = now_s();
         int st = plan_wait(f->plan);
         std::vector<ksw_extz_t> ez(f->tasks.size());
         for (size_t k = 0; k < f->tasks.size(); ++k) ez[k] = *f->tasks[k]->ez;
         if (st == KSW2_HIP_OK) st = plan_fetch(f->plan, ez.data(), nullptr);
+        const float dev_ms = f->plan->last_dp_ms + f->plan->last_bt_ms;
         plan_destroy(f->plan);
         const double t = now_s();
         lk.lock();
+        if (stats_) st_fetch_ += t - t_r0, st_device_ += dev_ms * 1e-3, ++st_flights_, st_tasks_ += f->tasks.size();
         for (size_t k = 0; k < f->tasks.size(); ++k) {
             Task *tk = f->tasks[k];
             if (st == KSW2_HIP_OK) {
@@ -367,51 +392,56 @@ bool Engine::reap_locked(std::unique_lock<std::mutex> &lk)
     return any;
 }
 
-// Forms and launches flights while streams and memory allow.
+// Forms and launches flights while streams and memory allow: each free
+// stream takes the waiting tasks of its size class, in queue order, as far as
+// pool memory and one device-full of waves allow; a class whose head does
+// not fit admits later tasks that do (backfill).
 bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
 {
     bool any = false;
     if (hold_) return false;
-    while (!queue_.empty()) {
-        int s = -1;
-        for (int k = 0; k < kStreams; ++k)
-            if (!busy_[k]) {
-                s = k;
+    for (int s = 0; s < kStreams; ++s) {
+        if (busy_[s]) continue;
+        // a stream serves its own class first; the long-task streams also take
+        // shorter tasks when no long ones wait, the others never take long ones
+        static const int pref[kClasses][kClasses] = {{0, 1, -1}, {1, 0, -1}, {2, 1, 0}};
+        const int own = kStreamClass[s];
+        int cls = -1;
+        for (int k = 0; k < kClasses && pref[own][k] >= 0; ++k)
+            if (!queue_[pref[own][k]].empty()) {
+                cls = pref[own][k];
                 break;
             }
-        if (s < 0) break;
+        if (cls < 0) continue;
+        std::deque<Task *> &q = queue_[cls];
         auto *pool = static_cast<FirstFitPool *>(pool_.get());
-        const uint64_t avail = pool->largest_free();
+        const uint64_t share = (uint64_t)(kFlightShare[cls] * (double)budget_);
+        uint64_t avail = std::min(pool->largest_free(), std::max<uint64_t>(share, 0));
+        if (flights_.empty()) avail = pool->largest_free();  // alone on the device: no reason to hold back
         std::vector<Task *> pick;
-        uint64_t bytes = kPlanSlack;
+        uint64_t bytes = kPlanSlack, seq = 0;
         int waves = 0;
-        const int wave_cap = resident_waves_;
-        // from the head: tasks of the head's size class, in queue order
-        const int cls = queue_.front()->cls;
-        for (auto it = queue_.begin(); it != queue_.end();) {
+        int skipped = 0;
+        for (auto it = q.begin(); it != q.end() && (int)pick.size() < kMaxFlightPairs;) {
             Task *t = *it;
-            if (t->cls != cls) {
+            if (!pick.empty() && waves + t->waves > resident_waves_) break;
+            if (bytes + t->footprint > avail) {
+                if (++skipped > 64) break;  // backfill window
                 ++it;
                 continue;
             }
-            if (bytes + t->footprint > avail || (!pick.empty() && waves + t->waves > wave_cap)) break;
             pick.push_back(t);
-            bytes += t->footprint, waves += t->waves;
-            it = queue_.erase(it);
-            if ((int)pick.size() >= kMaxFlightPairs) break;
+            bytes += t->footprint, waves += t->waves, seq += (uint64_t)t->pair.qlen + t->pair.tlen + 32;
+            it = q.erase(it);
         }
-        if (pick.empty()) {
-            // backfill from the back (the smallest tasks when longest-first)
-            while (!queue_.empty()) {
-                Task *t = queue_.back();
-                if (bytes + t->footprint > avail || waves + t->waves > wave_cap) break;
-                pick.push_back(t);
-                bytes += t->footprint, waves += t->waves;
-                queue_.pop_back();
-                if ((int)pick.size() >= kMaxFlightPairs) break;
-            }
+        if (pick.empty()) continue;  // wait for memory
+        // pinned upload buffer of this stream, grown to the flight's sequences
+        if (staging_bytes_[s] < seq) {
+            if (staging_[s]) (void)hipHostFree(staging_[s]);
+            staging_[s] = nullptr, staging_bytes_[s] = 0;
+            const uint64_t want = std::max<uint64_t>(seq * 2, 1 << 20);
+            if (hipHostMalloc((void **)&staging_[s], want, hipHostMallocDefault) == hipSuccess) staging_bytes_[s] = want;
         }
-        if (pick.empty()) break;  // wait for memory
         auto f = std::make_unique<Flight>();
         f->tasks = pick;
         f->stream = s;
@@ -423,17 +453,22 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
         PlanEnv env;
         env.pool = pool_.get();
         env.stream = streams_[s];
+        env.staging = staging_[s], env.staging_bytes = staging_bytes_[s];
+        env.h_stop = h_stop_, env.d_stop = d_stop_;
+        const double t_c0 = now_s();
         int st = plan_create(&f->plan, f->pairs.data(), idx.data(), (int)idx.size(), m_, cls_, gsc_, &env);
+        const double t_c1 = now_s();
         if (st == KSW2_HIP_OK) st = plan_launch(f->plan);
         const double t0 = now_s();
         lk.lock();
+        if (stats_) st_create_ += t_c1 - t_c0, st_launch_ += t0 - t_c1;
         if (st != KSW2_HIP_OK) {
             busy_[s] = false;
             if (f->plan) plan_destroy(f->plan);
             if (st == KSW2_HIP_ENOMEM && !flights_.empty()) {
                 // the estimate was short of the pool's fragmentation: requeue
-                for (auto it = pick.rbegin(); it != pick.rend(); ++it) queue_.push_front(*it);
-                break;
+                for (auto it = pick.rbegin(); it != pick.rend(); ++it) q.push_front(*it);
+                continue;
             }
             for (Task *t : pick) {
                 t->ez->zdropped = 1;

@@ -20,6 +20,14 @@ namespace kswhip {
 class Engine {
 public:
     static constexpr int kStreams = 4;             // flights in flight at once
+    static constexpr int kClasses = 3;             // task size classes
+    // stream -> size class it serves: two for the long tasks, one each for
+    // the medium and short ones, so short tasks never queue behind long ones
+    static constexpr int kStreamClass[kStreams] = {2, 2, 1, 0};
+    // share of the pool one flight of a class may hold: long pairs hold
+    // memory longest per byte (their speed is capped by one workgroup), so
+    // they never take the whole pool and shorter pairs keep the device full
+    static constexpr double kFlightShare[kClasses] = {0.25, 0.35, 0.3};
     static constexpr int kMaxFlightPairs = 8192;
     static constexpr uint64_t kPlanSlack = 1 << 16;
 
@@ -46,6 +54,7 @@ private:
     void loop();
     bool reap_locked(std::unique_lock<std::mutex> &lk);
     bool admit_locked(std::unique_lock<std::mutex> &lk);
+    bool queues_empty() const;
     void stop_all_locked();
 
     int8_t m_;
@@ -58,10 +67,13 @@ private:
     std::unique_ptr<DevicePool> pool_;
     hipStream_t streams_[kStreams] = {};
     bool busy_[kStreams] = {};
+    uint8_t *staging_[kStreams] = {};              // pinned upload buffers, per stream
+    uint64_t staging_bytes_[kStreams] = {};
+    int32_t *h_stop_ = nullptr, *d_stop_ = nullptr;  // stop flag shared by all flights
     std::thread worker_;
     std::mutex mu_;
     std::condition_variable cv_work_, cv_done_;
-    std::deque<Task *> queue_;
+    std::deque<Task *> queue_[kClasses];
     std::vector<std::unique_ptr<Flight>> flights_;
     std::deque<uint64_t> done_;
     std::unordered_map<uint64_t, Task *> tasks_;
@@ -70,6 +82,10 @@ private:
     int status_ = 0;
     bool quit_ = false;
     bool hold_ = false;
+    // KSW2_HIP_ENGINE_STATS: per-phase host time, printed when the engine ends
+    bool stats_ = false;
+    double st_create_ = 0, st_launch_ = 0, st_fetch_ = 0, st_device_ = 0;
+    int64_t st_flights_ = 0, st_tasks_ = 0;
 };
 
 }  // namespace kswhip

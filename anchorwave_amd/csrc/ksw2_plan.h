@@ -65,6 +65,10 @@ struct DevicePool {
 struct PlanEnv {
     DevicePool *pool = nullptr;
     hipStream_t stream = nullptr;
+    uint8_t *staging = nullptr;      // pinned upload buffer (>= staging_bytes), reused
+    uint64_t staging_bytes = 0;
+    int32_t *h_stop = nullptr;       // shared stop flag (pinned, mapped) and its
+    int32_t *d_stop = nullptr;       // device address
 };
 
 // A device-resident batch: sequences uploaded, arenas sized, jobs built.
@@ -98,6 +102,7 @@ struct Plan {
     uint64_t pool_off = 0, block_bytes = 0;
     bool own_streams = false;
     uint8_t *h_seq = nullptr;        // pinned staging kept while an upload may be in flight
+    bool own_seq = true, own_stop = true;
     int progress_calls = 0;
     uint8_t *d_seq = nullptr, *d_dir = nullptr;
     int32_t *d_bnd = nullptr;

@@ -544,11 +544,15 @@ static int resident_t()
     return per_cu * prop.multiProcessorCount;
 }
 
+// Cached per variant: the occupancy query reads the device properties.
 int stripe_resident_waves(bool right, bool score_only)
 {
-    if (score_only) return resident_t<false, true>();
-    if (right) return resident_t<true, false>();
-    return resident_t<false, false>();
+    static int cache[3] = {0, 0, 0};
+    const int v = score_only ? 2 : right ? 1 : 0;
+    if (cache[v] > 0) return cache[v];
+    const int n = score_only ? resident_t<false, true>() : right ? resident_t<true, false>() : resident_t<false, false>();
+    __atomic_store_n(&cache[v], n, __ATOMIC_RELAXED);
+    return n;
 }
 
 hipError_t launch_backtrack(const BacktrackArgs &a, hipStream_t st)

@@ -106,7 +106,7 @@ class Result:
 
     @staticmethod
     def from_ez(ez: KswExtz) -> "Result":
-        cig = tuple(ez.cigar[i] for i in range(ez.n_cigar)) if ez.n_cigar > 0 else ()
+        cig = tuple(np.ctypeslib.as_array(ez.cigar, (ez.n_cigar,)).tolist()) if ez.n_cigar > 0 else ()
         return Result(ez.max, ez.zdropped, ez.max_q, ez.max_t, ez.mqe, ez.mqe_t, ez.mte,
                       ez.mte_q, ez.score, ez.n_cigar, ez.reach_end, ez.stopped, cig)
 
@@ -242,16 +242,22 @@ def route(query, target, mat=None, m=5, q=8, e=2, q2=75, e2=1, w=-1, zdrop=-1, f
 
 
 def extd2_batch(pairs: Sequence[tuple], mat=None, m=5, q=8, e=2, q2=75, e2=1, w=-1, zdrop=-1,
-                end_bonus=0, flag=0, check=True) -> List[Result]:
+                end_bonus=0, flag=0, check=True, timing: dict = None) -> List[Result]:
     """``ksw_extd2_hip_batch`` over ``pairs``: tuples (query, target) or
     (query, target, flag[, w[, zdrop[, end_bonus]]]); the keyword values are
-    the defaults for fields a tuple leaves out.  Scoring is shared."""
+    the defaults for fields a tuple leaves out.  Scoring is shared.  When
+    ``timing`` is a dict, its "seconds" receives the wall time of the C call
+    alone (uploads, device work, CIGAR delivery)."""
+    import time
     L = lib()
     n = len(pairs)
     arr, keep = _pair_array(pairs, w=w, zdrop=zdrop, end_bonus=end_bonus, flag=flag)
     mat = np.ascontiguousarray(np.asarray(default_matrix() if mat is None else mat, dtype=np.int8))
     ezs = (KswExtz * max(n, 1))()
+    t0 = time.perf_counter()
     st = L.ksw_extd2_hip_batch(n, arr, m, mat.ctypes.data, q, e, q2, e2, ezs)
+    if timing is not None:
+        timing["seconds"] = time.perf_counter() - t0
     out = []
     for k in range(n):
         out.append(Result.from_ez(ezs[k]))
@@ -383,8 +389,11 @@ class Server:
 
     def poll(self, max_n: int = 1024, wait_ms: int = 0, results: bool = True):
         """Completed pairs: [(ticket, Result or None, latency_s), ...]."""
-        buf = (C.c_uint64 * max_n)()
-        lat = (C.c_double * max_n)()
+        if getattr(self, "_pbuf_n", 0) < max_n:
+            self._pbuf = (C.c_uint64 * max_n)()
+            self._plat = (C.c_double * max_n)()
+            self._pbuf_n = max_n
+        buf, lat = self._pbuf, self._plat
         n = self._L.ksw2_hip_server_poll(self._h, buf, lat, max_n, wait_ms)
         if n < 0:
             raise KswError(f"ksw2_hip_server_poll: {STATUS.get(n, n)}")

@@ -88,9 +88,9 @@ def uneven_leg(args) -> dict:
     pu = W.uneven(seed=3, n=args.uneven)
     cu = W.cells(pu)
     ksw2.extd2(pu[0][0][:64], pu[0][1][:64], mat=W_MAT, **SC)  # device initialised
-    t0 = time.perf_counter()
-    gres = ksw2.extd2_batch(pu, mat=W_MAT, **SC)
-    dt = time.perf_counter() - t0
+    tm = {}
+    gres = ksw2.extd2_batch(pu, mat=W_MAT, **SC, timing=tm)
+    dt = tm["seconds"]
     log(f"[bench] uneven batch: {cu:.3g} cells in {dt:.2f} s = {cu / dt / 1e9:.1f} GCUPS (host API)")
     rng = np.random.default_rng(7)
     idx = rng.choice(len(pu), size=min(args.single_sample or 24, len(pu)), replace=False)
@@ -130,19 +130,23 @@ def serving_leg(args) -> dict:
     cells = np.array([len(t.query) * len(t.target) for t in tasks], dtype=np.float64)
     srv = ksw2.Server(mat=W_MAT, **SC)
 
+    kinds = np.array([t.kind for t in tasks])
+
     def run(arrivals):
         n = len(tasks)
         lat = np.zeros(n)
+        kind_of = {}
         t0 = time.perf_counter()
         i = done = 0
         while done < n:
             now = time.perf_counter() - t0
             while i < n and (arrivals is None or arrivals[i] <= now):
-                srv.submit(tasks[i].query, tasks[i].target)
+                kind_of[srv.submit(tasks[i].query, tasks[i].target)] = i
                 i += 1
-            for tk, _, lt in srv.poll(max_n=4096, wait_ms=0 if i < n else 1, results=False):
-                lat[done] = lt
-                done += 1
+            if i >= n or arrivals is None or (i & 15) == 0:
+                for tk, _, lt in srv.poll(max_n=4096, wait_ms=0 if i < n else 1, results=False):
+                    lat[kind_of.pop(tk)] = lt
+                    done += 1
             if i < n and arrivals is not None:
                 gap = arrivals[i] - (time.perf_counter() - t0)
                 if gap > 2e-4:
@@ -160,9 +164,12 @@ def serving_leg(args) -> dict:
         wall, lat = run(arr)
         levels.append({"load": load, "offered_tasks_per_s": round(load * cap, 1),
                        "achieved_tasks_per_s": round(len(tasks) / wall, 1),
-                       "gcups": round(cells.sum() / wall / 1e9, 2),
+                       "gcups": round(float(cells.sum() / wall / 1e9), 2),
                        "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
-                       "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3)})
+                       "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+                       "by_kind_p50_p99_ms": {k: [round(float(np.percentile(lat[kinds == k], 50)) * 1e3, 2),
+                                                  round(float(np.percentile(lat[kinds == k], 99)) * 1e3, 2)]
+                                              for k in ("single", "fixed", "uneven")}})
         log(f"[bench] serving load {load:.2f}: {levels[-1]}")
     srv.close()
     # the same tasks one call at a time (seeded sample per kind)
