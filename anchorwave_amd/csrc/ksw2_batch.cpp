@@ -320,6 +320,12 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
                 return fail(KSW2_HIP_EDEVICE);
         }
     }
+    // host mirror of the stripe-kernel results (engine flights)
+    if (env && env->mirror_flag && p->n_fast > 0) {
+        p->h_mout = env->mirror_out, p->h_mcig = env->mirror_cig, p->h_mflag = env->mirror_flag;
+        p->d_mout = env->mirror_out_dev, p->d_mcig = env->mirror_cig_dev, p->d_mflag = env->mirror_flag_dev;
+        memset(p->h_mflag, 0, sizeof(int32_t) * p->n_fast);
+    }
     // stage sequences through pinned memory (kept until the plan is
     // destroyed when the copy is left in flight on the caller's stream)
     uint8_t *h_seq = nullptr;
@@ -410,8 +416,9 @@ int plan_launch(Plan *p, const Progress *prog)
         KSW_HIP_TRY(launch_general(ga, l.threads, st));
     }
     StripeArgs sa;
-    sa.seq = p->d_seq, sa.dir = p->d_dir, sa.bnd = p->d_bnd, sa.sc = p->sc;
+    sa.seq = p->d_seq, sa.dir = p->d_dir, sa.bnd = p->d_bnd, sa.cig = p->d_cig, sa.sc = p->sc;
     sa.jobs = p->d_jobs, sa.out = p->d_out, sa.recs = p->d_recs;
+    sa.host_out = p->d_mout, sa.host_cig = p->d_mcig, sa.host_flag = p->d_mflag;
     for (const Plan::Launch &l : p->launches) {
         hipStream_t st = lane(l.lane);
         if (!st) return KSW2_HIP_EDEVICE;
@@ -424,12 +431,6 @@ int plan_launch(Plan *p, const Progress *prog)
             KSW_HIP_TRY(hipStreamWaitEvent(p->stream, p->lane_done[k], 0));
         }
     KSW_HIP_TRY(hipEventRecord(p->ev[1], p->stream));
-    if (p->n_fast > 0) {
-        BacktrackArgs ba;
-        ba.dir = p->d_dir, ba.cig = p->d_cig, ba.jobs = p->d_jobs, ba.out = p->d_out, ba.recs = p->d_recs;
-        ba.njobs = p->n_fast, ba.sc = p->sc;
-        KSW_HIP_TRY(launch_backtrack(ba, p->stream));
-    }
     if (p->n_gen > 0) {
         GenBacktrackArgs gb;
         gb.dir = p->d_dir, gb.offs = p->d_offs, gb.cig = p->d_cig, gb.jobs = p->d_gjobs, gb.out = p->d_out;
@@ -475,11 +476,30 @@ int plan_run(Plan *p, const Progress *prog)
     return plan_wait(p, prog);
 }
 
-int plan_fetch(Plan *p, ksw_extz_t *ez, void *km)
+// One slot's result into the caller's ez (CIGAR grown with krealloc/realloc).
+int deliver_slot(const Plan *p, int k, const KswOut &o, const uint32_t *cig_arena, ksw_extz_t *z, void *km)
+{
+    z->max = (uint32_t)o.max, z->zdropped = o.zdropped;
+    z->max_q = o.max_q, z->max_t = o.max_t;
+    z->mqe = o.mqe, z->mqe_t = o.mqe_t, z->mte = o.mte, z->mte_q = o.mte_q;
+    z->score = o.score, z->reach_end = o.reach_end, z->stopped = o.stopped;
+    if (p->stop_requested && k < p->n_fast) {
+        // the stripe kernel has no stop point: a pair whose deadline passed
+        // while it ran is reported stopped with no path (callers discard
+        // stopped results, alignSlidingWindow.cpp:957-963)
+        z->stopped = 1, z->score = KSW_NEG_INF;
+        z->n_cigar = 0;
+        return KSW2_HIP_OK;
+    }
+    return deliver_cigar(z, cig_arena + p->cig_off[k], o.n_cigar, km);
+}
+
+int plan_fetch(Plan *p, ksw_extz_t *ez, void *km, const std::vector<char> *skip)
 {
     if (p->n == 0) return KSW2_HIP_OK;
     if (p->stop_requested && !p->started) {
-        for (int k = 0; k < p->n; ++k) ez[p->order[k]].stopped = 1;  // stopped at r = 0
+        for (int k = 0; k < p->n; ++k)
+            if (!skip || !(*skip)[k]) ez[p->order[k]].stopped = 1;  // stopped at r = 0
         return KSW2_HIP_OK;
     }
     std::vector<KswOut> outs(p->n);
@@ -488,21 +508,8 @@ int plan_fetch(Plan *p, ksw_extz_t *ez, void *km)
     KSW_HIP_TRY(hipMemcpyAsync(cig.data(), p->d_cig, p->cig_words * 4, hipMemcpyDeviceToHost, p->stream));
     KSW_HIP_TRY(hipStreamSynchronize(p->stream));
     for (int k = 0; k < p->n; ++k) {
-        ksw_extz_t *z = ez + p->order[k];
-        const KswOut &o = outs[k];
-        z->max = (uint32_t)o.max, z->zdropped = o.zdropped;
-        z->max_q = o.max_q, z->max_t = o.max_t;
-        z->mqe = o.mqe, z->mqe_t = o.mqe_t, z->mte = o.mte, z->mte_q = o.mte_q;
-        z->score = o.score, z->reach_end = o.reach_end, z->stopped = o.stopped;
-        if (p->stop_requested && k < p->n_fast) {
-            // the stripe kernel has no stop point: a pair whose deadline passed
-            // while it ran is reported stopped with no path (callers discard
-            // stopped results, alignSlidingWindow.cpp:957-963)
-            z->stopped = 1, z->score = KSW_NEG_INF;
-            z->n_cigar = 0;
-            continue;
-        }
-        const int st = deliver_cigar(z, cig.data() + p->cig_off[k], o.n_cigar, km);
+        if (skip && (*skip)[k]) continue;
+        const int st = deliver_slot(p, k, outs[k], cig.data(), ez + p->order[k], km);
         if (st != KSW2_HIP_OK) return st;
     }
     return KSW2_HIP_OK;

@@ -130,15 +130,20 @@ struct Engine::Flight {
     Plan *plan = nullptr;
     std::vector<Task *> tasks;
     std::vector<ksw2_pair_t> pairs;
+    // per plan slot; a delivered task may be freed by poll at once, so its
+    // entry in tasks is cleared and never read again
+    std::vector<char> delivered;
+    int undelivered_fast = 0;
     int stream = 0;
+    double cells = 0, t_start = 0;  // for the statistics
 };
 
 Engine::Engine(int8_t m, const int8_t *mat, const KswScore &sc, const ScoreClass &cls, uint64_t budget)
-    : m_(m), sc_(sc), cls_(cls), budget_(budget)
+    : m_(m), cls_(cls), budget_(budget)
 {
     make_gen_score(m, mat, sc, cls, &gsc_);
     const char *ev = getenv("KSW2_HIP_ENGINE_STATS");
-    stats_ = ev && atoi(ev) > 0;
+    stats_ = ev ? atoi(ev) : 0;
 }
 
 int Engine::start()
@@ -176,6 +181,7 @@ Engine::~Engine()
         if (s) (void)hipStreamDestroy(s);
     for (auto *b : staging_)
         if (b) (void)hipHostFree(b);
+    for (auto &m : mirror_) m.release();
     if (h_stop_) (void)hipHostFree(h_stop_);
     for (auto &kv : tasks_) delete kv.second;
     if (stats_)
@@ -355,7 +361,27 @@ bool Engine::reap_locked(std::unique_lock<std::mutex> &lk)
     bool any = false;
     for (size_t i = 0; i < flights_.size();) {
         Flight *f = flights_[i].get();
-        if (!plan_done(f->plan)) {
+        Plan *p = f->plan;
+        // pairs the stripe kernel has finished (completion flag in the host
+        // mirror) are delivered now, while the rest of the flight runs
+        if (p->h_mflag && f->undelivered_fast > 0) {
+            const double t = now_s();
+            bool got = false;
+            for (int k = 0; k < p->n_fast; ++k) {
+                if (f->delivered[k] || !__atomic_load_n(p->h_mflag + k, __ATOMIC_ACQUIRE)) continue;
+                Task *tk = f->tasks[p->order[k]];
+                const int st = deliver_slot(p, k, p->h_mout[k], p->h_mcig, tk->ez, nullptr);
+                if (st != KSW2_HIP_OK) tk->ez->zdropped = 1, status_ = st;
+                f->delivered[k] = 1, --f->undelivered_fast;
+                f->tasks[p->order[k]] = nullptr;
+                tk->t_done = t;
+                done_.push_back(tk->ticket);
+                --pending_;
+                got = true;
+            }
+            if (got) cv_done_.notify_all(), any = true;
+        }
+        if (!plan_done(p)) {
             ++i;
             continue;
         }
@@ -364,28 +390,42 @@ bool Engine::reap_locked(std::unique_lock<std::mutex> &lk)
         busy_[f->stream] = false;
         lk.unlock();
         const double t_r0 = now_s();
-        int st = plan_wait(f->plan);
+        int st = plan_wait(p);
         std::vector<ksw_extz_t> ez(f->tasks.size());
-        for (size_t k = 0; k < f->tasks.size(); ++k) ez[k] = *f->tasks[k]->ez;
-        if (st == KSW2_HIP_OK) st = plan_fetch(f->plan, ez.data(), nullptr);
-        const float dev_ms = f->plan->last_dp_ms + f->plan->last_bt_ms;
-        plan_destroy(f->plan);
+        for (size_t k = 0; k < f->tasks.size(); ++k)
+            if (f->tasks[k]) ez[k] = *f->tasks[k]->ez;
+        // the rest (general-kernel pairs, or every pair without a mirror)
+        if (st == KSW2_HIP_OK) st = plan_fetch(p, ez.data(), nullptr, &f->delivered);
+        const float dev_ms = p->last_dp_ms + p->last_bt_ms;
         const double t = now_s();
         lk.lock();
-        if (stats_) st_fetch_ += t - t_r0, st_device_ += dev_ms * 1e-3, ++st_flights_, st_tasks_ += f->tasks.size();
-        for (size_t k = 0; k < f->tasks.size(); ++k) {
-            Task *tk = f->tasks[k];
+        if (stats_) {
+            st_fetch_ += t - t_r0, st_device_ += dev_ms * 1e-3, ++st_flights_, st_tasks_ += f->tasks.size();
+            if (stats_ > 1) {
+                int maxteam = 0;
+                for (const auto &l : p->launches) maxteam = std::max(maxteam, l.team);
+                fprintf(stderr, "[ksw2 engine] flight stream %d tasks %zu cells %.3g dp %.1f ms bt %.1f ms "
+                        "fast %d gen %d maxteam %d wall %.1f ms\n", f->stream, f->tasks.size(), f->cells,
+                        p->last_dp_ms, p->last_bt_ms, p->n_fast, p->n_gen, maxteam,
+                        (t - f->t_start) * 1e3);
+            }
+        }
+        for (int k = 0; k < p->n; ++k) {
+            if (f->delivered[k]) continue;
+            Task *tk = f->tasks[p->order[k]];
+            ksw_extz_t &z = ez[p->order[k]];
             if (st == KSW2_HIP_OK) {
-                *tk->ez = ez[k];
+                *tk->ez = z;
             } else {
                 // keep the (possibly reallocated) CIGAR buffer with the caller
-                tk->ez->cigar = ez[k].cigar, tk->ez->m_cigar = ez[k].m_cigar;
+                tk->ez->cigar = z.cigar, tk->ez->m_cigar = z.m_cigar;
                 tk->ez->zdropped = 1, status_ = st;
             }
             tk->t_done = t;
             done_.push_back(tk->ticket);
             --pending_;
         }
+        plan_destroy(p);
         cv_done_.notify_all();
         any = true;
     }
@@ -416,22 +456,25 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
         std::deque<Task *> &q = queue_[cls];
         auto *pool = static_cast<FirstFitPool *>(pool_.get());
         const uint64_t share = (uint64_t)(kFlightShare[cls] * (double)budget_);
-        uint64_t avail = std::min(pool->largest_free(), std::max<uint64_t>(share, 0));
-        if (flights_.empty()) avail = pool->largest_free();  // alone on the device: no reason to hold back
+        const uint64_t free_block = pool->largest_free();
+        const uint64_t avail = std::min(free_block, share);
         std::vector<Task *> pick;
-        uint64_t bytes = kPlanSlack, seq = 0;
+        uint64_t bytes = kPlanSlack, seq = 0, cigw = 0;
         int waves = 0;
         int skipped = 0;
         for (auto it = q.begin(); it != q.end() && (int)pick.size() < kMaxFlightPairs;) {
             Task *t = *it;
             if (!pick.empty() && waves + t->waves > resident_waves_) break;
-            if (bytes + t->footprint > avail) {
+            // a flight holds at most its class's share of the pool, except that
+            // its first task may use any free block (a task larger than the share)
+            if (bytes + t->footprint > (pick.empty() ? free_block : avail)) {
                 if (++skipped > 64) break;  // backfill window
                 ++it;
                 continue;
             }
             pick.push_back(t);
             bytes += t->footprint, waves += t->waves, seq += (uint64_t)t->pair.qlen + t->pair.tlen + 32;
+            cigw += ((uint64_t)t->pair.qlen + t->pair.tlen + 2 + 15) / 16 * 16;
             it = q.erase(it);
         }
         if (pick.empty()) continue;  // wait for memory
@@ -442,9 +485,28 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
             const uint64_t want = std::max<uint64_t>(seq * 2, 1 << 20);
             if (hipHostMalloc((void **)&staging_[s], want, hipHostMallocDefault) == hipSuccess) staging_bytes_[s] = want;
         }
+        // host mirror of this stream: result slots, CIGAR words, flags
+        Mirror &mr = mirror_[s];
+        if (mr.slots < pick.size() || mr.words < cigw) {
+            mr.release();
+            const uint64_t slots = std::max<uint64_t>(pick.size() * 2, 4096);
+            const uint64_t words = std::max<uint64_t>(cigw * 2, 1 << 20);
+            const unsigned fl = hipHostMallocMapped | hipHostMallocCoherent;
+            if (hipHostMalloc((void **)&mr.out, slots * sizeof(KswOut), fl) == hipSuccess &&
+                hipHostMalloc((void **)&mr.cig, words * 4, fl) == hipSuccess &&
+                hipHostMalloc((void **)&mr.flag, slots * 4, fl) == hipSuccess &&
+                hipHostGetDevicePointer((void **)&mr.out_dev, mr.out, 0) == hipSuccess &&
+                hipHostGetDevicePointer((void **)&mr.cig_dev, mr.cig, 0) == hipSuccess &&
+                hipHostGetDevicePointer((void **)&mr.flag_dev, mr.flag, 0) == hipSuccess) {
+                mr.slots = slots, mr.words = words;
+            } else {
+                mr.release();  // no mirror: the flight completes as a whole
+            }
+        }
         auto f = std::make_unique<Flight>();
         f->tasks = pick;
         f->stream = s;
+        for (const Task *tk : pick) f->cells += tk->cells;
         for (Task *t : pick) f->pairs.push_back(t->pair);
         busy_[s] = true;
         lk.unlock();
@@ -455,8 +517,17 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
         env.stream = streams_[s];
         env.staging = staging_[s], env.staging_bytes = staging_bytes_[s];
         env.h_stop = h_stop_, env.d_stop = d_stop_;
+        if (mr.slots) {
+            env.mirror_out = mr.out, env.mirror_out_dev = mr.out_dev;
+            env.mirror_cig = mr.cig, env.mirror_cig_dev = mr.cig_dev;
+            env.mirror_flag = mr.flag, env.mirror_flag_dev = mr.flag_dev;
+        }
         const double t_c0 = now_s();
         int st = plan_create(&f->plan, f->pairs.data(), idx.data(), (int)idx.size(), m_, cls_, gsc_, &env);
+        if (st == KSW2_HIP_OK) {
+            f->delivered.assign(f->plan->n, 0);
+            f->undelivered_fast = f->plan->h_mflag ? f->plan->n_fast : 0;
+        }
         const double t_c1 = now_s();
         if (st == KSW2_HIP_OK) st = plan_launch(f->plan);
         const double t0 = now_s();
@@ -481,6 +552,7 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
             continue;
         }
         for (Task *t : pick) t->t_start = t0;
+        f->t_start = t0;
         flights_.push_back(std::move(f));
         any = true;
     }

@@ -58,7 +58,6 @@ private:
     void stop_all_locked();
 
     int8_t m_;
-    KswScore sc_;
     ScoreClass cls_;
     GenScore gsc_{};
     uint64_t budget_;
@@ -67,6 +66,21 @@ private:
     std::unique_ptr<DevicePool> pool_;
     hipStream_t streams_[kStreams] = {};
     bool busy_[kStreams] = {};
+    // per-stream host mirror (pinned, mapped, coherent) of flight results
+    struct Mirror {
+        KswOut *out = nullptr, *out_dev = nullptr;
+        uint32_t *cig = nullptr, *cig_dev = nullptr;
+        int32_t *flag = nullptr, *flag_dev = nullptr;
+        uint64_t slots = 0, words = 0;
+        void release()
+        {
+            if (out) (void)hipHostFree(out);
+            if (cig) (void)hipHostFree(cig);
+            if (flag) (void)hipHostFree(flag);
+            *this = Mirror();
+        }
+    };
+    Mirror mirror_[kStreams];
     uint8_t *staging_[kStreams] = {};              // pinned upload buffers, per stream
     uint64_t staging_bytes_[kStreams] = {};
     int32_t *h_stop_ = nullptr, *d_stop_ = nullptr;  // stop flag shared by all flights
@@ -83,7 +97,7 @@ private:
     bool quit_ = false;
     bool hold_ = false;
     // KSW2_HIP_ENGINE_STATS: per-phase host time, printed when the engine ends
-    bool stats_ = false;
+    int stats_ = 0;  // 1: totals, 2: also one line per flight
     double st_create_ = 0, st_launch_ = 0, st_fetch_ = 0, st_device_ = 0;
     int64_t st_flights_ = 0, st_tasks_ = 0;
 };

@@ -10,21 +10,17 @@ struct StripeArgs {
     const uint8_t *seq;
     uint8_t *dir;
     int32_t *bnd;
+    uint32_t *cig;
     const KswJob *jobs;
     KswOut *out;
     const int32_t *order;      // this launch's jobs, one workgroup each
     int njobs;
     StripeRec *recs;           // per stripe: mqe partials
-    KswScore sc;
-};
-
-struct BacktrackArgs {
-    const uint8_t *dir;
-    uint32_t *cig;
-    const KswJob *jobs;
-    KswOut *out;
-    const StripeRec *recs;
-    int njobs;
+    // optional host mirror (pinned, mapped): each pair's result, CIGAR and a
+    // completion flag raised once both are there; null when not wanted
+    KswOut *host_out;
+    uint32_t *host_cig;
+    int32_t *host_flag;
     KswScore sc;
 };
 
@@ -56,7 +52,6 @@ hipError_t launch_general_backtrack(const GenBacktrackArgs &a, hipStream_t st);
 
 // Stripe kernel: one workgroup of `team` waves (1..kMaxTeam) per job.
 hipError_t launch_stripe(const StripeArgs &a, bool right, bool score_only, int team, hipStream_t st);
-hipError_t launch_backtrack(const BacktrackArgs &a, hipStream_t st);
 // Waves of the stripe kernel that fit on the device at once.
 int stripe_resident_waves(bool right, bool score_only);
 

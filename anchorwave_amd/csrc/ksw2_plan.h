@@ -69,6 +69,12 @@ struct PlanEnv {
     uint64_t staging_bytes = 0;
     int32_t *h_stop = nullptr;       // shared stop flag (pinned, mapped) and its
     int32_t *d_stop = nullptr;       // device address
+    // host mirror for per-pair completion (pinned, mapped, coherent): result
+    // slots, CIGAR words (cig arena layout) and completion flags; host and
+    // device addresses
+    KswOut *mirror_out = nullptr, *mirror_out_dev = nullptr;
+    uint32_t *mirror_cig = nullptr, *mirror_cig_dev = nullptr;
+    int32_t *mirror_flag = nullptr, *mirror_flag_dev = nullptr;
 };
 
 // A device-resident batch: sequences uploaded, arenas sized, jobs built.
@@ -104,6 +110,9 @@ struct Plan {
     uint8_t *h_seq = nullptr;        // pinned staging kept while an upload may be in flight
     bool own_seq = true, own_stop = true;
     int progress_calls = 0;
+    KswOut *h_mout = nullptr, *d_mout = nullptr;   // host mirror (not owned)
+    uint32_t *h_mcig = nullptr, *d_mcig = nullptr;
+    int32_t *h_mflag = nullptr, *d_mflag = nullptr;
     uint8_t *d_seq = nullptr, *d_dir = nullptr;
     int32_t *d_bnd = nullptr;
     uint32_t *d_cig = nullptr;
@@ -137,7 +146,10 @@ bool plan_done(Plan *p);
 int plan_wait(Plan *p, const Progress *prog = nullptr);
 int plan_run(Plan *p, const Progress *prog = nullptr);
 bool progress_continue(const Progress *prog, int &calls, int total);
-int plan_fetch(Plan *p, ksw_extz_t *ez, void *km);
+// Results of the last run into ez[order[k]] for every slot k (those with
+// skip[k] set are left alone).
+int plan_fetch(Plan *p, ksw_extz_t *ez, void *km, const std::vector<char> *skip = nullptr);
+int deliver_slot(const Plan *p, int k, const KswOut &o, const uint32_t *cig_arena, ksw_extz_t *z, void *km);
 void plan_destroy(Plan *p);
 
 }  // namespace kswhip

@@ -384,42 +384,26 @@ __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc,
 
 }  // namespace
 
-// One workgroup (a team of blockDim.x / 64 waves) per pair of this launch's
-// job list; wave w computes passes w, w+W, ...
-template <bool RIGHT, bool SCORE_ONLY>
-__global__ void __launch_bounds__(kWave * kMaxTeam) ksw_stripe_kernel(StripeArgs a)
-{
-    __shared__ int32_t prog[kMaxTeam];
-    const int W = __builtin_amdgcn_readfirstlane(blockDim.x / kWave);
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int job = __builtin_amdgcn_readfirstlane(a.order[blockIdx.x]);
-    prog[w] = 0;  // every lane of wave w writes the same zero
-    __syncthreads();
-    const Consts kc = make_consts<RIGHT>(a.sc);
-    const KswJob jb = a.jobs[job];
-    const int qlen = __builtin_amdgcn_readfirstlane(jb.qlen), tlen = __builtin_amdgcn_readfirstlane(jb.tlen);
-    const int npass = ((tlen + SR - 1) / SR + 1) / 2;
-    for (int p = w; p < npass; p += W) pass_body<RIGHT, SCORE_ONLY>(a, kc, job, p, jb, qlen, tlen, prog, W);
-}
-
-// Device backtrack: one wave per pair.  The wave first folds the
-// per-stripe mqe partials in row order (ties to the later row, the '>=' of
-// minimap2/ksw2_extd2_sse.c:371), then walks the direction bytes from the
-// start cell chosen as in minimap2/ksw2_extd2_sse.c:398-418, running the
+// Epilogue of one pair, run by the whole workgroup once every pass is done:
+// folds the per-stripe mqe partials in row order (ties to the later row, the
+// '>=' of minimap2/ksw2_extd2_sse.c:371), picks the start cell as in
+// minimap2/ksw2_extd2_sse.c:398-418 and walks the direction bytes with the
 // state machine of ksw_backtrack (minimap2/ksw2.h:160-192) for an unbanded
 // matrix (no cell of the path can leave the band there, so the off/off_end
-// forcing never triggers).  The walk reads a tile of 8 lanes x 64 columns
-// (up to 64 rows x 64 columns of one stripe) staged in LDS; when the path
-// leaves the tile the wave stages the next one, so HBM latency is paid once
-// per tile rather than once per step.  Every lane runs the same walk
-// (uniform control flow, LDS broadcast reads); lane 0 writes the run-length
-// encoded CIGAR, which the wave reverses at the end unless KSW_EZ_REV_CIGAR.
-__global__ void __launch_bounds__(64) ksw_backtrack_kernel(BacktrackArgs a)
+// forcing never triggers).  The walk reads a tile of 8 lanes x 64 columns (up
+// to 64 rows x 64 columns of one stripe) staged in LDS by the workgroup; when
+// the path leaves the tile the next one is staged, so HBM latency is paid
+// once per tile rather than once per step.  Every thread runs the same walk
+// (uniform control flow, LDS broadcast reads) while other workgroups keep the
+// SIMDs busy; thread 0 writes the run-length encoded CIGAR, which is reversed
+// at the end unless KSW_EZ_REV_CIGAR.  With a host mirror, the result and the
+// CIGAR are then copied to pinned host memory and the pair's completion flag
+// is raised (system-scope release), so a caller can take this pair's result
+// while the launch still runs.
+__device__ __forceinline__ void pair_epilogue(const StripeArgs &a, int job_id, const KswJob &jb,
+                                              uint64_t (*tile)[64])
 {
-    __shared__ uint64_t tile[8][64];
-    const int job_id = blockIdx.x;
-    const int lane = threadIdx.x;
-    const KswJob jb = a.jobs[job_id];
+    const int tid = threadIdx.x, nthr = blockDim.x;
     KswOut o = a.out[job_id];
     const int flag = jb.flag;
     const int qlen = jb.qlen, tlen = jb.tlen;
@@ -450,9 +434,9 @@ __global__ void __launch_bounds__(64) ksw_backtrack_kernel(BacktrackArgs a)
         }
     }
     int n = 0;
+    uint32_t *cig = a.cig + jb.cig_off;
     if (i0 >= 0 || j0 >= 0) {
         const uint8_t *dir = a.dir + jb.dir_off;
-        uint32_t *cig = a.cig + jb.cig_off;
         const uint64_t nsteps = (uint64_t)qlen + kWave - 1 + kLag;
         int i = i0, j = j0, state = 0;
         int t_i_lo = 1, t_i_hi = 0, t_j_lo = 1, t_j_hi = 0;  // staged tile (empty)
@@ -461,7 +445,7 @@ __global__ void __launch_bounds__(64) ksw_backtrack_kernel(BacktrackArgs a)
             if (op == run_op) {
                 run_len += len;
             } else {
-                if (run_len > 0 && lane == 0) cig[n] = (uint32_t)run_len << 4 | (uint32_t)run_op;
+                if (run_len > 0 && tid == 0) cig[n] = (uint32_t)run_len << 4 | (uint32_t)run_op;
                 if (run_len > 0) ++n;
                 run_op = op, run_len = len;
             }
@@ -475,7 +459,7 @@ __global__ void __launch_bounds__(64) ksw_backtrack_kernel(BacktrackArgs a)
                 t_j_hi = j, t_j_lo = j >= 63 ? j - 63 : 0;
                 const int c_lo = t_j_lo + kb + kLag * h, c_hi = t_j_hi + k + kLag * h;
                 __syncthreads();  // the previous tile is no longer read
-                for (int c = c_lo + lane; c <= c_hi; c += kWave) {
+                for (int c = c_lo + tid; c <= c_hi; c += nthr) {
                     const uint4 *src = (const uint4 *)(dir + (((uint64_t)pp * nsteps + (uint64_t)c) * kWave + kb) * 16);
                     for (int kr = 0; kr <= k - kb; ++kr) {
                         const int jj = c - kLag * h - (kb + kr);
@@ -505,14 +489,45 @@ __global__ void __launch_bounds__(64) ksw_backtrack_kernel(BacktrackArgs a)
         push(-2, 0);  // flush the last run
         __syncthreads();
         if (!(flag & 0x80))
-            for (int x = lane; x < (n >> 1); x += kWave) {
+            for (int x = tid; x < (n >> 1); x += nthr) {
                 const uint32_t t = cig[x];
                 cig[x] = cig[n - 1 - x];
                 cig[n - 1 - x] = t;
             }
     }
     o.n_cigar = n;
-    if (lane == 0) a.out[job_id] = o;
+    if (tid == 0) a.out[job_id] = o;
+    if (a.host_flag) {
+        __syncthreads();
+        uint32_t *hc = a.host_cig + jb.cig_off;
+        for (int x = tid; x < n; x += nthr) hc[x] = cig[x];
+        if (tid == 0) a.host_out[job_id] = o;
+        __threadfence_system();
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(a.host_flag + job_id, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+// One workgroup (a team of blockDim.x / 64 waves) per pair of this launch's
+// job list; wave w computes passes w, w+W, ...; then the workgroup runs the
+// pair's epilogue (mqe, start cell, backtrack).
+template <bool RIGHT, bool SCORE_ONLY>
+__global__ void __launch_bounds__(kWave * kMaxTeam) ksw_stripe_kernel(StripeArgs a)
+{
+    __shared__ int32_t prog[kMaxTeam];
+    __shared__ uint64_t tile[8][64];
+    const int W = __builtin_amdgcn_readfirstlane(blockDim.x / kWave);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int job = __builtin_amdgcn_readfirstlane(a.order[blockIdx.x]);
+    prog[w] = 0;  // every lane of wave w writes the same zero
+    __syncthreads();
+    const Consts kc = make_consts<RIGHT>(a.sc);
+    const KswJob jb = a.jobs[job];
+    const int qlen = __builtin_amdgcn_readfirstlane(jb.qlen), tlen = __builtin_amdgcn_readfirstlane(jb.tlen);
+    const int npass = ((tlen + SR - 1) / SR + 1) / 2;
+    for (int p = w; p < npass; p += W) pass_body<RIGHT, SCORE_ONLY>(a, kc, job, p, jb, qlen, tlen, prog, W);
+    __syncthreads();  // every pass written: stripe partials, score / mte, direction bytes
+    pair_epilogue(a, job, jb, tile);
 }
 
 template <bool RIGHT, bool SCORE_ONLY>
@@ -553,13 +568,6 @@ int stripe_resident_waves(bool right, bool score_only)
     const int n = score_only ? resident_t<false, true>() : right ? resident_t<true, false>() : resident_t<false, false>();
     __atomic_store_n(&cache[v], n, __ATOMIC_RELAXED);
     return n;
-}
-
-hipError_t launch_backtrack(const BacktrackArgs &a, hipStream_t st)
-{
-    if (a.njobs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(ksw_backtrack_kernel, dim3(a.njobs), dim3(kWave), 0, st, a);
-    return hipGetLastError();
 }
 
 }  // namespace kswhip
