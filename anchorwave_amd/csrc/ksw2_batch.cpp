@@ -129,7 +129,7 @@ void make_gen_score(int8_t m, const int8_t *mat, const KswScore &sc, const Score
 // Threads of the general kernel's workgroup for one pair: about four cells
 // of the widest anti-diagonal per thread, a power of two in [64, 1024].
 // KSW2_HIP_GEN_THREADS forces a size (tuning and tests).
-static int gen_threads(const ksw2_pair_t &p)
+int gen_threads(const ksw2_pair_t &p)
 {
     const char *ev = getenv("KSW2_HIP_GEN_THREADS");
     const int forced = ev ? atoi(ev) : 0;
@@ -220,7 +220,7 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         double total_steps = 0;
         for (const KswJob &j : p->jobs) total_steps += pair_steps(j);
         const double T = std::max(1.0, total_steps / std::max(1, stripe_resident_waves(false, false)));
-        int forced = 0;
+        int forced = env ? env->team : 0;
         if (const char *ev = getenv("KSW2_HIP_TEAM")) forced = atoi(ev);
         std::vector<int> team(p->n_fast);
         for (int k = 0; k < p->n_fast; ++k) team[k] = choose_team(p->jobs[k], T, forced);

@@ -2,12 +2,19 @@
 //
 // Tasks (pairs) wait in a host queue and are admitted by their device
 // footprint (direction bytes, state, CIGAR slots) into a pool carved out of
-// one hipMalloc.  Admitted tasks are grouped into plans ("flights") of one
-// size class that run concurrently on a few streams; a flight's results are
-// delivered and its memory returned as soon as it completes, and the freed
-// memory admits the next tasks.  When the task at the head of the queue does
-// not fit, smaller tasks from the back of the queue are admitted instead
-// (backfill), so the device keeps working while memory drains.
+// one hipMalloc, and by waves: the admitted tasks never need more waves than
+// the device holds at once, so every launch starts at once and no task waits
+// inside the device behind another launch's pending workgroups.  Admitted
+// tasks are grouped into plans ("flights") of one size class and one team
+// size (one launch) that run concurrently on a few streams.  A task's team
+// follows the load: the per-wave step budget T is the work queued and in
+// flight over the device's waves, and a pair gets more waves while its steps
+// per wave exceed 2T, so at light load a pair runs on up to one wave per
+// pass (lowest latency) and at heavy load on one wave (highest throughput).
+// Each stripe-kernel pair is delivered the moment its completion flag shows;
+// a flight's memory returns when its last kernel completes.  When the task at
+// the head of the queue does not fit, later tasks that do are admitted
+// instead (backfill), so the device keeps working while memory drains.
 //
 // Two users: ksw_extd2_hip_batch for batches larger than the memory budget
 // (queue ordered longest first), and the serving API (first come, first
@@ -112,7 +119,7 @@ private:
 
 // Size classes keep a flight's members of similar duration, so short tasks
 // are not held back by long ones that share their plan.
-static int size_class(double cells) { return cells < 2e7 ? 0 : cells < 5e8 ? 1 : 2; }
+static int size_class(double cells) { return cells < 3e7 ? 0 : cells < 5e8 ? 1 : 2; }
 
 struct Engine::Task {
     uint64_t ticket = 0;
@@ -121,7 +128,11 @@ struct Engine::Task {
     ksw_extz_t *ez = nullptr;           // caller's output
     uint64_t footprint = 0;
     double cells = 0;
-    int waves = 1, cls = 0;
+    double steps = 0;  // stripe kernel: passes x steps per pass (one wave)
+    int passes = 1;    // stripe kernel: most useful team size
+    int waves = 1;     // general kernel: waves of its workgroup
+    int held = 0;      // waves held while admitted
+    int cls = 0;
     bool general = false;
     double t_submit = 0, t_start = 0, t_done = 0;
 };
@@ -230,7 +241,12 @@ int Engine::submit(const ksw2_pair_t &p, ksw_extz_t *ez, bool copy, uint64_t *ti
         }
         t->cells = (double)p.qlen * p.tlen;
         t->cls = size_class(t->cells);
-        t->waves = t->cls == 2 ? std::min(kMaxTeam, pass_count(p.tlen)) : 1;
+        if (t->general) {
+            t->waves = gen_threads(p) / kWave;
+        } else {
+            t->passes = pass_count(p.tlen);
+            t->steps = (double)t->passes * (double)pass_steps(p.qlen);
+        }
     }
     std::lock_guard<std::mutex> lk(mu_);
     t->ticket = ++next_ticket_;
@@ -242,6 +258,7 @@ int Engine::submit(const ksw2_pair_t &p, ksw_extz_t *ez, bool copy, uint64_t *ti
         cv_done_.notify_all();
     } else {
         queue_[t->cls].push_back(t);
+        queued_steps_ += t->steps;
         ++pending_;
     }
     if (ticket) *ticket = t->ticket;
@@ -261,6 +278,7 @@ void Engine::hold(bool on)
 void Engine::order_longest_first()
 {
     std::lock_guard<std::mutex> lk(mu_);
+    batch_ = true;
     for (auto &q : queue_)
         std::stable_sort(q.begin(), q.end(), [](const Task *a, const Task *b) { return a->cells > b->cells; });
 }
@@ -330,6 +348,7 @@ void Engine::stop_all_locked()
         }
         q.clear();
     }
+    queued_steps_ = 0;
     for (auto &f : flights_) f->plan->stop_requested = true;
     if (h_stop_) __atomic_store_n(h_stop_, 1, __ATOMIC_SEQ_CST);
     cv_done_.notify_all();
@@ -374,6 +393,7 @@ bool Engine::reap_locked(std::unique_lock<std::mutex> &lk)
                 if (st != KSW2_HIP_OK) tk->ez->zdropped = 1, status_ = st;
                 f->delivered[k] = 1, --f->undelivered_fast;
                 f->tasks[p->order[k]] = nullptr;
+                busy_waves_ -= tk->held, inflight_steps_ -= tk->steps;
                 tk->t_done = t;
                 done_.push_back(tk->ticket);
                 --pending_;
@@ -424,7 +444,10 @@ bool Engine::reap_locked(std::unique_lock<std::mutex> &lk)
             tk->t_done = t;
             done_.push_back(tk->ticket);
             --pending_;
+            busy_waves_ -= tk->held, inflight_steps_ -= tk->steps;
         }
+        if (busy_waves_ < 0 || flights_.empty()) busy_waves_ = 0;
+        if (inflight_steps_ < 0 || flights_.empty()) inflight_steps_ = 0;
         plan_destroy(p);
         cv_done_.notify_all();
         any = true;
@@ -432,20 +455,41 @@ bool Engine::reap_locked(std::unique_lock<std::mutex> &lk)
     return any;
 }
 
-// Forms and launches flights while streams and memory allow: each free
+// Team size of a queued task under the per-wave step budget T, at most
+// `cap` waves and one wave per pass; 0 for a general-kernel task (those form
+// flights of their own).
+static int team_for(int passes, double steps, bool general, double T, int cap)
+{
+    if (general) return 0;
+    int team = 1;
+    while (team < kMaxTeam && team < passes && 2 * team <= cap && steps / team > 2 * T) team <<= 1;
+    return std::min(team, passes);
+}
+
+// Forms and launches flights while streams, waves and memory allow: each free
 // stream takes the waiting tasks of its size class, in queue order, as far as
-// pool memory and one device-full of waves allow; a class whose head does
-// not fit admits later tasks that do (backfill).
+// pool memory and the waves allow; tasks that do not fit are passed over for
+// later ones that do (backfill).
+//  * Serving: the flights in the device together hold at most the device's
+//    waves, and a flight's tasks share the head's team size (one launch) and
+//    run within a factor of two of the head's steps per wave, so a stream is
+//    not held long after most of its flight is done.
+//  * Batch (longest first, latency irrelevant): each flight may hold a
+//    device-full of waves, with the plan choosing each pair's team, so the
+//    device always has workgroups waiting to fill the tail of a flight.
+
 bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
 {
     bool any = false;
     if (hold_) return false;
     for (int s = 0; s < kStreams; ++s) {
         if (busy_[s]) continue;
+        const int64_t free_waves = batch_ ? resident_waves_ : resident_waves_ - busy_waves_;
+        if (free_waves <= 0) break;  // the device is full: wait for pairs to complete
         // a stream serves its own class first; the long-task streams also take
         // shorter tasks when no long ones wait, the others never take long ones
         static const int pref[kClasses][kClasses] = {{0, 1, -1}, {1, 0, -1}, {2, 1, 0}};
-        const int own = kStreamClass[s];
+        const int own = batch_ ? kBatchStreamClass[s] : kStreamClass[s];
         int cls = -1;
         for (int k = 0; k < kClasses && pref[own][k] >= 0; ++k)
             if (!queue_[pref[own][k]].empty()) {
@@ -458,22 +502,41 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
         const uint64_t share = (uint64_t)(kFlightShare[cls] * (double)budget_);
         const uint64_t free_block = pool->largest_free();
         const uint64_t avail = std::min(free_block, share);
+        // per-wave step budget: work queued and in flight over the device's waves
+        const double T = std::max(1.0, (queued_steps_ + inflight_steps_) / resident_waves_);
+        const int cap = (int)std::min<int64_t>(free_waves, kMaxTeam);
+        const Task *head = q.front();
+        const int key = batch_ ? (head->general ? 0 : -1) : team_for(head->passes, head->steps, head->general, T, cap);
+        const double span = key > 0 ? head->steps / key : 0;  // the head's steps per wave
         std::vector<Task *> pick;
         uint64_t bytes = kPlanSlack, seq = 0, cigw = 0;
-        int waves = 0;
+        int64_t waves = 0;
+        double steps = 0;
         int skipped = 0;
         for (auto it = q.begin(); it != q.end() && (int)pick.size() < kMaxFlightPairs;) {
             Task *t = *it;
-            if (!pick.empty() && waves + t->waves > resident_waves_) break;
-            // a flight holds at most its class's share of the pool, except that
-            // its first task may use any free block (a task larger than the share)
-            if (bytes + t->footprint > (pick.empty() ? free_block : avail)) {
+            // serving: one team size (one launch) and similar durations per
+            // flight; batch: general and stripe pairs in flights of their own
+            const bool same = batch_ ? (key == 0) == t->general
+                                     : team_for(t->passes, t->steps, t->general, T, cap) == key &&
+                                           (key == 0 || (t->steps / key <= 2 * span && 2 * t->steps / key >= span));
+            if (!same) {
                 if (++skipped > 64) break;  // backfill window
                 ++it;
                 continue;
             }
+            const int w = key > 0 ? key : key < 0 ? (t->cls == 2 ? std::min(kMaxTeam, t->passes) : 1) : t->waves;
+            if (!pick.empty() && waves + w > free_waves) break;
+            // a flight holds at most its class's share of the pool, except that
+            // its first task may use any free block (a task larger than the share)
+            if (bytes + t->footprint > (pick.empty() ? free_block : avail)) {
+                if (++skipped > 64) break;
+                ++it;
+                continue;
+            }
             pick.push_back(t);
-            bytes += t->footprint, waves += t->waves, seq += (uint64_t)t->pair.qlen + t->pair.tlen + 32;
+            t->held = w;
+            bytes += t->footprint, waves += w, steps += t->steps, seq += (uint64_t)t->pair.qlen + t->pair.tlen + 32;
             cigw += ((uint64_t)t->pair.qlen + t->pair.tlen + 2 + 15) / 16 * 16;
             it = q.erase(it);
         }
@@ -517,6 +580,7 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
         env.stream = streams_[s];
         env.staging = staging_[s], env.staging_bytes = staging_bytes_[s];
         env.h_stop = h_stop_, env.d_stop = d_stop_;
+        env.team = std::max(key, 0);
         if (mr.slots) {
             env.mirror_out = mr.out, env.mirror_out_dev = mr.out_dev;
             env.mirror_cig = mr.cig, env.mirror_cig_dev = mr.cig_dev;
@@ -541,6 +605,7 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
                 for (auto it = pick.rbegin(); it != pick.rend(); ++it) q.push_front(*it);
                 continue;
             }
+            queued_steps_ -= steps;
             for (Task *t : pick) {
                 t->ez->zdropped = 1;
                 t->t_done = now_s();
@@ -553,6 +618,8 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
         }
         for (Task *t : pick) t->t_start = t0;
         f->t_start = t0;
+        busy_waves_ += waves, inflight_steps_ += steps, queued_steps_ -= steps;
+        if (queued_steps_ < 0 || queues_empty()) queued_steps_ = 0;
         flights_.push_back(std::move(f));
         any = true;
     }

@@ -21,9 +21,12 @@ class Engine {
 public:
     static constexpr int kStreams = 4;             // flights in flight at once
     static constexpr int kClasses = 3;             // task size classes
-    // stream -> size class it serves: two for the long tasks, one each for
-    // the medium and short ones, so short tasks never queue behind long ones
-    static constexpr int kStreamClass[kStreams] = {2, 2, 1, 0};
+    // stream -> size class it serves first: one for the long tasks, one for
+    // the medium and two for the short ones (most tasks), so short tasks
+    // never queue behind long ones
+    static constexpr int kStreamClass[kStreams] = {2, 1, 0, 0};
+    // batch mode: the long tasks hold most of the work, so two streams
+    static constexpr int kBatchStreamClass[kStreams] = {2, 2, 1, 0};
     // share of the pool one flight of a class may hold: long pairs hold
     // memory longest per byte (their speed is capped by one workgroup), so
     // they never take the whole pool and shorter pairs keep the device full
@@ -93,9 +96,14 @@ private:
     std::unordered_map<uint64_t, Task *> tasks_;
     uint64_t next_ticket_ = 0;
     int64_t pending_ = 0;
+    // admission by waves: waves held by admitted, undelivered tasks, and the
+    // stripe-kernel steps queued and in flight (for the per-wave step budget)
+    int64_t busy_waves_ = 0;
+    double queued_steps_ = 0, inflight_steps_ = 0;
     int status_ = 0;
     bool quit_ = false;
     bool hold_ = false;
+    bool batch_ = false;  // longest-first batch: throughput over latency
     // KSW2_HIP_ENGINE_STATS: per-phase host time, printed when the engine ends
     int stats_ = 0;  // 1: totals, 2: also one line per flight
     double st_create_ = 0, st_launch_ = 0, st_fetch_ = 0, st_device_ = 0;

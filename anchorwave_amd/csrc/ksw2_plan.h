@@ -50,6 +50,8 @@ int fast_path_reject(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls);
 bool takes_general(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls);
 // Device bytes a pair needs while its plan is alive.
 uint64_t pair_footprint(const ksw2_pair_t &p, bool general);
+// Threads of the general kernel's workgroup for one pair (64..1024).
+int gen_threads(const ksw2_pair_t &p);
 
 // Device memory pool of a serving engine: one hipMalloc carved first-fit.
 struct DevicePool {
@@ -75,6 +77,7 @@ struct PlanEnv {
     KswOut *mirror_out = nullptr, *mirror_out_dev = nullptr;
     uint32_t *mirror_cig = nullptr, *mirror_cig_dev = nullptr;
     int32_t *mirror_flag = nullptr, *mirror_flag_dev = nullptr;
+    int team = 0;                    // > 0: every stripe-kernel pair gets this team size
 };
 
 // A device-resident batch: sequences uploaded, arenas sized, jobs built.
