@@ -29,6 +29,7 @@ KSW_EZ_APPROX_DROP = 0x10
 KSW_EZ_EXTZ_ONLY = 0x40
 KSW_EZ_REV_CIGAR = 0x80
 KSW_EZ_SEMIGLOBAL_END = 0x800
+KSW_EZ_SINGLETRACK = 0x1000  # two-track (Singletrack) traceback
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "unsupported", -3: "out of memory",
           -4: "HIP runtime error", -5: "no usable gfx950 device"}

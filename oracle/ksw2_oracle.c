@@ -96,6 +96,96 @@ static void backtrack(int is_rev, const uint8_t *p, const int *off, const int *o
 }
 
 /* ksw_reset_extz, minimap2/ksw2.h:194-199 */
+/* Singletrack traceback (minimap2/ksw2_singletrack.cpp:29-176): walks the two
+ * stored difference tracks of the same anti-diagonal layout as the direction
+ * bytes, p = H(i,j) - H(i,j-1) and p2 = H(i,j) - H(i-1,j), from (tlen-1,
+ * qlen-1).  Cells outside [off, off_end] of their anti-diagonal read as
+ * KSW_NEG_INF; row/column -1 read the gap-cost difference of the boundary.
+ * A diagonal step is taken while p(i,j) + p2(i,j-1) equals the pair's
+ * substitution score; otherwise a gap length l = 1, 2, ... is searched whose
+ * accumulated difference equals -(q + l*e) or -(q2 + l*e2), deletion first.
+ * Returns 0 when no gap length is found (the reference then frees the CIGAR
+ * and reports zdropped). */
+static int32_t st_gap(int32_t len, int32_t go, int32_t ge, int32_t go2, int32_t ge2)
+{
+    int64_t a, b;
+    if (len <= 0) return 0;
+    a = (int64_t)go + (int64_t)len * ge, b = (int64_t)go2 + (int64_t)len * ge2;
+    return (int32_t)(a > b ? a : b);
+}
+
+static int32_t st_get(const int8_t *tr, int32_t i, int32_t j, size_t n_col, const int *off, const int *off_end,
+                      int32_t go, int32_t ge, int32_t go2, int32_t ge2)
+{
+    int64_t d;
+    if (i < 0 || j < 0) {
+        int32_t c = i < 0 ? j : i;
+        return st_gap(c + 1, go, ge, go2, ge2) - st_gap(c, go, ge, go2, ge2);
+    }
+    d = (int64_t)i + j;
+    if (i < off[d] || i > off_end[d]) return KSW_NEG_INF;
+    return tr[(size_t)d * n_col + (size_t)(i - off[d])];
+}
+
+static int st_fin(int32_t v) { return v > KSW_NEG_INF / 2 && v < -KSW_NEG_INF / 2; }
+
+static int st_backtrace(int qlen, const uint8_t *query, int tlen, const uint8_t *target, int8_t m,
+                        const int8_t *mat, int8_t q, int8_t e, int8_t q2, int8_t e2, size_t n_col,
+                        const int *off, const int *off_end, const int8_t *p, const int8_t *p2, ksw_extz_t *ez)
+{
+    const int32_t go = -q, ge = -e, go2 = -q2, ge2 = -e2;
+    int in_m = 1, n_cigar = 0, m_cigar = ez->m_cigar, k;
+    uint32_t *cigar = ez->cigar;
+    int32_t i = tlen - 1, j = qlen - 1, l = 0, dd = 0, di = 0;
+    while (i >= 0 && j >= 0) {
+        const int32_t diag = i + j;
+        if (i < off[diag]) { cigar = push_cigar(&n_cigar, &m_cigar, cigar, 1, 1); --j; in_m = 1; continue; }
+        if (i > off_end[diag]) { cigar = push_cigar(&n_cigar, &m_cigar, cigar, 2, 1); --i; in_m = 1; continue; }
+        if (in_m) {
+            int32_t h = st_get(p, i, j, n_col, off, off_end, go, ge, go2, ge2);
+            int32_t v = st_get(p2, i, j - 1, n_col, off, off_end, go, ge, go2, ge2);
+            int32_t cost = mat[target[i] * m + query[j]];
+            if (st_fin(h) && st_fin(v) && (int64_t)h + v == cost) {
+                cigar = push_cigar(&n_cigar, &m_cigar, cigar, 0, 1);
+                --i, --j;
+            } else {
+                in_m = 0, l = 0;
+                di = st_get(p, i, j, n_col, off, off_end, go, ge, go2, ge2);
+                dd = st_get(p2, i, j, n_col, off, off_end, go, ge, go2, ge2);
+            }
+        } else {
+            int32_t a1, a2, ji, id;
+            ++l;
+            if (l > i + 1 && l > j + 1) { free(cigar); ez->cigar = 0; ez->m_cigar = ez->n_cigar = 0; return 0; }
+            a1 = go + l * ge, a2 = go2 + l * ge2, ji = j - l, id = i - l;
+            if (id >= -1 && (dd == a1 || dd == a2)) {
+                cigar = push_cigar(&n_cigar, &m_cigar, cigar, 2, l);
+                i = id, in_m = 1;
+            } else if (ji >= -1 && (di == a1 || di == a2)) {
+                cigar = push_cigar(&n_cigar, &m_cigar, cigar, 1, l);
+                j = ji, in_m = 1;
+            } else {
+                if (ji >= 0) {
+                    int32_t hh = st_get(p, i, ji, n_col, off, off_end, go, ge, go2, ge2);
+                    di = st_fin(di) && st_fin(hh) ? di + hh : KSW_NEG_INF;
+                }
+                if (id >= 0) {
+                    int32_t vv = st_get(p2, id, j, n_col, off, off_end, go, ge, go2, ge2);
+                    dd = st_fin(dd) && st_fin(vv) ? dd + vv : KSW_NEG_INF;
+                }
+            }
+        }
+    }
+    if (i >= 0) cigar = push_cigar(&n_cigar, &m_cigar, cigar, 2, i + 1);
+    if (j >= 0) cigar = push_cigar(&n_cigar, &m_cigar, cigar, 1, j + 1);
+    for (k = 0; k < n_cigar >> 1; ++k) {
+        uint32_t tmp = cigar[k];
+        cigar[k] = cigar[n_cigar - 1 - k], cigar[n_cigar - 1 - k] = tmp;
+    }
+    ez->cigar = cigar, ez->m_cigar = m_cigar, ez->n_cigar = n_cigar;
+    return 1;
+}
+
 static void reset_extz(ksw_extz_t *ez)
 {
     ez->max_q = ez->max_t = ez->mqe_t = ez->mte_q = -1;
@@ -139,10 +229,13 @@ void oracle_ksw_extd2(int qlen, const uint8_t *query, int tlen, const uint8_t *t
     const int qe_orig = q + e;
     const int with_cigar = !(flag & KSW_EZ_SCORE_ONLY), approx_max = !!(flag & KSW_EZ_APPROX_MAX);
     const int right = !!(flag & KSW_EZ_RIGHT), generic = !!(flag & KSW_EZ_GENERIC_SC);
+    /* two-track traceback: the AVX2/AVX-512 kernels only (ksw2_extd2_avx.c:128);
+     * the SSE kernel ignores the flag */
+    const int singletrack = !!(flag & KSW_EZ_SINGLETRACK) && isa == ORACLE_ISA_AVX512;
     int r, t, tlen_, qlen_, n_col_, max_sc, min_sc, long_thres, long_diff, last_st, last_en;
     int32_t *H = 0, H0 = 0, last_H0_t = 0;
     int8_t qe8, qe28, sc_mch, sc_mis, sc_N, m1;
-    uint8_t *mem, *p = 0;
+    uint8_t *mem, *p = 0, *p2 = 0;
     int8_t *u, *v, *x, *y, *x2, *y2, *s;
     uint8_t *sf, *qr;
     int *off = 0, *off_end = 0;
@@ -174,6 +267,7 @@ void oracle_ksw_extd2(int qlen, const uint8_t *query, int tlen, const uint8_t *t
 
     if (with_cigar) {
         p = (uint8_t *)malloc(((size_t)(qlen + tlen - 1) * n_col_ + 1) * B);
+        if (singletrack) p2 = (uint8_t *)malloc(((size_t)(qlen + tlen - 1) * n_col_ + 1) * B);
         off = (int *)malloc((size_t)(qlen + tlen - 1) * sizeof(int) * 2);
         off_end = off + qlen + tlen - 1;
     }
@@ -311,7 +405,15 @@ void oracle_ksw_extd2(int qlen, const uint8_t *query, int tlen, const uint8_t *t
                 rng_commit();
                 if (wrapped) ++g_range.wraps;
             }
-            if (with_cigar) {
+            if (with_cigar && singletrack) {
+                /* p = z - u(t) (ksw2_extd2_avx.c:432-436): the new v; p2 = the new u */
+                p[(size_t)r * n_col_ * B + (size_t)(t - stB)] = (uint8_t)v[t];
+                p2[(size_t)r * n_col_ * B + (size_t)(t - stB)] = (uint8_t)u[t];
+                if (dir_ij && t >= st0 && t <= en0) {
+                    dir_ij[(size_t)t * qlen + (size_t)(r - t)] = (uint8_t)v[t];
+                    dir_ij[(size_t)tlen * qlen + (size_t)t * qlen + (size_t)(r - t)] = (uint8_t)u[t];
+                }
+            } else if (with_cigar) {
                 p[(size_t)r * n_col_ * B + (size_t)(t - stB)] = (uint8_t)d;
                 if (dir_ij && t >= st0 && t <= en0) dir_ij[(size_t)t * qlen + (size_t)(r - t)] = (uint8_t)d;
             }
@@ -369,7 +471,18 @@ void oracle_ksw_extd2(int qlen, const uint8_t *query, int tlen, const uint8_t *t
     }
     free(mem);
     free(H);
-    if (with_cigar) { /* ksw2_extd2_sse.c:398-418 */
+    if (with_cigar && singletrack) { /* ksw2_extd2_avx.c:752-762 */
+        if (!ez->stopped && !ez->zdropped && !(flag & (KSW_EZ_EXTZ_ONLY | KSW_EZ_SEMIGLOBAL_END))) {
+            st_backtrace(qlen, query, tlen, target, m, mat, q, e, q2, e2, (size_t)n_col_ * B, off, off_end,
+                         (const int8_t *)p, (const int8_t *)p2, ez);
+            if (ez->n_cigar == 0) ez->zdropped = 1;
+        } else if (!ez->stopped && !ez->zdropped) {
+            ez->zdropped = 1;
+        }
+        free(p);
+        free(p2);
+        free(off);
+    } else if (with_cigar) { /* ksw2_extd2_sse.c:398-418 */
         int rev = !!(flag & KSW_EZ_REV_CIGAR);
         size_t nc = (size_t)n_col_ * B;
         if (!ez->stopped && !ez->zdropped && (flag & KSW_EZ_SEMIGLOBAL_END)) {

@@ -25,7 +25,10 @@ extern "C" {
  *   isa      ORACLE_ISA_SSE or ORACLE_ISA_AVX512
  *   dir_ij   optional (may be NULL): tlen*qlen bytes, receives the direction
  *            byte of every in-band cell (i = target, j = query) at
- *            dir_ij[i*qlen + j]; other bytes are left untouched
+ *            dir_ij[i*qlen + j]; other bytes are left untouched.  With
+ *            KSW_EZ_SINGLETRACK (ISA 64) it holds 2*tlen*qlen bytes instead:
+ *            the two tracks, p = H(i,j)-H(i,j-1) at [i*qlen + j] and
+ *            p2 = H(i,j)-H(i-1,j) at [tlen*qlen + i*qlen + j] (int8)
  *   stop_r   anti-diagonal at which a progress callback would return 0
  *            (-1: never), emulating minimap2/ksw2_progress.c:30-38 */
 void oracle_ksw_extd2(int qlen, const uint8_t *query, int tlen, const uint8_t *target,

@@ -138,14 +138,19 @@ def default_matrix(match: int = 0, mismatch: int = -6, m: int = 5) -> np.ndarray
 def run_oracle(query, target, mat=None, m=5, q=8, e=2, q2=75, e2=1, w=-1, zdrop=-1,
                end_bonus=0, flag=0, isa=ISA_AVX512, want_dirs=False, stop_r=-1):
     """Runs the C restatement; returns ``Result`` (and the (tlen, qlen)
-    direction-byte matrix of in-band cells when ``want_dirs``)."""
+    direction-byte matrix of in-band cells when ``want_dirs``; with
+    KSW_EZ_SINGLETRACK a (2, tlen, qlen) int8 array of the two difference
+    tracks instead, p = H(i,j)-H(i,j-1) and p2 = H(i,j)-H(i-1,j))."""
     lib = oracle_lib()
     qs, ts = _u8(query), _u8(target)
     mat = _i8(default_matrix() if mat is None else mat)
     ez = KswExtz()
     dirs = None
     dptr = None
-    if want_dirs:
+    if want_dirs and (flag & 0x1000) and isa == ISA_AVX512:
+        dirs = np.zeros((2, len(ts), len(qs)), dtype=np.int8)
+        dptr = dirs.ctypes.data
+    elif want_dirs:
         dirs = np.zeros((len(ts), len(qs)), dtype=np.uint8)
         dptr = dirs.ctypes.data
     lib.oracle_ksw_extd2(len(qs), qs.ctypes.data, len(ts), ts.ctypes.data, m, mat.ctypes.data,

@@ -120,6 +120,46 @@ def main():
                               -1, -1, 0, flag | gflag, inline=False, src=src))
         cases.append(case(f"{qf}:{qi}~{tf}:{ti}/w500", qs, ts, O.default_matrix(), 5, 8, 2, 75, 1,
                           500, -1, 0, gflag, inline=False, src=src))
+    # KSW_EZ_SINGLETRACK (the two-track traceback, minimap2/ksw2_singletrack.cpp),
+    # drawn from a separate stream so the cases above stay as they were
+    rng2 = np.random.default_rng(20261203)
+    st_flags = [0x1000, 0x1000, 0x1000, 0x1002, 0x1080, 0x1004, 0x1040, 0x1800, 0x1001]
+    for k in range(120):
+        tl = int(rng2.integers(1, 400))
+        t = W.random_seq(rng2, tl)
+        if rng2.random() < 0.7:
+            q = W.mutate(rng2, t, 0.08, 0.03)
+            if rng2.random() < 0.3:
+                q = W.big_indel(rng2, q, 5, max(6, tl // 2))
+        else:
+            q = W.random_seq(rng2, int(rng2.integers(1, 400)))
+        if len(q) == 0:
+            q = W.random_seq(rng2, 1)
+        flag = int(rng2.choice(st_flags))
+        if rng2.random() < 0.15:
+            t = t.copy()
+            t[rng2.integers(0, len(t), max(1, len(t) // 50))] = 4
+            flag |= 0x04
+        w = -1 if rng2.random() < 0.6 else int(rng2.integers(0, 160))
+        zdrop = -1 if rng2.random() < 0.85 else int(rng2.integers(0, 150))
+        if rng2.random() < 0.6:
+            m, mat, pen = 5, O.default_matrix(), (8, 2, 75, 1)
+        else:
+            m, mat = 5, O.default_matrix(0, -int(rng2.integers(1, 10)))
+            pen = (int(rng2.integers(0, 20)), int(rng2.integers(1, 6)), int(rng2.integers(0, 90)),
+                   int(rng2.integers(0, 3)))
+        cases.append(case(f"st{k}", q, t, mat, m, *pen, w, zdrop, 0, flag))
+    for k, (q, t) in enumerate(W.fixed(seed=2, n=3)):
+        cases.append(case(f"st_fixed{k}", q, t, O.default_matrix(), 5, 8, 2, 75, 1, -1, -1, 0, 0x1000))
+    for qf, qi, tf, ti in pairs[:2]:
+        qs, qn = encode(fa[qf][qi][1])
+        ts, tn = encode(fa[tf][ti][1])
+        gflag = 0x04 if (qn or tn) else 0
+        src = {"query": [qf, qi], "target": [tf, ti]}
+        cases.append(case(f"{qf}:{qi}~{tf}:{ti}/st", qs, ts, O.default_matrix(), 5, 8, 2, 75, 1,
+                          -1, -1, 0, 0x1000 | gflag, inline=False, src=src))
+        cases.append(case(f"{qf}:{qi}~{tf}:{ti}/st-w500", qs, ts, O.default_matrix(), 5, 8, 2, 75, 1,
+                          500, -1, 0, 0x1000 | gflag, inline=False, src=src))
     out = os.path.join(HERE, "ksw2_golden.json.gz")
     with gzip.open(out, "wt") as f:
         json.dump({"generator": "tests/golden/make_golden.py", "reference": "ksw_extd2_avx512",

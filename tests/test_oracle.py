@@ -232,3 +232,121 @@ def test_planted_mistakes_are_caught():
     differing = [(c, q, t, kw) for c, q, t, kw in golden_cases() if not c["sse_same"]]
     for c, q, t, kw in differing:
         assert not same(O.run_oracle(q, t, isa=O.ISA_SSE, **kw), as_result(c["expect"]))
+
+
+# ------------------------------------------------------- Singletrack traceback
+def _gotoh_h(qs, ts, mat, m, q, e, q2, e2):
+    """Full textbook Gotoh matrix with two gap pieces, H[i+1][j+1] = H(i, j),
+    row/column 0 the boundary; independent of the difference recurrence."""
+    neg = -(1 << 40)
+    tl, ql = len(ts), len(qs)
+
+    def gap(k):
+        return -min(q + k * e, q2 + k * e2) if k > 0 else 0
+
+    H = [[0] * (ql + 1) for _ in range(tl + 1)]
+    E1 = [[neg] * (ql + 1) for _ in range(tl + 1)]
+    E2 = [[neg] * (ql + 1) for _ in range(tl + 1)]
+    F1 = [[neg] * (ql + 1) for _ in range(tl + 1)]
+    F2 = [[neg] * (ql + 1) for _ in range(tl + 1)]
+    for j in range(1, ql + 1):
+        H[0][j] = gap(j)
+    for i in range(1, tl + 1):
+        H[i][0] = gap(i)
+        for j in range(1, ql + 1):
+            E1[i][j] = max(H[i][j - 1] - q - e, E1[i][j - 1] - e)
+            E2[i][j] = max(H[i][j - 1] - q2 - e2, E2[i][j - 1] - e2)
+            F1[i][j] = max(H[i - 1][j] - q - e, F1[i - 1][j] - e)
+            F2[i][j] = max(H[i - 1][j] - q2 - e2, F2[i - 1][j] - e2)
+            H[i][j] = max(H[i - 1][j - 1] + int(mat[int(ts[i - 1]) * m + int(qs[j - 1])]),
+                          E1[i][j], E2[i][j], F1[i][j], F2[i][j])
+    return np.array(H, dtype=np.int64)
+
+
+def test_singletrack_tracks_are_gotoh_differences():
+    """The two stored tracks (minimap2/ksw2_extd2_avx.c:432-436: z - u and
+    z - v) are the horizontal and vertical differences of H.  Wherever no
+    int8 intermediate wraps and the match cap never binds, they equal the
+    differences of an independent textbook Gotoh matrix, cell by cell."""
+    rng = np.random.default_rng(31)
+    checked = 0
+    for _ in range(120):
+        t = W.random_seq(rng, int(rng.integers(1, 40)))
+        q = W.mutate(rng, t, 0.15, 0.08) if rng.random() < 0.7 else W.random_seq(rng, int(rng.integers(1, 40)))
+        q = q if len(q) else W.random_seq(rng, 1)
+        pen = (8, 2, 75, 1) if rng.random() < 0.6 else (int(rng.integers(1, 12)), int(rng.integers(1, 4)),
+                                                       int(rng.integers(12, 40)), 1)
+        mat = O.default_matrix(0, -int(rng.integers(1, 8)))
+        kw = dict(mat=mat, m=5, q=pen[0], e=pen[1], q2=pen[2], e2=pen[3])
+        with O.RangeProbe() as rp:
+            res, tr = O.run_oracle(q, t, flag=ksw2.KSW_EZ_SINGLETRACK, want_dirs=True, **kw)
+        if rp.wraps or rp.cap_binds or res.score <= -(1 << 29):
+            continue
+        H = _gotoh_h(q, t, np.asarray(mat).reshape(-1), 5, *pen)
+        assert np.array_equal(tr[0], H[1:, 1:] - H[1:, :-1])
+        assert np.array_equal(tr[1], H[1:, 1:] - H[:-1, 1:])
+        checked += 1
+    assert checked > 60
+
+
+def test_singletrack_cigar_properties():
+    """A Singletrack CIGAR (when the walk succeeds) consumes both sequences,
+    never scores above the optimum, and is a single match run for identical
+    sequences; a failed walk leaves no CIGAR and sets zdropped
+    (minimap2/ksw2_singletrack.cpp:160-165, ksw2_extd2_avx.c:756-757)."""
+    rng = np.random.default_rng(32)
+    valid = failed = 0
+    for _ in range(300):
+        t = W.random_seq(rng, int(rng.integers(1, 150)))
+        q = W.mutate(rng, t, 0.1, 0.05)
+        q = q if len(q) else W.random_seq(rng, 1)
+        kw = dict(mat=O.default_matrix(0, -6), m=5, q=8, e=2, q2=75, e2=1)
+        r = O.run_oracle(q, t, flag=ksw2.KSW_EZ_SINGLETRACK, **kw)
+        if r.n_cigar == 0:
+            assert r.zdropped == 1
+            failed += 1
+            continue
+        valid += 1
+        assert O.cigar_lengths(r.cigar) == (len(q), len(t))
+        assert O.cigar_score(r.cigar, q, t, **kw) <= O.gotoh2_global(q, t, **kw)
+    assert valid > 200
+    t = W.random_seq(rng, 300)
+    r = O.run_oracle(t, t, flag=ksw2.KSW_EZ_SINGLETRACK, mat=O.default_matrix(0, -6))
+    assert list(r.cigar) == [300 << 4] and r.score == 0
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference checker not built (oracle/_ref)")
+def test_singletrack_oracle_vs_reference_random():
+    """Live differential of the Singletrack path against the reference
+    compiled from its sources, over bands, z-drops, scorings and flags."""
+    rng = np.random.default_rng(33)
+    for it in range(400):
+        t = W.random_seq(rng, int(rng.integers(1, 300)))
+        q = W.mutate(rng, t, 0.1, 0.04) if rng.random() < 0.7 else W.random_seq(rng, int(rng.integers(1, 300)))
+        q = q if len(q) else W.random_seq(rng, 1)
+        if rng.random() < 0.3:
+            q = W.big_indel(rng, q, 5, max(6, len(q) // 2))
+        flag = ksw2.KSW_EZ_SINGLETRACK | int(rng.choice([0, 0, 0x02, 0x80, 0x04, 0x40, 0x800, 0x01]))
+        pen = (8, 2, 75, 1) if rng.random() < 0.5 else (int(rng.integers(0, 20)), int(rng.integers(1, 6)),
+                                                       int(rng.integers(0, 90)), int(rng.integers(0, 3)))
+        kw = dict(mat=O.default_matrix(int(rng.integers(0, 3)), -int(rng.integers(1, 9))),
+                  q=pen[0], e=pen[1], q2=pen[2], e2=pen[3],
+                  w=-1 if rng.random() < 0.6 else int(rng.integers(0, 120)),
+                  zdrop=-1 if rng.random() < 0.8 else int(rng.integers(0, 150)), flag=flag)
+        r = O.run_ref(q, t, isa="avx512", **kw)
+        o = O.run_oracle(q, t, **kw)
+        assert same(o, r), f"it={it} {kw['w']} {flag:#x}: {diff(o, r)}"
+
+
+def test_planted_singletrack_mistake_is_caught():
+    """The Singletrack golden vectors are sensitive to the walk: the same
+    calls without the flag (the direction-byte backtrack) differ on many."""
+    changed = n = 0
+    for c, q, t, kw in golden_cases():
+        if not (kw["flag"] & ksw2.KSW_EZ_SINGLETRACK) or kw["flag"] & 0x841:
+            continue
+        n += 1
+        kw2 = dict(kw, flag=kw["flag"] & ~ksw2.KSW_EZ_SINGLETRACK)
+        if not same(O.run_oracle(q, t, **kw2), as_result(c["expect"])):
+            changed += 1
+    assert n > 40 and changed > n // 4
