@@ -26,6 +26,7 @@ using namespace kswhip;
 // kalloc's krealloc when the host program links it (AnchorWave does); with
 // km == NULL kalloc itself falls back to realloc (minimap2/kalloc.c:167-175).
 extern "C" void *krealloc(void *km, void *ptr, size_t size) __attribute__((weak));
+extern "C" void kfree(void *km, void *ptr) __attribute__((weak));
 
 namespace {
 
@@ -176,6 +177,14 @@ void ksw_extd2_hip(void *km, int qlen, const uint8_t *query, int tlen, const uin
     set_err(st);
 }
 
+void ksw_extd2_singletrack_hip(void *km, int qlen, const uint8_t *query, int tlen, const uint8_t *target,
+                               int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q2, int8_t e2, int w,
+                               int zdrop, int end_bonus, int flag, ksw_extz_t *ez)
+{
+    ksw_extd2_hip(km, qlen, query, tlen, target, m, mat, q, e, q2, e2, w, zdrop, end_bonus,
+                  flag | KSW_EZ_SINGLETRACK, ez);
+}
+
 struct ksw2_hip_plan {
     kswhip::Plan *p;
     double cells;
@@ -198,7 +207,7 @@ int ksw2_hip_plan_create(ksw2_hip_plan **plan, int n, const ksw2_pair_t *pairs, 
         if (pairs[k].qlen <= 0 || pairs[k].tlen <= 0 || !pairs[k].query || !pairs[k].target)
             return set_err(KSW2_HIP_EINVAL);
         if (pairs[k].qlen > 0x7fffffff - pairs[k].tlen + 1) return set_err(KSW2_HIP_EINVAL);
-        if (pairs[k].flag & KSW_EZ_SINGLETRACK) return set_err(KSW2_HIP_EUNSUPPORTED);
+
         idx[k] = k;
         cells += (double)pairs[k].qlen * pairs[k].tlen;
     }
@@ -240,7 +249,7 @@ int ksw2_hip_plan_dirs(const ksw2_hip_plan *plan, int index, uint8_t *out, size_
         if (p->order[x] == index) k = x;
     if (k < 0 || k >= p->n_fast) return set_err(KSW2_HIP_EINVAL);  // general-kernel pair: other layout
     const kswhip::KswJob &j = p->jobs[k];
-    const uint64_t need = kswhip::dir_bytes(j.qlen, j.tlen);
+    const uint64_t need = (track_pair(j.flag) ? 2 : 1) * kswhip::dir_bytes(j.qlen, j.tlen);
     if (bytes < need || (j.flag & KSW_EZ_SCORE_ONLY)) return set_err(KSW2_HIP_EINVAL);
     if (hipMemcpy(out, p->d_dir + j.dir_off, need, hipMemcpyDeviceToHost) != hipSuccess)
         return set_err(KSW2_HIP_EDEVICE);
@@ -329,6 +338,20 @@ int deliver_cigar(ksw_extz_t *ez, const uint32_t *src, int n, void *km)
     }
     if (n > 0) memcpy(ez->cigar, src, (size_t)n * sizeof(uint32_t));
     ez->n_cigar = n;
+    return KSW2_HIP_OK;
+}
+
+// A failed Singletrack walk: the reference frees the CIGAR buffer and leaves
+// none (minimap2/ksw2_singletrack.cpp:160-165); the caller sees zdropped.
+int drop_cigar(ksw_extz_t *ez, void *km)
+{
+    if (ez->cigar) {
+        if (km && kfree) kfree(km, ez->cigar);
+        else if (km) return KSW2_HIP_EUNSUPPORTED;  // arena without kalloc linked
+        else free(ez->cigar);
+    }
+    ez->cigar = nullptr, ez->m_cigar = 0, ez->n_cigar = 0;
+    ez->zdropped = 1;
     return KSW2_HIP_OK;
 }
 

@@ -64,7 +64,7 @@ int fast_path_reject(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls)
     const int mx = std::max(p.qlen, p.tlen);
     if (!(p.w < 0 || p.w >= mx)) why |= kRejBand;
     if (p.zdrop >= 0) why |= kRejZdrop;
-    if (p.flag & (KSW_EZ_APPROX_MAX | KSW_EZ_SINGLETRACK)) why |= kRejFlags;
+    if (p.flag & KSW_EZ_APPROX_MAX) why |= kRejFlags;
     if (!cls.fast_ok) why |= kRejScoring;
     if (p.flag & KSW_EZ_GENERIC_SC) {
         if (!cls.generic_mm) why |= kRejScoring;
@@ -109,11 +109,13 @@ uint64_t pair_footprint(const ksw2_pair_t &p, bool general)
 {
     const uint64_t cig = 4ull * ((uint64_t)p.qlen + p.tlen + 2) + 64, seq = (uint64_t)p.qlen + p.tlen + 32;
     const bool cigar = !(p.flag & KSW_EZ_SCORE_ONLY);
+    const uint64_t tracks = track_pair(p) ? 2 : 1;  // Singletrack: two bytes per cell
     if (!general)
-        return (cigar ? dir_bytes(p.qlen, p.tlen) : 0) + 8ull * p.qlen + cig + seq + 16ull * stripe_count(p.tlen) + 512;
+        return (cigar ? tracks * dir_bytes(p.qlen, p.tlen) : 0) + 8ull * p.qlen + cig + seq +
+               16ull * stripe_count(p.tlen) + 512;
     const int tlenB = (p.tlen + 63) / 64 * 64, qlenB = (p.qlen + 63) / 64 * 64;
     const uint64_t diags = (uint64_t)p.qlen + p.tlen - 1;
-    const uint64_t rows = cigar ? diags * (uint64_t)gen_ncolB(p.qlen, p.tlen, norm_w(p)) + 256 : 0;
+    const uint64_t rows = cigar ? tracks * diags * (uint64_t)gen_ncolB(p.qlen, p.tlen, norm_w(p)) + 256 : 0;
     return gen_layout(tlenB, qlenB).bytes + rows + (cigar ? 8 * diags : 0) + cig + seq + 512;
 }
 
@@ -151,20 +153,23 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
     p->gsc = gsc;
     p->R = kRowsPerLane;
     const int R = p->R, SR = kWave * R;
-    // slots: stripe-kernel pairs by group (left, right, score-only), then
-    // general-kernel pairs
-    std::vector<int> g[3], gen;
+    // slots: stripe-kernel pairs by group (left, right, score-only, tracks),
+    // then general-kernel pairs
+    std::vector<int> g[kGroups], gen;
     for (int k = 0; k < nidx; ++k) {
         const ksw2_pair_t &pp = pairs[idx[k]];
         if (takes_general(pp, m, cls)) {
             gen.push_back(idx[k]);
             continue;
         }
-        const int grp = (pp.flag & KSW_EZ_SCORE_ONLY) ? 2 : (pp.flag & KSW_EZ_RIGHT) ? 1 : 0;
+        const int grp = (pp.flag & KSW_EZ_SCORE_ONLY) ? kGroupScoreOnly
+                        : track_pair(pp)               ? kGroupTracks
+                        : (pp.flag & KSW_EZ_RIGHT)     ? kGroupRight
+                                                       : kGroupLeft;
         g[grp].push_back(idx[k]);
     }
     std::vector<int> grp_of;
-    for (int grp = 0; grp < 3; ++grp)
+    for (int grp = 0; grp < kGroups; ++grp)
         for (int id : g[grp]) p->order.push_back(id), grp_of.push_back(grp);
     p->n_fast = (int)p->order.size();
     for (int id : gen) p->order.push_back(id);
@@ -185,7 +190,8 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         j.q_off = so, so = round_up(so + pp.qlen, 16);
         j.t_off = so, so = round_up(so + pp.tlen, 16);
         j.dir_off = dof;
-        if (!(pp.flag & KSW_EZ_SCORE_ONLY)) dof = round_up(dof + dir_bytes(pp.qlen, pp.tlen), 256);
+        if (!(pp.flag & KSW_EZ_SCORE_ONLY))
+            dof = round_up(dof + (track_pair(pp) ? 2 : 1) * dir_bytes(pp.qlen, pp.tlen), 256);
         j.bnd_off = bo, bo += round_up(2ull * pp.qlen, 64);
         j.cig_off = co, co += round_up((uint64_t)pp.qlen + pp.tlen + 2, 16);
         j.stripe_off = stripes, stripes += (pp.tlen + SR - 1) / SR;
@@ -195,6 +201,7 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
     for (int gk = 0; gk < p->n_gen; ++gk) {
         const int slot = p->n_fast + gk;
         const ksw2_pair_t &pp = pairs[p->order[slot]];
+        if (track_pair(pp)) ++p->n_tracks_gen;
         GenJob &j = p->gjobs[gk];
         j.qlen = pp.qlen, j.tlen = pp.tlen, j.w = norm_w(pp), j.zdrop = pp.zdrop;
         j.end_bonus = pp.end_bonus, j.flag = pp.flag;
@@ -208,7 +215,7 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         j.dir_off = dof;
         j.off_off = oo;
         if (!(pp.flag & KSW_EZ_SCORE_ONLY)) {
-            dof = round_up(dof + diags * (uint64_t)j.ncolB, 256);
+            dof = round_up(dof + (track_pair(pp) ? 2 : 1) * diags * (uint64_t)j.ncolB, 256);
             oo += round_up(2 * diags, 16);
         }
         j.cig_off = co, co += round_up((uint64_t)pp.qlen + pp.tlen + 2, 16);
@@ -219,12 +226,12 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
     if (p->n_fast > 0) {
         double total_steps = 0;
         for (const KswJob &j : p->jobs) total_steps += pair_steps(j);
-        const double T = std::max(1.0, total_steps / std::max(1, stripe_resident_waves(false, false)));
+        const double T = std::max(1.0, total_steps / std::max(1, stripe_resident_waves(kGroupLeft)));
         int forced = env ? env->team : 0;
         if (const char *ev = getenv("KSW2_HIP_TEAM")) forced = atoi(ev);
         std::vector<int> team(p->n_fast);
         for (int k = 0; k < p->n_fast; ++k) team[k] = choose_team(p->jobs[k], T, forced);
-        for (int grp = 0; grp < 3; ++grp)
+        for (int grp = 0; grp < kGroups; ++grp)
             for (int t = kMaxTeam; t >= 1; --t) {
                 std::vector<int> members;
                 for (int k = 0; k < p->n_fast; ++k)
@@ -423,7 +430,7 @@ int plan_launch(Plan *p, const Progress *prog)
         hipStream_t st = lane(l.lane);
         if (!st) return KSW2_HIP_EDEVICE;
         sa.order = p->d_order + l.off, sa.njobs = l.count;
-        KSW_HIP_TRY(launch_stripe(sa, l.grp == 1, l.grp == 2, l.team, st));
+        KSW_HIP_TRY(launch_stripe(sa, l.grp, l.team, st));
     }
     for (int k = 0; k < Plan::kLanes; ++k)
         if (used[k]) {
@@ -436,6 +443,12 @@ int plan_launch(Plan *p, const Progress *prog)
         gb.dir = p->d_dir, gb.offs = p->d_offs, gb.cig = p->d_cig, gb.jobs = p->d_gjobs, gb.out = p->d_out;
         gb.njobs = p->n_gen;
         KSW_HIP_TRY(launch_general_backtrack(gb, p->stream));
+        if (p->n_tracks_gen > 0) {
+            GenWalkArgs gw;
+            gw.seq = p->d_seq, gw.dir = p->d_dir, gw.offs = p->d_offs, gw.cig = p->d_cig;
+            gw.jobs = p->d_gjobs, gw.out = p->d_out, gw.sc = p->gsc, gw.njobs = p->n_gen;
+            KSW_HIP_TRY(launch_general_walk(gw, p->stream));
+        }
     }
     KSW_HIP_TRY(hipEventRecord(p->ev[2], p->stream));
     return KSW2_HIP_OK;
@@ -491,6 +504,7 @@ int deliver_slot(const Plan *p, int k, const KswOut &o, const uint32_t *cig_aren
         z->n_cigar = 0;
         return KSW2_HIP_OK;
     }
+    if (o.n_cigar < 0) return drop_cigar(z, km);  // failed Singletrack walk
     return deliver_cigar(z, cig_arena + p->cig_off[k], o.n_cigar, km);
 }
 
@@ -556,11 +570,6 @@ int run_batch_host(int n, const ksw2_pair_t *pairs, int8_t m, const int8_t *mat,
         if (!p.query || !p.target) return KSW2_HIP_EINVAL;
         if (p.qlen > 0x7fffffff - p.tlen + 1) {               // ksw2_extd2_avx.c:138-141
             ez[k].zdropped = 1;
-            continue;
-        }
-        if (p.flag & KSW_EZ_SINGLETRACK) {  // two-track traceback: not implemented
-            ez[k].zdropped = 1;
-            status = KSW2_HIP_EUNSUPPORTED;
             continue;
         }
         todo.push_back(k);

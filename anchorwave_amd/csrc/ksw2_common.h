@@ -161,4 +161,14 @@ KSW_HD inline uint64_t dir_bytes(int qlen, int tlen)
     return (uint64_t)pass_count(tlen) * pass_steps(qlen) * kWave * 16;
 }
 
+// Singletrack track byte of cell (i, j): 32 bytes per lane per step, the p
+// track (H(i,j) - H(i,j-1)) in the first 16 and the p2 track
+// (H(i,j) - H(i-1,j)) in the next 16, each in the direction-byte order;
+// the stored byte is the negated value.  Twice dir_bytes() per pair.
+KSW_HD inline uint64_t track_index(int track, int i, int j, int qlen)
+{
+    const uint64_t d = dir_index(i, j, qlen);
+    return (d >> 4) * 32 + (uint64_t)track * 16 + (d & 15);
+}
+
 }  // namespace kswhip

@@ -168,7 +168,7 @@ int Engine::start()
     if (st != KSW2_HIP_OK) return st;
     for (auto &s : streams_)
         if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return KSW2_HIP_EDEVICE;
-    resident_waves_ = std::max(1, stripe_resident_waves(false, false));
+    resident_waves_ = std::max(1, stripe_resident_waves(kGroupLeft));
     if (hipHostMalloc((void **)&h_stop_, sizeof(int32_t), hipHostMallocMapped) != hipSuccess) return KSW2_HIP_ENOMEM;
     *h_stop_ = 0;
     if (hipHostGetDevicePointer((void **)&d_stop_, h_stop_, 0) != hipSuccess) return KSW2_HIP_EDEVICE;
@@ -225,8 +225,6 @@ int Engine::submit(const ksw2_pair_t &p, ksw_extz_t *ez, bool copy, uint64_t *ti
         ez->zdropped = 1, why = KSW2_HIP_EINVAL;
     } else if (p.qlen > 0x7fffffff - p.tlen + 1) {
         ez->zdropped = 1;                                   // ksw2_extd2_avx.c:138-141
-    } else if (p.flag & KSW_EZ_SINGLETRACK) {
-        ez->zdropped = 1, why = KSW2_HIP_EUNSUPPORTED;      // two-track traceback: not implemented
     } else {
         t->general = takes_general(p, m_, cls_);
         t->footprint = pair_footprint(p, t->general);

@@ -24,6 +24,7 @@
 #include <stdint.h>
 #include "ksw2_common.h"
 #include "ksw2_kernels.h"
+#include "ksw2_walk.h"
 
 namespace kswhip {
 
@@ -31,7 +32,7 @@ namespace {
 
 constexpr int kMaxWaves = 16;
 constexpr int kFlagScoreOnly = 0x01, kFlagRight = 0x02, kFlagGeneric = 0x04, kFlagApproxMax = 0x08,
-              kFlagApproxDrop = 0x10;
+              kFlagApproxDrop = 0x10, kFlagExtzOnly = 0x40, kFlagSemiglobalEnd = 0x800, kFlagSingletrack = 0x1000;
 
 __device__ __forceinline__ int w8(int v) { return (int)(int8_t)(uint8_t)(v & 0xff); }
 
@@ -85,6 +86,9 @@ __global__ void __launch_bounds__(kWave * kMaxWaves) ksw_general_kernel(GenArgs 
     const int tlenB = jb.tlenB, qlenB = jb.qlenB;
     const bool with_cigar = !(flag & kFlagScoreOnly), right = flag & kFlagRight, generic = flag & kFlagGeneric;
     const bool approx = flag & kFlagApproxMax;
+    // Singletrack: the new v and u of every cell instead of its direction
+    // byte (minimap2/ksw2_extd2_avx.c:432-436), in two row arrays
+    const bool tracks = with_cigar && (flag & kFlagSingletrack);
     const GenLayout L = gen_layout(tlenB, qlenB);
     uint8_t *base = a.state + jb.st_off;
     int8_t *u = (int8_t *)(base + L.u), *y = (int8_t *)(base + L.y), *y2 = (int8_t *)(base + L.y2);
@@ -97,6 +101,7 @@ __global__ void __launch_bounds__(kWave * kMaxWaves) ksw_general_kernel(GenArgs 
     uint8_t *p = a.dir + jb.dir_off;
     int32_t *off = a.offs + jb.off_off, *off_end = off + (qlen + tlen - 1);
     const int ncolB = jb.ncolB;
+    uint8_t *p2 = p + (uint64_t)(qlen + tlen - 1) * (uint64_t)ncolB;
 
     const int q = sc.q, e = sc.e, q2 = sc.q2, e2 = sc.e2;
     const int qe8 = w8(q + e), qe28 = w8(q2 + e2);
@@ -293,7 +298,12 @@ __global__ void __launch_bounds__(kWave * kMaxWaves) ksw_general_kernel(GenArgs 
             }
             u[t] = (int8_t)un, Vc[t] = (int8_t)vn, Xc[t] = (int8_t)xn;
             y[t] = (int8_t)yn, X2c[t] = (int8_t)x2n, y2[t] = (int8_t)y2n;
-            if (with_cigar) p[(uint64_t)r * ncolB + (uint64_t)(t - stB)] = (uint8_t)d;
+            if (tracks) {
+                p[(uint64_t)r * ncolB + (uint64_t)(t - stB)] = (uint8_t)vn;
+                p2[(uint64_t)r * ncolB + (uint64_t)(t - stB)] = (uint8_t)un;
+            } else if (with_cigar) {
+                p[(uint64_t)r * ncolB + (uint64_t)(t - stB)] = (uint8_t)d;
+            }
             // ---- H bookkeeping (ksw2_extd2_sse.c:332-375)
             if (!approx) {
                 if (t == en0) F.u_en0 = un, F.v_en0 = vn, F.h_old_en0 = H[t];
@@ -354,6 +364,7 @@ __global__ void __launch_bounds__(64) ksw_general_backtrack_kernel(GenBacktrackA
     const GenJob jb = a.jobs[k];
     KswOut o = a.out[jb.out];
     const int flag = jb.flag, qlen = jb.qlen, tlen = jb.tlen;
+    if ((flag & kFlagSingletrack) && !(flag & kFlagScoreOnly)) return;  // ksw_general_walk_kernel
     int i0 = -1, j0 = -1;
     if (!(flag & kFlagScoreOnly)) {
         if (!o.stopped && !o.zdropped && (flag & 0x800)) {          // SEMIGLOBAL_END
@@ -408,6 +419,59 @@ __global__ void __launch_bounds__(64) ksw_general_backtrack_kernel(GenBacktrackA
     a.out[jb.out] = o;
 }
 
+// Singletrack tracks of one general pair: reference layout, two row arrays
+// of n_col*64 bytes per anti-diagonal, off[]/off_end[] per anti-diagonal.
+struct GenTracks {
+    const uint8_t *p, *p2;
+    const int32_t *off, *off_end;
+    const uint8_t *qs, *ts;
+    const int8_t *mat;
+    int32_t ncolB, m;
+    __device__ int32_t get(int track, int i, int j) const
+    {
+        const int d = i + j;
+        const int o = off[d];
+        // (a stored range always fits its row; the width test only keeps a
+        // damaged row from reading outside its own bytes)
+        if (i < o || i > off_end[d] || i - o >= ncolB) return KSW_NEG_INF_I;
+        return (int32_t)(int8_t)(track ? p2 : p)[(uint64_t)d * ncolB + (uint64_t)(i - o)];
+    }
+    __device__ int forced(int i, int j) const
+    {
+        const int d = i + j;
+        return i < off[d] ? 1 : i > off_end[d] ? 2 : 0;
+    }
+    __device__ int32_t cost(int i, int j) const { return mat[(ts[i] * m + qs[j]) & (kMaxM * kMaxM - 1)]; }
+};
+
+// One wavefront per general pair with KSW_EZ_SINGLETRACK: the dispatch of
+// minimap2/ksw2_extd2_avx.c:752-760 and the Singletrack walk (track_walk);
+// other pairs return at once.
+__global__ void __launch_bounds__(64) ksw_general_walk_kernel(GenWalkArgs a)
+{
+    const GenJob jb = a.jobs[blockIdx.x];
+    const int flag = __builtin_amdgcn_readfirstlane(jb.flag);
+    if (!(flag & kFlagSingletrack) || (flag & kFlagScoreOnly)) return;
+    KswOut o = a.out[jb.out];
+    const int qlen = jb.qlen, tlen = jb.tlen;
+    int n = 0;
+    if (!o.stopped && !o.zdropped && !(flag & (kFlagExtzOnly | kFlagSemiglobalEnd))) {
+        GenTracks A;
+        A.p = a.dir + jb.dir_off;
+        A.p2 = A.p + (uint64_t)(qlen + tlen - 1) * (uint64_t)jb.ncolB;
+        A.off = a.offs + jb.off_off, A.off_end = A.off + (qlen + tlen - 1);
+        A.qs = a.seq + jb.q_off, A.ts = a.seq + jb.t_off;
+        A.mat = a.sc.mat, A.ncolB = jb.ncolB, A.m = a.sc.m;
+        const WalkGaps g = {-a.sc.k.q, -a.sc.k.e, -a.sc.k.q2, -a.sc.k.e2};
+        n = track_walk(A, qlen, tlen, g, a.cig + jb.cig_off);
+        if (n < 0) o.zdropped = 1;
+    } else if (!o.stopped && !o.zdropped) {
+        o.zdropped = 1;
+    }
+    o.n_cigar = n;
+    if (threadIdx.x == 0) a.out[jb.out] = o;
+}
+
 hipError_t launch_general(const GenArgs &a, int threads, hipStream_t st)
 {
     if (a.njobs <= 0) return hipSuccess;
@@ -420,6 +484,13 @@ hipError_t launch_general_backtrack(const GenBacktrackArgs &a, hipStream_t st)
 {
     if (a.njobs <= 0) return hipSuccess;
     hipLaunchKernelGGL(ksw_general_backtrack_kernel, dim3((a.njobs + 63) / 64), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_general_walk(const GenWalkArgs &a, hipStream_t st)
+{
+    if (a.njobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ksw_general_walk_kernel, dim3(a.njobs), dim3(kWave), 0, st, a);
     return hipGetLastError();
 }
 

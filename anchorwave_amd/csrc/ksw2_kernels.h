@@ -46,13 +46,31 @@ struct GenBacktrackArgs {
     int njobs;
 };
 
+// Singletrack walk of general pairs: one wavefront per job (all general
+// jobs of the plan; those without KSW_EZ_SINGLETRACK return at once).
+struct GenWalkArgs {
+    const uint8_t *seq;
+    const uint8_t *dir;
+    const int32_t *offs;
+    uint32_t *cig;
+    const GenJob *jobs;
+    KswOut *out;
+    GenScore sc;
+    int njobs;
+};
+
 // General kernel: one workgroup of `threads` (64..1024) per job.
 hipError_t launch_general(const GenArgs &a, int threads, hipStream_t st);
 hipError_t launch_general_backtrack(const GenBacktrackArgs &a, hipStream_t st);
+hipError_t launch_general_walk(const GenWalkArgs &a, hipStream_t st);
+
+// Stripe-kernel variants ("groups" of a plan): direction bytes with left or
+// right gap alignment, score only, or the two Singletrack tracks.
+enum StripeGroup { kGroupLeft = 0, kGroupRight = 1, kGroupScoreOnly = 2, kGroupTracks = 3, kGroups = 4 };
 
 // Stripe kernel: one workgroup of `team` waves (1..kMaxTeam) per job.
-hipError_t launch_stripe(const StripeArgs &a, bool right, bool score_only, int team, hipStream_t st);
+hipError_t launch_stripe(const StripeArgs &a, int group, int team, hipStream_t st);
 // Waves of the stripe kernel that fit on the device at once.
-int stripe_resident_waves(bool right, bool score_only);
+int stripe_resident_waves(int group);
 
 }  // namespace kswhip

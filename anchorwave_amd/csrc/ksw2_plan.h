@@ -33,6 +33,10 @@ struct Progress {
 int run_batch_host(int n, const ksw2_pair_t *pairs, int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q2,
                    int8_t e2, ksw_extz_t *ez, void *km, const Progress *prog = nullptr);
 int deliver_cigar(ksw_extz_t *ez, const uint32_t *src, int n, void *km);
+int drop_cigar(ksw_extz_t *ez, void *km);
+// A pair whose CIGAR comes from the Singletrack walk over two stored tracks.
+inline bool track_pair(int flag) { return (flag & KSW_EZ_SINGLETRACK) && !(flag & KSW_EZ_SCORE_ONLY); }
+inline bool track_pair(const ksw2_pair_t &p) { return track_pair(p.flag); }
 void reset_ez(ksw_extz_t *ez);
 
 // Why a pair cannot take the fast (unbanded stripe) kernel; 0 when it can.
@@ -88,6 +92,7 @@ struct Plan {
     KswScore sc{};
     GenScore gsc{};
     int n = 0;                       // pairs in the plan
+    int n_tracks_gen = 0;            // general pairs with the Singletrack walk
     int n_fast = 0, n_gen = 0;       // slots [0, n_fast) stripe kernel, then general kernel
     int R = kRowsPerLane;            // rows per lane of the stripe kernel
     std::vector<KswJob> jobs;        // host copy, stripe-kernel slots

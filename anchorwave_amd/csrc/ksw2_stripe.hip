@@ -40,12 +40,16 @@
 //    H never exceeds 0, so ez->max stays 0), there is no band and no z-drop.
 //
 // Outputs per pair: the direction byte of every cell (bits as in
-// minimap2/ksw2.h:156-159, layout dir_index()), and the H bookkeeping of
-// minimap2/ksw2_extd2_sse.c:369-375 (score, mqe/mqe_t, mte/mte_q).
+// minimap2/ksw2.h:156-159, layout dir_index()), or, for KSW_EZ_SINGLETRACK,
+// the two difference tracks of every cell (the new v and u,
+// minimap2/ksw2_extd2_avx.c:432-436, layout track_index()); and the H
+// bookkeeping of minimap2/ksw2_extd2_sse.c:369-375 (score, mqe/mqe_t,
+// mte/mte_q).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "ksw2_common.h"
 #include "ksw2_kernels.h"
+#include "ksw2_walk.h"
 
 namespace kswhip {
 
@@ -56,6 +60,9 @@ constexpr int kDppWaveRor1 = 0x13C;
 constexpr int R = kRowsPerLane;  // rows per lane
 constexpr int SR = kWave * R;    // rows per stripe
 constexpr int kLag = kWave;      // steps the high stripe trails the low one
+
+// What a pass stores per cell.
+enum Mode { kDirs = 0, kScoreOnly = 1, kTracks = 2 };
 
 typedef short v2s __attribute__((ext_vector_type(2)));
 typedef unsigned short v2u __attribute__((ext_vector_type(2)));
@@ -225,7 +232,7 @@ __device__ __forceinline__ void stripe_rec(const v2s (&Us)[R], bool hi, int i0, 
 // stripe's bottom row.  Lanes compute every step; each half's left state is
 // (re)initialised at its first column and U is snapshot at its last one, so
 // the cells a half computes outside its columns never reach a result.
-template <bool RIGHT, bool SCORE_ONLY>
+template <bool RIGHT, int MODE>
 __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc, int job_id, int p, const KswJob &jb,
                                           int qlen, int tlen, int32_t *prog, int W)
 {
@@ -312,13 +319,17 @@ __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc,
         int d[R];
         if (!last) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) d[r] = cell2<RIGHT, SCORE_ONLY>(kc, TB[r], qb, X, X2, V, U[r], Y[r], Y2[r]);
+            for (int r = 0; r < R; ++r) {
+                d[r] = cell2<RIGHT, MODE != kDirs>(kc, TB[r], qb, X, X2, V, U[r], Y[r], Y2[r]);
+                if (MODE == kTracks) d[r] = bits(V);
+            }
         } else {
             // the last pass also follows V along row tlen-1
             v2s vl = V;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                d[r] = cell2<RIGHT, SCORE_ONLY>(kc, TB[r], qb, X, X2, V, U[r], Y[r], Y2[r]);
+                d[r] = cell2<RIGHT, MODE != kDirs>(kc, TB[r], qb, X, X2, V, U[r], Y[r], Y2[r]);
+                if (MODE == kTracks) d[r] = bits(V);
                 if (r == rho_last) vl = V;
             }
             const int jh = jl - kLag * h_last;
@@ -337,7 +348,7 @@ __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc,
             }
         }
         Xo = bits(X), X2o = bits(X2), Vo = bits(V), Qo = Qw;
-        if (!SCORE_ONLY) {
+        if (MODE == kDirs) {
             // 16 bytes per lane: byte 4g + 2h + (rho & 1) is row 2g + (rho & 1)
             // of half h
             uint4 w;
@@ -346,6 +357,27 @@ __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc,
             w.z = __builtin_amdgcn_perm((unsigned)d[5], (unsigned)d[4], 0x06020400u);
             w.w = __builtin_amdgcn_perm((unsigned)d[7], (unsigned)d[6], 0x06020400u);
             *(uint4 *)(dir + (((uint64_t)p * nsteps + c) * kWave + lane) * 16) = w;
+        } else if (MODE == kTracks) {
+            // 32 bytes per lane: the p track (new V) then the p2 track (new
+            // U), each in the byte order of the direction bytes; a value is
+            // stored as -8w >> 3 = -w (exact: both are multiples of 8)
+            int vt[R], ut[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                vt[r] = bits(vec(d[r]) >> (v2s){3, 3});
+                ut[r] = bits(U[r] >> (v2s){3, 3});
+            }
+            uint4 wv, wu;
+            wv.x = __builtin_amdgcn_perm((unsigned)vt[1], (unsigned)vt[0], 0x06020400u);
+            wv.y = __builtin_amdgcn_perm((unsigned)vt[3], (unsigned)vt[2], 0x06020400u);
+            wv.z = __builtin_amdgcn_perm((unsigned)vt[5], (unsigned)vt[4], 0x06020400u);
+            wv.w = __builtin_amdgcn_perm((unsigned)vt[7], (unsigned)vt[6], 0x06020400u);
+            wu.x = __builtin_amdgcn_perm((unsigned)ut[1], (unsigned)ut[0], 0x06020400u);
+            wu.y = __builtin_amdgcn_perm((unsigned)ut[3], (unsigned)ut[2], 0x06020400u);
+            wu.z = __builtin_amdgcn_perm((unsigned)ut[5], (unsigned)ut[4], 0x06020400u);
+            wu.w = __builtin_amdgcn_perm((unsigned)ut[7], (unsigned)ut[6], 0x06020400u);
+            uint4 *dst = (uint4 *)(dir + (((uint64_t)p * nsteps + c) * kWave + lane) * 32);
+            dst[0] = wv, dst[1] = wu;
         }
         if (!last) {
             // lane 63 appends the high stripe's bottom-row output (column
@@ -400,8 +432,33 @@ __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc,
 // CIGAR are then copied to pinned host memory and the pair's completion flag
 // is raised (system-scope release), so a caller can take this pair's result
 // while the launch still runs.
+//
+// With the two Singletrack tracks instead of direction bytes, the first wave
+// runs the Singletrack walk (track_walk) for a global alignment; with
+// KSW_EZ_EXTZ_ONLY or KSW_EZ_SEMIGLOBAL_END the result is zdropped with no
+// path (minimap2/ksw2_extd2_avx.c:752-760), and a failed walk reports
+// n_cigar = -1 (no CIGAR, zdropped).
+
+// Singletrack tracks of one stripe-kernel pair (layout of the tracks mode).
+struct StripeTracks {
+    const uint8_t *dir;
+    const uint8_t *qs, *ts;
+    uint64_t nsteps;
+    int32_t mch, mis;
+    __device__ int32_t get(int track, int i, int j) const
+    {
+        const int s = i / SR, pp = s >> 1, h = s & 1, k = (i % SR) / R, rho = i % R;
+        const uint64_t c = (uint64_t)(j + k + kLag * h);
+        const uint8_t b =
+            dir[((pp * nsteps + c) * kWave + k) * 32 + track * 16 + 4 * (rho >> 1) + 2 * h + (rho & 1)];
+        return -(int32_t)(int8_t)b;
+    }
+    __device__ int forced(int, int) const { return 0; }
+    __device__ int32_t cost(int i, int j) const { return ts[i] == qs[j] ? mch : mis; }
+};
+
 __device__ __forceinline__ void pair_epilogue(const StripeArgs &a, int job_id, const KswJob &jb,
-                                              uint64_t (*tile)[64])
+                                              uint64_t (*tile)[64], bool tracks, int *shared_n)
 {
     const int tid = threadIdx.x, nthr = blockDim.x;
     KswOut o = a.out[job_id];
@@ -421,7 +478,26 @@ __device__ __forceinline__ void pair_epilogue(const StripeArgs &a, int job_id, c
         o.mqe_t = best_i;
     }
     int i0 = -1, j0 = -1;
-    if (!(flag & 0x01)) {  // KSW_EZ_SCORE_ONLY: no path
+    int n = 0;
+    uint32_t *cig = a.cig + jb.cig_off;
+    if (tracks) {
+        if (!o.stopped && !o.zdropped && !(flag & (0x40 | 0x800))) {
+            if (tid < kWave) {
+                StripeTracks A;
+                A.dir = a.dir + jb.dir_off, A.qs = a.seq + jb.q_off, A.ts = a.seq + jb.t_off;
+                A.nsteps = (uint64_t)qlen + kWave - 1 + kLag;
+                A.mch = a.sc.mch, A.mis = a.sc.mis;
+                const WalkGaps g = {-a.sc.q, -a.sc.e, -a.sc.q2, -a.sc.e2};
+                const int nn = track_walk(A, qlen, tlen, g, cig);
+                if (tid == 0) *shared_n = nn;
+            }
+            __syncthreads();
+            n = *shared_n;
+            if (n < 0) o.zdropped = 1;
+        } else if (!o.stopped && !o.zdropped) {
+            o.zdropped = 1;
+        }
+    } else if (!(flag & 0x01)) {  // KSW_EZ_SCORE_ONLY: no path
         if (!o.stopped && !o.zdropped && (flag & 0x800)) {          // SEMIGLOBAL_END
             if (o.mqe > o.mte) o.score = o.mqe, i0 = o.mqe_t, j0 = qlen - 1;
             else o.score = o.mte, i0 = tlen - 1, j0 = o.mte_q;
@@ -433,8 +509,6 @@ __device__ __forceinline__ void pair_epilogue(const StripeArgs &a, int job_id, c
             i0 = o.max_t, j0 = o.max_q;
         }
     }
-    int n = 0;
-    uint32_t *cig = a.cig + jb.cig_off;
     if (i0 >= 0 || j0 >= 0) {
         const uint8_t *dir = a.dir + jb.dir_off;
         const uint64_t nsteps = (uint64_t)qlen + kWave - 1 + kLag;
@@ -500,6 +574,7 @@ __device__ __forceinline__ void pair_epilogue(const StripeArgs &a, int job_id, c
     if (a.host_flag) {
         __syncthreads();
         uint32_t *hc = a.host_cig + jb.cig_off;
+        if (n < 0) n = 0;
         for (int x = tid; x < n; x += nthr) hc[x] = cig[x];
         if (tid == 0) a.host_out[job_id] = o;
         __threadfence_system();
@@ -511,11 +586,12 @@ __device__ __forceinline__ void pair_epilogue(const StripeArgs &a, int job_id, c
 // One workgroup (a team of blockDim.x / 64 waves) per pair of this launch's
 // job list; wave w computes passes w, w+W, ...; then the workgroup runs the
 // pair's epilogue (mqe, start cell, backtrack).
-template <bool RIGHT, bool SCORE_ONLY>
+template <bool RIGHT, int MODE>
 __global__ void __launch_bounds__(kWave * kMaxTeam) ksw_stripe_kernel(StripeArgs a)
 {
     __shared__ int32_t prog[kMaxTeam];
     __shared__ uint64_t tile[8][64];
+    __shared__ int shared_n;
     const int W = __builtin_amdgcn_readfirstlane(blockDim.x / kWave);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int job = __builtin_amdgcn_readfirstlane(a.order[blockIdx.x]);
@@ -525,34 +601,38 @@ __global__ void __launch_bounds__(kWave * kMaxTeam) ksw_stripe_kernel(StripeArgs
     const KswJob jb = a.jobs[job];
     const int qlen = __builtin_amdgcn_readfirstlane(jb.qlen), tlen = __builtin_amdgcn_readfirstlane(jb.tlen);
     const int npass = ((tlen + SR - 1) / SR + 1) / 2;
-    for (int p = w; p < npass; p += W) pass_body<RIGHT, SCORE_ONLY>(a, kc, job, p, jb, qlen, tlen, prog, W);
+    for (int p = w; p < npass; p += W) pass_body<RIGHT, MODE>(a, kc, job, p, jb, qlen, tlen, prog, W);
     __syncthreads();  // every pass written: stripe partials, score / mte, direction bytes
-    pair_epilogue(a, job, jb, tile);
+    pair_epilogue(a, job, jb, tile, MODE == kTracks, &shared_n);
 }
 
-template <bool RIGHT, bool SCORE_ONLY>
+template <bool RIGHT, int MODE>
 static hipError_t launch_stripe_t(const StripeArgs &a, int team, hipStream_t st)
 {
-    hipLaunchKernelGGL((ksw_stripe_kernel<RIGHT, SCORE_ONLY>), dim3(a.njobs), dim3(kWave * team), 0, st, a);
+    hipLaunchKernelGGL((ksw_stripe_kernel<RIGHT, MODE>), dim3(a.njobs), dim3(kWave * team), 0, st, a);
     return hipGetLastError();
 }
 
-hipError_t launch_stripe(const StripeArgs &a, bool right, bool score_only, int team, hipStream_t st)
+hipError_t launch_stripe(const StripeArgs &a, int group, int team, hipStream_t st)
 {
     if (a.njobs <= 0) return hipSuccess;
     if (team < 1 || team > kMaxTeam) return hipErrorInvalidValue;
-    if (score_only) return launch_stripe_t<false, true>(a, team, st);
-    if (right) return launch_stripe_t<true, false>(a, team, st);
-    return launch_stripe_t<false, false>(a, team, st);
+    switch (group) {
+    case kGroupLeft: return launch_stripe_t<false, kDirs>(a, team, st);
+    case kGroupRight: return launch_stripe_t<true, kDirs>(a, team, st);
+    case kGroupScoreOnly: return launch_stripe_t<false, kScoreOnly>(a, team, st);
+    case kGroupTracks: return launch_stripe_t<false, kTracks>(a, team, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
-template <bool RIGHT, bool SCORE_ONLY>
+template <bool RIGHT, int MODE>
 static int resident_t()
 {
     int per_cu = 0, dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 1024;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ksw_stripe_kernel<RIGHT, SCORE_ONLY>, kWave, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ksw_stripe_kernel<RIGHT, MODE>, kWave, 0) !=
             hipSuccess ||
         per_cu <= 0)
         return prop.multiProcessorCount * 4;
@@ -560,13 +640,19 @@ static int resident_t()
 }
 
 // Cached per variant: the occupancy query reads the device properties.
-int stripe_resident_waves(bool right, bool score_only)
+int stripe_resident_waves(int group)
 {
-    static int cache[3] = {0, 0, 0};
-    const int v = score_only ? 2 : right ? 1 : 0;
-    if (cache[v] > 0) return cache[v];
-    const int n = score_only ? resident_t<false, true>() : right ? resident_t<true, false>() : resident_t<false, false>();
-    __atomic_store_n(&cache[v], n, __ATOMIC_RELAXED);
+    static int cache[kGroups] = {};
+    if (group < 0 || group >= kGroups) group = kGroupLeft;
+    if (cache[group] > 0) return cache[group];
+    int n = 0;
+    switch (group) {
+    case kGroupRight: n = resident_t<true, kDirs>(); break;
+    case kGroupScoreOnly: n = resident_t<false, kScoreOnly>(); break;
+    case kGroupTracks: n = resident_t<false, kTracks>(); break;
+    default: n = resident_t<false, kDirs>(); break;
+    }
+    __atomic_store_n(&cache[group], n, __ATOMIC_RELAXED);
     return n;
 }
 

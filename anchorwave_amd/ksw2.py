@@ -37,6 +37,7 @@ STATUS = {0: "ok", -1: "invalid argument", -2: "unsupported", -3: "out of memory
 # Symbols include/ksw2_hip.h declares; tests check the library exports each.
 EXPORTS = (
     "ksw_extd2_hip",
+    "ksw_extd2_singletrack_hip",
     "ksw_extd2_hip_batch",
     "ksw2_hip_last_error",
     "ksw2_hip_error_string",
@@ -131,6 +132,7 @@ def lib():
         L.ksw_extd2_hip.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int8,
                                     C.c_void_p, C.c_int8, C.c_int8, C.c_int8, C.c_int8, C.c_int,
                                     C.c_int, C.c_int, C.c_int, C.POINTER(KswExtz)]
+        L.ksw_extd2_singletrack_hip.argtypes = L.ksw_extd2_hip.argtypes
         L.ksw_extd2_hip.restype = None
         L.ksw_extd2_hip_batch.argtypes = [C.c_int, C.POINTER(KswPair), C.c_int8, C.c_void_p, C.c_int8,
                                           C.c_int8, C.c_int8, C.c_int8, C.POINTER(KswExtz)]
@@ -230,6 +232,25 @@ def extd2(query, target, mat=None, m=5, q=8, e=2, q2=75, e2=1, w=-1, zdrop=-1, e
         _libc.free(C.cast(ez.cigar, C.c_void_p))
     if check and st != 0:
         raise KswError(f"ksw_extd2_hip: {STATUS.get(st, st)}")
+    return res
+
+
+def extd2_singletrack(query, target, mat=None, m=5, q=8, e=2, q2=75, e2=1, w=-1, zdrop=-1, end_bonus=0,
+                      flag=0, check=True) -> Result:
+    """One call of ``ksw_extd2_singletrack_hip`` (km = NULL): ``extd2`` with
+    KSW_EZ_SINGLETRACK, through its own entry point."""
+    L = lib()
+    qs, ts = _u8(query), _u8(target)
+    mat = np.ascontiguousarray(np.asarray(default_matrix() if mat is None else mat, dtype=np.int8))
+    ez = KswExtz()
+    L.ksw_extd2_singletrack_hip(None, len(qs), qs.ctypes.data, len(ts), ts.ctypes.data, m, mat.ctypes.data,
+                                q, e, q2, e2, w, zdrop, end_bonus, flag, C.byref(ez))
+    st = L.ksw2_hip_last_error()
+    res = Result.from_ez(ez)
+    if ez.cigar:
+        _libc.free(C.cast(ez.cigar, C.c_void_p))
+    if check and st != 0:
+        raise KswError(f"ksw_extd2_singletrack_hip: {STATUS.get(st, st)}")
     return res
 
 
@@ -349,6 +370,29 @@ class Plan:
         c = j + k + 64 * h
         idx = ((p * nsteps + c) * 64 + k) * 16 + 4 * (rho // 2) + 2 * h + rho % 2
         return buf[idx]
+
+    def tracks(self, index: int, qlen: int, tlen: int) -> np.ndarray:
+        """Singletrack tracks of pair ``index`` (flag KSW_EZ_SINGLETRACK) as a
+        (2, tlen, qlen) int8 array: p = H(i,j)-H(i,j-1) and p2 =
+        H(i,j)-H(i-1,j).  Same cell order as ``dirs``, 32 bytes per lane and
+        step (the negated p track, then the negated p2 track)."""
+        npass = ((tlen + 511) // 512 + 1) // 2
+        nsteps = qlen + 127
+        nbytes = npass * nsteps * 64 * 32
+        buf = np.zeros(nbytes, dtype=np.uint8)
+        r = C.c_int()
+        st = lib().ksw2_hip_plan_dirs(self.h, index, buf.ctypes.data, nbytes, C.byref(r))
+        if st != 0:
+            raise KswError(f"ksw2_hip_plan_dirs: {STATUS.get(st, st)}")
+        i = np.arange(tlen)[:, None]
+        j = np.arange(qlen)[None, :]
+        s, rem = i // 512, i % 512
+        p, h = s // 2, s % 2
+        k, rho = rem // 8, rem % 8
+        c = j + k + 64 * h
+        idx = ((p * nsteps + c) * 64 + k) * 32 + 4 * (rho // 2) + 2 * h + rho % 2
+        raw = buf.view(np.int8)
+        return -np.stack([raw[idx], raw[idx + 16]]).astype(np.int16)
 
     def close(self):
         if self.h:

@@ -35,7 +35,7 @@
 #define KSW_EZ_SPLICE_REV     0x200
 #define KSW_EZ_SPLICE_FLANK   0x400
 #define KSW_EZ_SEMIGLOBAL_END 0x800  /* traceback from the better sequence end */
-#define KSW_EZ_SINGLETRACK    0x1000 /* reference-internal two-track traceback */
+#define KSW_EZ_SINGLETRACK    0x1000 /* two-track (Singletrack) traceback */
 
 #ifdef __cplusplus
 extern "C" {
@@ -86,6 +86,21 @@ void ksw_extd2_hip(void *km, int qlen, const uint8_t *query, int tlen, const uin
                    int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q2, int8_t e2,
                    int w, int zdrop, int end_bonus, int flag, ksw_extz_t *ez);
 
+/*
+ * Singletrack variant, same contract as ksw_extd2_singletrack_avx512
+ * (minimap2/ksw2_singletrack.hpp:41-46, body ksw2_singletrack.cpp:191-201):
+ * ksw_extd2_hip with KSW_EZ_SINGLETRACK added to flag.  The DP stores the two
+ * score-difference tracks instead of direction bytes and the CIGAR comes from
+ * the Singletrack walk (ksw2_singletrack.cpp:63-176), always in forward
+ * order.  Only a global alignment is walked: with KSW_EZ_EXTZ_ONLY or
+ * KSW_EZ_SEMIGLOBAL_END the result is zdropped with no CIGAR, and a walk that
+ * finds no path frees ez->cigar, leaves it NULL and sets zdropped
+ * (minimap2/ksw2_extd2_avx.c:752-760).
+ */
+void ksw_extd2_singletrack_hip(void *km, int qlen, const uint8_t *query, int tlen, const uint8_t *target,
+                               int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q2, int8_t e2,
+                               int w, int zdrop, int end_bonus, int flag, ksw_extz_t *ez);
+
 /* Status of the last entry point called on this host thread, and its text. */
 int ksw2_hip_last_error(void);
 const char *ksw2_hip_error_string(int status);
@@ -96,8 +111,8 @@ int ksw2_hip_init(int device);
 
 /* Host-only (no GPU needed): which device kernel a ksw_extd2_hip call with
  * these arguments takes.  0 = the unbanded fast kernel; otherwise a bit mask
- * of why not: 1 band narrower than the matrix, 2 z-drop on, 4 APPROX_MAX or
- * SINGLETRACK flag, 8 scoring outside the fast kernel's exactness domain,
+ * of why not: 1 band narrower than the matrix, 2 z-drop on, 4 APPROX_MAX
+ * flag, 8 scoring outside the fast kernel's exactness domain,
  * 16 wildcard code m-1 present in match/mismatch mode.  -1 when the
  * reference returns before aligning (m <= 1, empty sequence, or
  * -min(mat) > 2(q+e), minimap2/ksw2_extd2_sse.c:81,105). */
@@ -180,7 +195,11 @@ void ksw2_hip_server_destroy(ksw2_hip_server *server);
  * lane k = (i%512)/8, row rho = i%8, and cell (i, j) is byte
  * ((p*(qlen+127) + j+k+64h)*64 + k)*16 + 4*(rho/2) + 2h + rho%2) into `out`
  * and reports the rows per lane (8); `bytes` must be at least
- * ceil(ceil(tlen/512)/2)*(qlen+127)*1024.  Returns a status code. */
+ * ceil(ceil(tlen/512)/2)*(qlen+127)*1024.  A KSW_EZ_SINGLETRACK pair has
+ * twice that: per lane and step 32 bytes, the negated p track
+ * (H(i,j)-H(i,j-1)) in the first 16 and the negated p2 track
+ * (H(i,j)-H(i-1,j)) in the next 16, each in the byte order above.
+ * Returns a status code. */
 int ksw2_hip_plan_dirs(const ksw2_hip_plan *plan, int index, uint8_t *out, size_t bytes, int *R);
 
 #ifdef __cplusplus

@@ -1,0 +1,130 @@
+"""KSW_EZ_SINGLETRACK on the device vs the oracle (bit-exact: every ez field
+and the CIGAR, including failed walks: no CIGAR and zdropped).
+
+The oracle's Singletrack path is pinned by the reference's own vectors
+(tests/golden, written by ksw_extd2_avx512 with the flag), a live
+differential against the compiled reference, and its tracks against an
+independent Gotoh matrix (tests/test_oracle.py)."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+from anchorwave_amd import ksw2, workloads as W
+from helpers import diff, random_pairs, same
+
+pytestmark = pytest.mark.gpu
+
+ST = ksw2.KSW_EZ_SINGLETRACK
+FLAGS = [ST, ST | ksw2.KSW_EZ_RIGHT, ST | ksw2.KSW_EZ_REV_CIGAR, ST | ksw2.KSW_EZ_EXTZ_ONLY,
+         ST | ksw2.KSW_EZ_SEMIGLOBAL_END, ST | ksw2.KSW_EZ_SCORE_ONLY]
+
+
+def check_batch(pairs, flag=ST, **kw):
+    got = ksw2.extd2_batch(pairs, flag=flag, **kw)
+    bad = []
+    for k, ((q, t), g) in enumerate(zip(pairs, got)):
+        ref = O.run_oracle(q, t, flag=flag, **kw)
+        if not same(g, ref):
+            bad.append(f"pair {k} qlen={len(q)} tlen={len(t)} flag={flag:#x}: {diff(g, ref)}")
+    assert not bad, "\n".join(bad[:5]) + f"\n({len(bad)} of {len(pairs)} differ)"
+    return got
+
+
+@pytest.mark.parametrize("flag", FLAGS)
+def test_singletrack_random_fast_kernel(flag):
+    pairs = random_pairs(500 + flag % 97, 60, 1, 900)
+    assert all(ksw2.route(q, t, flag=flag) == 0 for q, t in pairs)
+    check_batch(pairs, flag=flag)
+
+
+def test_singletrack_walk_failures_and_successes():
+    """Under this scoring (mismatch -3, gaps 6/1 and 20/1; general kernel)
+    the walk fails on some unrelated pairs (no gap length fits): no CIGAR,
+    zdropped, as the reference.  Both outcomes are exact."""
+    pairs = random_pairs(510, 80, 20, 400, mutated=0.3)
+    got = check_batch(pairs, mat=O.default_matrix(0, -3), q=6, e=1, q2=20, e2=1)
+    failed = sum(1 for g in got if g.n_cigar == 0)
+    assert 0 < failed < len(pairs)
+
+
+def test_singletrack_general_kernel():
+    """Bands, z-drop, wildcards, generic matrices and other scorings run on
+    the general kernel with the reference-layout tracks and the banded walk
+    (forced moves at off/off_end, out-of-band cells poison gap sums)."""
+    rng = np.random.default_rng(520)
+    pairs, kws = random_pairs(521, 70, 1, 500), []
+    for k, (q, t) in enumerate(pairs):
+        w = -1 if k % 3 == 0 else int(rng.integers(0, 120))
+        zdrop = -1 if k % 4 else int(rng.integers(20, 150))
+        kws.append((w, zdrop))
+    flags = [ST, ST | ksw2.KSW_EZ_RIGHT, ST | ksw2.KSW_EZ_GENERIC_SC, ST | ksw2.KSW_EZ_EXTZ_ONLY]
+    tuples = [(q, t, flags[k % 4], w, z) for k, ((q, t), (w, z)) in enumerate(zip(pairs, kws))]
+    for mat, pen in ((O.default_matrix(), (8, 2, 75, 1)), (O.default_matrix(1, -3), (5, 1, 30, 0))):
+        got = ksw2.extd2_batch(tuples, mat=mat, q=pen[0], e=pen[1], q2=pen[2], e2=pen[3])
+        for k, ((q, t, f, w, z), g) in enumerate(zip(tuples, got)):
+            ref = O.run_oracle(q, t, mat=mat, q=pen[0], e=pen[1], q2=pen[2], e2=pen[3], w=w, zdrop=z, flag=f)
+            assert same(g, ref), f"pair {k} w={w} zdrop={z} flag={f:#x}: {diff(g, ref)}"
+
+
+def test_singletrack_wildcards():
+    rng = np.random.default_rng(530)
+    pairs = []
+    for q, t in random_pairs(531, 30, 50, 600):
+        t = t.copy()
+        t[rng.integers(0, len(t), max(1, len(t) // 40))] = 4
+        pairs.append((q, t))
+    check_batch(pairs, flag=ST | ksw2.KSW_EZ_GENERIC_SC)
+    check_batch(pairs, flag=ST)
+
+
+def test_singletrack_forced_general(monkeypatch):
+    """Fast-domain pairs forced onto the general kernel give the same results."""
+    monkeypatch.setenv("KSW2_HIP_FORCE_GENERAL", "1")
+    check_batch(random_pairs(540, 40, 1, 700))
+
+
+@pytest.mark.parametrize("team", [1, 3, 16])
+def test_singletrack_team_sizes(team, monkeypatch):
+    monkeypatch.setenv("KSW2_HIP_TEAM", str(team))
+    check_batch(random_pairs(550 + team, 10, 600, 4000))
+
+
+def test_singletrack_tracks_every_cell():
+    """Every stored track value (not only those on the path) equals the
+    reference's value for that cell."""
+    pairs = random_pairs(560, 5, 300, 1400) + [W.fixed(seed=2, n=1)[0]]
+    plan = ksw2.Plan(pairs, flag=ST)
+    plan.run()
+    got = plan.fetch()
+    for k, (q, t) in enumerate(pairs):
+        ref, rt = O.run_oracle(q, t, flag=ST, want_dirs=True)
+        gt = plan.tracks(k, len(q), len(t))
+        nbad = int((gt != rt).sum())
+        assert nbad == 0, f"pair {k}: {nbad} cells differ, first {np.argwhere(gt != rt)[:4].tolist()}"
+        assert same(got[k], ref)
+    plan.close()
+
+
+def test_singletrack_benchmark_pairs():
+    """Benchmark-shaped pairs (2 kbp, 5 kbp, multi-stripe uneven) against the
+    compiled reference when present, else the oracle."""
+    from concurrent.futures import ThreadPoolExecutor
+    pairs = W.fixed(seed=2, n=24) + W.single(seed=1) + \
+        [p for p in W.uneven(seed=3, n=300) if len(p[0]) * len(p[1]) <= 6e7][:24]
+    fn = (lambda p: O.run_ref(p[0], p[1], flag=ST)) if O.ref_available() else \
+        (lambda p: O.run_oracle(p[0], p[1], flag=ST))
+    got = ksw2.extd2_batch(pairs, flag=ST)
+    with ThreadPoolExecutor(16) as ex:
+        exp = list(ex.map(fn, pairs))
+    bad = [f"pair {k} qlen={len(q)} tlen={len(t)}: {diff(g, r)}"
+           for k, ((q, t), g, r) in enumerate(zip(pairs, got, exp)) if not same(g, r)]
+    assert not bad, "\n".join(bad[:5]) + f"\n({len(bad)} of {len(pairs)} differ)"
+
+
+def test_singletrack_entry_point_and_one_call():
+    """ksw_extd2_singletrack_hip equals ksw_extd2_hip with the flag, and the
+    one-pair call equals the batched one."""
+    for q, t in random_pairs(570, 12, 1, 800):
+        a = ksw2.extd2_singletrack(q, t)
+        b = ksw2.extd2(q, t, flag=ST)
+        assert same(a, b) and same(a, O.run_oracle(q, t, flag=ST))
