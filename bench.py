@@ -58,16 +58,16 @@ def read_traffic(name: str):
         return None
 
 
-def cpu_baseline(pairs, threads: int):
+def cpu_baseline(pairs, threads: int, flag: int = 0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py as O  # checker / baseline only
 
     if O.ref_available():
         kind = "reference"
-        fn = lambda p: O.run_ref(p[0], p[1], mat=W_MAT, **SC)  # noqa: E731
+        fn = lambda p: O.run_ref(p[0], p[1], mat=W_MAT, flag=flag, **SC)  # noqa: E731
     else:
         kind = "port"
-        fn = lambda p: O.run_oracle(p[0], p[1], mat=W_MAT, **SC)  # noqa: E731
+        fn = lambda p: O.run_oracle(p[0], p[1], mat=W_MAT, flag=flag, **SC)  # noqa: E731
     # largest-cells-first, one pair per task
     order = sorted(range(len(pairs)), key=lambda k: -len(pairs[k][0]) * len(pairs[k][1]))
     res = [None] * len(pairs)
@@ -77,6 +77,35 @@ def cpu_baseline(pairs, threads: int):
             res[k] = r
     dt = time.perf_counter() - t0
     return kind, dt, res
+
+
+def singletrack_leg(args, pairs, cells) -> dict:
+    """The fixed batch with KSW_EZ_SINGLETRACK (two difference tracks per
+    cell, the Singletrack walk): resident inputs, timed like the headline,
+    checked against the CPU reference with the same flag."""
+    plan = ksw2.Plan(pairs, mat=W_MAT, flag=ksw2.KSW_EZ_SINGLETRACK, **SC)
+    plan.run()
+    dp = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.run()
+        dp.append(plan.timing()[0])
+    wall = time.perf_counter() - t0
+    res = plan.fetch()
+    plan.close()
+    gcups = cells / (wall / args.steps) / 1e9
+    out = {"gcups": round(gcups, 2), "ms_per_step": round(wall * 1e3 / args.steps, 3),
+           "device_ms": round(float(np.mean(dp)), 3),
+           "track_bytes_per_cell": 2, "walk": "wave-parallel Singletrack walk in the stripe kernel's epilogue"}
+    log(f"[bench] singletrack: {gcups:.1f} GCUPS ({out['device_ms']:.2f} ms device)")
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        kind, dt, cpu_res = cpu_baseline(pairs, threads, flag=ksw2.KSW_EZ_SINGLETRACK)
+        exact = sum(1 for g, c in zip(res, cpu_res) if g.__dict__ == c.__dict__)
+        out["exact"] = {"agree": exact, "of": len(pairs), "frac": exact / len(pairs)}
+        out["cpu"] = {"kind": kind, "gcups": round(cells / dt / 1e9, 3), "cores": threads}
+        log(f"[bench] singletrack CPU {cells / dt / 1e9:.2f} GCUPS; exact {exact}/{len(pairs)}")
+    return out
 
 
 def uneven_leg(args) -> dict:
@@ -225,6 +254,8 @@ def main() -> int:
                     help="tasks per offered-load level in the serving leg (0: skip it)")
     ap.add_argument("--loads", default="0.2,0.5,0.8,1.0,1.2",
                     help="offered loads of the serving leg, as fractions of measured capacity")
+    ap.add_argument("--singletrack", type=int, default=1,
+                    help="also time the fixed batch with KSW_EZ_SINGLETRACK (0: skip)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (PMC byte-counter scale)")
     args = ap.parse_args()
@@ -312,6 +343,8 @@ def main() -> int:
         "one_at_a_time_gcups": None if single is None else round(single, 3),
         "batched_over_one_at_a_time": None if not single else round(gcups / single, 1),
     }
+    if args.singletrack:
+        out["singletrack"] = singletrack_leg(args, pairs, cells)
     if args.uneven > 0:
         out["uneven"] = uneven_leg(args)
     if args.serve > 0:
