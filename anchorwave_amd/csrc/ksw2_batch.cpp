@@ -131,6 +131,17 @@ void make_gen_score(int8_t m, const int8_t *mat, const KswScore &sc, const Score
 // Threads of the general kernel's workgroup for one pair: about four cells
 // of the widest anti-diagonal per thread, a power of two in [64, 1024].
 // KSW2_HIP_GEN_THREADS forces a size (tuning and tests).
+// LDS ring slots of the general kernel for one pair (0: state in HBM).
+// APPROX_MAX pairs keep their state in HBM: their path tracker may read a
+// position far behind the band.  KSW2_HIP_GEN_RING=0 turns the rings off
+// (tests of both forms).
+int gen_ring(const ksw2_pair_t &p)
+{
+    const char *ev = getenv("KSW2_HIP_GEN_RING");
+    if ((ev && atoi(ev) == 0) || (p.flag & KSW_EZ_APPROX_MAX)) return 0;
+    return gen_ring_slots(std::min(std::min(p.qlen, p.tlen), norm_w(p) + 1) + 128);
+}
+
 int gen_threads(const ksw2_pair_t &p)
 {
     const char *ev = getenv("KSW2_HIP_GEN_THREADS");
@@ -245,23 +256,30 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
                 order.insert(order.end(), members.begin(), members.end());
             }
     }
-    // general-kernel launches: per block size, most anti-diagonal cells first
+    // general-kernel launches: per block size and LDS ring size, most
+    // anti-diagonal cells first
     std::vector<int32_t> gorder;
     {
-        std::vector<int> thr(p->n_gen);
-        for (int gk = 0; gk < p->n_gen; ++gk) thr[gk] = gen_threads(pairs[p->order[p->n_fast + gk]]);
-        for (int t = 1024; t >= 64; t >>= 1) {
-            std::vector<int> members;
-            for (int gk = 0; gk < p->n_gen; ++gk)
-                if (thr[gk] == t) members.push_back(gk);
-            if (members.empty()) continue;
-            std::stable_sort(members.begin(), members.end(), [&](int x, int y) {
-                const GenJob &a = p->gjobs[x], &b = p->gjobs[y];
-                return (double)(a.qlen + a.tlen) * a.ncolB > (double)(b.qlen + b.tlen) * b.ncolB;
-            });
-            p->glaunches.push_back({t, (int)gorder.size(), (int)members.size(), t >= 512 ? 0 : t >= 256 ? 1 : 2});
-            gorder.insert(gorder.end(), members.begin(), members.end());
+        std::vector<int> thr(p->n_gen), ring(p->n_gen);
+        for (int gk = 0; gk < p->n_gen; ++gk) {
+            const ksw2_pair_t &pp = pairs[p->order[p->n_fast + gk]];
+            thr[gk] = gen_threads(pp);
+            ring[gk] = gen_ring(pp);
         }
+        for (int t = 1024; t >= 64; t >>= 1)
+            for (int rs = kGenRingMax; rs >= 0; rs = rs > 256 ? rs >> 1 : rs - 256) {
+                std::vector<int> members;
+                for (int gk = 0; gk < p->n_gen; ++gk)
+                    if (thr[gk] == t && ring[gk] == rs) members.push_back(gk);
+                if (members.empty()) continue;
+                std::stable_sort(members.begin(), members.end(), [&](int x, int y) {
+                    const GenJob &a = p->gjobs[x], &b = p->gjobs[y];
+                    return (double)(a.qlen + a.tlen) * a.ncolB > (double)(b.qlen + b.tlen) * b.ncolB;
+                });
+                p->glaunches.push_back({t, rs, (int)gorder.size(), (int)members.size(),
+                                        t >= 512 ? 0 : t >= 256 ? 1 : 2});
+                gorder.insert(gorder.end(), members.begin(), members.end());
+            }
     }
     p->seq_bytes = std::max<uint64_t>(so, 16), p->dir_bytes = std::max<uint64_t>(dof, 256);
     p->bnd_dwords = std::max<uint64_t>(bo, 64), p->cig_words = std::max<uint64_t>(co, 16);
@@ -419,7 +437,7 @@ int plan_launch(Plan *p, const Progress *prog)
     for (const Plan::GLaunch &l : p->glaunches) {
         hipStream_t st = lane(l.lane);
         if (!st) return KSW2_HIP_EDEVICE;
-        ga.order = p->d_gorder + l.off, ga.njobs = l.count;
+        ga.order = p->d_gorder + l.off, ga.njobs = l.count, ga.ring = l.ring;
         KSW_HIP_TRY(launch_general(ga, l.threads, st));
     }
     StripeArgs sa;

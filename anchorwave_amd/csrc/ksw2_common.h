@@ -93,6 +93,19 @@ KSW_HD inline GenLayout gen_layout(int tlenB, int qlenB)
     return L;
 }
 
+// LDS rings of the general kernel: ten int8 arrays and int32 H per slot.
+constexpr int kGenRingMax = 8192;
+KSW_HD inline uint32_t gen_ring_bytes(int ring) { return (uint32_t)ring * 14u; }
+// Ring slots for a pair whose band (plus the 64/16 rounding) spans at most
+// `window` positions: a power of two at least window + 384, or 0 when that
+// exceeds kGenRingMax (the state then stays in HBM).
+KSW_HD inline int gen_ring_slots(int window)
+{
+    int r = 256;
+    while (r < window + 384) r <<= 1;
+    return r <= kGenRingMax ? r : 0;
+}
+
 // The reference's direction-row width for 64-byte blocks
 // (minimap2/ksw2_extd2_avx.c:198-206): n_col_ = ceil(min(qlen, tlen, w+1)/64) + 1.
 KSW_HD inline int gen_ncolB(int qlen, int tlen, int w)

@@ -72,9 +72,18 @@ __device__ __forceinline__ int edge_i8(const KswScore &sc, int r) { return edge_
 
 }  // namespace
 
+// RING: the per-position state lives in LDS, in rings of a.ring slots
+// indexed by t mod a.ring, instead of the pair's state block in HBM.  The
+// band [stB, en16] moves right monotonically, so a ring a few hundred
+// slots wider than the band holds every position a cell reads (t-1, st16-1,
+// stB+15, en0-1); a slot is reset to the reference's initial values
+// (ksw2_extd2_avx.c:226-245) just before its position first enters.  The
+// cells compute exactly what they compute from HBM, at LDS latency.
+template <bool RING>
 __global__ void __launch_bounds__(kWave * kMaxWaves) ksw_general_kernel(GenArgs a)
 {
     __shared__ Fin fin[2];
+    extern __shared__ __align__(16) uint8_t gen_ring[];
     const int job = __builtin_amdgcn_readfirstlane(a.order[blockIdx.x]);
     const GenJob jb = a.jobs[job];
     const GenScore &gs = a.sc;
@@ -91,13 +100,19 @@ __global__ void __launch_bounds__(kWave * kMaxWaves) ksw_general_kernel(GenArgs 
     const bool tracks = with_cigar && (flag & kFlagSingletrack);
     const GenLayout L = gen_layout(tlenB, qlenB);
     uint8_t *base = a.state + jb.st_off;
-    int8_t *u = (int8_t *)(base + L.u), *y = (int8_t *)(base + L.y), *y2 = (int8_t *)(base + L.y2);
-    int8_t *s = (int8_t *)(base + L.s);
-    int8_t *const X0 = (int8_t *)(base + L.x0), *const X1 = (int8_t *)(base + L.x1);
-    int8_t *const V0 = (int8_t *)(base + L.v0), *const V1 = (int8_t *)(base + L.v1);
-    int8_t *const XX0 = (int8_t *)(base + L.xx0), *const XX1 = (int8_t *)(base + L.xx1);
+    // state arrays: ring slots in LDS (index & mk) or the state block (mk = -1)
+    const int RS = RING ? a.ring : 0;
+    const int mk = RING ? RS - 1 : -1;
+    uint8_t *sb = RING ? gen_ring : base;
+    int8_t *u = (int8_t *)(sb + (RING ? 0 * RS : L.u)), *y = (int8_t *)(sb + (RING ? 1 * RS : L.y));
+    int8_t *y2 = (int8_t *)(sb + (RING ? 2 * RS : L.y2));
+    int8_t *s = (int8_t *)(sb + (RING ? 3 * RS : L.s));
+    int8_t *const X0 = (int8_t *)(sb + (RING ? 4 * RS : L.x0)), *const X1 = (int8_t *)(sb + (RING ? 5 * RS : L.x1));
+    int8_t *const V0 = (int8_t *)(sb + (RING ? 6 * RS : L.v0)), *const V1 = (int8_t *)(sb + (RING ? 7 * RS : L.v1));
+    int8_t *const XX0 = (int8_t *)(sb + (RING ? 8 * RS : L.xx0));
+    int8_t *const XX1 = (int8_t *)(sb + (RING ? 9 * RS : L.xx1));
+    int32_t *H = (int32_t *)(sb + (RING ? 10 * RS : L.H));
     uint8_t *sf = base + L.sf, *qr = base + L.qr;
-    int32_t *H = (int32_t *)(base + L.H);
     uint8_t *p = a.dir + jb.dir_off;
     int32_t *off = a.offs + jb.off_off, *off_end = off + (qlen + tlen - 1);
     const int ncolB = jb.ncolB;
@@ -111,13 +126,13 @@ __global__ void __launch_bounds__(kWave * kMaxWaves) ksw_general_kernel(GenArgs 
     // -(q2+e2), s = 0, H = NEG_INF, sf = target, qr = reversed query
     {
         const uint8_t *tq = a.seq + jb.t_off, *qq = a.seq + jb.q_off;
-        for (int t = tid; t < tlenB; t += nthr) {
+        for (int t = tid; t < (RING ? RS : tlenB); t += nthr) {
             u[t] = y[t] = X0[t] = X1[t] = V0[t] = V1[t] = (int8_t)w8(-q - e);
             y2[t] = XX0[t] = XX1[t] = (int8_t)w8(-q2 - e2);
             s[t] = 0;
-            sf[t] = t < tlen ? tq[t] : 0;
             H[t] = KSW_NEG_INF_I;
         }
+        for (int t = tid; t < tlenB; t += nthr) sf[t] = t < tlen ? tq[t] : 0;
         for (int t = tid; t < qlenB + 128; t += nthr) qr[t] = t < qlen ? qq[qlen - 1 - t] : 0;
     }
     __syncthreads();
@@ -127,6 +142,7 @@ __global__ void __launch_bounds__(kWave * kMaxWaves) ksw_general_kernel(GenArgs 
     z.mte = KSW_NEG_INF_I, z.mte_q = -1, z.score = KSW_NEG_INF_I, z.zdropped = 0, z.stopped = 0;
     int last_st = -1, last_en = -1;
     int H0 = 0, last_H0_t = 0;
+    int init_end = RS - 1;  // RING: positions < init_end + 1 have their slots set
     // band of the previous anti-diagonal, for its deferred bookkeeping
     int p_st0 = 0, p_en0 = 0, p_en1 = 0;
     const int nr = qlen + tlen - 1;
@@ -162,7 +178,7 @@ __global__ void __launch_bounds__(kWave * kMaxWaves) ksw_general_kernel(GenArgs 
                 const int h_st0 = st0 == en0 ? h_en0 : F.h_st0;
                 if (en0 == tlen - 1 && h_en0 >= z.mte) z.mte = h_en0, z.mte_q = rr - en0;
                 if (rr - st0 == qlen - 1 && h_st0 >= z.mqe) z.mqe = h_st0, z.mqe_t = st0;
-                if (en0 % nthr == tid) H[en0] = h_en0;  // owner keeps H[en0]
+                if (en0 % nthr == tid) H[en0 & mk] = h_en0;  // owner keeps H[en0]
                 if (apply_zdrop(z, max_H, rr, max_t, jb.zdrop, e2)) stop = true;
                 else if (rr == qlen + tlen - 2 && en0 == tlen - 1) z.score = h_en0;
             } else {
@@ -217,7 +233,7 @@ __global__ void __launch_bounds__(kWave * kMaxWaves) ksw_general_kernel(GenArgs 
         int x1, x21, v1;
         if (st16 > 0) {
             if (st16 - 1 >= last_st && st16 - 1 <= last_en)
-                x1 = Xp[st16 - 1], x21 = X2p[st16 - 1], v1 = Vp[st16 - 1];
+                x1 = Xp[(st16 - 1) & mk], x21 = X2p[(st16 - 1) & mk], v1 = Vp[(st16 - 1) & mk];
             else
                 x1 = w8(-q - e), x21 = w8(-q2 - e2), v1 = w8(-q - e);
         } else {
@@ -227,7 +243,7 @@ __global__ void __launch_bounds__(kWave * kMaxWaves) ksw_general_kernel(GenArgs 
         // (ksw2_extd2_avx.c:72-85): lane stB reads position stB+15
         const bool wrap = st16 != stB;
         int wx = 0, wvv = 0, wx2 = 0;
-        if (wrap) wx = Xp[stB + 15], wvv = Vp[stB + 15], wx2 = X2p[stB + 15];
+        if (wrap) wx = Xp[(stB + 15) & mk], wvv = Vp[(stB + 15) & mk], wx2 = X2p[(stB + 15) & mk];
         const bool inject = en16 >= r;  // y[r], y2[r], u[r] re-initialised
         const int sc_end = generic ? en0 : min(st0 + (en0 - st0 + 16) / 16 * 16 - 1, tlenB - 1);
         const int t_end = max(en16, sc_end);
@@ -251,18 +267,18 @@ __global__ void __launch_bounds__(kWave * kMaxWaves) ksw_general_kernel(GenArgs 
                 } else {
                     sct = gs.mat[(sq * m + sk) & (kMaxM * kMaxM - 1)];
                 }
-                s[t] = (int8_t)sct;
+                s[t & mk] = (int8_t)sct;
             }
             if (t > en16) continue;
             // ---- one cell (ksw2_extd2_sse.c:43-71, 237-331)
             int xl, vl, x2l;
             if (t == st16) xl = x1, vl = v1, x2l = x21;
             else if (wrap && t == stB) xl = wx, vl = wvv, x2l = wx2;
-            else xl = Xp[t - 1], vl = Vp[t - 1], x2l = X2p[t - 1];
+            else xl = Xp[(t - 1) & mk], vl = Vp[(t - 1) & mk], x2l = X2p[(t - 1) & mk];
             int ut, yt, y2t;
             if (inject && t == r) ut = edge_i8(sc, r), yt = w8(-q - e), y2t = w8(-q2 - e2);
-            else ut = u[t], yt = y[t], y2t = y2[t];
-            int zz = in_sc ? sct : (int)s[t];
+            else ut = u[t & mk], yt = y[t & mk], y2t = y2[t & mk];
+            int zz = in_sc ? sct : (int)s[t & mk];
             int A = w8(xl + vl), B = w8(yt + ut), A2 = w8(x2l + vl), B2 = w8(y2t + ut);
             int d = 0;
             if (!with_cigar) {
@@ -296,8 +312,9 @@ __global__ void __launch_bounds__(kWave * kMaxWaves) ksw_general_kernel(GenArgs 
                 x2n = w8((A2 < 0 ? 0 : A2) - qe28); d |= A2 < 0 ? 0 : 0x20;
                 y2n = w8((B2 < 0 ? 0 : B2) - qe28); d |= B2 < 0 ? 0 : 0x40;
             }
-            u[t] = (int8_t)un, Vc[t] = (int8_t)vn, Xc[t] = (int8_t)xn;
-            y[t] = (int8_t)yn, X2c[t] = (int8_t)x2n, y2[t] = (int8_t)y2n;
+            const int tm = t & mk;
+            u[tm] = (int8_t)un, Vc[tm] = (int8_t)vn, Xc[tm] = (int8_t)xn;
+            y[tm] = (int8_t)yn, X2c[tm] = (int8_t)x2n, y2[tm] = (int8_t)y2n;
             if (tracks) {
                 p[(uint64_t)r * ncolB + (uint64_t)(t - stB)] = (uint8_t)vn;
                 p2[(uint64_t)r * ncolB + (uint64_t)(t - stB)] = (uint8_t)un;
@@ -306,12 +323,12 @@ __global__ void __launch_bounds__(kWave * kMaxWaves) ksw_general_kernel(GenArgs 
             }
             // ---- H bookkeeping (ksw2_extd2_sse.c:332-375)
             if (!approx) {
-                if (t == en0) F.u_en0 = un, F.v_en0 = vn, F.h_old_en0 = H[t];
+                if (t == en0) F.u_en0 = un, F.v_en0 = vn, F.h_old_en0 = H[tm];
                 if (r > 0) {
-                    if (t == en0 - 1) F.h_old_en0m1 = H[t];
+                    if (t == en0 - 1) F.h_old_en0m1 = H[tm];
                     if (t >= st0 && t < en0) {
-                        const int hn = H[t] + vn;
-                        H[t] = hn;
+                        const int hn = H[tm] + vn;
+                        H[tm] = hn;
                         if (t == st0) F.h_st0 = hn;
                         const int k = t - st0;
                         const uint32_t rank = t < en1 ? 1u + ((uint32_t)(k & 3) << 29) + (uint32_t)(k >> 2)
@@ -325,10 +342,11 @@ __global__ void __launch_bounds__(kWave * kMaxWaves) ksw_general_kernel(GenArgs 
             }
         }
         // values the bookkeeping needs at positions outside [stB, en16]
-        if (!approx && r > 0 && en0 >= 1 && en0 - 1 < stB && (en0 - 1) % nthr == tid) F.h_old_en0m1 = H[en0 - 1];
+        if (!approx && r > 0 && en0 >= 1 && en0 - 1 < stB && (en0 - 1) % nthr == tid)
+            F.h_old_en0m1 = H[(en0 - 1) & mk];
         if (approx && r > 0) {
             const int t1 = last_H0_t + 1;
-            if ((t1 < stB || t1 > en16) && t1 < tlenB && t1 % nthr == tid) F.u_h0 = u[t1];
+            if ((t1 < stB || t1 > en16) && t1 < tlenB && t1 % nthr == tid) F.u_h0 = u[t1 & mk];
         }
         if (!approx) {
             // wave reduction of (H, rank): larger H, then smaller rank
@@ -342,6 +360,22 @@ __global__ void __launch_bounds__(kWave * kMaxWaves) ksw_general_kernel(GenArgs 
         }
         p_st0 = st0, p_en0 = en0, p_en1 = en1;
         last_st = st16, last_en = en16;
+        if (RING) {
+            // reset the slots of the positions anti-diagonal r+1 reaches first
+            // (its band's right end moves by at most 17 past this one's)
+            int en_n = tlen - 1;
+            if (en_n > r + 1) en_n = r + 1;
+            if (en_n > (r + 1 + w) >> 1) en_n = (r + 1 + w) >> 1;
+            const int reach = min(max((en_n + 16) / 16 * 16 - 1, en_n / 16 * 16 + 15) + 16, tlenB - 1);
+            for (int t = init_end + 1 + tid; t <= reach; t += nthr) {
+                const int tm = t & mk;
+                u[tm] = y[tm] = X0[tm] = X1[tm] = V0[tm] = V1[tm] = (int8_t)w8(-q - e);
+                y2[tm] = XX0[tm] = XX1[tm] = (int8_t)w8(-q2 - e2);
+                s[tm] = 0;
+                H[tm] = KSW_NEG_INF_I;
+            }
+            init_end = max(init_end, reach);
+        }
         __syncthreads();
     }
     if (tid == 0) {
@@ -476,7 +510,20 @@ hipError_t launch_general(const GenArgs &a, int threads, hipStream_t st)
 {
     if (a.njobs <= 0) return hipSuccess;
     if (threads < kWave || threads > kWave * kMaxWaves || threads % kWave) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(ksw_general_kernel, dim3(a.njobs), dim3(threads), 0, st, a);
+    if (a.ring > 0) {
+        if (a.ring < 256 || a.ring > kGenRingMax || (a.ring & (a.ring - 1))) return hipErrorInvalidValue;
+        static bool attr_set = false;  // allow the largest ring's dynamic LDS (once)
+        if (!attr_set) {
+            const hipError_t e = hipFuncSetAttribute((const void *)ksw_general_kernel<true>,
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                     (int)gen_ring_bytes(kGenRingMax));
+            if (e != hipSuccess) return e;
+            attr_set = true;
+        }
+        hipLaunchKernelGGL(ksw_general_kernel<true>, dim3(a.njobs), dim3(threads), gen_ring_bytes(a.ring), st, a);
+    } else {
+        hipLaunchKernelGGL(ksw_general_kernel<false>, dim3(a.njobs), dim3(threads), 0, st, a);
+    }
     return hipGetLastError();
 }
 

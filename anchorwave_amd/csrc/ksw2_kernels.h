@@ -35,6 +35,7 @@ struct GenArgs {
     KswOut *out;
     const int32_t *stop;       // host-raised stop flag (pinned), or null
     GenScore sc;
+    int ring;                  // > 0: state in LDS rings of this many slots (gen_ring_slots)
 };
 
 struct GenBacktrackArgs {

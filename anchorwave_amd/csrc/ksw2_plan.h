@@ -56,6 +56,8 @@ bool takes_general(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls);
 uint64_t pair_footprint(const ksw2_pair_t &p, bool general);
 // Threads of the general kernel's workgroup for one pair (64..1024).
 int gen_threads(const ksw2_pair_t &p);
+// LDS ring slots of the general kernel for one pair (0: state in HBM).
+int gen_ring(const ksw2_pair_t &p);
 
 // Device memory pool of a serving engine: one hipMalloc carved first-fit.
 struct DevicePool {
@@ -106,9 +108,9 @@ struct Plan {
     };
     std::vector<Launch> launches;
     struct GLaunch {
-        int threads, off, count, lane;
+        int threads, ring, off, count, lane;
     };
-    std::vector<GLaunch> glaunches;  // general kernel, per block size
+    std::vector<GLaunch> glaunches;  // general kernel, per block size and ring size
     int nstripes_total = 0;
     uint64_t seq_bytes = 0, dir_bytes = 0, bnd_dwords = 0, cig_words = 0;
     uint8_t *block = nullptr;        // own device block (no pool)

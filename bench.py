@@ -58,16 +58,16 @@ def read_traffic(name: str):
         return None
 
 
-def cpu_baseline(pairs, threads: int, flag: int = 0):
+def cpu_baseline(pairs, threads: int, flag: int = 0, w: int = -1):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py as O  # checker / baseline only
 
     if O.ref_available():
         kind = "reference"
-        fn = lambda p: O.run_ref(p[0], p[1], mat=W_MAT, flag=flag, **SC)  # noqa: E731
+        fn = lambda p: O.run_ref(p[0], p[1], mat=W_MAT, flag=flag, w=w, **SC)  # noqa: E731
     else:
         kind = "port"
-        fn = lambda p: O.run_oracle(p[0], p[1], mat=W_MAT, flag=flag, **SC)  # noqa: E731
+        fn = lambda p: O.run_oracle(p[0], p[1], mat=W_MAT, flag=flag, w=w, **SC)  # noqa: E731
     # largest-cells-first, one pair per task
     order = sorted(range(len(pairs)), key=lambda k: -len(pairs[k][0]) * len(pairs[k][1]))
     res = [None] * len(pairs)
@@ -106,6 +106,69 @@ def singletrack_leg(args, pairs, cells) -> dict:
         out["cpu"] = {"kind": kind, "gcups": round(cells / dt / 1e9, 3), "cores": threads}
         log(f"[bench] singletrack CPU {cells / dt / 1e9:.2f} GCUPS; exact {exact}/{len(pairs)}")
     return out
+
+
+def band_cells(qlen: int, tlen: int, w: int) -> int:
+    """In-band cells the reference visits: the sum over anti-diagonals of
+    en0 - st0 + 1 (minimap2/ksw2_extd2_sse.c:147-156), up to a band exit."""
+    r = np.arange(qlen + tlen - 1, dtype=np.int64)
+    st = np.maximum(np.maximum(0, r - qlen + 1), (r - w + 1) >> 1)
+    en = np.minimum(np.minimum(tlen - 1, r), (r + w) >> 1)
+    bad = np.nonzero(st > en)[0]
+    stop = int(bad[0]) if len(bad) else len(r)
+    return int((en[:stop] - st[:stop] + 1).sum())
+
+
+def single_pair_leg(args) -> dict:
+    """One 5 kbp pair (8% substitutions, 2% indels; global, full CIGAR): one
+    GPU call (median of repeats) against the reference on one host core."""
+    q, t = W.single(seed=1)[0]
+    ksw2.extd2(q, t, mat=W_MAT, **SC)
+    ts = []
+    for _ in range(9):
+        t1 = time.perf_counter()
+        g = ksw2.extd2(q, t, mat=W_MAT, **SC)
+        ts.append(time.perf_counter() - t1)
+    cells = len(q) * len(t)
+    out = {"qlen": len(q), "tlen": len(t), "gpu_ms": round(float(np.median(ts)) * 1e3, 3),
+           "gpu_gcups": round(cells / float(np.median(ts)) / 1e9, 3),
+           "timing": "wall clock of one ksw_extd2_hip call (upload, DP, backtrack, CIGAR), median of 9"}
+    if not args.no_cpu:
+        kind, dt, res = cpu_baseline([(q, t)], 1)
+        out["cpu"] = {"kind": kind, "ms": round(dt * 1e3, 3), "gcups": round(cells / dt / 1e9, 3), "cores": 1}
+        out["exact"] = g.__dict__ == res[0].__dict__
+    log(f"[bench] single 5 kbp pair: {out}")
+    return out
+
+
+def banded_leg(args) -> dict:
+    """64 long pairs (100-300 kbp; 10% substitutions, 2% indels, 2-5
+    structural indels of 5-50 kbp), w = 4096, zdrop off, full traceback,
+    through ksw_extd2_hip_batch; GCUPS over in-band cells.  Pairs whose
+    length difference exceeds the band leave it (zdropped), as in the
+    reference."""
+    log(f"[bench] banded batch: generating {args.banded} pairs")
+    pb = W.banded(seed=4, n=args.banded)
+    w = 4096
+    cells = sum(band_cells(len(q), len(t), w) for q, t in pb)
+    ksw2.extd2(pb[0][0][:64], pb[0][1][:64], mat=W_MAT, **SC)
+    tm = {}
+    gres = ksw2.extd2_batch(pb, mat=W_MAT, w=w, **SC, timing=tm)
+    dt = tm["seconds"]
+    exits = sum(1 for g in gres if g.zdropped)
+    log(f"[bench] banded batch: {cells:.3g} in-band cells in {dt:.2f} s = {cells / dt / 1e9:.1f} GCUPS "
+        f"({exits} band exits)")
+    res = {"pairs": len(pb), "w": w, "in_band_cells": cells, "seconds": round(dt, 3),
+           "gcups": round(cells / dt / 1e9, 2), "band_exits": exits,
+           "timing": "wall clock of ksw_extd2_hip_batch: upload, DP, backtrack, CIGAR download"}
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        kind, cdt, cres = cpu_baseline(pb, threads, w=w)
+        exact = sum(1 for g, c in zip(gres, cres) if g.__dict__ == c.__dict__)
+        res["exact"] = {"agree": exact, "of": len(pb), "frac": exact / len(pb)}
+        res["cpu"] = {"kind": kind, "gcups": round(cells / cdt / 1e9, 3), "cores": threads}
+        log(f"[bench] banded CPU {cells / cdt / 1e9:.2f} GCUPS; exact {exact}/{len(pb)}")
+    return res
 
 
 def uneven_leg(args) -> dict:
@@ -254,6 +317,10 @@ def main() -> int:
                     help="tasks per offered-load level in the serving leg (0: skip it)")
     ap.add_argument("--loads", default="0.2,0.5,0.8,1.0,1.2",
                     help="offered loads of the serving leg, as fractions of measured capacity")
+    ap.add_argument("--single-pair", type=int, default=1,
+                    help="also time one 5 kbp pair on the GPU and on one host core (0: skip)")
+    ap.add_argument("--banded", type=int, default=64,
+                    help="pairs in the banded long-pair leg, w = 4096 (0: skip it)")
     ap.add_argument("--singletrack", type=int, default=1,
                     help="also time the fixed batch with KSW_EZ_SINGLETRACK (0: skip)")
     ap.add_argument("--copy-calib", type=int, default=0,
@@ -343,8 +410,12 @@ def main() -> int:
         "one_at_a_time_gcups": None if single is None else round(single, 3),
         "batched_over_one_at_a_time": None if not single else round(gcups / single, 1),
     }
+    if args.single_pair:
+        out["single_pair"] = single_pair_leg(args)
     if args.singletrack:
         out["singletrack"] = singletrack_leg(args, pairs, cells)
+    if args.banded > 0:
+        out["banded"] = banded_leg(args)
     if args.uneven > 0:
         out["uneven"] = uneven_leg(args)
     if args.serve > 0:

@@ -121,3 +121,33 @@ def test_band_exit_long_pair():
             g = ksw2.extd2(q, t, w=w, flag=flag)
             r = O.run_oracle(q, t, w=w, flag=flag)
             assert same(g, r), f"w={w} flag={flag:#x}: {diff(g, r)}"
+
+
+@pytest.mark.parametrize("name", ["band_medium", "zdrop_band", "wrapping", "positive_match"])
+def test_scenarios_state_in_hbm(name, monkeypatch):
+    """The general kernel with its per-position state in HBM (the form for
+    bands wider than its LDS rings) on the same scenarios."""
+    monkeypatch.setenv("KSW2_HIP_GEN_RING", "0")
+    test_scenarios_vs_oracle(name)
+
+
+@pytest.mark.parametrize("ring", ["1", "0"])
+def test_long_banded_pairs(ring, monkeypatch):
+    """Long pairs under a band: the band crosses the LDS rings many times
+    over (positions far beyond the ring size), exact with the state in LDS
+    rings and in HBM.  Includes a w = 4096 pair (the banded long-pair
+    workload's band) and band exits."""
+    monkeypatch.setenv("KSW2_HIP_GEN_RING", ring)
+    rng = np.random.default_rng(61)
+    t1 = W.random_seq(rng, 20000)
+    q1 = W.mutate(rng, t1, 0.08, 0.02)
+    t2 = W.random_seq(rng, 40000)
+    q2 = W.big_indel(rng, W.mutate(rng, t2, 0.1, 0.02), 1000, 3000)
+    t3 = W.random_seq(rng, 12000)
+    q3 = np.concatenate([t3[:5000], t3[9000:]])  # length difference beyond a w = 300 band
+    cases = [(q1, t1, dict(w=300)), (q1, t1, dict(w=300, zdrop=200)), (q2, t2, dict(w=4096)),
+             (q3, t3, dict(w=300)), (q1, t1, dict(w=300, mat=O.default_matrix(2, -4)))]
+    for k, (q, t, kw) in enumerate(cases):
+        g = ksw2.extd2(q, t, **kw)
+        r = O.run_oracle(q, t, **kw)
+        assert same(g, r), f"case {k} ring={ring}: {diff(g, r)}"
