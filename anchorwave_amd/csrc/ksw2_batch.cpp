@@ -116,7 +116,8 @@ uint64_t pair_footprint(const ksw2_pair_t &p, bool general)
     const int tlenB = (p.tlen + 63) / 64 * 64, qlenB = (p.qlen + 63) / 64 * 64;
     const uint64_t diags = (uint64_t)p.qlen + p.tlen - 1;
     const uint64_t rows = cigar ? tracks * diags * (uint64_t)gen_ncolB(p.qlen, p.tlen, norm_w(p)) + 256 : 0;
-    return gen_layout(tlenB, qlenB).bytes + rows + (cigar ? 8 * diags : 0) + cig + seq + 512;
+    // (+ the band kernel's two stripe buffers and redo flag, whichever kernel runs it)
+    return gen_layout(tlenB, qlenB).bytes + rows + (cigar ? 8 * diags : 0) + cig + seq + 16 * diags + 516;
 }
 
 void make_gen_score(int8_t m, const int8_t *mat, const KswScore &sc, const ScoreClass &cls, GenScore *g)
@@ -131,6 +132,35 @@ void make_gen_score(int8_t m, const int8_t *mat, const KswScore &sc, const Score
 // Threads of the general kernel's workgroup for one pair: about four cells
 // of the widest anti-diagonal per thread, a power of two in [64, 1024].
 // KSW2_HIP_GEN_THREADS forces a size (tuning and tests).
+bool takes_band(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls)
+{
+    const char *ev = getenv("KSW2_HIP_BAND");
+    if (ev && atoi(ev) == 0) return false;
+    const char *fg = getenv("KSW2_HIP_FORCE_GENERAL");
+    if (fg && atoi(fg) > 0) return false;
+    if (p.flag & KSW_EZ_SINGLETRACK) return false;
+    const char *mn = getenv("KSW2_HIP_BAND_MIN");
+    const int min_width = mn ? atoi(mn) : 512;
+    // progress counts stripe * (qlen + tlen + 1) + anti-diagonal must fit int32
+    const int64_t nstripes = ((int64_t)p.tlen + 63) / 64 * 64 / kStripeRows + 1;
+    if ((nstripes + 1) * ((int64_t)p.qlen + p.tlen + 1) >= INT32_MAX) return false;
+    return fast_path_reject(p, m, cls) == kRejBand && norm_w(p) + 1 >= min_width;
+}
+
+int band_r_end(int qlen, int tlen, int w)
+{
+    const int nr = qlen + tlen - 1;
+    for (int r = 0; r < nr; ++r) {
+        int st = 0, en = tlen - 1;
+        if (st < r - qlen + 1) st = r - qlen + 1;
+        if (en > r) en = r;
+        if (st < (r - w + 1) >> 1) st = (r - w + 1) >> 1;
+        if (en > (r + w) >> 1) en = (r + w) >> 1;
+        if (st > en) return r;  // minimap2/ksw2_extd2_sse.c:152-155
+    }
+    return nr;
+}
+
 // LDS ring slots of the general kernel for one pair (0: state in HBM).
 // APPROX_MAX pairs keep their state in HBM: their path tracker may read a
 // position far behind the band.  KSW2_HIP_GEN_RING=0 turns the rings off
@@ -209,6 +239,8 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         p->cig_off[k] = j.cig_off;
     }
     p->nstripes_total = stripes;
+    std::vector<char> band(p->n_gen, 0);
+    for (int gk = 0; gk < p->n_gen; ++gk) band[gk] = takes_band(pairs[p->order[p->n_fast + gk]], m, cls);
     for (int gk = 0; gk < p->n_gen; ++gk) {
         const int slot = p->n_fast + gk;
         const ksw2_pair_t &pp = pairs[p->order[slot]];
@@ -219,6 +251,11 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         j.tlenB = (pp.tlen + 63) / 64 * 64, j.qlenB = (pp.qlen + 63) / 64 * 64;
         j.ncolB = gen_ncolB(pp.qlen, pp.tlen, j.w);
         j.out = slot;
+        j.r_end = 0, j.bnd_off = 0;
+        if (band[gk]) {
+            j.r_end = band_r_end(pp.qlen, pp.tlen, j.w);
+            j.bnd_off = bo, bo += round_up(4ull * ((uint64_t)pp.qlen + pp.tlen - 1), 64);
+        }
         j.q_off = so, so = round_up(so + pp.qlen, 16);
         j.t_off = so, so = round_up(so + pp.tlen, 16);
         j.st_off = sto, sto += gen_layout(j.tlenB, j.qlenB).bytes;
@@ -270,7 +307,7 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
             for (int rs = kGenRingMax; rs >= 0; rs = rs > 256 ? rs >> 1 : rs - 256) {
                 std::vector<int> members;
                 for (int gk = 0; gk < p->n_gen; ++gk)
-                    if (thr[gk] == t && ring[gk] == rs) members.push_back(gk);
+                    if (!band[gk] && thr[gk] == t && ring[gk] == rs) members.push_back(gk);
                 if (members.empty()) continue;
                 std::stable_sort(members.begin(), members.end(), [&](int x, int y) {
                     const GenJob &a = p->gjobs[x], &b = p->gjobs[y];
@@ -280,6 +317,30 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
                                         t >= 512 ? 0 : t >= 256 ? 1 : 2});
                 gorder.insert(gorder.end(), members.begin(), members.end());
             }
+        // band-kernel launches: per group and team (one wave per 512-position
+        // stripe, at most kMaxTeam), longest first
+        p->band_off = (int)gorder.size();
+        const char *bt = getenv("KSW2_HIP_BAND_TEAM");  // forces a team size (tests)
+        const int band_team = bt ? std::max(1, std::min(kBandTeam, atoi(bt))) : 0;
+        for (int grp = 0; grp < 3; ++grp)
+            for (int team = kBandTeam; team >= 1; --team) {
+                std::vector<int> members;
+                for (int gk = 0; gk < p->n_gen; ++gk) {
+                    if (!band[gk]) continue;
+                    const GenJob &j = p->gjobs[gk];
+                    const int g = (j.flag & KSW_EZ_SCORE_ONLY) ? 2 : (j.flag & KSW_EZ_RIGHT) ? 1 : 0;
+                    const int tm = band_team ? band_team : std::min(kBandTeam, (j.tlenB + kStripeRows - 1) / kStripeRows);
+                    if (g == grp && tm == team) members.push_back(gk);
+                }
+                if (members.empty()) continue;
+                std::stable_sort(members.begin(), members.end(), [&](int x, int y) {
+                    const GenJob &a = p->gjobs[x], &b = p->gjobs[y];
+                    return (double)(a.qlen + a.tlen) * a.ncolB > (double)(b.qlen + b.tlen) * b.ncolB;
+                });
+                p->blaunches.push_back({grp, team, (int)gorder.size(), (int)members.size(), 0});
+                gorder.insert(gorder.end(), members.begin(), members.end());
+            }
+        p->n_band = (int)gorder.size() - p->band_off;
     }
     p->seq_bytes = std::max<uint64_t>(so, 16), p->dir_bytes = std::max<uint64_t>(dof, 256);
     p->bnd_dwords = std::max<uint64_t>(bo, 64), p->cig_words = std::max<uint64_t>(co, 16);
@@ -319,6 +380,7 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         o_state = take(p->state_bytes), o_offs = take(p->offs_ints * 4);
         o_gjobs = take(sizeof(GenJob) * p->n_gen), o_gorder = take(sizeof(int32_t) * gorder.size());
     }
+    const uint64_t o_redo = p->n_band > 0 ? take(sizeof(int32_t) * p->n_gen) : 0;
     p->block_bytes = at;
     uint8_t *base = nullptr;
     if (env && env->pool) {
@@ -335,6 +397,7 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
     if (p->n_gen > 0) {
         p->d_state = base + o_state, p->d_offs = (int32_t *)(base + o_offs);
         p->d_gjobs = (GenJob *)(base + o_gjobs), p->d_gorder = (int32_t *)(base + o_gorder);
+        if (p->n_band > 0) p->d_redo = (int32_t *)(base + o_redo);
         if (env && env->h_stop) {
             p->h_stop = env->h_stop, p->d_stop = env->d_stop, p->own_stop = false;
         } else {
@@ -437,8 +500,25 @@ int plan_launch(Plan *p, const Progress *prog)
     for (const Plan::GLaunch &l : p->glaunches) {
         hipStream_t st = lane(l.lane);
         if (!st) return KSW2_HIP_EDEVICE;
-        ga.order = p->d_gorder + l.off, ga.njobs = l.count, ga.ring = l.ring;
+        ga.order = p->d_gorder + l.off, ga.njobs = l.count, ga.ring = l.ring, ga.only_if = nullptr;
         KSW_HIP_TRY(launch_general(ga, l.threads, st));
+    }
+    if (!p->blaunches.empty()) {
+        BandArgs ba;
+        ba.seq = p->d_seq, ba.dir = p->d_dir, ba.offs = p->d_offs, ba.bnd = p->d_bnd, ba.jobs = p->d_gjobs;
+        ba.out = p->d_out, ba.redo = p->d_redo, ba.sc = p->gsc;
+        const char *fr = getenv("KSW2_HIP_BAND_REDO");
+        ba.force_redo = fr && atoi(fr) > 0 ? 1 : 0;
+        hipStream_t st = lane(0);
+        if (!st) return KSW2_HIP_EDEVICE;
+        for (const Plan::BLaunch &l : p->blaunches) {
+            ba.order = p->d_gorder + l.off, ba.njobs = l.count;
+            KSW_HIP_TRY(launch_band(ba, l.team, l.grp == 1, l.grp == 2, st));
+        }
+        // pairs whose H went above 0 (the band kernel does not track the
+        // maximum) run again on the general kernel, on the same stream
+        ga.order = p->d_gorder + p->band_off, ga.njobs = p->n_band, ga.ring = 0, ga.only_if = p->d_redo;
+        KSW_HIP_TRY(launch_general(ga, 1024, st));
     }
     StripeArgs sa;
     sa.seq = p->d_seq, sa.dir = p->d_dir, sa.bnd = p->d_bnd, sa.cig = p->d_cig, sa.sc = p->sc;
@@ -514,8 +594,8 @@ int deliver_slot(const Plan *p, int k, const KswOut &o, const uint32_t *cig_aren
     z->max_q = o.max_q, z->max_t = o.max_t;
     z->mqe = o.mqe, z->mqe_t = o.mqe_t, z->mte = o.mte, z->mte_q = o.mte_q;
     z->score = o.score, z->reach_end = o.reach_end, z->stopped = o.stopped;
-    if (p->stop_requested && k < p->n_fast) {
-        // the stripe kernel has no stop point: a pair whose deadline passed
+    if (p->stop_requested && (k < p->n_fast || p->gjobs[k - p->n_fast].r_end > 0)) {
+        // the stripe and band kernels have no stop point: a pair whose deadline passed
         // while it ran is reported stopped with no path (callers discard
         // stopped results, alignSlidingWindow.cpp:957-963)
         z->stopped = 1, z->score = KSW_NEG_INF;

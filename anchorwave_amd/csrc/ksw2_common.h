@@ -74,6 +74,8 @@ struct GenJob {
     int32_t ncolB;             // bytes per direction row: n_col_ * 64
     int32_t tlenB, qlenB;      // lengths rounded up to 64
     int32_t out;               // slot in the output array
+    int32_t r_end;             // band kernel: anti-diagonals the reference completes
+    uint64_t bnd_off;          // band kernel: dword offset of its two stripe buffers
 };
 
 // State block of one general pair: ten int8 arrays of tlenB (u, y, y2, s and

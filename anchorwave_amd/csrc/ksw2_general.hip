@@ -85,6 +85,7 @@ __global__ void __launch_bounds__(kWave * kMaxWaves) ksw_general_kernel(GenArgs 
     __shared__ Fin fin[2];
     extern __shared__ __align__(16) uint8_t gen_ring[];
     const int job = __builtin_amdgcn_readfirstlane(a.order[blockIdx.x]);
+    if (a.only_if && __builtin_amdgcn_readfirstlane(a.only_if[job]) == 0) return;  // uniform
     const GenJob jb = a.jobs[job];
     const GenScore &gs = a.sc;
     const KswScore &sc = gs.k;

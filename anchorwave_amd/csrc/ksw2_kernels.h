@@ -36,6 +36,7 @@ struct GenArgs {
     const int32_t *stop;       // host-raised stop flag (pinned), or null
     GenScore sc;
     int ring;                  // > 0: state in LDS rings of this many slots (gen_ring_slots)
+    const int32_t *only_if;    // non-null: run job j only when only_if[j] != 0
 };
 
 struct GenBacktrackArgs {
@@ -46,6 +47,23 @@ struct GenBacktrackArgs {
     KswOut *out;
     int njobs;
 };
+
+// Band kernel (ksw2_band.hip): one workgroup of `team` waves per job.
+struct BandArgs {
+    const uint8_t *seq;
+    uint8_t *dir;              // direction rows (reference layout)
+    int32_t *offs;             // off[] / off_end[] per pair
+    int32_t *bnd;              // per pair: two stripe hand-off buffers of 2 * (qlen+tlen-1) dwords
+    const GenJob *jobs;
+    const int32_t *order;      // this launch's jobs, one workgroup each
+    int njobs;
+    KswOut *out;
+    int32_t *redo;             // per job: 1 when H > 0 was seen (the general kernel must run it)
+    GenScore sc;
+    int force_redo;            // tests: flag every job for the general kernel
+};
+constexpr int kBandTeam = 8;  // waves per workgroup of the band kernel, at most
+hipError_t launch_band(const BandArgs &a, int team, bool right, bool score_only, hipStream_t st);
 
 // Singletrack walk of general pairs: one wavefront per job (all general
 // jobs of the plan; those without KSW_EZ_SINGLETRACK return at once).

@@ -58,6 +58,13 @@ uint64_t pair_footprint(const ksw2_pair_t &p, bool general);
 int gen_threads(const ksw2_pair_t &p);
 // LDS ring slots of the general kernel for one pair (0: state in HBM).
 int gen_ring(const ksw2_pair_t &p);
+// True when a general-kernel pair runs on the band kernel instead: its only
+// departure from the stripe kernel's domain is a band narrower than the
+// matrix, no Singletrack, and a band at least KSW2_HIP_BAND_MIN (512) wide.
+bool takes_band(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls);
+// Anti-diagonals the reference completes before the band leaves the
+// matrix (qlen + tlen - 1 when it never does).
+int band_r_end(int qlen, int tlen, int w);
 
 // Device memory pool of a serving engine: one hipMalloc carved first-fit.
 struct DevicePool {
@@ -111,6 +118,13 @@ struct Plan {
         int threads, ring, off, count, lane;
     };
     std::vector<GLaunch> glaunches;  // general kernel, per block size and ring size
+    // band kernel, per (group, team); `off`/`count` index d_gorder
+    struct BLaunch {
+        int grp, team, off, count, lane;
+    };
+    std::vector<BLaunch> blaunches;
+    int n_band = 0, band_off = 0;    // band jobs: d_gorder[band_off .. band_off + n_band)
+    int32_t *d_redo = nullptr;       // per general job: the band kernel asks the general kernel to rerun it
     int nstripes_total = 0;
     uint64_t seq_bytes = 0, dir_bytes = 0, bnd_dwords = 0, cig_words = 0;
     uint8_t *block = nullptr;        // own device block (no pool)
