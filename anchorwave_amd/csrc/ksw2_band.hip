@@ -52,8 +52,8 @@ namespace kswhip {
 
 namespace {
 
-constexpr int R = kRowsPerLane;  // positions per lane
-constexpr int SR = kWave * R;    // positions per stripe
+constexpr int R = kBandRowsPerLane;  // positions per lane (4: one step of a stripe is short)
+constexpr int SR = kWave * R;        // positions per stripe
 constexpr int kBlock = 64;       // anti-diagonals between progress publications
 constexpr int kLead = 128;       // lead a stripe waits for on its predecessor
 constexpr int kDppWaveShr1 = 0x138;
@@ -116,10 +116,70 @@ __device__ __forceinline__ int pack3(int x, int v, int x2)
 }
 __device__ __forceinline__ int byte_s(int w, int k) { return (int)(int8_t)(uint8_t)((unsigned)w >> (8 * k)); }
 
+// A lane's R direction bytes (one aligned store).
+__device__ __forceinline__ void store_dirs(uint8_t *dst, const uint32_t (&dw)[2])
+{
+    if (R == 8) {
+        uint2 w8;
+        w8.x = dw[0], w8.y = dw[1];
+        *(uint2 *)dst = w8;
+    } else {
+        *(uint32_t *)dst = dw[0];
+    }
+}
+
 }  // namespace
 
-// Up to kBandTeam waves per pair (two per SIMD), so each wave may hold up to
-// 256 VGPRs: the eight positions' state and both step forms fit unspilled.
+// Scaled, tagged state (the stripe kernel's representation): every
+// difference w is kept as -8w (u, v) or -8(w + G) - BIAS (x, y with G1; x2,
+// y2 with G2), BIAS = 0 for left and 8 for right alignment, so that one
+// chain of mins over s, a, b, a2, b2 (each -8*value + a 3-bit tag) gives
+// both the reference's maximum and the index its strict '>' (left) or '>='
+// (right) chain picks; T = (x + v) + 8e - 8z is -8a' - BIAS, whose sign is
+// the continuation flag and min(T, -BIAS) the new x.
+struct BandConsts {
+    int CA, CB, CA2, CB2, E1, E2, LIM, SM, SMM, CAP;
+};
+
+template <bool RIGHT>
+__device__ __forceinline__ BandConsts band_consts(const KswScore &sc)
+{
+    const int bias = RIGHT ? 8 : 0;
+    const int tS = RIGHT ? 7 : 0, tA = RIGHT ? 6 : 1, tB = RIGHT ? 5 : 2, tA2 = RIGHT ? 4 : 3, tB2 = RIGHT ? 3 : 4;
+    BandConsts k;
+    k.CA = 8 * sc.G1 + tA + bias, k.CB = 8 * sc.G1 + tB + bias;
+    k.CA2 = 8 * sc.G2 + tA2 + bias, k.CB2 = 8 * sc.G2 + tB2 + bias;
+    k.E1 = 8 * sc.e, k.E2 = 8 * sc.e2, k.LIM = -bias;
+    k.SM = -8 * sc.mch + tS, k.SMM = -8 * sc.mis + tS;
+    k.CAP = -8 * sc.mch;  // z = min(z, mat[0]) in the -8 domain
+    return k;
+}
+
+// One cell in the scaled form.  In: x, v, x2 of the position above, u, y,
+// y2 of this position (scaled), mm = mismatch.  Out: the new state; returns
+// the direction byte (0 when SCORE_ONLY).
+template <bool RIGHT, bool SCORE_ONLY>
+__device__ __forceinline__ int band_cell(const BandConsts &k, int X, int V, int X2, int U, int Y, int Y2, bool mm,
+                                         int &Un, int &Vn, int &Xn, int &Yn, int &X2n, int &Y2n)
+{
+    const int S = mm ? k.SMM : k.SM;
+    const int A = X + V + k.CA, B = Y + U + k.CB, A2 = X2 + V + k.CA2, B2 = Y2 + U + k.CB2;
+    const int Z = min(min(min(min(S, A), B), A2), B2);
+    const int Z8 = max(Z & ~7, k.CAP);
+    const int M1 = k.E1 - Z8, M2 = k.E2 - Z8;
+    const int Ta = X + V + M1, Tb = Y + U + M1, Ta2 = X2 + V + M2, Tb2 = Y2 + U + M2;
+    Un = Z8 - V, Vn = Z8 - U;
+    Xn = min(Ta, k.LIM), Yn = min(Tb, k.LIM), X2n = min(Ta2, k.LIM), Y2n = min(Tb2, k.LIM);
+    if (SCORE_ONLY) return 0;
+    int d = RIGHT ? (~Z & 7) : (Z & 7);
+    d |= (int)(((uint32_t)Ta >> 31) << 3) | (int)(((uint32_t)Tb >> 31) << 4);
+    d |= (int)(((uint32_t)Ta2 >> 31) << 5) | (int)(((uint32_t)Tb2 >> 31) << 6);
+    return d;
+}
+
+// Up to kBandTeam waves per pair.  A pair needs about (2w + 2 SR + 300) /
+// (2 SR + 190) stripes in flight (a stripe's anti-diagonals start 2 SR
+// after its predecessor's, plus the lead), 13 for w = 4096 at SR = 256.
 template <bool RIGHT, bool SCORE_ONLY>
 __global__ void __launch_bounds__(kWave * kBandTeam) ksw_band_kernel(BandArgs a)
 {
@@ -136,7 +196,11 @@ __global__ void __launch_bounds__(kWave * kBandTeam) ksw_band_kernel(BandArgs a)
     const bool generic = flag & 0x04;
     const int nr = qlen + tlen - 1;
     const KswScore &sc = a.sc.k;
-    const int G1 = sc.G1, G2 = sc.G2, q = sc.q, q2 = sc.q2, mch = sc.mch, mis = sc.mis;
+    const BandConsts kc = band_consts<RIGHT>(sc);
+    const int G1 = sc.G1, G2 = sc.G2, bias = RIGHT ? 8 : 0;
+    // scaled constants: u, v = -G1 -> 8 G1; x, y, x2, y2 at their initial
+    // values (-G1, -G2) -> -BIAS
+    const int U0 = 8 * G1, L0 = -bias;
     const uint8_t *qs = a.seq + jb.q_off, *ts = a.seq + jb.t_off;
     uint8_t *p = a.dir + jb.dir_off;
     int32_t *off = a.offs + jb.off_off, *off_end = off + nr;
@@ -162,26 +226,25 @@ __global__ void __launch_bounds__(kWave * kBandTeam) ksw_band_kernel(BandArgs a)
         const int pw = (s + W - 1) % W, pbase = (s - 1) / W * (nr + 2);  // producer wave, its count base
         int32_t *out_buf = buf + (uint64_t)(s & 1) * 2 * nr;
         const int32_t *in_buf = buf + (uint64_t)((s + 1) & 1) * 2 * nr;
-        // per position: the in-place state and H; the score s as one bit
-        // (set: mismatch; s is mch or mis in this domain, 0 at first), the
-        // target codes packed four to a register (zero past tlen, as the
-        // reference's zero-padded target copy)
+        // per position: the in-place state (scaled) and H; the score s as
+        // one bit (set: mismatch; s is mch or mis in this domain, mch = 0 at
+        // first), the target codes packed four to a register (zero past
+        // tlen, as the reference's zero-padded target copy)
         int U[R], Y[R], Y2[R], X[R], V[R], X2[R], H[R];
         uint32_t smask = 0, tcw[2] = {0u, 0u};
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            U[k] = Y[k] = X[k] = V[k] = -G1;
-            Y2[k] = X2[k] = -G2;
+            U[k] = V[k] = U0;
+            X[k] = Y[k] = X2[k] = Y2[k] = L0;
             H[k] = KSW_NEG_INF_I;
             const uint32_t c = tl + k < tlen ? (uint32_t)ts[tl + k] : 0u;
             tcw[k >> 2] |= c << (8 * (k & 3));
         }
-        // the predecessor's bottom position once it is final (read once)
-        int fx = -G1, fv = -G1, fx2 = -G2, fh = KSW_NEG_INF_I;
-        bool have_final = false;
         // query window: byte k of (qwin[1]:qwin[0]) is the query code of
         // position tl + k's column r - (tl + k) (0 outside the query, as the
-        // reference's zero-tailed reversed query); one new byte per step
+        // reference's zero-tailed reversed query).  Each step lane k's new
+        // byte is the one lane k-1 drops (its byte 7); lane 0 takes
+        // Q(r - t0) from a queue loaded four steps ahead.
         uint32_t qwin[2] = {0u, 0u};
 #pragma unroll
         for (int k = 0; k < R; ++k) {
@@ -189,6 +252,14 @@ __global__ void __launch_bounds__(kWave * kBandTeam) ksw_band_kernel(BandArgs a)
             const uint32_t c = j >= 0 && j < qlen ? (uint32_t)qs[j] : 0u;
             qwin[k >> 2] |= c << (8 * (k & 3));
         }
+        // per block of kBlock steps starting at r_b, lane k holds in
+        // registers what lane 0 needs at step r_b + k: the query code
+        // Q(r_b + k - t0) and the predecessor's entry r_b + k - 1 (unscaled
+        // x, v, x2 and H; initial values before its first anti-diagonal, its
+        // last entry after its last).  Loaded once per block, after the wait
+        // that guarantees the entries are written, so no step waits on memory.
+        uint32_t blk_q = 0;
+        int blk_pk = 0, blk_h = KSW_NEG_INF_I;
         int lst16 = -1, len16 = -1;  // st16, en16 of anti-diagonal r-1
         if (r_lo > 0) {
             const Band bp = band_of(r_lo - 1, qlen, tlen, bw, tlenB, generic);
@@ -196,48 +267,46 @@ __global__ void __launch_bounds__(kWave * kBandTeam) ksw_band_kernel(BandArgs a)
         }
         int hmax = KSW_NEG_INF_I;  // largest H on interior steps (checked against 0 at the end)
         for (int r = r_lo; r <= r_hi; ++r) {
-            if (s > 0 && (r - r_lo) % kBlock == 0)
-                known = wait_count(prog + pw, pbase + min(r + kLead, p_hi) + 1, known);
+            const int rb = r - (r - r_lo) % kBlock;  // block start
+            if (r == rb) {
+                if (s > 0) known = wait_count(prog + pw, pbase + min(r + kLead, p_hi) + 1, known);
+                const int j = r + lane - t0;
+                blk_q = j >= 0 && j < qlen ? (uint32_t)qs[j] : 0u;
+                const int e = min(r + lane - 1, p_hi);
+                if (s > 0 && p_hi >= p_lo && e >= p_lo) {
+                    blk_pk = in_buf[2 * e], blk_h = in_buf[2 * e + 1];
+                } else {
+                    blk_pk = pack3(-G1, -G1, -G2), blk_h = KSW_NEG_INF_I;
+                }
+                // take the loads here: the steps of the block then read the
+                // registers without waiting on memory (the step's own stores
+                // would otherwise be waited for on every step)
+                asm volatile("" : "+v"(blk_q), "+v"(blk_pk), "+v"(blk_h));
+            }
             const Band b = band_of(r, qlen, tlen, bw, tlenB, generic);
             {
-                const int j = r - tl;
-                const uint32_t c = j >= 0 && j < qlen ? (uint32_t)qs[j] : 0u;
-                qwin[1] = qwin[1] << 8 | qwin[0] >> 24;
+                const uint32_t down = (uint32_t)shr1((int)qwin[(R - 1) >> 2]) >> 24;
+                const uint32_t c0 = (uint32_t)__builtin_amdgcn_readlane((int)blk_q, r - rb);
+                const uint32_t c = lane == 0 ? c0 : down;
+                if (R > 4) qwin[1] = qwin[1] << 8 | qwin[0] >> 24;
                 qwin[0] = qwin[0] << 8 | c;
             }
-            // position t0-1 as anti-diagonal r-1 left it (lane 0's upper neighbour)
-            int cx = -G1, cv = -G1, cx2 = -G2, ch = KSW_NEG_INF_I;
-            if (s > 0 && p_hi >= p_lo && r - 1 >= p_lo) {
-                if (r - 1 >= p_hi) {
-                    if (!have_final) {
-                        const int e = p_hi;
-                        const int pk = in_buf[2 * e], hh = in_buf[2 * e + 1];
-                        fx = byte_s(pk, 0), fv = byte_s(pk, 1), fx2 = byte_s(pk, 2), fh = hh;
-                        have_final = true;
-                    }
-                    cx = fx, cv = fv, cx2 = fx2, ch = fh;
-                } else if (lane == 0) {
-                    const int pk = in_buf[2 * (r - 1)], hh = in_buf[2 * (r - 1) + 1];
-                    cx = byte_s(pk, 0), cv = byte_s(pk, 1), cx2 = byte_s(pk, 2), ch = hh;
-                }
+            // position t0-1 as anti-diagonal r-1 left it (lane 0's upper
+            // neighbour), from the block registers
+            int cx, cv, cx2, ch;
+            {
+                const int pk = __builtin_amdgcn_readlane(blk_pk, r - rb);
+                ch = __builtin_amdgcn_readlane(blk_h, r - rb);
+                cx = -8 * (byte_s(pk, 0) + G1) - bias, cv = -8 * byte_s(pk, 1);
+                cx2 = -8 * (byte_s(pk, 2) + G2) - bias;
             }
             {
                 const int sx = shr1(X[R - 1]), sv = shr1(V[R - 1]), sx2 = shr1(X2[R - 1]), sh = shr1(H[R - 1]);
                 if (lane != 0) cx = sx, cv = sv, cx2 = sx2, ch = sh;
             }
-            // the AVX-512 lane wrap: position stB reads position stB+15
-            // (row 7 of the next lane) from anti-diagonal r-1
-            const bool wrap = b.st16 != b.stB;
-            int wx = 0, wvv = 0, wx2 = 0;
-            if (wrap && b.stB >= t0 && b.stB < t0 + SR) {
-                const int src = (b.stB - t0) / R + 1;
-                wx = __builtin_amdgcn_readlane(X[R - 1], src);
-                wvv = __builtin_amdgcn_readlane(V[R - 1], src);
-                wx2 = __builtin_amdgcn_readlane(X2[R - 1], src);
-            }
             // interior step: every position of the stripe strictly inside
             // (st0, en0) -- all computed and scored, no boundary rule, no H
-            // entry, no mqe/mte/score position -- the lean form below
+            // entry, no mqe/mte/score position -- the lean form
             if (b.st0 < t0 && t0 + SR - 1 < b.en0) {
                 uint32_t dw[2] = {0u, 0u}, sm = 0;
 #pragma unroll
@@ -248,142 +317,87 @@ __global__ void __launch_bounds__(kWave * kBandTeam) ksw_band_kernel(BandArgs a)
                     const uint32_t qc = (qwin[k >> 2] >> (8 * (k & 3))) & 0xffu;
                     const bool mm = tc != qc;
                     sm |= mm ? 1u << k : 0u;
-                    int z = mm ? mis : mch, d = 0;
-                    int aa = xin + vin, bb = Y[k] + U[k], a2 = x2in + vin, b2 = Y2[k] + U[k];
-                    if (SCORE_ONLY) {
-                        z = max(max(max(max(z, aa), bb), a2), b2);
-                    } else if (!RIGHT) {
-                        d = aa > z ? 1 : 0;  z = max(z, aa);
-                        d = bb > z ? 2 : d;  z = max(z, bb);
-                        d = a2 > z ? 3 : d;  z = max(z, a2);
-                        d = b2 > z ? 4 : d;  z = max(z, b2);
-                    } else {
-                        d = z > aa ? 0 : 1;  z = max(z, aa);
-                        d = z > bb ? d : 2;  z = max(z, bb);
-                        d = z > a2 ? d : 3;  z = max(z, a2);
-                        d = z > b2 ? d : 4;  z = max(z, b2);
+                    int un, vn, xn, yn, x2n, y2n;
+                    const int d = band_cell<RIGHT, SCORE_ONLY>(kc, xin, vin, x2in, U[k], Y[k], Y2[k], mm, un, vn, xn,
+                                                               yn, x2n, y2n);
+                    if (!SCORE_ONLY) {
+                        dw[k >> 2] |= (uint32_t)d << (8 * (k & 3));
+                        asm volatile("" : "+v"(dw[k >> 2]));
                     }
-                    z = min(z, mch);
-                    const int un = z - vin, vn = z - U[k];
-                    int tmp = z - q;
-                    aa -= tmp, bb -= tmp;
-                    tmp = z - q2;
-                    a2 -= tmp, b2 -= tmp;
-                    if (!RIGHT) {
-                        d |= aa > 0 ? 0x08 : 0;
-                        d |= bb > 0 ? 0x10 : 0;
-                        d |= a2 > 0 ? 0x20 : 0;
-                        d |= b2 > 0 ? 0x40 : 0;
-                    } else {
-                        d |= aa >= 0 ? 0x08 : 0;
-                        d |= bb >= 0 ? 0x10 : 0;
-                        d |= a2 >= 0 ? 0x20 : 0;
-                        d |= b2 >= 0 ? 0x40 : 0;
-                    }
-                    dw[k >> 2] |= (uint32_t)d << (8 * (k & 3));
-                    asm volatile("" : "+v"(dw[k >> 2]));
-                    X[k] = max(aa, 0) - G1, Y[k] = max(bb, 0) - G1;
-                    X2[k] = max(a2, 0) - G2, Y2[k] = max(b2, 0) - G2;
-                    U[k] = un, V[k] = vn;
-                    H[k] += vn;
+                    U[k] = un, V[k] = vn, X[k] = xn, Y[k] = yn, X2[k] = x2n, Y2[k] = y2n;
+                    H[k] -= vn >> 3;  // v = -(V / 8)
                     hmax = max(hmax, H[k]);
                 }
                 smask = sm;
-                if (!SCORE_ONLY) {
-                    uint2 w8;
-                    w8.x = dw[0], w8.y = dw[1];
-                    *(uint2 *)(p + (uint64_t)r * ncolB + (uint64_t)(tl - b.stB)) = w8;
-                }
+                if (!SCORE_ONLY) store_dirs(p + (uint64_t)r * ncolB + (uint64_t)(tl - b.stB), dw);
             } else {
-            const bool keep_prev = b.st16 > 0 && b.st16 - 1 >= lst16 && b.st16 - 1 <= len16;
-            const int edge = edge_diff(sc, r);
-            const bool inject = b.en16 >= r;
-            const bool la = tl >= b.stB && tl + R - 1 <= b.en16;
-            uint32_t dw[2] = {0u, 0u};
+                // the AVX-512 lane wrap: position stB reads position stB+15
+                // (row 7 of the next lane) from anti-diagonal r-1
+                const bool wrap = b.st16 != b.stB;
+                int wx = 0, wvv = 0, wx2 = 0;
+                if (wrap && b.stB >= t0 && b.stB < t0 + SR) {
+                    const int src = (b.stB + 15 - t0) / R;  // position stB+15 is row R-1 of this lane
+                    wx = __builtin_amdgcn_readlane(X[R - 1], src);
+                    wvv = __builtin_amdgcn_readlane(V[R - 1], src);
+                    wx2 = __builtin_amdgcn_readlane(X2[R - 1], src);
+                }
+                const bool keep_prev = b.st16 > 0 && b.st16 - 1 >= lst16 && b.st16 - 1 <= len16;
+                const int edgeV = -8 * edge_diff(sc, r);
+                const bool inject = b.en16 >= r;
+                const bool la = tl >= b.stB && tl + R - 1 <= b.en16;
+                uint32_t dw[2] = {0u, 0u};
 #pragma unroll
-            for (int k = R - 1; k >= 0; --k) {
-                const int t = tl + k;
-                int xin = k > 0 ? X[k - 1] : cx, vin = k > 0 ? V[k - 1] : cv, x2in = k > 0 ? X2[k - 1] : cx2;
-                if (t == b.st16) {
-                    if (b.st16 == 0) xin = -G1, vin = edge, x2in = -G2;
-                    else if (!keep_prev) xin = -G1, vin = -G1, x2in = -G2;
-                } else if (wrap && t == b.stB) {
-                    xin = wx, vin = wvv, x2in = wx2;
-                }
-                int uin = U[k], yin = Y[k], y2in = Y2[k];
-                if (inject && t == r) uin = edge, yin = -G1, y2in = -G2;
-                if (t >= b.st0 && t <= b.sc_end) {
-                    const uint32_t qc = (qwin[k >> 2] >> (8 * (k & 3))) & 0xffu;
-                    const uint32_t tc = (tcw[k >> 2] >> (8 * (k & 3))) & 0xffu;
-                    smask = (smask & ~(1u << k)) | (tc != qc ? 1u << k : 0u);
-                }
-                int z = (smask >> k & 1u) ? mis : mch, d = 0;
-                int aa = xin + vin, bb = yin + uin, a2 = x2in + vin, b2 = y2in + uin;
-                if (SCORE_ONLY) {
-                    z = max(max(max(max(z, aa), bb), a2), b2);
-                } else if (!RIGHT) {
-                    d = aa > z ? 1 : 0;  z = max(z, aa);
-                    d = bb > z ? 2 : d;  z = max(z, bb);
-                    d = a2 > z ? 3 : d;  z = max(z, a2);
-                    d = b2 > z ? 4 : d;  z = max(z, b2);
-                } else {
-                    d = z > aa ? 0 : 1;  z = max(z, aa);
-                    d = z > bb ? d : 2;  z = max(z, bb);
-                    d = z > a2 ? d : 3;  z = max(z, a2);
-                    d = z > b2 ? d : 4;  z = max(z, b2);
-                }
-                z = min(z, mch);
-                const int un = z - vin, vn = z - uin;
-                int tmp = z - q;
-                aa -= tmp, bb -= tmp;
-                tmp = z - q2;
-                a2 -= tmp, b2 -= tmp;
-                int xn, yn, x2n, y2n;
-                if (!RIGHT) {
-                    xn = max(aa, 0) - G1;   d |= aa > 0 ? 0x08 : 0;
-                    yn = max(bb, 0) - G1;   d |= bb > 0 ? 0x10 : 0;
-                    x2n = max(a2, 0) - G2;  d |= a2 > 0 ? 0x20 : 0;
-                    y2n = max(b2, 0) - G2;  d |= b2 > 0 ? 0x40 : 0;
-                } else {
-                    xn = max(aa, 0) - G1;   d |= aa >= 0 ? 0x08 : 0;
-                    yn = max(bb, 0) - G1;   d |= bb >= 0 ? 0x10 : 0;
-                    x2n = max(a2, 0) - G2;  d |= a2 >= 0 ? 0x20 : 0;
-                    y2n = max(b2, 0) - G2;  d |= b2 >= 0 ? 0x40 : 0;
-                }
-                dw[k >> 2] |= (uint32_t)d << (8 * (k & 3));
-                // materialise the byte now (the compiler would otherwise sink
-                // the direction logic of all eight positions to the store and
-                // keep their operands alive)
-                asm volatile("" : "+v"(dw[k >> 2]));
-                if (la) {
-                    // H bookkeeping on [st0, en0]; the position above still
-                    // holds its H of anti-diagonal r-1
-                    if (t == b.en0) {
-                        const int hab = k > 0 ? H[k - 1] : ch;
-                        const int h = r == 0 ? vn - sc.qe_orig : b.en0 > 0 ? hab + un : H[k] + vn;
-                        H[k] = h;
-                        if (b.en0 == tlen - 1 && h >= mte_h) mte_h = h, mte_q = r - b.en0;
-                        if (r == nr - 1 && b.en0 == tlen - 1) score = h;
-                    } else if (t >= b.st0 && t < b.en0) {
-                        H[k] += vn;
+                for (int k = R - 1; k >= 0; --k) {
+                    const int t = tl + k;
+                    int xin = k > 0 ? X[k - 1] : cx, vin = k > 0 ? V[k - 1] : cv, x2in = k > 0 ? X2[k - 1] : cx2;
+                    if (t == b.st16) {
+                        if (b.st16 == 0) xin = L0, vin = edgeV, x2in = L0;
+                        else if (!keep_prev) xin = L0, vin = U0, x2in = L0;
+                    } else if (wrap && t == b.stB) {
+                        xin = wx, vin = wvv, x2in = wx2;
                     }
-                    if (t >= b.st0 && t <= b.en0) {
-                        if (H[k] > 0) pos = 1;
-                        if (t == b.st0 && r - b.st0 == qlen - 1 && H[k] >= mqe_h) mqe_h = H[k], mqe_t = t;
+                    int uin = U[k], yin = Y[k], y2in = Y2[k];
+                    if (inject && t == r) uin = edgeV, yin = L0, y2in = L0;
+                    if (t >= b.st0 && t <= b.sc_end) {
+                        const uint32_t qc = (qwin[k >> 2] >> (8 * (k & 3))) & 0xffu;
+                        const uint32_t tc = (tcw[k >> 2] >> (8 * (k & 3))) & 0xffu;
+                        smask = (smask & ~(1u << k)) | (tc != qc ? 1u << k : 0u);
                     }
-                    U[k] = un, V[k] = vn, X[k] = xn, Y[k] = yn, X2[k] = x2n, Y2[k] = y2n;
+                    int un, vn, xn, yn, x2n, y2n;
+                    const int d = band_cell<RIGHT, SCORE_ONLY>(kc, xin, vin, x2in, uin, yin, y2in, (smask >> k) & 1u,
+                                                               un, vn, xn, yn, x2n, y2n);
+                    if (!SCORE_ONLY) {
+                        dw[k >> 2] |= (uint32_t)d << (8 * (k & 3));
+                        asm volatile("" : "+v"(dw[k >> 2]));
+                    }
+                    if (la) {
+                        // H bookkeeping on [st0, en0]; the position above still
+                        // holds its H of anti-diagonal r-1
+                        const int ut = -(un >> 3), vt = -(vn >> 3);
+                        if (t == b.en0) {
+                            const int hab = k > 0 ? H[k - 1] : ch;
+                            const int h = r == 0 ? vt - sc.qe_orig : b.en0 > 0 ? hab + ut : H[k] + vt;
+                            H[k] = h;
+                            if (b.en0 == tlen - 1 && h >= mte_h) mte_h = h, mte_q = r - b.en0;
+                            if (r == nr - 1 && b.en0 == tlen - 1) score = h;
+                        } else if (t >= b.st0 && t < b.en0) {
+                            H[k] += vt;
+                        }
+                        if (t >= b.st0 && t <= b.en0) {
+                            if (H[k] > 0) pos = 1;
+                            if (t == b.st0 && r - b.st0 == qlen - 1 && H[k] >= mqe_h) mqe_h = H[k], mqe_t = t;
+                        }
+                        U[k] = un, V[k] = vn, X[k] = xn, Y[k] = yn, X2[k] = x2n, Y2[k] = y2n;
+                    }
                 }
-            }
-            if (!SCORE_ONLY && la) {
-                uint2 w8;
-                w8.x = dw[0], w8.y = dw[1];
-                *(uint2 *)(p + (uint64_t)r * ncolB + (uint64_t)(tl - b.stB)) = w8;
-            }
+                if (!SCORE_ONLY && la) store_dirs(p + (uint64_t)r * ncolB + (uint64_t)(tl - b.stB), dw);
             }
             lst16 = b.st16, len16 = b.en16;
-            // this stripe's bottom position for the next stripe
+            // this stripe's bottom position for the next stripe, unscaled
             if (lane == kWave - 1) {
-                out_buf[2 * r] = pack3(X[R - 1], V[R - 1], X2[R - 1]);
+                const int x = -((X[R - 1] + bias) >> 3) - G1, v = -(V[R - 1] >> 3);
+                const int x2 = -((X2[R - 1] + bias) >> 3) - G2;
+                out_buf[2 * r] = pack3(x, v, x2);
                 out_buf[2 * r + 1] = H[R - 1];
             }
             if ((r - r_lo) % kBlock == kBlock - 1) publish(prog + wv, kk * (nr + 2) + r + 1);

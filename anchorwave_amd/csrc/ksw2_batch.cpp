@@ -142,7 +142,7 @@ bool takes_band(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls)
     const char *mn = getenv("KSW2_HIP_BAND_MIN");
     const int min_width = mn ? atoi(mn) : 512;
     // progress counts stripe * (qlen + tlen + 1) + anti-diagonal must fit int32
-    const int64_t nstripes = ((int64_t)p.tlen + 63) / 64 * 64 / kStripeRows + 1;
+    const int64_t nstripes = ((int64_t)p.tlen + 63) / 64 * 64 / (kWave * kBandRowsPerLane) + 1;
     if ((nstripes + 1) * ((int64_t)p.qlen + p.tlen + 1) >= INT32_MAX) return false;
     return fast_path_reject(p, m, cls) == kRejBand && norm_w(p) + 1 >= min_width;
 }
@@ -329,7 +329,8 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
                     if (!band[gk]) continue;
                     const GenJob &j = p->gjobs[gk];
                     const int g = (j.flag & KSW_EZ_SCORE_ONLY) ? 2 : (j.flag & KSW_EZ_RIGHT) ? 1 : 0;
-                    const int tm = band_team ? band_team : std::min(kBandTeam, (j.tlenB + kStripeRows - 1) / kStripeRows);
+                    const int brows = kWave * kBandRowsPerLane;
+                    const int tm = band_team ? band_team : std::min(kBandTeam, (j.tlenB + brows - 1) / brows);
                     if (g == grp && tm == team) members.push_back(gk);
                 }
                 if (members.empty()) continue;

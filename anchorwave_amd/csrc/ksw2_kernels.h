@@ -62,7 +62,8 @@ struct BandArgs {
     GenScore sc;
     int force_redo;            // tests: flag every job for the general kernel
 };
-constexpr int kBandTeam = 8;  // waves per workgroup of the band kernel, at most
+constexpr int kBandTeam = 16;       // waves per workgroup of the band kernel, at most
+constexpr int kBandRowsPerLane = 4;  // positions per lane; a stripe is 64 lanes of them
 hipError_t launch_band(const BandArgs &a, int team, bool right, bool score_only, hipStream_t st);
 
 // Singletrack walk of general pairs: one wavefront per job (all general
