@@ -12,8 +12,8 @@
 // wrap at a non-64-aligned start, row r re-initialised at column 0).  This
 // kernel evaluates the same cells in (t, r) coordinates:
 //
-//  * Stripes.  The positions are cut into stripes of 512; lane k of a wave
-//    holds positions t0 + 8k .. t0 + 8k + 7 of its stripe in registers (the
+//  * Stripes.  The positions are cut into stripes of 256; lane k of a wave
+//    holds positions t0 + 4k .. t0 + 4k + 3 of its stripe in registers (the
 //    in-place state u, v, x, y, x2, y2, s and H of each) and steps through
 //    the anti-diagonals that reach the stripe.  Within a lane the positions
 //    are updated from the highest down, so each still reads its upper
@@ -22,14 +22,24 @@
 //    lane 0 takes the previous stripe's bottom position from a per-pair
 //    buffer indexed by r.  A position outside [stB, en16] is not updated:
 //    its stale value is simply the register it keeps.
-//  * Because stB is a multiple of 64 and en16 + 1 of 16, a lane's eight
+//  * Because stB is a multiple of 64 and en16 + 1 of 16, a lane's four
 //    positions are all inside [stB, en16] or all outside: the commit and the
-//    direction-byte store are per lane (eight bytes, aligned).
-//  * Teams.  One workgroup of W waves per pair; wave w takes stripes w,
-//    w + W, ...; stripe s trails stripe s-1 by at least 128 anti-diagonals
-//    through workgroup-scope progress counters in LDS (as the stripe kernel).
-//    The buffer is double-buffered by stripe parity; that lead keeps a
-//    producer two stripes ahead from overwriting an entry before it is read.
+//    direction-byte store are per lane (four bytes, aligned).
+//  * Teams.  W waves per pair (W = 16 for w = 4096: a stripe's
+//    anti-diagonals start 2 SR after its predecessor's, so about 2w / 2 SR
+//    stripes are in flight); wave w takes stripes w, w + W, ...  The waves
+//    sit in workgroups of four (one wave per SIMD) that the dispatcher
+//    spreads over the CUs, so a pair's waves do not share SIMDs.  Stripe s
+//    trails stripe s-1 by at least 64 anti-diagonals through progress
+//    counts in device memory; counts and hand-off entries use device-scope
+//    (sc1) accesses, coherent across the XCDs' L2s, and a count is stored
+//    only after the entries it covers are complete.  Three buffers by
+//    stripe mod 3: a producer three stripes ahead reuses a buffer, and the
+//    leads between the stripes in between keep it from overwriting an entry
+//    before it is read.
+//  * A wait that outlasts kWaitTicks (a workgroup the dispatcher could not
+//    place) gives up: the pair is flagged for the general kernel, which
+//    recomputes it, and every wave still runs to its end.
 //  * H bookkeeping (minimap2/ksw2_extd2_sse.c:332-375): H[en0] from the
 //    position above plus u, H[t] += v on [st0, en0), mte on position tlen-1,
 //    mqe where column qlen-1 is the band's start, the score at the last
@@ -54,9 +64,12 @@ namespace {
 
 constexpr int R = kBandRowsPerLane;  // positions per lane (4: one step of a stripe is short)
 constexpr int SR = kWave * R;        // positions per stripe
-constexpr int kBlock = 64;       // anti-diagonals between progress publications
-constexpr int kLead = 128;       // lead a stripe waits for on its predecessor
+constexpr int kBlock = 64;       // anti-diagonals per block of hand-off loads
+constexpr int kPublish = 32;     // anti-diagonals between progress publications
+constexpr int kLead = 64;        // lead a stripe waits for on its predecessor
+constexpr int kBufs = 3;         // hand-off buffers, by stripe mod 3
 constexpr int kDppWaveShr1 = 0x138;
+constexpr uint64_t kWaitTicks = 1000000000ull;  // 10 s of the 100 MHz real-time clock
 
 struct Band {
     int st0, en0, st16, en16, stB, sc_end;
@@ -91,22 +104,43 @@ __device__ __forceinline__ void stripe_range(int t0, int w, int r_end, int &lo, 
 
 __device__ __forceinline__ int shr1(int v) { return __builtin_amdgcn_update_dpp(0, v, kDppWaveShr1, 0xF, 0xF, false); }
 
+// Progress count: stored after the wave's earlier hand-off stores are
+// complete (s_waitcnt vmcnt(0)); both are device-scope stores that bypass
+// the XCD-private caches.
 __device__ __forceinline__ void publish(int32_t *slot, int value)
 {
-    __hip_atomic_store(slot, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xF70);  // vmcnt(0)
+    asm volatile("" ::: "memory");
+    if (__lane_id() == 0) __hip_atomic_store(slot, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int load_count(const int32_t *slot)
+{
+    return __builtin_amdgcn_readfirstlane(__hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
 
 // Spins until the producer's published count reaches `need` (uniform exit
 // test on lane 0's value).  A producer always ends a stripe by publishing a
-// count past every need of its consumer, so the wait ends.
-__device__ __forceinline__ int wait_count(const int32_t *slot, int need, int known)
+// count past every need of its consumer, so the wait ends once the
+// producer runs; past kWaitTicks it gives up, sets `late` and returns a
+// count that satisfies every later wait of the wave.
+__device__ __forceinline__ int wait_count(const int32_t *slot, int need, int known, int &late)
 {
     if (known >= need) return known;
-    int seen = __builtin_amdgcn_readfirstlane(__hip_atomic_load(slot, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
-    while (seen < need) {
-        __builtin_amdgcn_s_sleep(1);
-        seen = __builtin_amdgcn_readfirstlane(__hip_atomic_load(slot, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+    int seen = load_count(slot);
+    if (seen < need) {
+        const uint64_t t_start = wall_clock64();
+        while (seen < need) {
+            __builtin_amdgcn_s_sleep(2);
+            seen = load_count(slot);
+            if (seen < need && wall_clock64() - t_start > kWaitTicks) {
+                late = 1;
+                seen = INT32_MAX;
+            }
+        }
     }
+    asm volatile("" ::: "memory");  // the entry loads stay behind the wait
     return seen;
 }
 
@@ -177,17 +211,18 @@ __device__ __forceinline__ int band_cell(const BandConsts &k, int X, int V, int 
     return d;
 }
 
-// Up to kBandTeam waves per pair.  A pair needs about (2w + 2 SR + 300) /
-// (2 SR + 190) stripes in flight (a stripe's anti-diagonals start 2 SR
-// after its predecessor's, plus the lead), 13 for w = 4096 at SR = 256.
+// Up to kBandTeam waves per pair in workgroups of up to kBandGroup.  A
+// pair needs about (2w + 2 SR + 300) / (2 SR + 190) stripes in flight (a
+// stripe's anti-diagonals start 2 SR after its predecessor's, plus the
+// lead), 13 for w = 4096 at SR = 256.
 template <bool RIGHT, bool SCORE_ONLY>
-__global__ void __launch_bounds__(kWave * kBandTeam) ksw_band_kernel(BandArgs a)
+__global__ void __launch_bounds__(kWave * kBandGroup) ksw_band_kernel(BandArgs a)
 {
-    __shared__ int32_t prog[kBandTeam];
-    __shared__ int32_t red[kBandTeam][8];
     const int nthr = blockDim.x, tid = threadIdx.x, lane = tid % kWave;
-    const int W = __builtin_amdgcn_readfirstlane(nthr / kWave), wv = __builtin_amdgcn_readfirstlane(tid / kWave);
-    const int job = __builtin_amdgcn_readfirstlane(a.order[blockIdx.x]);
+    const int wgs = a.wgs, g = blockIdx.x % wgs;
+    const int W = __builtin_amdgcn_readfirstlane(wgs * (nthr / kWave));
+    const int wv = __builtin_amdgcn_readfirstlane(g * (nthr / kWave) + tid / kWave);
+    const int job = __builtin_amdgcn_readfirstlane(a.order[blockIdx.x / wgs]);
     const GenJob jb = a.jobs[job];
     const int qlen = __builtin_amdgcn_readfirstlane(jb.qlen), tlen = __builtin_amdgcn_readfirstlane(jb.tlen);
     const int bw = __builtin_amdgcn_readfirstlane(jb.w), flag = __builtin_amdgcn_readfirstlane(jb.flag);
@@ -204,28 +239,28 @@ __global__ void __launch_bounds__(kWave * kBandTeam) ksw_band_kernel(BandArgs a)
     const uint8_t *qs = a.seq + jb.q_off, *ts = a.seq + jb.t_off;
     uint8_t *p = a.dir + jb.dir_off;
     int32_t *off = a.offs + jb.off_off, *off_end = off + nr;
-    int32_t *buf = a.bnd + jb.bnd_off;  // two buffers of 2 * nr dwords
+    int32_t *ctl = a.bnd + jb.bnd_off;      // control block (zeroed before the launch)
+    int32_t *buf = ctl + kBandCtl;          // kBufs buffers of 2 * nr dwords
+    int32_t *prog = ctl;                    // count of wave w at prog[w * kBandCtlStride]
 
     if (!SCORE_ONLY)
-        for (int r = tid; r < r_end; r += nthr) {
+        for (int r = g * nthr + tid; r < r_end; r += wgs * nthr) {
             const Band b = band_of(r, qlen, tlen, bw, tlenB, generic);
             off[r] = b.stB, off_end[r] = b.en16;
         }
-    if (lane == 0) prog[wv] = 0;
-    __syncthreads();
 
     // per-lane results over this wave's stripes
     int mqe_h = KSW_NEG_INF_I, mqe_t = -1, mte_h = KSW_NEG_INF_I, mte_q = -1, score = KSW_NEG_INF_I, pos = 0;
     const int nstripes = (tlenB + SR - 1) / SR;
-    int known = 0;
+    int known = 0, late = 0;
     for (int s = wv, kk = 0; s < nstripes; s += W, ++kk) {
         const int t0 = s * SR, tl = t0 + lane * R;
         int r_lo, r_hi, p_lo = 0, p_hi = -1;
         stripe_range(t0, bw, r_end, r_lo, r_hi);
         if (s > 0) stripe_range(t0 - SR, bw, r_end, p_lo, p_hi);
         const int pw = (s + W - 1) % W, pbase = (s - 1) / W * (nr + 2);  // producer wave, its count base
-        int32_t *out_buf = buf + (uint64_t)(s & 1) * 2 * nr;
-        const int32_t *in_buf = buf + (uint64_t)((s + 1) & 1) * 2 * nr;
+        int32_t *out_buf = buf + (uint64_t)(s % kBufs) * 2 * nr;
+        const int32_t *in_buf = buf + (uint64_t)((s + kBufs - 1) % kBufs) * 2 * nr;
         // per position: the in-place state (scaled) and H; the score s as
         // one bit (set: mismatch; s is mch or mis in this domain, mch = 0 at
         // first), the target codes packed four to a register (zero past
@@ -269,12 +304,15 @@ __global__ void __launch_bounds__(kWave * kBandTeam) ksw_band_kernel(BandArgs a)
         for (int r = r_lo; r <= r_hi; ++r) {
             const int rb = r - (r - r_lo) % kBlock;  // block start
             if (r == rb) {
-                if (s > 0) known = wait_count(prog + pw, pbase + min(r + kLead, p_hi) + 1, known);
+                if (s > 0)
+                    known = wait_count(prog + pw * kBandCtlStride, pbase + min(r + kLead, p_hi) + 1, known, late);
                 const int j = r + lane - t0;
                 blk_q = j >= 0 && j < qlen ? (uint32_t)qs[j] : 0u;
                 const int e = min(r + lane - 1, p_hi);
                 if (s > 0 && p_hi >= p_lo && e >= p_lo) {
-                    blk_pk = in_buf[2 * e], blk_h = in_buf[2 * e + 1];
+                    const uint64_t en = __hip_atomic_load((const uint64_t *)(in_buf + 2 * e), __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT);
+                    blk_pk = (int)(uint32_t)en, blk_h = (int)(uint32_t)(en >> 32);
                 } else {
                     blk_pk = pack3(-G1, -G1, -G2), blk_h = KSW_NEG_INF_I;
                 }
@@ -330,6 +368,62 @@ __global__ void __launch_bounds__(kWave * kBandTeam) ksw_band_kernel(BandArgs a)
                 }
                 smask = sm;
                 if (!SCORE_ONLY) store_dirs(p + (uint64_t)r * ncolB + (uint64_t)(tl - b.stB), dw);
+            } else if (b.st16 > 0 && r > b.en16 && b.en0 > 0 && r != nr - 1 && b.en0 != tlen - 1 &&
+                       r - b.st0 != qlen - 1) {
+                // band-edge step without the rare rules (no injection at
+                // t == r, no column-0 start, no mte/mqe/score position): the
+                // interior form plus per-position selects for the st16 and
+                // wrap inputs, the score-bit update range, the H entry at
+                // en0, and the lane commit
+                const bool wrap = b.st16 != b.stB;
+                const bool fix16 = b.st16 >= t0 && b.st16 < t0 + SR &&
+                                   !(b.st16 - 1 >= lst16 && b.st16 - 1 <= len16);
+                const bool fixw = wrap && b.stB >= t0 && b.stB < t0 + SR;
+                int wx = 0, wvv = 0, wx2 = 0;
+                if (fixw) {
+                    const int src = (b.stB + 15 - t0) / R;
+                    wx = __builtin_amdgcn_readlane(X[R - 1], src);
+                    wvv = __builtin_amdgcn_readlane(V[R - 1], src);
+                    wx2 = __builtin_amdgcn_readlane(X2[R - 1], src);
+                }
+                const bool la = tl >= b.stB && tl + R - 1 <= b.en16;
+                // score bits: positions [st0, sc_end] take this anti-diagonal's
+                // mismatch, the others keep theirs
+                uint32_t mmw = 0;
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    const uint32_t tc = (tcw[k >> 2] >> (8 * (k & 3))) & 0xffu;
+                    const uint32_t qc = (qwin[k >> 2] >> (8 * (k & 3))) & 0xffu;
+                    mmw |= tc != qc ? 1u << k : 0u;
+                }
+                const int lo = min(max(b.st0 - tl, 0), R), hi = min(max(b.sc_end - tl + 1, 0), R);
+                const uint32_t upd = ((1u << hi) - 1u) & ~((1u << lo) - 1u);
+                smask = (smask & ~upd) | (mmw & upd);
+                uint32_t dw[2] = {0u, 0u};
+#pragma unroll
+                for (int k = R - 1; k >= 0; --k) {
+                    const int t = tl + k;
+                    int xin = k > 0 ? X[k - 1] : cx, vin = k > 0 ? V[k - 1] : cv, x2in = k > 0 ? X2[k - 1] : cx2;
+                    if (fix16 && t == b.st16) xin = L0, vin = U0, x2in = L0;
+                    if (fixw && t == b.stB) xin = wx, vin = wvv, x2in = wx2;
+                    int un, vn, xn, yn, x2n, y2n;
+                    const int d = band_cell<RIGHT, SCORE_ONLY>(kc, xin, vin, x2in, U[k], Y[k], Y2[k], (smask >> k) & 1u,
+                                                               un, vn, xn, yn, x2n, y2n);
+                    if (!SCORE_ONLY) {
+                        dw[k >> 2] |= (uint32_t)d << (8 * (k & 3));
+                        asm volatile("" : "+v"(dw[k >> 2]));
+                    }
+                    // H: set at en0 from the position above, += v on
+                    // [st0, en0); below st0 it is kept (a band one position
+                    // wide reads it), above en0 it is never read before the
+                    // position enters at en0, so it may drift there
+                    const int hab = k > 0 ? H[k - 1] : ch;
+                    const int hn = t == b.en0 ? hab - (un >> 3) : H[k] - (t >= b.st0 ? vn >> 3 : 0);
+                    H[k] = hn;
+                    hmax = max(hmax, t >= b.st0 && t <= b.en0 ? hn : KSW_NEG_INF_I);
+                    if (la) U[k] = un, V[k] = vn, X[k] = xn, Y[k] = yn, X2[k] = x2n, Y2[k] = y2n;
+                }
+                if (!SCORE_ONLY && la) store_dirs(p + (uint64_t)r * ncolB + (uint64_t)(tl - b.stB), dw);
             } else {
                 // the AVX-512 lane wrap: position stB reads position stB+15
                 // (row 7 of the next lane) from anti-diagonal r-1
@@ -397,12 +491,12 @@ __global__ void __launch_bounds__(kWave * kBandTeam) ksw_band_kernel(BandArgs a)
             if (lane == kWave - 1) {
                 const int x = -((X[R - 1] + bias) >> 3) - G1, v = -(V[R - 1] >> 3);
                 const int x2 = -((X2[R - 1] + bias) >> 3) - G2;
-                out_buf[2 * r] = pack3(x, v, x2);
-                out_buf[2 * r + 1] = H[R - 1];
+                const uint64_t en = (uint64_t)(uint32_t)pack3(x, v, x2) | (uint64_t)(uint32_t)H[R - 1] << 32;
+                __hip_atomic_store((uint64_t *)(out_buf + 2 * r), en, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-            if ((r - r_lo) % kBlock == kBlock - 1) publish(prog + wv, kk * (nr + 2) + r + 1);
+            if ((r - r_lo) % kPublish == kPublish - 1) publish(prog + wv * kBandCtlStride, kk * (nr + 2) + r + 1);
         }
-        publish(prog + wv, kk * (nr + 2) + nr + 1);  // past every need of the next stripe
+        publish(prog + wv * kBandCtlStride, kk * (nr + 2) + nr + 1);  // past every need of the next stripe
         if (hmax > 0) pos = 1;
     }
 
@@ -417,38 +511,81 @@ __global__ void __launch_bounds__(kWave * kBandTeam) ksw_band_kernel(BandArgs a)
         score = max(score, __shfl_xor(score, o, kWave));
         pos |= __shfl_xor(pos, o, kWave);
     }
+    // each wave leaves its record; the last to take a ticket combines them
+    // in wave order and writes the pair's results
+    int32_t *rec = ctl + kBandCtlRec;
+    pos |= late;
+    int last = 0;
     if (lane == 0) {
-        red[wv][0] = mqe_h, red[wv][1] = mqe_t, red[wv][2] = mte_h, red[wv][3] = mte_q;
-        red[wv][4] = score, red[wv][5] = pos;
+        int32_t *m = rec + 8 * wv;
+        m[0] = mqe_h, m[1] = mqe_t, m[2] = mte_h, m[3] = mte_q, m[4] = score, m[5] = pos;
+        last = __hip_atomic_fetch_add(ctl + kBandCtlTicket, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == W - 1;
     }
-    __syncthreads();
-    if (tid == 0) {
-        for (int k = 1; k < W; ++k) {
-            if (red[k][0] > mqe_h || (red[k][0] == mqe_h && red[k][1] > mqe_t)) mqe_h = red[k][0], mqe_t = red[k][1];
-            if (red[k][3] >= 0 && (mte_q < 0 || red[k][2] > mte_h)) mte_h = red[k][2], mte_q = red[k][3];
-            score = max(score, red[k][4]);
-            pos |= red[k][5];
+    if (lane == 0 && last) {
+        mqe_h = KSW_NEG_INF_I, mqe_t = -1, mte_h = KSW_NEG_INF_I, mte_q = -1, score = KSW_NEG_INF_I, pos = 0;
+        for (int k = 0; k < W; ++k) {
+            const int32_t *m = rec + 8 * k;
+            if (m[0] > mqe_h || (m[0] == mqe_h && m[1] > mqe_t)) mqe_h = m[0], mqe_t = m[1];
+            if (m[3] >= 0 && (mte_q < 0 || m[2] > mte_h)) mte_h = m[2], mte_q = m[3];
+            score = max(score, m[4]);
+            pos |= m[5];
         }
         KswOut o;
         o.score = score, o.mqe = mqe_h, o.mqe_t = mqe_t, o.mte = mte_h, o.mte_q = mte_q;
         o.max = 0, o.max_t = -1, o.max_q = -1;
         o.zdropped = r_end < nr ? 1 : 0, o.stopped = 0, o.reach_end = 0, o.n_cigar = 0;
         a.out[jb.out] = o;
-        a.redo[job] = pos | a.force_redo;
+        a.redo[job] = (pos ? 1 : 0) | a.force_redo;
     }
 }
 
-hipError_t launch_band(const BandArgs &a, int team, bool right, bool score_only, hipStream_t st)
+// Zeroes the control blocks of a launch's jobs.
+__global__ void __launch_bounds__(256) ksw_band_ctl_kernel(int32_t *bnd, const GenJob *jobs, const int32_t *order)
 {
-    if (a.njobs <= 0) return hipSuccess;
+    int32_t *ctl = bnd + jobs[order[blockIdx.x]].bnd_off;
+    for (int k = threadIdx.x; k < kBandCtl; k += blockDim.x) ctl[k] = 0;
+}
+
+namespace {
+
+// Workgroups of one band-kernel variant that the device holds at once.
+int band_capacity(const void *fn, int threads)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0) != hipSuccess) return 0;
+    return cus * per_cu;
+}
+
+}  // namespace
+
+hipError_t launch_band(const BandArgs &a0, int team, bool right, bool score_only, hipStream_t st)
+{
+    if (a0.njobs <= 0) return hipSuccess;
     if (team < 1 || team > kBandTeam) return hipErrorInvalidValue;
-    if (score_only)
-        hipLaunchKernelGGL((ksw_band_kernel<false, true>), dim3(a.njobs), dim3(kWave * team), 0, st, a);
-    else if (right)
-        hipLaunchKernelGGL((ksw_band_kernel<true, false>), dim3(a.njobs), dim3(kWave * team), 0, st, a);
-    else
-        hipLaunchKernelGGL((ksw_band_kernel<false, false>), dim3(a.njobs), dim3(kWave * team), 0, st, a);
-    return hipGetLastError();
+    BandArgs a = a0;
+    const int wpg = team < kBandGroup ? team : kBandGroup;
+    a.wgs = (team + wpg - 1) / wpg;
+    const int threads = kWave * wpg;
+    void (*fn)(BandArgs) = score_only ? ksw_band_kernel<false, true>
+                           : right    ? ksw_band_kernel<true, false>
+                                      : ksw_band_kernel<false, false>;
+    // half the device's capacity per launch: the workgroups of a launch
+    // wait on one another, so all of them must be resident together even
+    // beside the kernels of other streams (which finish on their own)
+    const int cap = band_capacity((const void *)fn, threads);
+    if (cap < 2 * a.wgs) return hipErrorInvalidConfiguration;
+    const int chunk = cap / 2 / a.wgs;
+    for (int off = 0; off < a0.njobs; off += chunk) {
+        a.order = a0.order + off;
+        a.njobs = a0.njobs - off < chunk ? a0.njobs - off : chunk;
+        hipLaunchKernelGGL(ksw_band_ctl_kernel, dim3(a.njobs), dim3(256), 0, st, a.bnd, a.jobs, a.order);
+        hipLaunchKernelGGL(fn, dim3(a.njobs * a.wgs), dim3(threads), 0, st, a);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace kswhip

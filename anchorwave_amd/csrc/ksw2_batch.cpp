@@ -42,6 +42,22 @@ uint64_t mem_budget()
 
 }  // namespace
 
+namespace {
+std::mutex g_band_mu;
+hipStream_t g_band_stream = nullptr;
+
+// The stream every band-kernel launch of the process goes through.  The
+// workgroups of a band launch wait on one another, so a launch is sized to
+// be resident as a whole; one stream keeps two of them from running at once.
+hipStream_t band_stream()
+{
+    std::lock_guard<std::mutex> lk(g_band_mu);
+    if (!g_band_stream && hipStreamCreateWithFlags(&g_band_stream, hipStreamNonBlocking) != hipSuccess)
+        g_band_stream = nullptr;
+    return g_band_stream;
+}
+}  // namespace
+
 int device_init(int device)
 {
     std::lock_guard<std::mutex> lk(g_init_mu);
@@ -116,8 +132,8 @@ uint64_t pair_footprint(const ksw2_pair_t &p, bool general)
     const int tlenB = (p.tlen + 63) / 64 * 64, qlenB = (p.qlen + 63) / 64 * 64;
     const uint64_t diags = (uint64_t)p.qlen + p.tlen - 1;
     const uint64_t rows = cigar ? tracks * diags * (uint64_t)gen_ncolB(p.qlen, p.tlen, norm_w(p)) + 256 : 0;
-    // (+ the band kernel's two stripe buffers and redo flag, whichever kernel runs it)
-    return gen_layout(tlenB, qlenB).bytes + rows + (cigar ? 8 * diags : 0) + cig + seq + 16 * diags + 516;
+    // (+ the band kernel's control block, three stripe buffers and redo flag, whichever kernel runs it)
+    return gen_layout(tlenB, qlenB).bytes + rows + (cigar ? 8 * diags : 0) + cig + seq + 24 * diags + 4 * kBandCtl + 516;
 }
 
 void make_gen_score(int8_t m, const int8_t *mat, const KswScore &sc, const ScoreClass &cls, GenScore *g)
@@ -254,7 +270,7 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         j.r_end = 0, j.bnd_off = 0;
         if (band[gk]) {
             j.r_end = band_r_end(pp.qlen, pp.tlen, j.w);
-            j.bnd_off = bo, bo += round_up(4ull * ((uint64_t)pp.qlen + pp.tlen - 1), 64);
+            j.bnd_off = bo, bo += kBandCtl + round_up(6ull * ((uint64_t)pp.qlen + pp.tlen - 1), 64);
         }
         j.q_off = so, so = round_up(so + pp.qlen, 16);
         j.t_off = so, so = round_up(so + pp.tlen, 16);
@@ -364,6 +380,9 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
     }
     for (auto &e : p->ev)
         if (hipEventCreate(&e) != hipSuccess) return fail(KSW2_HIP_EDEVICE);
+    if (p->n_band > 0)
+        for (auto &e : p->ev_band)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return fail(KSW2_HIP_EDEVICE);
     // one device block holds every arena
     uint64_t at = 0;
     auto take = [&](uint64_t bytes) {
@@ -510,12 +529,16 @@ int plan_launch(Plan *p, const Progress *prog)
         ba.out = p->d_out, ba.redo = p->d_redo, ba.sc = p->gsc;
         const char *fr = getenv("KSW2_HIP_BAND_REDO");
         ba.force_redo = fr && atoi(fr) > 0 ? 1 : 0;
-        hipStream_t st = lane(0);
-        if (!st) return KSW2_HIP_EDEVICE;
+        hipStream_t st = lane(0), bs = band_stream();
+        if (!st || !bs) return KSW2_HIP_EDEVICE;
+        KSW_HIP_TRY(hipEventRecord(p->ev_band[0], st));
+        KSW_HIP_TRY(hipStreamWaitEvent(bs, p->ev_band[0], 0));
         for (const Plan::BLaunch &l : p->blaunches) {
             ba.order = p->d_gorder + l.off, ba.njobs = l.count;
-            KSW_HIP_TRY(launch_band(ba, l.team, l.grp == 1, l.grp == 2, st));
+            KSW_HIP_TRY(launch_band(ba, l.team, l.grp == 1, l.grp == 2, bs));
         }
+        KSW_HIP_TRY(hipEventRecord(p->ev_band[1], bs));
+        KSW_HIP_TRY(hipStreamWaitEvent(st, p->ev_band[1], 0));
         // pairs whose H went above 0 (the band kernel does not track the
         // maximum) run again on the general kernel, on the same stream
         ga.order = p->d_gorder + p->band_off, ga.njobs = p->n_band, ga.ring = 0, ga.only_if = p->d_redo;
@@ -636,6 +659,8 @@ void plan_destroy(Plan *p)
     if (p->h_stop && p->own_stop) (void)hipHostFree(p->h_stop);
     if (p->h_seq && p->own_seq) (void)hipHostFree(p->h_seq);
     for (auto &e : p->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : p->ev_band)
         if (e) (void)hipEventDestroy(e);
     if (p->own_streams) {
         for (int l = 0; l < Plan::kLanes; ++l) {

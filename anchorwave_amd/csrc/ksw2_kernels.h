@@ -53,17 +53,31 @@ struct BandArgs {
     const uint8_t *seq;
     uint8_t *dir;              // direction rows (reference layout)
     int32_t *offs;             // off[] / off_end[] per pair
-    int32_t *bnd;              // per pair: two stripe hand-off buffers of 2 * (qlen+tlen-1) dwords
+    int32_t *bnd;              // per pair: kBandCtl control dwords, then three stripe hand-off
+                               // buffers of 2 * (qlen+tlen-1) dwords
     const GenJob *jobs;
-    const int32_t *order;      // this launch's jobs, one workgroup each
+    const int32_t *order;      // this launch's jobs, `wgs` workgroups each
     int njobs;
+    int wgs;                   // workgroups per job (set by launch_band)
     KswOut *out;
     int32_t *redo;             // per job: 1 when H > 0 was seen (the general kernel must run it)
     GenScore sc;
     int force_redo;            // tests: flag every job for the general kernel
 };
-constexpr int kBandTeam = 16;       // waves per workgroup of the band kernel, at most
-constexpr int kBandRowsPerLane = 4;  // positions per lane; a stripe is 64 lanes of them
+constexpr int kBandTeam = 16;       // waves per pair, at most
+constexpr int kBandGroup = 4;       // waves per workgroup: one per SIMD of a CU
+constexpr int kBandRowsPerLane = 4; // positions per lane; a stripe is 64 lanes of them
+// Control block of a band job: progress counts (one per team wave, 16
+// dwords apart), the finish ticket, per-wave result records.
+constexpr int kBandCtlStride = 16;
+constexpr int kBandCtlTicket = kBandTeam * kBandCtlStride;
+constexpr int kBandCtlRec = kBandCtlTicket + 16;
+constexpr int kBandCtl = 512;
+static_assert(kBandCtlRec + 8 * kBandTeam <= kBandCtl, "band control block");
+// Band kernel for a team of `team` waves per job: kBandGroup-wave
+// workgroups on different CUs that hand stripes on through device-scope
+// counters.  Launches go in chunks whose workgroups all fit the device at
+// once, so no spinning workgroup waits on one that cannot be placed.
 hipError_t launch_band(const BandArgs &a, int team, bool right, bool score_only, hipStream_t st);
 
 // Singletrack walk of general pairs: one wavefront per job (all general
