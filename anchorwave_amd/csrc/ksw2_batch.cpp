@@ -564,6 +564,9 @@ int plan_launch(Plan *p, const Progress *prog)
         GenBacktrackArgs gb;
         gb.dir = p->d_dir, gb.offs = p->d_offs, gb.cig = p->d_cig, gb.jobs = p->d_gjobs, gb.out = p->d_out;
         gb.njobs = p->n_gen;
+        // pairs of qlen + tlen >= 512 walk one per wave (KSW2_HIP_TILED_MIN: tests)
+        const char *tm = getenv("KSW2_HIP_TILED_MIN");
+        gb.tiled_min = tm ? atoi(tm) : 512;
         KSW_HIP_TRY(launch_general_backtrack(gb, p->stream));
         if (p->n_tracks_gen > 0) {
             GenWalkArgs gw;

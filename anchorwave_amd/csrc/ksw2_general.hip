@@ -392,27 +392,50 @@ __global__ void __launch_bounds__(kWave * kMaxWaves) ksw_general_kernel(GenArgs 
 // is_rot = 1 (minimap2/ksw2.h:160-192), including the forcing of cells
 // outside [off[r], off_end[r]], from the start cell of
 // minimap2/ksw2_extd2_sse.c:398-418.  One thread per pair.
+// Where the walk of a general pair starts (minimap2/ksw2_extd2_sse.c:
+// the end-of-DP cases before ksw_backtrack): adjusts o's score and
+// reach_end as the reference does; (-1, -1) when there is no CIGAR.
+__device__ __forceinline__ void backtrack_start(const GenJob &jb, KswOut &o, int &i0, int &j0)
+{
+    const int flag = jb.flag, qlen = jb.qlen, tlen = jb.tlen;
+    i0 = -1, j0 = -1;
+    if (flag & kFlagScoreOnly) return;
+    if (!o.stopped && !o.zdropped && (flag & 0x800)) {          // SEMIGLOBAL_END
+        if (o.mqe > o.mte) o.score = o.mqe, i0 = o.mqe_t, j0 = qlen - 1;
+        else o.score = o.mte, i0 = tlen - 1, j0 = o.mte_q;
+    } else if (!o.stopped && !o.zdropped && !(flag & 0x40)) {   // global
+        i0 = tlen - 1, j0 = qlen - 1;
+    } else if (!o.zdropped && (flag & 0x40) && o.mqe + jb.end_bonus > o.max) {  // EXTZ_ONLY
+        o.reach_end = 1, i0 = o.mqe_t, j0 = qlen - 1;
+    } else if (o.max_t >= 0 && o.max_q >= 0) {
+        i0 = o.max_t, j0 = o.max_q;
+    }
+}
+
+// One step of ksw_backtrack's state machine (minimap2/ksw2.h:142-170):
+// `tmp` is the direction byte (0 when forced or outside the row), `force`
+// 1 or 2 outside [off, off_end], else -1.  Returns the CIGAR op: 0 (i and j
+// step back), 2 (i only), 1 (j only).
+__device__ __forceinline__ int backtrack_op(int tmp, int force, int &state)
+{
+    if (state == 0) state = tmp & 7;
+    else if (!(tmp >> (state + 2) & 1)) state = 0;
+    if (state == 0) state = tmp & 7;
+    if (force >= 0) state = force;
+    return state == 0 ? 0 : state == 1 || state == 3 ? 2 : 1;
+}
+
 __global__ void __launch_bounds__(64) ksw_general_backtrack_kernel(GenBacktrackArgs a)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.njobs) return;
     const GenJob jb = a.jobs[k];
-    KswOut o = a.out[jb.out];
     const int flag = jb.flag, qlen = jb.qlen, tlen = jb.tlen;
     if ((flag & kFlagSingletrack) && !(flag & kFlagScoreOnly)) return;  // ksw_general_walk_kernel
-    int i0 = -1, j0 = -1;
-    if (!(flag & kFlagScoreOnly)) {
-        if (!o.stopped && !o.zdropped && (flag & 0x800)) {          // SEMIGLOBAL_END
-            if (o.mqe > o.mte) o.score = o.mqe, i0 = o.mqe_t, j0 = qlen - 1;
-            else o.score = o.mte, i0 = tlen - 1, j0 = o.mte_q;
-        } else if (!o.stopped && !o.zdropped && !(flag & 0x40)) {   // global
-            i0 = tlen - 1, j0 = qlen - 1;
-        } else if (!o.zdropped && (flag & 0x40) && o.mqe + jb.end_bonus > o.max) {  // EXTZ_ONLY
-            o.reach_end = 1, i0 = o.mqe_t, j0 = qlen - 1;
-        } else if (o.max_t >= 0 && o.max_q >= 0) {
-            i0 = o.max_t, j0 = o.max_q;
-        }
-    }
+    if (qlen + tlen >= a.tiled_min) return;                             // ksw_general_backtrack_tiled_kernel
+    KswOut o = a.out[jb.out];
+    int i0, j0;
+    backtrack_start(jb, o, i0, j0);
     int n = 0;
     if (i0 >= 0 || j0 >= 0) {
         const uint8_t *p = a.dir + jb.dir_off;
@@ -433,13 +456,9 @@ __global__ void __launch_bounds__(64) ksw_general_backtrack_kernel(GenBacktrackA
             // keeps a damaged row from reading outside its own bytes)
             const bool inrow = (uint32_t)(i - off[r]) < (uint32_t)jb.ncolB;
             const int tmp = force < 0 && inrow ? p[(uint64_t)r * jb.ncolB + (uint64_t)(i - off[r])] : 0;
-            if (state == 0) state = tmp & 7;
-            else if (!(tmp >> (state + 2) & 1)) state = 0;
-            if (state == 0) state = tmp & 7;
-            if (force >= 0) state = force;
-            if (state == 0) push(0, 1), --i, --j;
-            else if (state == 1 || state == 3) push(2, 1), --i;
-            else push(1, 1), --j;
+            const int op = backtrack_op(tmp, force, state);
+            i -= op != 1, j -= op != 2;
+            push(op, 1);
         }
         if (i >= 0) push(2, i + 1);
         if (j >= 0) push(1, j + 1);
@@ -452,6 +471,123 @@ __global__ void __launch_bounds__(64) ksw_general_backtrack_kernel(GenBacktrackA
     }
     o.n_cigar = n;
     a.out[jb.out] = o;
+}
+
+// The same walk for long pairs, one wave per pair.  A step moves to
+// anti-diagonal r-1 or r-2 and to target position i or i-1, so the next 64
+// anti-diagonals stay within positions [i-127, i] of the current (r, i):
+// the wave stages those 64 rows (132 bytes each, from the 4-byte boundary
+// at or below i-127, zero outside the row) and their off/off_end in LDS,
+// and the walk reads LDS until it leaves the tile.  The walk itself is
+// uniform (scalar); the CIGAR runs go out one store per run, and the wave
+// reverses them at the end.
+constexpr int kTileRows = 64, kTileDw = 33;
+
+__global__ void __launch_bounds__(64) ksw_general_backtrack_tiled_kernel(GenBacktrackArgs a)
+{
+    __shared__ uint32_t tile[kTileRows * kTileDw];
+    __shared__ int32_t t_off[kTileRows], t_end[kTileRows], t_base[kTileRows];
+    const int lane = threadIdx.x;
+    const GenJob jb = a.jobs[blockIdx.x];
+    const int flag = __builtin_amdgcn_readfirstlane(jb.flag);
+    const int qlen = __builtin_amdgcn_readfirstlane(jb.qlen), tlen = __builtin_amdgcn_readfirstlane(jb.tlen);
+    if ((flag & kFlagSingletrack) && !(flag & kFlagScoreOnly)) return;
+    if (qlen + tlen < a.tiled_min) return;
+    KswOut o = a.out[jb.out];
+    int i0, j0;
+    backtrack_start(jb, o, i0, j0);
+    int n = 0;
+    if (i0 >= 0 || j0 >= 0) {
+        const int ncolB = __builtin_amdgcn_readfirstlane(jb.ncolB);
+        const uint32_t *p32 = (const uint32_t *)(a.dir + jb.dir_off);
+        const int32_t *off = a.offs + jb.off_off, *off_end = off + (qlen + tlen - 1);
+        uint32_t *cig = a.cig + jb.cig_off;
+        const uint8_t *tb = (const uint8_t *)tile;
+        int i = __builtin_amdgcn_readfirstlane(i0), j = __builtin_amdgcn_readfirstlane(j0), state = 0;
+        int run_op = -1, run_len = 0;
+        int r0 = -1, ws = 0;  // tile: anti-diagonals (r0 - 64, r0], positions from ws
+        while (i >= 0 && j >= 0) {
+            const int r = i + j;
+            if (r0 < 0 || r <= r0 - kTileRows || i < ws) {
+                r0 = r, ws = i - 127;
+                __syncthreads();  // the walk's last reads of the old tile are done
+                const int row = r0 - lane;
+                int ro = 0, re = -1, base = 0;
+                if (row >= 0) {
+                    ro = off[row], re = off_end[row];
+                    const int d0 = (ws - ro) >> 2;  // floor
+                    base = ro + 4 * d0;
+                    const uint32_t *src = p32 + (uint64_t)row * (uint32_t)(ncolB >> 2);
+                    // all loads in flight at once (clamped in-row addresses),
+                    // then the bytes outside the row zeroed
+                    uint32_t v[kTileDw];
+#pragma unroll
+                    for (int q = 0; q < kTileDw; ++q) v[q] = src[min(max(d0 + q, 0), (ncolB >> 2) - 1)];
+#pragma unroll
+                    for (int q = 0; q < kTileDw; ++q) {
+                        const int d = d0 + q;
+                        tile[lane * kTileDw + q] = d >= 0 && d < (ncolB >> 2) ? v[q] : 0u;
+                    }
+                }
+                t_off[lane] = ro, t_end[lane] = re, t_base[lane] = base;
+                __syncthreads();
+            }
+            const int L = r0 - r;
+            const int ro = __builtin_amdgcn_readfirstlane(t_off[L]), re = __builtin_amdgcn_readfirstlane(t_end[L]);
+            int force = -1;
+            if (i < ro) force = 2;
+            if (i > re) force = 1;
+            int tmp = 0;
+            if (force < 0 && (uint32_t)(i - ro) < (uint32_t)ncolB) {
+                const int b = i - __builtin_amdgcn_readfirstlane(t_base[L]);
+                tmp = __builtin_amdgcn_readfirstlane((int)tb[L * kTileDw * 4 + b]);
+            }
+            const int op = backtrack_op(tmp, force, state);
+            i -= op != 1, j -= op != 2;
+            if (op == run_op) {
+                ++run_len;
+            } else {
+                if (run_len > 0 && lane == 0) cig[n] = (uint32_t)run_len << 4 | (uint32_t)run_op;
+                if (run_len > 0) ++n;
+                run_op = op, run_len = 1;
+            }
+        }
+        // the tails merge with the last run (push of ksw_backtrack)
+        if (i >= 0) {
+            if (run_op == 2) run_len += i + 1;
+            else {
+                if (run_len > 0 && lane == 0) cig[n] = (uint32_t)run_len << 4 | (uint32_t)run_op;
+                if (run_len > 0) ++n;
+                run_op = 2, run_len = i + 1;
+            }
+        }
+        if (j >= 0) {
+            if (run_op == 1) run_len += j + 1;
+            else {
+                if (run_len > 0 && lane == 0) cig[n] = (uint32_t)run_len << 4 | (uint32_t)run_op;
+                if (run_len > 0) ++n;
+                run_op = 1, run_len = j + 1;
+            }
+        }
+        if (run_len > 0 && lane == 0) cig[n] = (uint32_t)run_len << 4 | (uint32_t)run_op;
+        if (run_len > 0) ++n;
+        if (!(flag & 0x80)) {
+            // lane 0's stores complete before the wave reads them back with
+            // device-scope loads (past the CU's L1)
+            __builtin_amdgcn_s_waitcnt(0xF70);  // vmcnt(0)
+            __syncthreads();
+            for (int x = lane; x < (n >> 1); x += kWave) {
+                const uint32_t lo = __hip_atomic_load(cig + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t hi = __hip_atomic_load(cig + n - 1 - x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                cig[x] = hi;
+                cig[n - 1 - x] = lo;
+            }
+        }
+    }
+    if (lane == 0) {
+        o.n_cigar = n;
+        a.out[jb.out] = o;
+    }
 }
 
 // Singletrack tracks of one general pair: reference layout, two row arrays
@@ -532,6 +668,7 @@ hipError_t launch_general_backtrack(const GenBacktrackArgs &a, hipStream_t st)
 {
     if (a.njobs <= 0) return hipSuccess;
     hipLaunchKernelGGL(ksw_general_backtrack_kernel, dim3((a.njobs + 63) / 64), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(ksw_general_backtrack_tiled_kernel, dim3(a.njobs), dim3(64), 0, st, a);
     return hipGetLastError();
 }
 

@@ -46,6 +46,7 @@ struct GenBacktrackArgs {
     const GenJob *jobs;
     KswOut *out;
     int njobs;
+    int tiled_min;             // pairs with qlen + tlen >= this walk one per wave (staged rows)
 };
 
 // Band kernel (ksw2_band.hip): one workgroup of `team` waves per job.

@@ -151,3 +151,15 @@ def test_long_banded_pairs(ring, monkeypatch):
         g = ksw2.extd2(q, t, **kw)
         r = O.run_oracle(q, t, **kw)
         assert same(g, r), f"case {k} ring={ring}: {diff(g, r)}"
+
+
+@pytest.mark.parametrize("name", ["band_narrow", "zdrop", "zdrop_band"])
+def test_wave_walk_on_every_pair(name, monkeypatch):
+    """The one-wave walk with staged rows (long pairs' backtrack) forced on
+    every general pair, short ones included: the reference vectors and the
+    oracle scenarios over every flag (left/right alignment, reversed CIGAR,
+    EXTZ_ONLY, SEMIGLOBAL_END, z-drop stops)."""
+    monkeypatch.setenv("KSW2_HIP_TILED_MIN", "0")
+    test_scenarios_vs_oracle(name)
+    if name == "band_narrow":
+        test_golden_vectors_batched()
