@@ -64,11 +64,13 @@ namespace {
 
 constexpr int R = kBandRowsPerLane;  // positions per lane (4: one step of a stripe is short)
 constexpr int SR = kWave * R;        // positions per stripe
+static_assert(R == 4, "a lane's direction bytes and query window are one dword");
 constexpr int kBlock = 64;       // anti-diagonals per block of hand-off loads
 constexpr int kPublish = 32;     // anti-diagonals between progress publications
 constexpr int kLead = 64;        // lead a stripe waits for on its predecessor
 constexpr int kBufs = 3;         // hand-off buffers, by stripe mod 3
-constexpr int kDppWaveShr1 = 0x138;
+static_assert(kBlock % kPublish == 0, "publications fall on block offsets");
+constexpr int kDppWaveShr1 = 0x138, kDppWaveShl1 = 0x130;
 constexpr uint64_t kWaitTicks = 1000000000ull;  // 10 s of the 100 MHz real-time clock
 
 struct Band {
@@ -86,10 +88,11 @@ __device__ __forceinline__ Band band_of(int r, int qlen, int tlen, int w, int tl
     if (en > r) en = r;
     if (st < (r - w + 1) >> 1) st = (r - w + 1) >> 1;
     if (en > (r + w) >> 1) en = (r + w) >> 1;
+    // (0 <= st <= en: the 16- and 64-roundings are masks)
     Band b;
     b.st0 = st, b.en0 = en;
-    b.st16 = st / 16 * 16, b.en16 = (en + 16) / 16 * 16 - 1, b.stB = st / 64 * 64;
-    b.sc_end = generic ? en : min(st + (en - st + 16) / 16 * 16 - 1, tlenB - 1);
+    b.st16 = st & ~15, b.en16 = en | 15, b.stB = st & ~63;
+    b.sc_end = generic ? en : min(st + ((en - st + 16) & ~15) - 1, tlenB - 1);
     return b;
 }
 
@@ -102,7 +105,13 @@ __device__ __forceinline__ void stripe_range(int t0, int w, int r_end, int &lo, 
     hi = min(r_end - 1, 2 * (t0 + SR - 1) + w + 130);
 }
 
-__device__ __forceinline__ int shr1(int v) { return __builtin_amdgcn_update_dpp(0, v, kDppWaveShr1, 0xF, 0xF, false); }
+// lane k takes lane k-1's v, lane 0 keeps `old`
+__device__ __forceinline__ int dpp_shr1(int old, int v)
+{
+    return __builtin_amdgcn_update_dpp(old, v, kDppWaveShr1, 0xF, 0xF, false);
+}
+// lane k takes lane k+1's v (lane 63 keeps its own)
+__device__ __forceinline__ int dpp_shl1(int v) { return __builtin_amdgcn_update_dpp(v, v, kDppWaveShl1, 0xF, 0xF, false); }
 
 // Progress count: stored after the wave's earlier hand-off stores are
 // complete (s_waitcnt vmcnt(0)); both are device-scope stores that bypass
@@ -211,6 +220,87 @@ __device__ __forceinline__ int band_cell(const BandConsts &k, int X, int V, int 
     return d;
 }
 
+// A band-edge step in a block clear of the rare boundary rules: the
+// interior form plus what the band's edges change.  LEAD: the band starts
+// above the stripe (st0 < t0), so no start rule applies; TRAIL: it ends
+// below the stripe (en0 > t0 + SR - 1), so no end rule applies.
+//  * Inputs of the start (minimap2/ksw2_extd2_sse.c:158-172): position stB
+//    reads position stB+15 of anti-diagonal r-1 when st16 != stB (the
+//    AVX-512 lane wrap), position st16 the initial x, v, x2 unless position
+//    st16-1 was computed on r-1.  Both become the inputs of that position
+//    (the above-values of a position are read by the one below only).
+//  * Score bits: positions [st0, sc_end] take this anti-diagonal's mismatch.
+//  * H: set at en0 from the position above, += v on [st0, en0); kept below
+//    st0 (a band one position wide reads it), free to drift above en0 (a
+//    position enters at en0 from the position above).
+//  * Commit: a lane above en16 keeps its state (it enters the band later
+//    with it); below stB a lane's positions are never read again.
+template <bool RIGHT, bool SCORE_ONLY, bool LEAD, bool TRAIL>
+__device__ __forceinline__ void edge_step(const BandConsts &kc, const Band &b, int t0, int tl, int lane, int (&U)[R],
+                                          int (&V)[R], int (&X)[R], int (&Y)[R], int (&X2)[R], int (&Y2)[R],
+                                          int (&H)[R], uint32_t &smask, uint32_t tcw, uint32_t qwin, int cx, int cv,
+                                          int cx2, int ch, bool keep_prev, int U0, int L0, int &hmax, uint8_t *prow)
+{
+    if (!LEAD) {
+        if (b.st16 != b.stB && b.stB >= t0 && b.stB < t0 + SR) {
+            const int src = (b.stB + 15 - t0) / R;  // position stB+15 is row R-1 of this lane
+            const int wx = __builtin_amdgcn_readlane(X[R - 1], src), wv = __builtin_amdgcn_readlane(V[R - 1], src);
+            const int wx2 = __builtin_amdgcn_readlane(X2[R - 1], src);
+            const bool me = lane == (b.stB - t0) / R;
+            cx = me ? wx : cx, cv = me ? wv : cv, cx2 = me ? wx2 : cx2;
+        }
+        if (!keep_prev && b.st16 >= t0 && b.st16 < t0 + SR) {
+            const int k16 = (b.st16 - t0) % R;
+            const bool me = lane == (b.st16 - t0) / R;
+            if (k16 == 0) {
+                cx = me ? L0 : cx, cv = me ? U0 : cv, cx2 = me ? L0 : cx2;
+            }
+#pragma unroll
+            for (int k = 1; k < R; ++k)
+                if (k16 == k) {
+                    X[k - 1] = me ? L0 : X[k - 1], V[k - 1] = me ? U0 : V[k - 1];
+                    X2[k - 1] = me ? L0 : X2[k - 1];
+                }
+        }
+    }
+    const bool la = (LEAD || tl >= b.stB) && (TRAIL || tl + R - 1 <= b.en16);
+    const int lam = la ? -1 : 0;
+    uint32_t mmw = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) mmw |= ((tcw ^ qwin) >> (8 * k) & 0xffu) != 0 ? 1u << k : 0u;
+    const int lo = LEAD ? 0 : min(max(b.st0 - tl, 0), R);
+    const int hi = TRAIL ? R : min(max(b.sc_end - tl + 1, 0), R);
+    const uint32_t upd = ((1u << hi) - 1u) & ~((1u << lo) - 1u);
+    smask = (smask & ~upd) | (mmw & upd);
+    uint32_t dw = 0;
+#pragma unroll
+    for (int k = R - 1; k >= 0; --k) {
+        const int t = tl + k;
+        const int xin = k > 0 ? X[k - 1] : cx, vin = k > 0 ? V[k - 1] : cv, x2in = k > 0 ? X2[k - 1] : cx2;
+        int un, vn, xn, yn, x2n, y2n;
+        const int d = band_cell<RIGHT, SCORE_ONLY>(kc, xin, vin, x2in, U[k], Y[k], Y2[k], (smask >> k) & 1u, un, vn,
+                                                   xn, yn, x2n, y2n);
+        if (!SCORE_ONLY) {
+            dw |= (uint32_t)d << (8 * k);
+            asm volatile("" : "+v"(dw));
+        }
+        const int hab = k > 0 ? H[k - 1] : ch;
+        const int vdec = (LEAD || t >= b.st0) ? vn >> 3 : 0;  // v = -(V / 8)
+        const int hn = !TRAIL && t == b.en0 ? hab - (un >> 3) : H[k] - vdec;
+        H[k] = hn;
+        const bool inb = (LEAD || t >= b.st0) && (TRAIL || t <= b.en0);
+        hmax = max(hmax, inb ? hn : KSW_NEG_INF_I);
+        if (TRAIL) {
+            U[k] = un, V[k] = vn, X[k] = xn, Y[k] = yn, X2[k] = x2n, Y2[k] = y2n;
+        } else {
+            U[k] = (un & lam) | (U[k] & ~lam), V[k] = (vn & lam) | (V[k] & ~lam);
+            X[k] = (xn & lam) | (X[k] & ~lam), Y[k] = (yn & lam) | (Y[k] & ~lam);
+            X2[k] = (x2n & lam) | (X2[k] & ~lam), Y2[k] = (y2n & lam) | (Y2[k] & ~lam);
+        }
+    }
+    if (!SCORE_ONLY && la) *(uint32_t *)(prow + (tl - b.stB)) = dw;
+}
+
 // Up to kBandTeam waves per pair in workgroups of up to kBandGroup.  A
 // pair needs about (2w + 2 SR + 300) / (2 SR + 190) stripes in flight (a
 // stripe's anti-diagonals start 2 SR after its predecessor's, plus the
@@ -266,7 +356,7 @@ __global__ void __launch_bounds__(kWave * kBandGroup) ksw_band_kernel(BandArgs a
         // first), the target codes packed four to a register (zero past
         // tlen, as the reference's zero-padded target copy)
         int U[R], Y[R], Y2[R], X[R], V[R], X2[R], H[R];
-        uint32_t smask = 0, tcw[2] = {0u, 0u};
+        uint32_t smask = 0, tcw[1] = {0u};
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             U[k] = V[k] = U0;
@@ -275,226 +365,188 @@ __global__ void __launch_bounds__(kWave * kBandGroup) ksw_band_kernel(BandArgs a
             const uint32_t c = tl + k < tlen ? (uint32_t)ts[tl + k] : 0u;
             tcw[k >> 2] |= c << (8 * (k & 3));
         }
-        // query window: byte k of (qwin[1]:qwin[0]) is the query code of
-        // position tl + k's column r - (tl + k) (0 outside the query, as the
+        // query window: byte k of qwin[0] is the query code of position
+        // tl + k's column r - (tl + k) (0 outside the query, as the
         // reference's zero-tailed reversed query).  Each step lane k's new
-        // byte is the one lane k-1 drops (its byte 7); lane 0 takes
-        // Q(r - t0) from a queue loaded four steps ahead.
-        uint32_t qwin[2] = {0u, 0u};
+        // byte is the one lane k-1 drops (its byte 3); lane 0 takes Q(r - t0)
+        // from the block registers.
+        uint32_t qwin[1] = {0u};
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const int j = r_lo - 1 - tl - k;
             const uint32_t c = j >= 0 && j < qlen ? (uint32_t)qs[j] : 0u;
             qwin[k >> 2] |= c << (8 * (k & 3));
         }
-        // per block of kBlock steps starting at r_b, lane k holds in
-        // registers what lane 0 needs at step r_b + k: the query code
-        // Q(r_b + k - t0) and the predecessor's entry r_b + k - 1 (unscaled
-        // x, v, x2 and H; initial values before its first anti-diagonal, its
+        // per block of kBlock steps starting at rb, lane k holds in
+        // registers what lane 0 needs at step rb + k: the query code
+        // Q(rb + k - t0) and the predecessor's entry rb + k - 1 (x, v, x2
+        // scaled, and H; initial values before its first anti-diagonal, its
         // last entry after its last).  Loaded once per block, after the wait
-        // that guarantees the entries are written, so no step waits on memory.
-        uint32_t blk_q = 0;
-        int blk_pk = 0, blk_h = KSW_NEG_INF_I;
+        // that guarantees the entries are written, so no step waits on
+        // memory; every step moves them down one lane, so lane 0 holds the
+        // current step's.
+        int bq = 0, bx = 0, bv = 0, bx2 = 0, bh = KSW_NEG_INF_I;
         int lst16 = -1, len16 = -1;  // st16, en16 of anti-diagonal r-1
         if (r_lo > 0) {
             const Band bp = band_of(r_lo - 1, qlen, tlen, bw, tlenB, generic);
             lst16 = bp.st16, len16 = bp.en16;
         }
-        int hmax = KSW_NEG_INF_I;  // largest H on interior steps (checked against 0 at the end)
-        for (int r = r_lo; r <= r_hi; ++r) {
-            const int rb = r - (r - r_lo) % kBlock;  // block start
-            if (r == rb) {
-                if (s > 0)
-                    known = wait_count(prog + pw * kBandCtlStride, pbase + min(r + kLead, p_hi) + 1, known, late);
-                const int j = r + lane - t0;
-                blk_q = j >= 0 && j < qlen ? (uint32_t)qs[j] : 0u;
-                const int e = min(r + lane - 1, p_hi);
+        int hmax = KSW_NEG_INF_I;  // largest in-band H (checked against 0 at the end)
+        uint8_t *prow = p + (uint64_t)r_lo * ncolB;  // direction row of anti-diagonal r
+        for (int rb = r_lo; rb <= r_hi; rb += kBlock) {
+            if (s > 0) known = wait_count(prog + pw * kBandCtlStride, pbase + min(rb + kLead, p_hi) + 1, known, late);
+            {
+                const int j = rb + lane - t0;
+                bq = j >= 0 && j < qlen ? (int)qs[j] : 0;
+                const int e = min(rb + lane - 1, p_hi);
+                int pk = pack3(-G1, -G1, -G2), h = KSW_NEG_INF_I;
                 if (s > 0 && p_hi >= p_lo && e >= p_lo) {
                     const uint64_t en = __hip_atomic_load((const uint64_t *)(in_buf + 2 * e), __ATOMIC_RELAXED,
                                                           __HIP_MEMORY_SCOPE_AGENT);
-                    blk_pk = (int)(uint32_t)en, blk_h = (int)(uint32_t)(en >> 32);
-                } else {
-                    blk_pk = pack3(-G1, -G1, -G2), blk_h = KSW_NEG_INF_I;
+                    pk = (int)(uint32_t)en, h = (int)(uint32_t)(en >> 32);
                 }
                 // take the loads here: the steps of the block then read the
                 // registers without waiting on memory (the step's own stores
                 // would otherwise be waited for on every step)
-                asm volatile("" : "+v"(blk_q), "+v"(blk_pk), "+v"(blk_h));
+                asm volatile("" : "+v"(bq), "+v"(pk), "+v"(h));
+                bx = -8 * (byte_s(pk, 0) + G1) - bias, bv = -8 * byte_s(pk, 1);
+                bx2 = -8 * (byte_s(pk, 2) + G2) - bias, bh = h;
             }
-            const Band b = band_of(r, qlen, tlen, bw, tlenB, generic);
+            const int rbe = min(rb + kBlock - 1, r_hi);
+            // whether the block is clear of the rare boundary rules (the
+            // injection at t == r, a start below 16, the last position or
+            // anti-diagonal, the mqe column): r - en0, st0, en0 and r - st0
+            // never decrease with r, so the block's ends decide
+            bool plain;
             {
-                const uint32_t down = (uint32_t)shr1((int)qwin[(R - 1) >> 2]) >> 24;
-                const uint32_t c0 = (uint32_t)__builtin_amdgcn_readlane((int)blk_q, r - rb);
-                const uint32_t c = lane == 0 ? c0 : down;
-                if (R > 4) qwin[1] = qwin[1] << 8 | qwin[0] >> 24;
-                qwin[0] = qwin[0] << 8 | c;
+                const Band b0 = band_of(rb, qlen, tlen, bw, tlenB, generic);
+                const Band b1 = band_of(rbe, qlen, tlen, bw, tlenB, generic);
+                plain = rb - b0.en0 > 15 && b0.st0 >= 16 && b0.en0 > 0 && b1.en0 < tlen - 1 && rbe < nr - 1 &&
+                        rbe - b1.st0 < qlen - 1;
             }
-            // position t0-1 as anti-diagonal r-1 left it (lane 0's upper
-            // neighbour), from the block registers
-            int cx, cv, cx2, ch;
-            {
-                const int pk = __builtin_amdgcn_readlane(blk_pk, r - rb);
-                ch = __builtin_amdgcn_readlane(blk_h, r - rb);
-                cx = -8 * (byte_s(pk, 0) + G1) - bias, cv = -8 * byte_s(pk, 1);
-                cx2 = -8 * (byte_s(pk, 2) + G2) - bias;
-            }
-            {
-                const int sx = shr1(X[R - 1]), sv = shr1(V[R - 1]), sx2 = shr1(X2[R - 1]), sh = shr1(H[R - 1]);
-                if (lane != 0) cx = sx, cv = sv, cx2 = sx2, ch = sh;
-            }
-            // interior step: every position of the stripe strictly inside
-            // (st0, en0) -- all computed and scored, no boundary rule, no H
-            // entry, no mqe/mte/score position -- the lean form
-            if (b.st0 < t0 && t0 + SR - 1 < b.en0) {
-                uint32_t dw[2] = {0u, 0u}, sm = 0;
+            for (int r = rb; r <= rbe; ++r, prow += ncolB) {
+                const Band b = band_of(r, qlen, tlen, bw, tlenB, generic);
+                // query window: each lane's new byte is the one the lane
+                // above drops, lane 0's the block's current query code
+                qwin[0] = qwin[0] << 8 | ((uint32_t)dpp_shr1(bq << 24, (int)qwin[0]) >> 24);
+                // position t0-1 as anti-diagonal r-1 left it: the lane
+                // above's bottom position, lane 0's from the block registers
+                int cx = dpp_shr1(bx, X[R - 1]), cv = dpp_shr1(bv, V[R - 1]);
+                int cx2 = dpp_shr1(bx2, X2[R - 1]), ch = dpp_shr1(bh, H[R - 1]);
+                bq = dpp_shl1(bq), bx = dpp_shl1(bx), bv = dpp_shl1(bv), bx2 = dpp_shl1(bx2), bh = dpp_shl1(bh);
+                // interior step: every position of the stripe strictly inside
+                // (st0, en0) -- all computed and scored, no boundary rule, no H
+                // entry, no mqe/mte/score position -- the lean form
+                if (b.st0 < t0 && t0 + SR - 1 < b.en0) {
+                    uint32_t dw[2] = {0u, 0u}, sm = 0;
 #pragma unroll
-                for (int k = R - 1; k >= 0; --k) {
-                    const int xin = k > 0 ? X[k - 1] : cx, vin = k > 0 ? V[k - 1] : cv;
-                    const int x2in = k > 0 ? X2[k - 1] : cx2;
-                    const uint32_t tc = (tcw[k >> 2] >> (8 * (k & 3))) & 0xffu;
-                    const uint32_t qc = (qwin[k >> 2] >> (8 * (k & 3))) & 0xffu;
-                    const bool mm = tc != qc;
-                    sm |= mm ? 1u << k : 0u;
-                    int un, vn, xn, yn, x2n, y2n;
-                    const int d = band_cell<RIGHT, SCORE_ONLY>(kc, xin, vin, x2in, U[k], Y[k], Y2[k], mm, un, vn, xn,
-                                                               yn, x2n, y2n);
-                    if (!SCORE_ONLY) {
-                        dw[k >> 2] |= (uint32_t)d << (8 * (k & 3));
-                        asm volatile("" : "+v"(dw[k >> 2]));
-                    }
-                    U[k] = un, V[k] = vn, X[k] = xn, Y[k] = yn, X2[k] = x2n, Y2[k] = y2n;
-                    H[k] -= vn >> 3;  // v = -(V / 8)
-                    hmax = max(hmax, H[k]);
-                }
-                smask = sm;
-                if (!SCORE_ONLY) store_dirs(p + (uint64_t)r * ncolB + (uint64_t)(tl - b.stB), dw);
-            } else if (b.st16 > 0 && r > b.en16 && b.en0 > 0 && r != nr - 1 && b.en0 != tlen - 1 &&
-                       r - b.st0 != qlen - 1) {
-                // band-edge step without the rare rules (no injection at
-                // t == r, no column-0 start, no mte/mqe/score position): the
-                // interior form plus per-position selects for the st16 and
-                // wrap inputs, the score-bit update range, the H entry at
-                // en0, and the lane commit
-                const bool wrap = b.st16 != b.stB;
-                const bool fix16 = b.st16 >= t0 && b.st16 < t0 + SR &&
-                                   !(b.st16 - 1 >= lst16 && b.st16 - 1 <= len16);
-                const bool fixw = wrap && b.stB >= t0 && b.stB < t0 + SR;
-                int wx = 0, wvv = 0, wx2 = 0;
-                if (fixw) {
-                    const int src = (b.stB + 15 - t0) / R;
-                    wx = __builtin_amdgcn_readlane(X[R - 1], src);
-                    wvv = __builtin_amdgcn_readlane(V[R - 1], src);
-                    wx2 = __builtin_amdgcn_readlane(X2[R - 1], src);
-                }
-                const bool la = tl >= b.stB && tl + R - 1 <= b.en16;
-                // score bits: positions [st0, sc_end] take this anti-diagonal's
-                // mismatch, the others keep theirs
-                uint32_t mmw = 0;
-#pragma unroll
-                for (int k = 0; k < R; ++k) {
-                    const uint32_t tc = (tcw[k >> 2] >> (8 * (k & 3))) & 0xffu;
-                    const uint32_t qc = (qwin[k >> 2] >> (8 * (k & 3))) & 0xffu;
-                    mmw |= tc != qc ? 1u << k : 0u;
-                }
-                const int lo = min(max(b.st0 - tl, 0), R), hi = min(max(b.sc_end - tl + 1, 0), R);
-                const uint32_t upd = ((1u << hi) - 1u) & ~((1u << lo) - 1u);
-                smask = (smask & ~upd) | (mmw & upd);
-                uint32_t dw[2] = {0u, 0u};
-#pragma unroll
-                for (int k = R - 1; k >= 0; --k) {
-                    const int t = tl + k;
-                    int xin = k > 0 ? X[k - 1] : cx, vin = k > 0 ? V[k - 1] : cv, x2in = k > 0 ? X2[k - 1] : cx2;
-                    if (fix16 && t == b.st16) xin = L0, vin = U0, x2in = L0;
-                    if (fixw && t == b.stB) xin = wx, vin = wvv, x2in = wx2;
-                    int un, vn, xn, yn, x2n, y2n;
-                    const int d = band_cell<RIGHT, SCORE_ONLY>(kc, xin, vin, x2in, U[k], Y[k], Y2[k], (smask >> k) & 1u,
-                                                               un, vn, xn, yn, x2n, y2n);
-                    if (!SCORE_ONLY) {
-                        dw[k >> 2] |= (uint32_t)d << (8 * (k & 3));
-                        asm volatile("" : "+v"(dw[k >> 2]));
-                    }
-                    // H: set at en0 from the position above, += v on
-                    // [st0, en0); below st0 it is kept (a band one position
-                    // wide reads it), above en0 it is never read before the
-                    // position enters at en0, so it may drift there
-                    const int hab = k > 0 ? H[k - 1] : ch;
-                    const int hn = t == b.en0 ? hab - (un >> 3) : H[k] - (t >= b.st0 ? vn >> 3 : 0);
-                    H[k] = hn;
-                    hmax = max(hmax, t >= b.st0 && t <= b.en0 ? hn : KSW_NEG_INF_I);
-                    if (la) U[k] = un, V[k] = vn, X[k] = xn, Y[k] = yn, X2[k] = x2n, Y2[k] = y2n;
-                }
-                if (!SCORE_ONLY && la) store_dirs(p + (uint64_t)r * ncolB + (uint64_t)(tl - b.stB), dw);
-            } else {
-                // the AVX-512 lane wrap: position stB reads position stB+15
-                // (row 7 of the next lane) from anti-diagonal r-1
-                const bool wrap = b.st16 != b.stB;
-                int wx = 0, wvv = 0, wx2 = 0;
-                if (wrap && b.stB >= t0 && b.stB < t0 + SR) {
-                    const int src = (b.stB + 15 - t0) / R;  // position stB+15 is row R-1 of this lane
-                    wx = __builtin_amdgcn_readlane(X[R - 1], src);
-                    wvv = __builtin_amdgcn_readlane(V[R - 1], src);
-                    wx2 = __builtin_amdgcn_readlane(X2[R - 1], src);
-                }
-                const bool keep_prev = b.st16 > 0 && b.st16 - 1 >= lst16 && b.st16 - 1 <= len16;
-                const int edgeV = -8 * edge_diff(sc, r);
-                const bool inject = b.en16 >= r;
-                const bool la = tl >= b.stB && tl + R - 1 <= b.en16;
-                uint32_t dw[2] = {0u, 0u};
-#pragma unroll
-                for (int k = R - 1; k >= 0; --k) {
-                    const int t = tl + k;
-                    int xin = k > 0 ? X[k - 1] : cx, vin = k > 0 ? V[k - 1] : cv, x2in = k > 0 ? X2[k - 1] : cx2;
-                    if (t == b.st16) {
-                        if (b.st16 == 0) xin = L0, vin = edgeV, x2in = L0;
-                        else if (!keep_prev) xin = L0, vin = U0, x2in = L0;
-                    } else if (wrap && t == b.stB) {
-                        xin = wx, vin = wvv, x2in = wx2;
-                    }
-                    int uin = U[k], yin = Y[k], y2in = Y2[k];
-                    if (inject && t == r) uin = edgeV, yin = L0, y2in = L0;
-                    if (t >= b.st0 && t <= b.sc_end) {
-                        const uint32_t qc = (qwin[k >> 2] >> (8 * (k & 3))) & 0xffu;
+                    for (int k = R - 1; k >= 0; --k) {
+                        const int xin = k > 0 ? X[k - 1] : cx, vin = k > 0 ? V[k - 1] : cv;
+                        const int x2in = k > 0 ? X2[k - 1] : cx2;
                         const uint32_t tc = (tcw[k >> 2] >> (8 * (k & 3))) & 0xffu;
-                        smask = (smask & ~(1u << k)) | (tc != qc ? 1u << k : 0u);
-                    }
-                    int un, vn, xn, yn, x2n, y2n;
-                    const int d = band_cell<RIGHT, SCORE_ONLY>(kc, xin, vin, x2in, uin, yin, y2in, (smask >> k) & 1u,
-                                                               un, vn, xn, yn, x2n, y2n);
-                    if (!SCORE_ONLY) {
-                        dw[k >> 2] |= (uint32_t)d << (8 * (k & 3));
-                        asm volatile("" : "+v"(dw[k >> 2]));
-                    }
-                    if (la) {
-                        // H bookkeeping on [st0, en0]; the position above still
-                        // holds its H of anti-diagonal r-1
-                        const int ut = -(un >> 3), vt = -(vn >> 3);
-                        if (t == b.en0) {
-                            const int hab = k > 0 ? H[k - 1] : ch;
-                            const int h = r == 0 ? vt - sc.qe_orig : b.en0 > 0 ? hab + ut : H[k] + vt;
-                            H[k] = h;
-                            if (b.en0 == tlen - 1 && h >= mte_h) mte_h = h, mte_q = r - b.en0;
-                            if (r == nr - 1 && b.en0 == tlen - 1) score = h;
-                        } else if (t >= b.st0 && t < b.en0) {
-                            H[k] += vt;
-                        }
-                        if (t >= b.st0 && t <= b.en0) {
-                            if (H[k] > 0) pos = 1;
-                            if (t == b.st0 && r - b.st0 == qlen - 1 && H[k] >= mqe_h) mqe_h = H[k], mqe_t = t;
+                        const uint32_t qc = (qwin[k >> 2] >> (8 * (k & 3))) & 0xffu;
+                        const bool mm = tc != qc;
+                        sm |= mm ? 1u << k : 0u;
+                        int un, vn, xn, yn, x2n, y2n;
+                        const int d = band_cell<RIGHT, SCORE_ONLY>(kc, xin, vin, x2in, U[k], Y[k], Y2[k], mm, un, vn,
+                                                                   xn, yn, x2n, y2n);
+                        if (!SCORE_ONLY) {
+                            dw[k >> 2] |= (uint32_t)d << (8 * (k & 3));
+                            asm volatile("" : "+v"(dw[k >> 2]));
                         }
                         U[k] = un, V[k] = vn, X[k] = xn, Y[k] = yn, X2[k] = x2n, Y2[k] = y2n;
+                        H[k] -= vn >> 3;  // v = -(V / 8)
+                        hmax = max(hmax, H[k]);
                     }
+                    smask = sm;
+                    if (!SCORE_ONLY) store_dirs(prow + (tl - b.stB), dw);
+                } else if (plain) {
+                    const bool keep_prev = b.st16 - 1 >= lst16 && b.st16 - 1 <= len16;
+                    if (b.st0 < t0)
+                        edge_step<RIGHT, SCORE_ONLY, true, false>(kc, b, t0, tl, lane, U, V, X, Y, X2, Y2, H, smask,
+                                                                  tcw[0], qwin[0], cx, cv, cx2, ch, keep_prev, U0, L0,
+                                                                  hmax, prow);
+                    else if (t0 + SR - 1 < b.en0)
+                        edge_step<RIGHT, SCORE_ONLY, false, true>(kc, b, t0, tl, lane, U, V, X, Y, X2, Y2, H, smask,
+                                                                  tcw[0], qwin[0], cx, cv, cx2, ch, keep_prev, U0, L0,
+                                                                  hmax, prow);
+                    else
+                        edge_step<RIGHT, SCORE_ONLY, false, false>(kc, b, t0, tl, lane, U, V, X, Y, X2, Y2, H, smask,
+                                                                   tcw[0], qwin[0], cx, cv, cx2, ch, keep_prev, U0,
+                                                                   L0, hmax, prow);
+                } else {
+                    // the AVX-512 lane wrap: position stB reads position stB+15
+                    // (row 7 of the next lane) from anti-diagonal r-1
+                    const bool wrap = b.st16 != b.stB;
+                    int wx = 0, wvv = 0, wx2 = 0;
+                    if (wrap && b.stB >= t0 && b.stB < t0 + SR) {
+                        const int src = (b.stB + 15 - t0) / R;  // position stB+15 is row R-1 of this lane
+                        wx = __builtin_amdgcn_readlane(X[R - 1], src);
+                        wvv = __builtin_amdgcn_readlane(V[R - 1], src);
+                        wx2 = __builtin_amdgcn_readlane(X2[R - 1], src);
+                    }
+                    const bool keep_prev = b.st16 > 0 && b.st16 - 1 >= lst16 && b.st16 - 1 <= len16;
+                    const int edgeV = -8 * edge_diff(sc, r);
+                    const bool inject = b.en16 >= r;
+                    const bool la = tl >= b.stB && tl + R - 1 <= b.en16;
+                    uint32_t dw[2] = {0u, 0u};
+#pragma unroll
+                    for (int k = R - 1; k >= 0; --k) {
+                        const int t = tl + k;
+                        int xin = k > 0 ? X[k - 1] : cx, vin = k > 0 ? V[k - 1] : cv, x2in = k > 0 ? X2[k - 1] : cx2;
+                        if (t == b.st16) {
+                            if (b.st16 == 0) xin = L0, vin = edgeV, x2in = L0;
+                            else if (!keep_prev) xin = L0, vin = U0, x2in = L0;
+                        } else if (wrap && t == b.stB) {
+                            xin = wx, vin = wvv, x2in = wx2;
+                        }
+                        int uin = U[k], yin = Y[k], y2in = Y2[k];
+                        if (inject && t == r) uin = edgeV, yin = L0, y2in = L0;
+                        if (t >= b.st0 && t <= b.sc_end) {
+                            const uint32_t qc = (qwin[k >> 2] >> (8 * (k & 3))) & 0xffu;
+                            const uint32_t tc = (tcw[k >> 2] >> (8 * (k & 3))) & 0xffu;
+                            smask = (smask & ~(1u << k)) | (tc != qc ? 1u << k : 0u);
+                        }
+                        int un, vn, xn, yn, x2n, y2n;
+                        const int d = band_cell<RIGHT, SCORE_ONLY>(kc, xin, vin, x2in, uin, yin, y2in,
+                                                                   (smask >> k) & 1u, un, vn, xn, yn, x2n, y2n);
+                        if (!SCORE_ONLY) {
+                            dw[k >> 2] |= (uint32_t)d << (8 * (k & 3));
+                            asm volatile("" : "+v"(dw[k >> 2]));
+                        }
+                        if (la) {
+                            // H bookkeeping on [st0, en0]; the position above still
+                            // holds its H of anti-diagonal r-1
+                            const int ut = -(un >> 3), vt = -(vn >> 3);
+                            if (t == b.en0) {
+                                const int hab = k > 0 ? H[k - 1] : ch;
+                                const int h = r == 0 ? vt - sc.qe_orig : b.en0 > 0 ? hab + ut : H[k] + vt;
+                                H[k] = h;
+                                if (b.en0 == tlen - 1 && h >= mte_h) mte_h = h, mte_q = r - b.en0;
+                                if (r == nr - 1 && b.en0 == tlen - 1) score = h;
+                            } else if (t >= b.st0 && t < b.en0) {
+                                H[k] += vt;
+                            }
+                            if (t >= b.st0 && t <= b.en0) {
+                                if (H[k] > 0) pos = 1;
+                                if (t == b.st0 && r - b.st0 == qlen - 1 && H[k] >= mqe_h) mqe_h = H[k], mqe_t = t;
+                            }
+                            U[k] = un, V[k] = vn, X[k] = xn, Y[k] = yn, X2[k] = x2n, Y2[k] = y2n;
+                        }
+                    }
+                    if (!SCORE_ONLY && la) store_dirs(prow + (tl - b.stB), dw);
                 }
-                if (!SCORE_ONLY && la) store_dirs(p + (uint64_t)r * ncolB + (uint64_t)(tl - b.stB), dw);
+                lst16 = b.st16, len16 = b.en16;
+                // this stripe's bottom position for the next stripe, unscaled
+                if (lane == kWave - 1) {
+                    const int x = -((X[R - 1] + bias) >> 3) - G1, v = -(V[R - 1] >> 3);
+                    const int x2 = -((X2[R - 1] + bias) >> 3) - G2;
+                    const uint64_t en = (uint64_t)(uint32_t)pack3(x, v, x2) | (uint64_t)(uint32_t)H[R - 1] << 32;
+                    __hip_atomic_store((uint64_t *)(out_buf + 2 * r), en, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                if (((r - rb) & (kPublish - 1)) == kPublish - 1)
+                    publish(prog + wv * kBandCtlStride, kk * (nr + 2) + r + 1);
             }
-            lst16 = b.st16, len16 = b.en16;
-            // this stripe's bottom position for the next stripe, unscaled
-            if (lane == kWave - 1) {
-                const int x = -((X[R - 1] + bias) >> 3) - G1, v = -(V[R - 1] >> 3);
-                const int x2 = -((X2[R - 1] + bias) >> 3) - G2;
-                const uint64_t en = (uint64_t)(uint32_t)pack3(x, v, x2) | (uint64_t)(uint32_t)H[R - 1] << 32;
-                __hip_atomic_store((uint64_t *)(out_buf + 2 * r), en, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            if ((r - r_lo) % kPublish == kPublish - 1) publish(prog + wv * kBandCtlStride, kk * (nr + 2) + r + 1);
         }
         publish(prog + wv * kBandCtlStride, kk * (nr + 2) + nr + 1);  // past every need of the next stripe
         if (hmax > 0) pos = 1;
