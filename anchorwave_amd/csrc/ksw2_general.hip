@@ -418,10 +418,10 @@ __device__ __forceinline__ void backtrack_start(const GenJob &jb, KswOut &o, int
 // step back), 2 (i only), 1 (j only).
 __device__ __forceinline__ int backtrack_op(int tmp, int force, int &state)
 {
-    if (state == 0) state = tmp & 7;
-    else if (!(tmp >> (state + 2) & 1)) state = 0;
-    if (state == 0) state = tmp & 7;
-    if (force >= 0) state = force;
+    // a state continues when its bit (tmp >> (state + 2)) is set, else the
+    // byte's own state (tmp & 7) takes over
+    const bool cont = state != 0 && (tmp >> (state + 2) & 1);
+    state = force >= 0 ? force : cont ? state : tmp & 7;
     return state == 0 ? 0 : state == 1 || state == 3 ? 2 : 1;
 }
 
@@ -486,7 +486,6 @@ constexpr int kTileRows = 64, kTileDw = 33;
 __global__ void __launch_bounds__(64) ksw_general_backtrack_tiled_kernel(GenBacktrackArgs a)
 {
     __shared__ uint32_t tile[kTileRows * kTileDw];
-    __shared__ int32_t t_off[kTileRows], t_end[kTileRows], t_base[kTileRows];
     const int lane = threadIdx.x;
     const GenJob jb = a.jobs[blockIdx.x];
     const int flag = __builtin_amdgcn_readfirstlane(jb.flag);
@@ -506,13 +505,16 @@ __global__ void __launch_bounds__(64) ksw_general_backtrack_tiled_kernel(GenBack
         int i = __builtin_amdgcn_readfirstlane(i0), j = __builtin_amdgcn_readfirstlane(j0), state = 0;
         int run_op = -1, run_len = 0;
         int r0 = -1, ws = 0;  // tile: anti-diagonals (r0 - 64, r0], positions from ws
+        // lane L: off[], off_end[] of row r0 - L and the position of its
+        // first staged byte
+        int ro = 0, re = -1, base = 0;
         while (i >= 0 && j >= 0) {
             const int r = i + j;
             if (r0 < 0 || r <= r0 - kTileRows || i < ws) {
                 r0 = r, ws = i - 127;
                 __syncthreads();  // the walk's last reads of the old tile are done
                 const int row = r0 - lane;
-                int ro = 0, re = -1, base = 0;
+                ro = 0, re = -1, base = 0;
                 if (row >= 0) {
                     ro = off[row], re = off_end[row];
                     const int d0 = (ws - ro) >> 2;  // floor
@@ -529,19 +531,16 @@ __global__ void __launch_bounds__(64) ksw_general_backtrack_tiled_kernel(GenBack
                         tile[lane * kTileDw + q] = d >= 0 && d < (ncolB >> 2) ? v[q] : 0u;
                     }
                 }
-                t_off[lane] = ro, t_end[lane] = re, t_base[lane] = base;
                 __syncthreads();
             }
             const int L = r0 - r;
-            const int ro = __builtin_amdgcn_readfirstlane(t_off[L]), re = __builtin_amdgcn_readfirstlane(t_end[L]);
-            int force = -1;
-            if (i < ro) force = 2;
-            if (i > re) force = 1;
-            int tmp = 0;
-            if (force < 0 && (uint32_t)(i - ro) < (uint32_t)ncolB) {
-                const int b = i - __builtin_amdgcn_readfirstlane(t_base[L]);
-                tmp = __builtin_amdgcn_readfirstlane((int)tb[L * kTileDw * 4 + b]);
-            }
+            const int o_r = __builtin_amdgcn_readlane(ro, L), e_r = __builtin_amdgcn_readlane(re, L);
+            const int b_r = __builtin_amdgcn_readlane(base, L);
+            // (the row's byte is read whether or not it is used: the tile
+            // holds zeros outside the row, and i - b_r is within the tile)
+            const int byte = __builtin_amdgcn_readfirstlane((int)tb[L * kTileDw * 4 + (i - b_r)]);
+            const int force = i > e_r ? 1 : i < o_r ? 2 : -1;
+            const int tmp = force < 0 && (uint32_t)(i - o_r) < (uint32_t)ncolB ? byte : 0;
             const int op = backtrack_op(tmp, force, state);
             i -= op != 1, j -= op != 2;
             if (op == run_op) {
