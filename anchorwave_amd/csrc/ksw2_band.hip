@@ -220,6 +220,111 @@ __device__ __forceinline__ int band_cell(const BandConsts &k, int X, int V, int 
     return d;
 }
 
+// Two positions per register: position 2m of a lane in the low half of
+// register m, 2m+1 in the high half.  The scaled values and every sum the
+// cell forms fit in 16 bits in this domain (the stripe kernel's argument:
+// int8 differences times 8 plus constants below 2^11), so the packed cell
+// gives the 32-bit cell's values.
+typedef short v2s __attribute__((ext_vector_type(2)));
+typedef unsigned short v2u __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ int bits(v2s v) { return __builtin_bit_cast(int, v); }
+__device__ __forceinline__ int bits(v2u v) { return __builtin_bit_cast(int, v); }
+__device__ __forceinline__ v2s vec(int x) { return __builtin_bit_cast(v2s, x); }
+__device__ __forceinline__ v2u uvec(int x) { return __builtin_bit_cast(v2u, x); }
+__device__ __forceinline__ v2s splat(int x) { return (v2s){(short)x, (short)x}; }
+__device__ __forceinline__ v2s vmin(v2s a, v2s b) { return __builtin_elementwise_min(a, b); }
+__device__ __forceinline__ v2s vmax(v2s a, v2s b) { return __builtin_elementwise_max(a, b); }
+__device__ __forceinline__ int lo16(int x) { return (int)(int16_t)(x & 0xffff); }
+__device__ __forceinline__ int hi16(int x) { return x >> 16; }
+// (x >> 3) of each half, sign-extended to 32 bits
+__device__ __forceinline__ int lo16_div8(int x) { return __builtin_amdgcn_sbfe(x, 3, 13); }
+__device__ __forceinline__ int hi16_div8(int x) { return x >> 19; }
+// the high half of `hi` and the low half of `lo`: positions (2m-1, 2m)
+__device__ __forceinline__ int above_pair(int lo_reg, int hi_reg)
+{
+    return (int)__builtin_amdgcn_alignbit((unsigned)hi_reg, (unsigned)lo_reg, 16);
+}
+
+struct BandConsts2 {
+    v2s CA, CB, CA2, CB2, E1, E2, LIM, SM, SD, CAP;
+};
+
+__device__ __forceinline__ BandConsts2 pack_consts(const BandConsts &k)
+{
+    BandConsts2 p;
+    p.CA = splat(k.CA), p.CB = splat(k.CB), p.CA2 = splat(k.CA2), p.CB2 = splat(k.CB2);
+    p.E1 = splat(k.E1), p.E2 = splat(k.E2), p.LIM = splat(k.LIM);
+    p.SM = splat(k.SM), p.SD = splat(k.SMM - k.SM), p.CAP = splat(k.CAP);
+    return p;
+}
+
+// band_cell on two positions.  X/V/X2: the positions above; U/Y/Y2 this
+// pair's, replaced with the new values, as are X/V/X2 (the new values of
+// this pair); mm 0/1 per half.  Returns the two direction bytes in the low
+// byte of each half.
+template <bool RIGHT, bool SCORE_ONLY>
+__device__ __forceinline__ int band_cell2(const BandConsts2 &k, v2u mm, v2s &X, v2s &V, v2s &X2, v2s &U, v2s &Y,
+                                          v2s &Y2)
+{
+    v2u m = mm;
+    asm volatile("" : "+v"(m));  // keep the multiply-add form
+    const v2s S = k.SM + __builtin_bit_cast(v2s, m) * k.SD;
+    const v2s XV = X + V, X2V = X2 + V, YU = Y + U, Y2U = Y2 + U;
+    const v2s A = XV + k.CA, B = YU + k.CB, A2 = X2V + k.CA2, B2 = Y2U + k.CB2;
+    const v2s Z = vmin(vmin(vmin(vmin(S, A), B), A2), B2);
+    const v2s Z8 = vmax(vec(bits(Z) & (int)0xFFF8FFF8u), k.CAP);
+    const v2s M1 = k.E1 - Z8, M2 = k.E2 - Z8;
+    const v2s Ta = XV + M1, Tb = YU + M1, Ta2 = X2V + M2, Tb2 = Y2U + M2;
+    const v2s Un = Z8 - V, Vn = Z8 - U;
+    X = vmin(Ta, k.LIM), Y = vmin(Tb, k.LIM), X2 = vmin(Ta2, k.LIM), Y2 = vmin(Tb2, k.LIM);
+    U = Un, V = Vn;
+    if (SCORE_ONLY) return 0;
+    int d = RIGHT ? (~bits(Z) & 0x00070007) : (bits(Z) & 0x00070007);
+    d |= bits(__builtin_bit_cast(v2u, Ta) >> (v2u){12, 12}) & 0x00080008;
+    d |= bits(__builtin_bit_cast(v2u, Tb) >> (v2u){11, 11}) & 0x00100010;
+    d |= bits(__builtin_bit_cast(v2u, Ta2) >> (v2u){10, 10}) & 0x00200020;
+    d |= bits(__builtin_bit_cast(v2u, Tb2) >> (v2u){9, 9}) & 0x00400040;
+    return d;
+}
+
+// Per-stripe state of a lane: four positions in two packed registers per
+// quantity, the score bit of each position as 0/1 halves, H in 32 bits.
+struct Lane4 {
+    v2s U[2], V[2], X[2], Y[2], X2[2], Y2[2];
+    v2u MM[2];
+    int H[4];
+};
+
+// The inputs from the positions above for the lane's two pairs: (p-1, p0)
+// and (p1, p2), with position -1 from `top` (high half).
+struct Above {
+    v2s X[2], V[2], X2[2];
+};
+
+__device__ __forceinline__ Above above_of(const Lane4 &st, int tx, int tv, int tx2)
+{
+    Above a;
+    a.X[0] = vec(above_pair(tx, bits(st.X[0]))), a.X[1] = vec(above_pair(bits(st.X[0]), bits(st.X[1])));
+    a.V[0] = vec(above_pair(tv, bits(st.V[0]))), a.V[1] = vec(above_pair(bits(st.V[0]), bits(st.V[1])));
+    a.X2[0] = vec(above_pair(tx2, bits(st.X2[0]))), a.X2[1] = vec(above_pair(bits(st.X2[0]), bits(st.X2[1])));
+    return a;
+}
+
+// The four direction bytes of a lane, positions in order.
+__device__ __forceinline__ uint32_t pack_dirs(int d0, int d1)
+{
+    return __builtin_amdgcn_perm((unsigned)d1, (unsigned)d0, 0x06040200u);
+}
+
+// Query codes of the lane's pairs from its window: (q0, q1), (q2, q3).
+__device__ __forceinline__ void query_pairs(uint32_t qwin, v2u &q0, v2u &q1)
+{
+    q0 = uvec((int)__builtin_amdgcn_perm(0u, qwin, 0x0c010c00u));
+    q1 = uvec((int)__builtin_amdgcn_perm(0u, qwin, 0x0c030c02u));
+}
+
+__device__ __forceinline__ v2u mismatch(v2u tb, v2u qb) { return __builtin_elementwise_min(tb ^ qb, (v2u){1, 1}); }
+
 // A band-edge step in a block clear of the rare boundary rules: the
 // interior form plus what the band's edges change.  LEAD: the band starts
 // above the stripe (st0 < t0), so no start rule applies; TRAIL: it ends
@@ -236,69 +341,95 @@ __device__ __forceinline__ int band_cell(const BandConsts &k, int X, int V, int 
 //  * Commit: a lane above en16 keeps its state (it enters the band later
 //    with it); below stB a lane's positions are never read again.
 template <bool RIGHT, bool SCORE_ONLY, bool LEAD, bool TRAIL>
-__device__ __forceinline__ void edge_step(const BandConsts &kc, const Band &b, int t0, int tl, int lane, int (&U)[R],
-                                          int (&V)[R], int (&X)[R], int (&Y)[R], int (&X2)[R], int (&Y2)[R],
-                                          int (&H)[R], uint32_t &smask, uint32_t tcw, uint32_t qwin, int cx, int cv,
-                                          int cx2, int ch, bool keep_prev, int U0, int L0, int &hmax, uint8_t *prow)
+__device__ __forceinline__ void edge_step(const BandConsts2 &kc, const Band &b, int t0, int tl, int lane, Lane4 &st,
+                                          const v2u (&TB)[2], uint32_t qwin, int tx, int tv, int tx2, int ch,
+                                          bool keep_prev, int U0, int L0, int &hmax, uint8_t *prow)
 {
+    Above a = above_of(st, tx, tv, tx2);
     if (!LEAD) {
         if (b.st16 != b.stB && b.stB >= t0 && b.stB < t0 + SR) {
-            const int src = (b.stB + 15 - t0) / R;  // position stB+15 is row R-1 of this lane
-            const int wx = __builtin_amdgcn_readlane(X[R - 1], src), wv = __builtin_amdgcn_readlane(V[R - 1], src);
-            const int wx2 = __builtin_amdgcn_readlane(X2[R - 1], src);
-            const bool me = lane == (b.stB - t0) / R;
-            cx = me ? wx : cx, cv = me ? wv : cv, cx2 = me ? wx2 : cx2;
+            // position stB (low half of pair 0 of its lane) takes position
+            // stB+15 (high half of pair 1 of lane (stB+15-t0)/R)
+            const int src = (b.stB + 15 - t0) / R;
+            const int wx = __builtin_amdgcn_readlane(bits(st.X[1]), src);
+            const int wv = __builtin_amdgcn_readlane(bits(st.V[1]), src);
+            const int wx2 = __builtin_amdgcn_readlane(bits(st.X2[1]), src);
+            if (lane == (b.stB - t0) / R) {
+                a.X[0] = vec((bits(a.X[0]) & (int)0xffff0000u) | ((unsigned)wx >> 16));
+                a.V[0] = vec((bits(a.V[0]) & (int)0xffff0000u) | ((unsigned)wv >> 16));
+                a.X2[0] = vec((bits(a.X2[0]) & (int)0xffff0000u) | ((unsigned)wx2 >> 16));
+            }
         }
         if (!keep_prev && b.st16 >= t0 && b.st16 < t0 + SR) {
-            const int k16 = (b.st16 - t0) % R;
-            const bool me = lane == (b.st16 - t0) / R;
-            if (k16 == 0) {
-                cx = me ? L0 : cx, cv = me ? U0 : cv, cx2 = me ? L0 : cx2;
-            }
-#pragma unroll
-            for (int k = 1; k < R; ++k)
-                if (k16 == k) {
-                    X[k - 1] = me ? L0 : X[k - 1], V[k - 1] = me ? U0 : V[k - 1];
-                    X2[k - 1] = me ? L0 : X2[k - 1];
+            const int k16 = (b.st16 - t0) % R, m = k16 >> 1;
+            const int hm = (k16 & 1) ? (int)0xffff0000u : 0xffff;
+            if (lane == (b.st16 - t0) / R) {
+                const int l0 = bits(splat(L0)) & hm, u0 = bits(splat(U0)) & hm;
+                if (m == 0) {
+                    a.X[0] = vec((bits(a.X[0]) & ~hm) | l0), a.V[0] = vec((bits(a.V[0]) & ~hm) | u0);
+                    a.X2[0] = vec((bits(a.X2[0]) & ~hm) | l0);
+                } else {
+                    a.X[1] = vec((bits(a.X[1]) & ~hm) | l0), a.V[1] = vec((bits(a.V[1]) & ~hm) | u0);
+                    a.X2[1] = vec((bits(a.X2[1]) & ~hm) | l0);
                 }
+            }
         }
     }
     const bool la = (LEAD || tl >= b.stB) && (TRAIL || tl + R - 1 <= b.en16);
     const int lam = la ? -1 : 0;
-    uint32_t mmw = 0;
+    // score bits: positions [st0, sc_end] take this anti-diagonal's mismatch
+    {
+        v2u q0, q1;
+        query_pairs(qwin, q0, q1);
+        const int lo = LEAD ? 0 : min(max(b.st0 - tl, 0), R);
+        const int hi = TRAIL ? R : min(max(b.sc_end - tl + 1, 0), R);
+        const uint32_t upd = ((1u << hi) - 1u) & ~((1u << lo) - 1u);
+        const int um0 = (int)((upd & 1u) * 0xffffu | (upd & 2u) * 0x7fff8000u);
+        const int um1 = (int)((upd >> 2 & 1u) * 0xffffu | (upd >> 2 & 2u) * 0x7fff8000u);
+        st.MM[0] = uvec((bits(mismatch(TB[0], q0)) & um0) | (bits(st.MM[0]) & ~um0));
+        st.MM[1] = uvec((bits(mismatch(TB[1], q1)) & um1) | (bits(st.MM[1]) & ~um1));
+    }
+    Lane4 nw;
 #pragma unroll
-    for (int k = 0; k < R; ++k) mmw |= ((tcw ^ qwin) >> (8 * k) & 0xffu) != 0 ? 1u << k : 0u;
-    const int lo = LEAD ? 0 : min(max(b.st0 - tl, 0), R);
-    const int hi = TRAIL ? R : min(max(b.sc_end - tl + 1, 0), R);
-    const uint32_t upd = ((1u << hi) - 1u) & ~((1u << lo) - 1u);
-    smask = (smask & ~upd) | (mmw & upd);
-    uint32_t dw = 0;
+    for (int m = 0; m < 2; ++m) {
+        nw.X[m] = a.X[m], nw.V[m] = a.V[m], nw.X2[m] = a.X2[m];
+        nw.U[m] = st.U[m], nw.Y[m] = st.Y[m], nw.Y2[m] = st.Y2[m];
+    }
+    const int d0 = band_cell2<RIGHT, SCORE_ONLY>(kc, st.MM[0], nw.X[0], nw.V[0], nw.X2[0], nw.U[0], nw.Y[0], nw.Y2[0]);
+    const int d1 = band_cell2<RIGHT, SCORE_ONLY>(kc, st.MM[1], nw.X[1], nw.V[1], nw.X2[1], nw.U[1], nw.Y[1], nw.Y2[1]);
+    // H: set at en0 from the position above, += v on [st0, en0); kept below
+    // st0, free to drift above en0
+    const int un3[4] = {lo16_div8(bits(nw.U[0])), hi16_div8(bits(nw.U[0])), lo16_div8(bits(nw.U[1])),
+                        hi16_div8(bits(nw.U[1]))};
+    const int vn3[4] = {lo16_div8(bits(nw.V[0])), hi16_div8(bits(nw.V[0])), lo16_div8(bits(nw.V[1])),
+                        hi16_div8(bits(nw.V[1]))};
 #pragma unroll
     for (int k = R - 1; k >= 0; --k) {
         const int t = tl + k;
-        const int xin = k > 0 ? X[k - 1] : cx, vin = k > 0 ? V[k - 1] : cv, x2in = k > 0 ? X2[k - 1] : cx2;
-        int un, vn, xn, yn, x2n, y2n;
-        const int d = band_cell<RIGHT, SCORE_ONLY>(kc, xin, vin, x2in, U[k], Y[k], Y2[k], (smask >> k) & 1u, un, vn,
-                                                   xn, yn, x2n, y2n);
-        if (!SCORE_ONLY) {
-            dw |= (uint32_t)d << (8 * k);
-            asm volatile("" : "+v"(dw));
-        }
-        const int hab = k > 0 ? H[k - 1] : ch;
-        const int vdec = (LEAD || t >= b.st0) ? vn >> 3 : 0;  // v = -(V / 8)
-        const int hn = !TRAIL && t == b.en0 ? hab - (un >> 3) : H[k] - vdec;
-        H[k] = hn;
+        const int hab = k > 0 ? st.H[k - 1] : ch;
+        const int vdec = (LEAD || t >= b.st0) ? vn3[k] : 0;
+        const int hn = !TRAIL && t == b.en0 ? hab - un3[k] : st.H[k] - vdec;
+        st.H[k] = hn;
         const bool inb = (LEAD || t >= b.st0) && (TRAIL || t <= b.en0);
         hmax = max(hmax, inb ? hn : KSW_NEG_INF_I);
+    }
+    // commit: a lane above en16 keeps its state (it enters the band later
+    // with it); below stB a lane's positions are never read again
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
         if (TRAIL) {
-            U[k] = un, V[k] = vn, X[k] = xn, Y[k] = yn, X2[k] = x2n, Y2[k] = y2n;
+            st.U[m] = nw.U[m], st.V[m] = nw.V[m], st.X[m] = nw.X[m];
+            st.Y[m] = nw.Y[m], st.X2[m] = nw.X2[m], st.Y2[m] = nw.Y2[m];
         } else {
-            U[k] = (un & lam) | (U[k] & ~lam), V[k] = (vn & lam) | (V[k] & ~lam);
-            X[k] = (xn & lam) | (X[k] & ~lam), Y[k] = (yn & lam) | (Y[k] & ~lam);
-            X2[k] = (x2n & lam) | (X2[k] & ~lam), Y2[k] = (y2n & lam) | (Y2[k] & ~lam);
+            st.U[m] = vec((bits(nw.U[m]) & lam) | (bits(st.U[m]) & ~lam));
+            st.V[m] = vec((bits(nw.V[m]) & lam) | (bits(st.V[m]) & ~lam));
+            st.X[m] = vec((bits(nw.X[m]) & lam) | (bits(st.X[m]) & ~lam));
+            st.Y[m] = vec((bits(nw.Y[m]) & lam) | (bits(st.Y[m]) & ~lam));
+            st.X2[m] = vec((bits(nw.X2[m]) & lam) | (bits(st.X2[m]) & ~lam));
+            st.Y2[m] = vec((bits(nw.Y2[m]) & lam) | (bits(st.Y2[m]) & ~lam));
         }
     }
-    if (!SCORE_ONLY && la) *(uint32_t *)(prow + (tl - b.stB)) = dw;
+    if (!SCORE_ONLY && la) *(uint32_t *)(prow + (tl - b.stB)) = pack_dirs(d0, d1);
 }
 
 // Up to kBandTeam waves per pair in workgroups of up to kBandGroup.  A
@@ -322,6 +453,7 @@ __global__ void __launch_bounds__(kWave * kBandGroup) ksw_band_kernel(BandArgs a
     const int nr = qlen + tlen - 1;
     const KswScore &sc = a.sc.k;
     const BandConsts kc = band_consts<RIGHT>(sc);
+    const BandConsts2 kc2 = pack_consts(kc);
     const int G1 = sc.G1, G2 = sc.G2, bias = RIGHT ? 8 : 0;
     // scaled constants: u, v = -G1 -> 8 G1; x, y, x2, y2 at their initial
     // values (-G1, -G2) -> -BIAS
@@ -351,20 +483,23 @@ __global__ void __launch_bounds__(kWave * kBandGroup) ksw_band_kernel(BandArgs a
         const int pw = (s + W - 1) % W, pbase = (s - 1) / W * (nr + 2);  // producer wave, its count base
         int32_t *out_buf = buf + (uint64_t)(s % kBufs) * 2 * nr;
         const int32_t *in_buf = buf + (uint64_t)((s + kBufs - 1) % kBufs) * 2 * nr;
-        // per position: the in-place state (scaled) and H; the score s as
-        // one bit (set: mismatch; s is mch or mis in this domain, mch = 0 at
-        // first), the target codes packed four to a register (zero past
-        // tlen, as the reference's zero-padded target copy)
-        int U[R], Y[R], Y2[R], X[R], V[R], X2[R], H[R];
-        uint32_t smask = 0, tcw[1] = {0u};
+        // per position: the in-place state (scaled, two positions per
+        // register) and H; the score s as a 0/1 half (1: mismatch; s is mch
+        // or mis in this domain, mch = 0 at first); the target codes as
+        // 16-bit halves (zero past tlen, as the reference's zero-padded
+        // target copy)
+        Lane4 st;
+        v2u TB[2];
 #pragma unroll
-        for (int k = 0; k < R; ++k) {
-            U[k] = V[k] = U0;
-            X[k] = Y[k] = X2[k] = Y2[k] = L0;
-            H[k] = KSW_NEG_INF_I;
-            const uint32_t c = tl + k < tlen ? (uint32_t)ts[tl + k] : 0u;
-            tcw[k >> 2] |= c << (8 * (k & 3));
+        for (int m = 0; m < 2; ++m) {
+            st.U[m] = st.V[m] = splat(U0);
+            st.X[m] = st.Y[m] = st.X2[m] = st.Y2[m] = splat(L0);
+            st.MM[m] = (v2u){0, 0};
+            const int ta = tl + 2 * m, tb = ta + 1;
+            TB[m] = (v2u){(unsigned short)(ta < tlen ? ts[ta] : 0), (unsigned short)(tb < tlen ? ts[tb] : 0)};
         }
+#pragma unroll
+        for (int k = 0; k < R; ++k) st.H[k] = KSW_NEG_INF_I;
         // query window: byte k of qwin[0] is the query code of position
         // tl + k's column r - (tl + k) (0 outside the query, as the
         // reference's zero-tailed reversed query).  Each step lane k's new
@@ -409,8 +544,9 @@ __global__ void __launch_bounds__(kWave * kBandGroup) ksw_band_kernel(BandArgs a
                 // registers without waiting on memory (the step's own stores
                 // would otherwise be waited for on every step)
                 asm volatile("" : "+v"(bq), "+v"(pk), "+v"(h));
-                bx = -8 * (byte_s(pk, 0) + G1) - bias, bv = -8 * byte_s(pk, 1);
-                bx2 = -8 * (byte_s(pk, 2) + G2) - bias, bh = h;
+                // scaled, in the high half (the position above pair 0)
+                bx = (-8 * (byte_s(pk, 0) + G1) - bias) << 16, bv = (-8 * byte_s(pk, 1)) << 16;
+                bx2 = (-8 * (byte_s(pk, 2) + G2) - bias) << 16, bh = h;
             }
             const int rbe = min(rb + kBlock - 1, r_hi);
             // whether the block is clear of the rare boundary rules (the
@@ -429,52 +565,60 @@ __global__ void __launch_bounds__(kWave * kBandGroup) ksw_band_kernel(BandArgs a
                 // query window: each lane's new byte is the one the lane
                 // above drops, lane 0's the block's current query code
                 qwin[0] = qwin[0] << 8 | ((uint32_t)dpp_shr1(bq << 24, (int)qwin[0]) >> 24);
-                // position t0-1 as anti-diagonal r-1 left it: the lane
-                // above's bottom position, lane 0's from the block registers
-                int cx = dpp_shr1(bx, X[R - 1]), cv = dpp_shr1(bv, V[R - 1]);
-                int cx2 = dpp_shr1(bx2, X2[R - 1]), ch = dpp_shr1(bh, H[R - 1]);
+                // position t0-1 as anti-diagonal r-1 left it (high halves): the
+                // lane above's bottom position, lane 0's from the block
+                // registers
+                const int tx = dpp_shr1(bx, bits(st.X[1])), tv = dpp_shr1(bv, bits(st.V[1]));
+                const int tx2 = dpp_shr1(bx2, bits(st.X2[1])), ch = dpp_shr1(bh, st.H[R - 1]);
                 bq = dpp_shl1(bq), bx = dpp_shl1(bx), bv = dpp_shl1(bv), bx2 = dpp_shl1(bx2), bh = dpp_shl1(bh);
                 // interior step: every position of the stripe strictly inside
                 // (st0, en0) -- all computed and scored, no boundary rule, no H
                 // entry, no mqe/mte/score position -- the lean form
                 if (b.st0 < t0 && t0 + SR - 1 < b.en0) {
-                    uint32_t dw[2] = {0u, 0u}, sm = 0;
-#pragma unroll
-                    for (int k = R - 1; k >= 0; --k) {
-                        const int xin = k > 0 ? X[k - 1] : cx, vin = k > 0 ? V[k - 1] : cv;
-                        const int x2in = k > 0 ? X2[k - 1] : cx2;
-                        const uint32_t tc = (tcw[k >> 2] >> (8 * (k & 3))) & 0xffu;
-                        const uint32_t qc = (qwin[k >> 2] >> (8 * (k & 3))) & 0xffu;
-                        const bool mm = tc != qc;
-                        sm |= mm ? 1u << k : 0u;
-                        int un, vn, xn, yn, x2n, y2n;
-                        const int d = band_cell<RIGHT, SCORE_ONLY>(kc, xin, vin, x2in, U[k], Y[k], Y2[k], mm, un, vn,
-                                                                   xn, yn, x2n, y2n);
-                        if (!SCORE_ONLY) {
-                            dw[k >> 2] |= (uint32_t)d << (8 * (k & 3));
-                            asm volatile("" : "+v"(dw[k >> 2]));
-                        }
-                        U[k] = un, V[k] = vn, X[k] = xn, Y[k] = yn, X2[k] = x2n, Y2[k] = y2n;
-                        H[k] -= vn >> 3;  // v = -(V / 8)
-                        hmax = max(hmax, H[k]);
-                    }
-                    smask = sm;
-                    if (!SCORE_ONLY) store_dirs(prow + (tl - b.stB), dw);
+                    Above ab = above_of(st, tx, tv, tx2);
+                    v2u q0, q1;
+                    query_pairs(qwin[0], q0, q1);
+                    st.MM[0] = mismatch(TB[0], q0), st.MM[1] = mismatch(TB[1], q1);
+                    st.X[0] = ab.X[0], st.V[0] = ab.V[0], st.X2[0] = ab.X2[0];
+                    st.X[1] = ab.X[1], st.V[1] = ab.V[1], st.X2[1] = ab.X2[1];
+                    const int d0 = band_cell2<RIGHT, SCORE_ONLY>(kc2, st.MM[0], st.X[0], st.V[0], st.X2[0], st.U[0],
+                                                                 st.Y[0], st.Y2[0]);
+                    const int d1 = band_cell2<RIGHT, SCORE_ONLY>(kc2, st.MM[1], st.X[1], st.V[1], st.X2[1], st.U[1],
+                                                                 st.Y[1], st.Y2[1]);
+                    // v = -(V / 8)
+                    st.H[0] -= lo16_div8(bits(st.V[0])), st.H[1] -= hi16_div8(bits(st.V[0]));
+                    st.H[2] -= lo16_div8(bits(st.V[1])), st.H[3] -= hi16_div8(bits(st.V[1]));
+                    hmax = max(hmax, max(max(st.H[0], st.H[1]), max(st.H[2], st.H[3])));
+                    if (!SCORE_ONLY) *(uint32_t *)(prow + (tl - b.stB)) = pack_dirs(d0, d1);
                 } else if (plain) {
                     const bool keep_prev = b.st16 - 1 >= lst16 && b.st16 - 1 <= len16;
                     if (b.st0 < t0)
-                        edge_step<RIGHT, SCORE_ONLY, true, false>(kc, b, t0, tl, lane, U, V, X, Y, X2, Y2, H, smask,
-                                                                  tcw[0], qwin[0], cx, cv, cx2, ch, keep_prev, U0, L0,
-                                                                  hmax, prow);
+                        edge_step<RIGHT, SCORE_ONLY, true, false>(kc2, b, t0, tl, lane, st, TB, qwin[0], tx, tv, tx2,
+                                                                  ch, keep_prev, U0, L0, hmax, prow);
                     else if (t0 + SR - 1 < b.en0)
-                        edge_step<RIGHT, SCORE_ONLY, false, true>(kc, b, t0, tl, lane, U, V, X, Y, X2, Y2, H, smask,
-                                                                  tcw[0], qwin[0], cx, cv, cx2, ch, keep_prev, U0, L0,
-                                                                  hmax, prow);
+                        edge_step<RIGHT, SCORE_ONLY, false, true>(kc2, b, t0, tl, lane, st, TB, qwin[0], tx, tv, tx2,
+                                                                  ch, keep_prev, U0, L0, hmax, prow);
                     else
-                        edge_step<RIGHT, SCORE_ONLY, false, false>(kc, b, t0, tl, lane, U, V, X, Y, X2, Y2, H, smask,
-                                                                   tcw[0], qwin[0], cx, cv, cx2, ch, keep_prev, U0,
-                                                                   L0, hmax, prow);
+                        edge_step<RIGHT, SCORE_ONLY, false, false>(kc2, b, t0, tl, lane, st, TB, qwin[0], tx, tv, tx2,
+                                                                   ch, keep_prev, U0, L0, hmax, prow);
                 } else {
+                    // every rule, on the 32-bit form of the lane's state
+                    int U[R], Y[R], Y2[R], X[R], V[R], X2[R];
+                    int *H = st.H;
+#pragma unroll
+                    for (int m = 0; m < 2; ++m) {
+                        U[2 * m] = lo16(bits(st.U[m])), U[2 * m + 1] = hi16(bits(st.U[m]));
+                        V[2 * m] = lo16(bits(st.V[m])), V[2 * m + 1] = hi16(bits(st.V[m]));
+                        X[2 * m] = lo16(bits(st.X[m])), X[2 * m + 1] = hi16(bits(st.X[m]));
+                        Y[2 * m] = lo16(bits(st.Y[m])), Y[2 * m + 1] = hi16(bits(st.Y[m]));
+                        X2[2 * m] = lo16(bits(st.X2[m])), X2[2 * m + 1] = hi16(bits(st.X2[m]));
+                        Y2[2 * m] = lo16(bits(st.Y2[m])), Y2[2 * m + 1] = hi16(bits(st.Y2[m]));
+                    }
+                    uint32_t smask = (bits(st.MM[0]) & 1) | (bits(st.MM[0]) >> 15 & 2) |
+                                     (bits(st.MM[1]) & 1) << 2 | (bits(st.MM[1]) >> 13 & 8);
+                    const uint32_t tcw[1] = {(uint32_t)bits(TB[0]) & 0xffu | ((uint32_t)bits(TB[0]) >> 8 & 0xff00u) |
+                                             ((uint32_t)bits(TB[1]) & 0xffu) << 16 | ((uint32_t)bits(TB[1]) & 0xff0000u) << 8};
+                    const int cx = hi16(tx), cv = hi16(tv), cx2 = hi16(tx2);
                     // the AVX-512 lane wrap: position stB reads position stB+15
                     // (row 7 of the next lane) from anti-diagonal r-1
                     const bool wrap = b.st16 != b.stB;
@@ -535,13 +679,26 @@ __global__ void __launch_bounds__(kWave * kBandGroup) ksw_band_kernel(BandArgs a
                         }
                     }
                     if (!SCORE_ONLY && la) store_dirs(prow + (tl - b.stB), dw);
+                    // back to the packed form
+#pragma unroll
+                    for (int m = 0; m < 2; ++m) {
+                        st.U[m] = vec((int)__builtin_amdgcn_perm((unsigned)U[2 * m + 1], (unsigned)U[2 * m], 0x05040100u));
+                        st.V[m] = vec((int)__builtin_amdgcn_perm((unsigned)V[2 * m + 1], (unsigned)V[2 * m], 0x05040100u));
+                        st.X[m] = vec((int)__builtin_amdgcn_perm((unsigned)X[2 * m + 1], (unsigned)X[2 * m], 0x05040100u));
+                        st.Y[m] = vec((int)__builtin_amdgcn_perm((unsigned)Y[2 * m + 1], (unsigned)Y[2 * m], 0x05040100u));
+                        st.X2[m] =
+                            vec((int)__builtin_amdgcn_perm((unsigned)X2[2 * m + 1], (unsigned)X2[2 * m], 0x05040100u));
+                        st.Y2[m] =
+                            vec((int)__builtin_amdgcn_perm((unsigned)Y2[2 * m + 1], (unsigned)Y2[2 * m], 0x05040100u));
+                        st.MM[m] = uvec((int)((smask >> (2 * m) & 1u) | (smask >> (2 * m + 1) & 1u) << 16));
+                    }
                 }
                 lst16 = b.st16, len16 = b.en16;
                 // this stripe's bottom position for the next stripe, unscaled
                 if (lane == kWave - 1) {
-                    const int x = -((X[R - 1] + bias) >> 3) - G1, v = -(V[R - 1] >> 3);
-                    const int x2 = -((X2[R - 1] + bias) >> 3) - G2;
-                    const uint64_t en = (uint64_t)(uint32_t)pack3(x, v, x2) | (uint64_t)(uint32_t)H[R - 1] << 32;
+                    const int x = -((hi16(bits(st.X[1])) + bias) >> 3) - G1, v = -(hi16(bits(st.V[1])) >> 3);
+                    const int x2 = -((hi16(bits(st.X2[1])) + bias) >> 3) - G2;
+                    const uint64_t en = (uint64_t)(uint32_t)pack3(x, v, x2) | (uint64_t)(uint32_t)st.H[R - 1] << 32;
                     __hip_atomic_store((uint64_t *)(out_buf + 2 * r), en, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
                 if (((r - rb) & (kPublish - 1)) == kPublish - 1)

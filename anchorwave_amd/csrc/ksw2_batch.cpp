@@ -43,19 +43,15 @@ uint64_t mem_budget()
 }  // namespace
 
 namespace {
-std::mutex g_band_mu;
-hipStream_t g_band_stream = nullptr;
-
-// The stream every band-kernel launch of the process goes through.  The
+// Band-kernel launches of the process run one after another.  The
 // workgroups of a band launch wait on one another, so a launch is sized to
-// be resident as a whole; one stream keeps two of them from running at once.
-hipStream_t band_stream()
-{
-    std::lock_guard<std::mutex> lk(g_band_mu);
-    if (!g_band_stream && hipStreamCreateWithFlags(&g_band_stream, hipStreamNonBlocking) != hipSuccess)
-        g_band_stream = nullptr;
-    return g_band_stream;
-}
+// be resident as a whole; two at once could each hold part of the device.
+// Each sequence of band launches waits (on its own stream) for the previous
+// sequence's completion event, then records the event again, under a lock.
+// An event and not a stream of its own: a stream would hold one of the
+// process's few hardware queues for good.
+std::mutex g_band_mu;
+hipEvent_t g_band_done = nullptr;
 }  // namespace
 
 int device_init(int device)
@@ -380,9 +376,6 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
     }
     for (auto &e : p->ev)
         if (hipEventCreate(&e) != hipSuccess) return fail(KSW2_HIP_EDEVICE);
-    if (p->n_band > 0)
-        for (auto &e : p->ev_band)
-            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return fail(KSW2_HIP_EDEVICE);
     // one device block holds every arena
     uint64_t at = 0;
     auto take = [&](uint64_t bytes) {
@@ -529,16 +522,21 @@ int plan_launch(Plan *p, const Progress *prog)
         ba.out = p->d_out, ba.redo = p->d_redo, ba.sc = p->gsc;
         const char *fr = getenv("KSW2_HIP_BAND_REDO");
         ba.force_redo = fr && atoi(fr) > 0 ? 1 : 0;
-        hipStream_t st = lane(0), bs = band_stream();
-        if (!st || !bs) return KSW2_HIP_EDEVICE;
-        KSW_HIP_TRY(hipEventRecord(p->ev_band[0], st));
-        KSW_HIP_TRY(hipStreamWaitEvent(bs, p->ev_band[0], 0));
-        for (const Plan::BLaunch &l : p->blaunches) {
-            ba.order = p->d_gorder + l.off, ba.njobs = l.count;
-            KSW_HIP_TRY(launch_band(ba, l.team, l.grp == 1, l.grp == 2, bs));
+        hipStream_t st = lane(0);
+        if (!st) return KSW2_HIP_EDEVICE;
+        {
+            std::lock_guard<std::mutex> lk(g_band_mu);
+            if (!g_band_done && hipEventCreateWithFlags(&g_band_done, hipEventDisableTiming) != hipSuccess) {
+                g_band_done = nullptr;
+                return KSW2_HIP_EDEVICE;
+            }
+            KSW_HIP_TRY(hipStreamWaitEvent(st, g_band_done, 0));
+            for (const Plan::BLaunch &l : p->blaunches) {
+                ba.order = p->d_gorder + l.off, ba.njobs = l.count;
+                KSW_HIP_TRY(launch_band(ba, l.team, l.grp == 1, l.grp == 2, st));
+            }
+            KSW_HIP_TRY(hipEventRecord(g_band_done, st));
         }
-        KSW_HIP_TRY(hipEventRecord(p->ev_band[1], bs));
-        KSW_HIP_TRY(hipStreamWaitEvent(st, p->ev_band[1], 0));
         // pairs whose H went above 0 (the band kernel does not track the
         // maximum) run again on the general kernel, on the same stream
         ga.order = p->d_gorder + p->band_off, ga.njobs = p->n_band, ga.ring = 0, ga.only_if = p->d_redo;
@@ -663,8 +661,7 @@ void plan_destroy(Plan *p)
     if (p->h_seq && p->own_seq) (void)hipHostFree(p->h_seq);
     for (auto &e : p->ev)
         if (e) (void)hipEventDestroy(e);
-    for (auto &e : p->ev_band)
-        if (e) (void)hipEventDestroy(e);
+
     if (p->own_streams) {
         for (int l = 0; l < Plan::kLanes; ++l) {
             if (p->lane_done[l]) (void)hipEventDestroy(p->lane_done[l]);

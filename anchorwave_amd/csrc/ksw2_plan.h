@@ -152,7 +152,6 @@ struct Plan {
     int32_t *d_stop = nullptr;
     StripeRec *d_recs = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    hipEvent_t ev_band[2] = {nullptr, nullptr};  // into and out of the band stream
     // launches of different team sizes overlap on their own streams
     static constexpr int kLanes = 4;
     hipStream_t lane_stream[kLanes] = {nullptr, nullptr, nullptr, nullptr};

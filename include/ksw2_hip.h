@@ -111,7 +111,9 @@ int ksw2_hip_init(int device);
 
 /* Host-only (no GPU needed): which device kernel a ksw_extd2_hip call with
  * these arguments takes.  0 = the unbanded fast kernel; otherwise a bit mask
- * of why not: 1 band narrower than the matrix, 2 z-drop on, 4 APPROX_MAX
+ * of why not: 1 band narrower than the matrix (exactly 1: the band kernel
+ * when the band is at least 512 wide and KSW_EZ_SINGLETRACK is off, else
+ * the general kernel; any other mask: the general kernel), 2 z-drop on, 4 APPROX_MAX
  * flag, 8 scoring outside the fast kernel's exactness domain,
  * 16 wildcard code m-1 present in match/mismatch mode.  -1 when the
  * reference returns before aligning (m <= 1, empty sequence, or
