@@ -55,6 +55,8 @@ EXPORTS = (
     "ksw2_hip_plan_timing",
     "ksw2_hip_plan_destroy",
     "ksw2_hip_plan_dirs",
+    "ksw2_hip_certify_batch",
+    "ksw2_hip_certify_free",
 )
 
 
@@ -157,6 +159,11 @@ def lib():
         L.ksw2_hip_plan_timing.restype = C.c_int
         L.ksw2_hip_plan_dirs.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_int)]
         L.ksw2_hip_plan_dirs.restype = C.c_int
+        L.ksw2_hip_certify_batch.argtypes = [C.c_int, C.POINTER(CertifyTask), C.c_int32, C.c_int32, C.c_int32,
+                                             C.c_int32, C.c_int32, C.c_double, C.POINTER(CertifyResult)]
+        L.ksw2_hip_certify_batch.restype = C.c_int
+        L.ksw2_hip_certify_free.argtypes = [C.c_int, C.POINTER(CertifyResult)]
+        L.ksw2_hip_certify_free.restype = None
         L.ksw2_hip_plan_destroy.argtypes = [C.c_void_p]
         L.ksw2_hip_plan_destroy.restype = None
         L.ksw2_hip_set_progress_callback.argtypes = [PROGRESS_CB, C.c_void_p, C.c_int]
@@ -251,6 +258,75 @@ def extd2_singletrack(query, target, mat=None, m=5, q=8, e=2, q2=75, e2=1, w=-1,
         _libc.free(C.cast(ez.cigar, C.c_void_p))
     if check and st != 0:
         raise KswError(f"ksw_extd2_singletrack_hip: {STATUS.get(st, st)}")
+    return res
+
+
+class CertifyTask(C.Structure):
+    """``ksw2_certify_task_t`` (include/ksw2_hip.h)."""
+
+    _fields_ = [("query", C.c_char_p), ("target", C.c_char_p), ("qlen", C.c_int64), ("tlen", C.c_int64),
+                ("initial_band", C.c_int64), ("maximum_band", C.c_int64),
+                ("precomputed_optimal_score", C.POINTER(C.c_int64))]
+
+
+class CertifyResult(C.Structure):
+    """``ksw2_certify_result_t`` (include/ksw2_hip.h)."""
+
+    _fields_ = [("status", C.c_int32), ("score", C.c_int64), ("optimal_score", C.c_int64),
+                ("final_band_width", C.c_int64), ("traceback_attempts", C.c_uint64),
+                ("cigar", C.POINTER(C.c_uint32)), ("n_cigar", C.c_int32),
+                ("best_effort_cigar", C.POINTER(C.c_uint32)), ("n_best_effort", C.c_int32)]
+
+
+CERT_COMPLETED, CERT_TIME_LIMIT, CERT_NOT_CERTIFIED, CERT_FAILED = 0, 1, 2, 3
+
+
+@dataclass
+class Certified:
+    """One task's result of ``certify_batch`` (the fields of
+    Ksw2CertifiedResult, alignments as CIGARs)."""
+
+    status: int
+    score: int
+    optimal_score: int
+    final_band_width: int
+    traceback_attempts: int
+    cigar: List[int]
+    best_effort_cigar: List[int]
+
+
+def certify_batch(tasks: Sequence[tuple], mismatch=-6, open1=-8, extend1=-2, open2=-75, extend2=-1,
+                  max_seconds: float = 0.0) -> List[Certified]:
+    """``ksw2_hip_certify_batch``: tasks are (query, target, initial_band,
+    maximum_band[, precomputed_optimal_score]) with str or bytes sequences;
+    the scoring is AnchorWave's penalties (negative), as
+    alignScoreCertifiedKsw2 takes them."""
+    L = lib()
+    n = len(tasks)
+    arr = (CertifyTask * max(n, 1))()
+    keep = []
+    for k, t in enumerate(tasks):
+        qs = t[0].encode() if isinstance(t[0], str) else bytes(t[0])
+        ts = t[1].encode() if isinstance(t[1], str) else bytes(t[1])
+        keep += [qs, ts]
+        arr[k].query, arr[k].target = qs, ts
+        arr[k].qlen, arr[k].tlen = len(qs), len(ts)
+        arr[k].initial_band, arr[k].maximum_band = int(t[2]), int(t[3])
+        if len(t) > 4 and t[4] is not None:
+            v = C.c_int64(int(t[4]))
+            keep.append(v)
+            arr[k].precomputed_optimal_score = C.pointer(v)
+    out = (CertifyResult * max(n, 1))()
+    st = L.ksw2_hip_certify_batch(n, arr, mismatch, open1, extend1, open2, extend2, float(max_seconds), out)
+    if st != 0:
+        raise KswError(f"ksw2_hip_certify_batch: {STATUS.get(st, st)}")
+    res = []
+    for k in range(n):
+        o = out[k]
+        res.append(Certified(o.status, o.score, o.optimal_score, o.final_band_width, o.traceback_attempts,
+                             [o.cigar[i] for i in range(o.n_cigar)],
+                             [o.best_effort_cigar[i] for i in range(o.n_best_effort)]))
+    L.ksw2_hip_certify_free(n, out)
     return res
 
 

@@ -192,6 +192,57 @@ int ksw2_hip_server_poll(ksw2_hip_server *server, uint64_t *tickets, double *lat
 /* Finishes every submitted pair, then releases the server. */
 void ksw2_hip_server_destroy(ksw2_hip_server *server);
 
+/* ---- Score-certified banded alignment, batched.  Replaces
+ * alignScoreCertifiedKsw2 (src/myImportandFunction/alignSlidingWindow.cpp:
+ * 1197-1400) for many intervals at once, without its chained-traceback
+ * selection plan (selectionPlan == NULL there).  Per task, as the reference:
+ * the optimal score from a full score-only ksw_extd2 call (ksw2ScoreOnly,
+ * :723-796) unless precomputed; then banded ksw_extd2 tracebacks
+ * (alignSlidingWindow_minimap2, :800-890) at bands starting from twice
+ * max(|qlen - tlen|, initial_band, 1) and doubling up to
+ * min(max(qlen, tlen), maximum_band), until the rescored alignment
+ * (scoreGappedTwoPieceAffineAlignment, :677-721) equals the optimal score
+ * and both rows reconstruct their sequences; on failure the skipped first
+ * band is tried last.  Every task's next call goes into one batched launch
+ * per round, so a batch of n tasks costs one score-only launch and about
+ * log2(maximum/initial) + 2 banded launches.
+ * Sequences are characters (A/C/G/T in either case; anything else is the
+ * wildcard code 4 with KSW_EZ_GENERIC_SC, :644-660); the scoring is
+ * AnchorWave's penalties (mismatch and the two gap pieces, negative), used
+ * as the reference uses them.  The deadline is per batch: max_seconds > 0
+ * stops the batch's device work once that much time has passed, and every
+ * task not yet certified ends with status TimeLimit. */
+enum {
+    KSW2_CERT_COMPLETED = 0,
+    KSW2_CERT_TIME_LIMIT = 1,
+    KSW2_CERT_NOT_CERTIFIED = 2,
+    KSW2_CERT_FAILED = 3
+};
+typedef struct {
+    const char *query;      /* dna_q */
+    const char *target;     /* dna_d */
+    int64_t qlen, tlen;
+    int64_t initial_band, maximum_band;
+    const int64_t *precomputed_optimal_score;  /* NULL: computed by a score-only call */
+} ksw2_certify_task_t;
+typedef struct {
+    int32_t status;             /* KSW2_CERT_* (Ksw2CertifiedStatus order) */
+    int64_t score, optimal_score, final_band_width;
+    uint64_t traceback_attempts;
+    /* COMPLETED: the certified alignment, len<<4 | op as in ksw_extz_t.  A
+     * task whose certification failed keeps its best-scoring valid banded
+     * alignment in best_effort_cigar.  Both are malloc'd (released by
+     * ksw2_hip_certify_free). */
+    uint32_t *cigar;
+    int32_t n_cigar;
+    uint32_t *best_effort_cigar;
+    int32_t n_best_effort;
+} ksw2_certify_result_t;
+int ksw2_hip_certify_batch(int n, const ksw2_certify_task_t *tasks, int32_t mismatching_penalty,
+                           int32_t open_gap1, int32_t extend_gap1, int32_t open_gap2, int32_t extend_gap2,
+                           double max_seconds, ksw2_certify_result_t *results);
+void ksw2_hip_certify_free(int n, ksw2_certify_result_t *results);
+
 /* Diagnostics: copies pair `index`'s raw direction bytes (stripe-kernel
  * pairs only; pass layout: stripe s = i/512 is half h = s&1 of pass p = s>>1,
  * lane k = (i%512)/8, row rho = i%8, and cell (i, j) is byte

@@ -1,0 +1,140 @@
+"""Pins the restatement of AnchorWave's score-certified banding
+(oracle/certify_py.py), CPU only.  Its ksw_extd2 calls are the oracle's
+(pinned by the golden vectors); these checks pin the host logic around them:
+- a certified alignment rescores to the optimum of an independent textbook
+  two-piece Gotoh DP (oracle_gotoh2_global), and its rows rebuild both
+  sequences;
+- the band schedule (twice the first band, doubling to the cap, the skipped
+  first band last) by its closed form, on tasks that can never certify;
+- the reference's edge cases: an empty side, a cap below the length
+  difference, a KSW_NEG_INF optimum, case-sensitive rescoring.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+import certify_py as CP
+import oracle_py as O
+
+PEN = dict(mis=-6, o1=-8, e1=-2, o2=-75, e2=-1)
+KS = dict(q=8, e=2, q2=75, e2=1)
+
+
+def _dna(rng, n):
+    return "".join("ACGT"[i] for i in rng.integers(0, 4, n))
+
+
+def _mutate(rng, s, sub=0.08, indel=0.03):
+    out = []
+    for c in s:
+        r = rng.random()
+        if r < sub:
+            out.append("ACGT"[(("ACGT".index(c)) + int(rng.integers(1, 4))) % 4])
+        elif r < sub + indel / 2:
+            continue
+        elif r < sub + indel:
+            out.append(c)
+            out.append("ACGT"[int(rng.integers(0, 4))])
+        else:
+            out.append(c)
+    return "".join(out)
+
+
+def _schedule(diff, initial, maximum, qlen, tlen):
+    """Bands the reference tries on a task that never certifies."""
+    max_band = min(max(qlen, tlen), maximum)
+    req = max(diff, max(1, initial))
+    band = min(max_band, max(req, 2 * req))
+    bands = [band]
+    while band != max_band:
+        band = min(max_band, max(band + 1, 2 * band))
+        bands.append(band)
+    if req < min(max_band, 2 * req):
+        bands.append(req)
+    return bands, band
+
+
+def test_certified_alignments_are_optimal():
+    rng = np.random.default_rng(71)
+    done = 0
+    for k in range(40):
+        t = _dna(rng, int(rng.integers(20, 160)))
+        q = _mutate(rng, t)
+        r = CP.certify(q, t, int(rng.integers(0, 20)), 10_000, **PEN)
+        assert r.status == CP.COMPLETED, (k, r)
+        qs, _ = CP.encode(q)
+        ts, _ = CP.encode(t)
+        opt = O.gotoh2_global(qs, ts, **KS)
+        assert r.optimal_score == opt and r.score == opt, (k, r.score, opt)
+        aq, ad = CP.rows(r.cigar, q, t)
+        assert CP.reconstructs(aq, q) and CP.reconstructs(ad, t)
+        assert CP.rescore(aq, ad, **PEN) == opt
+        done += 1
+    assert done == 40
+
+
+def test_band_schedule_when_never_certified():
+    """A precomputed optimum above any alignment's score never certifies:
+    every band of the schedule is tried, then the skipped first band."""
+    rng = np.random.default_rng(72)
+    for initial, maximum in [(0, 200), (3, 64), (10, 10_000), (7, 7), (50, 51)]:
+        t = _dna(rng, 120)
+        q = _mutate(rng, t)
+        diff = abs(len(q) - len(t))
+        if maximum < diff:
+            continue
+        r = CP.certify(q, t, initial, maximum, precomputed=1, **PEN)
+        bands, last = _schedule(diff, initial, maximum, len(q), len(t))
+        assert r.status == CP.NOT_CERTIFIED
+        assert r.traceback_attempts == len(bands), (initial, maximum, r, bands)
+        assert r.final_band_width == last
+        assert r.best_effort_cigar, "a valid banded alignment is kept"
+        aq, ad = CP.rows(r.best_effort_cigar, q, t)
+        assert CP.rescore(aq, ad, **PEN) == r.score
+
+
+def test_edge_cases():
+    # one side empty: all gap, closed form
+    r = CP.certify("", "ACGTA", 5, 5, **PEN)
+    assert (r.status, r.score, r.final_band_width, r.traceback_attempts) == (CP.COMPLETED, -18, 0, 1)
+    assert r.cigar == [5 << 4 | 2]
+    r = CP.certify("ACG", "", 5, 5, **PEN)
+    assert r.score == max(-8 - 2 * 3, -75 - 3) and r.cigar == [3 << 4 | 1]
+    r = CP.certify("", "", 5, 5, **PEN)
+    assert r.status == CP.COMPLETED and r.cigar == [] and r.score == -8
+    # a cap below the length difference, or negative bands: not certified, nothing run
+    for a, b in [(5, 2), (-1, 50), (5, -1)]:
+        r = CP.certify("ACGTACGTAC", "ACGTA", a, b, **PEN)
+        assert (r.status, r.traceback_attempts, r.final_band_width) == (CP.NOT_CERTIFIED, 0, -1)
+    # a KSW_NEG_INF optimum fails
+    r = CP.certify("ACGT", "ACGT", 1, 10, precomputed=CP.KSW_NEG_INF, **PEN)
+    assert r.status == CP.FAILED
+    # rescoring compares characters: lower case against upper case is a
+    # mismatch there (ksw_extd2 sees equal codes), so nothing certifies
+    r = CP.certify("acgtacgtac", "ACGTACGTAC", 1, 100, **PEN)
+    assert r.optimal_score == 0 and r.status == CP.NOT_CERTIFIED and r.score == -60
+    # a wildcard takes the generic matrix (N on its diagonal scores 0, N
+    # against a base the mismatch), and the rescoring agrees
+    r = CP.certify("ACGNNACGT", "ACGNNACGT", 1, 100, **PEN)
+    assert r.status == CP.COMPLETED and r.optimal_score == 0
+    r = CP.certify("ACGNNACGT", "ACGTTACGT", 1, 100, **PEN)
+    assert r.status == CP.COMPLETED and r.optimal_score == -12
+
+
+def test_planted_mistakes_are_caught(monkeypatch):
+    """A gap piece dropped from the rescoring stops certification (its
+    score no longer reaches the Gotoh optimum); a schedule without the
+    skipped first band attempts one band fewer than the reference."""
+    original = CP.rescore
+
+    def one_piece(aq, ad, mis, o1, e1, o2, e2):
+        return original(aq, ad, mis, o1, e1, o1, e1)
+
+    rng = np.random.default_rng(73)
+    t = _dna(rng, 500)
+    q = t[:150] + t[300:]  # a 150 bp deletion: the long-gap piece is the optimum
+    assert CP.certify(q, t, 1, 10_000, **PEN).status == CP.COMPLETED
+    monkeypatch.setattr(CP, "rescore", one_piece)
+    assert CP.certify(q, t, 1, 10_000, **PEN).status == CP.NOT_CERTIFIED
+    bands, _ = _schedule(0, 3, 64, 120, 120)
+    assert bands[-1] == 3 and len(bands[:-1]) == len(bands) - 1
