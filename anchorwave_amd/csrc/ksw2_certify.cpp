@@ -14,24 +14,14 @@
 
 #include "../../include/ksw2_hip.h"
 #include "ksw2_plan.h"
+#include "ksw2_rows.h"
 
 namespace {
 
+using kswhip::code_of;
 using kswhip::Progress;
 
 constexpr int32_t kNegInf = -0x40000000;  // KSW_NEG_INF (minimap2/ksw2.h:18)
-
-// AnchorWave's code table (alignSlidingWindow.cpp:644-660, :820-838).
-uint8_t code_of(char c)
-{
-    switch (c) {
-    case 'A': case 'a': return 0;
-    case 'C': case 'c': return 1;
-    case 'G': case 'g': return 2;
-    case 'T': case 't': return 3;
-    default: return 4;
-    }
-}
 
 struct Task {
     int index = 0;
@@ -57,54 +47,12 @@ Rescored rescore(const char *dq, int64_t ql, const char *dd, int64_t tl, const u
 {
     std::string aq, ad;
     aq.reserve((size_t)(ql + tl)), ad.reserve((size_t)(ql + tl));
-    int64_t qp = 0, tp = 0;  // text_pos (query), pattern_pos (target)
-    for (int i = 0; i < n; ++i) {
-        const int64_t len = cigar[i] >> 4;
-        const int op = (int)(cigar[i] & 0xf);
-        if (op == 0 || op == 7 || op == 8) {
-            aq.append(dq + qp, (size_t)std::max<int64_t>(0, std::min(len, ql - qp)));
-            ad.append(dd + tp, (size_t)std::max<int64_t>(0, std::min(len, tl - tp)));
-            qp += len, tp += len;
-        } else if (op == 1) {
-            aq.append(dq + qp, (size_t)std::max<int64_t>(0, std::min(len, ql - qp)));
-            ad.append((size_t)len, '-');
-            qp += len;
-        } else if (op == 2) {
-            aq.append((size_t)len, '-');
-            ad.append(dd + tp, (size_t)std::max<int64_t>(0, std::min(len, tl - tp)));
-            tp += len;
-        }
-    }
+    int64_t qp = 0, tp = 0;
+    kswhip::append_rows(aq, ad, dq, ql, qp, dd, tl, tp, cigar, n);
     Rescored r;
-    auto reconstructs = [](const std::string &row, const char *seq, int64_t len) {
-        int64_t pos = 0;
-        for (const char c : row) {
-            if (c == '-') continue;
-            if (pos >= len || seq[pos] != c) return false;
-            ++pos;
-        }
-        return pos == len;
-    };
-    r.valid = aq.size() == ad.size() && reconstructs(aq, dq, ql) && reconstructs(ad, dd, tl);
-    if (aq.size() != ad.size()) return r;  // the reference throws here; not reachable from a CIGAR
-    size_t col = 0;
-    while (col < aq.size()) {
-        const bool qgap = aq[col] == '-', dgap = ad[col] == '-';
-        if (qgap && dgap) {  // the reference throws: no score, not valid
-            r.valid = false;
-            return r;
-        }
-        if (qgap || dgap) {
-            size_t end = col + 1;
-            while (end < aq.size() && (aq[end] == '-') == qgap && (ad[end] == '-') != qgap) ++end;
-            const int64_t l = (int64_t)(end - col);
-            r.score += std::max<int64_t>((int64_t)o1 + (int64_t)e1 * l, (int64_t)o2 + (int64_t)e2 * l);
-            col = end;
-            continue;
-        }
-        if (aq[col] != ad[col]) r.score += mis;
-        ++col;
-    }
+    bool ok = false;
+    r.score = kswhip::rescore_rows(aq, ad, mis, o1, e1, o2, e2, &ok);
+    r.valid = ok && kswhip::reconstructs(aq, dq, ql) && kswhip::reconstructs(ad, dd, tl);
     return r;
 }
 

@@ -57,6 +57,8 @@ EXPORTS = (
     "ksw2_hip_plan_dirs",
     "ksw2_hip_certify_batch",
     "ksw2_hip_certify_free",
+    "ksw2_hip_window_batch",
+    "ksw2_hip_window_free",
 )
 
 
@@ -164,6 +166,11 @@ def lib():
         L.ksw2_hip_certify_batch.restype = C.c_int
         L.ksw2_hip_certify_free.argtypes = [C.c_int, C.POINTER(CertifyResult)]
         L.ksw2_hip_certify_free.restype = None
+        L.ksw2_hip_window_batch.argtypes = [C.c_int, C.POINTER(WindowTask), C.c_int64, C.c_int32, C.c_int32,
+                                            C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(WindowResult)]
+        L.ksw2_hip_window_batch.restype = C.c_int
+        L.ksw2_hip_window_free.argtypes = [C.c_int, C.POINTER(WindowResult)]
+        L.ksw2_hip_window_free.restype = None
         L.ksw2_hip_plan_destroy.argtypes = [C.c_void_p]
         L.ksw2_hip_plan_destroy.restype = None
         L.ksw2_hip_set_progress_callback.argtypes = [PROGRESS_CB, C.c_void_p, C.c_int]
@@ -327,6 +334,52 @@ def certify_batch(tasks: Sequence[tuple], mismatch=-6, open1=-8, extend1=-2, ope
                              [o.cigar[i] for i in range(o.n_cigar)],
                              [o.best_effort_cigar[i] for i in range(o.n_best_effort)]))
     L.ksw2_hip_certify_free(n, out)
+    return res
+
+
+class WindowTask(C.Structure):
+    """``ksw2_window_task_t`` (include/ksw2_hip.h)."""
+
+    _fields_ = [("query", C.c_char_p), ("target", C.c_char_p), ("qlen", C.c_int64), ("tlen", C.c_int64)]
+
+
+class WindowResult(C.Structure):
+    """``ksw2_window_result_t`` (include/ksw2_hip.h)."""
+
+    _fields_ = [("status", C.c_int32), ("score", C.c_int64), ("windows", C.c_int32),
+                ("cigar", C.POINTER(C.c_uint32)), ("n_cigar", C.c_int32)]
+
+
+@dataclass
+class Windowed:
+    """One task's result of ``window_batch``."""
+
+    status: int
+    score: int
+    windows: int
+    cigar: List[int]
+
+
+def window_batch(tasks: Sequence[tuple], window: int, matching=0, mismatch=-6, open1=-8, extend1=-2, open2=-75,
+                 extend2=-1) -> List[Windowed]:
+    """``ksw2_hip_window_batch``: tasks are (query, target) str or bytes;
+    alignSlidingWindow's window size and penalties."""
+    L = lib()
+    n = len(tasks)
+    arr = (WindowTask * max(n, 1))()
+    keep = []
+    for k, t in enumerate(tasks):
+        qs = t[0].encode() if isinstance(t[0], str) else bytes(t[0])
+        ts = t[1].encode() if isinstance(t[1], str) else bytes(t[1])
+        keep += [qs, ts]
+        arr[k].query, arr[k].target, arr[k].qlen, arr[k].tlen = qs, ts, len(qs), len(ts)
+    out = (WindowResult * max(n, 1))()
+    st = L.ksw2_hip_window_batch(n, arr, int(window), matching, mismatch, open1, extend1, open2, extend2, out)
+    if st != 0:
+        raise KswError(f"ksw2_hip_window_batch: {STATUS.get(st, st)}")
+    res = [Windowed(out[k].status, out[k].score, out[k].windows, [out[k].cigar[i] for i in range(out[k].n_cigar)])
+           for k in range(n)]
+    L.ksw2_hip_window_free(n, out)
     return res
 
 

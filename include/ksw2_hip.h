@@ -243,6 +243,39 @@ int ksw2_hip_certify_batch(int n, const ksw2_certify_task_t *tasks, int32_t mism
                            double max_seconds, ksw2_certify_result_t *results);
 void ksw2_hip_certify_free(int n, ksw2_certify_result_t *results);
 
+/* ---- Sliding-window alignment, batched.  Replaces alignSlidingWindow
+ * (src/myImportandFunction/alignSlidingWindow.cpp:457-590) for many
+ * intervals at once.  Per task, as the reference: when
+ * (tlen/W)(qlen/W) <= 1 one global ksw_extd2 alignment
+ * (minimap2AlignmentWithArena, :241-332); otherwise windows of W+1 bases
+ * from the current start of each sequence, each aligned with
+ * KSW_EZ_SEMIGLOBAL_END under the shifted penalties of :535-539
+ * (alignmentMinimap2WithArena, :334-431), the next window starting where
+ * the better end (mqe > mte: the query's end, else the target's) left
+ * off; then the unaligned tails as one target gap and one query gap
+ * (:559-580).  The score is the rescored end-to-end alignment under the
+ * given penalties (:587-589).  Window k of every task runs in one batched
+ * launch (the windows of one task depend on each other).
+ * Sequences are characters as in ksw2_hip_certify_batch; window_size >= 0. */
+typedef struct {
+    const char *query;      /* dna_q */
+    const char *target;     /* dna_d */
+    int64_t qlen, tlen;
+} ksw2_window_task_t;
+typedef struct {
+    int32_t status;         /* 0 done; 1 a window's CIGAR did not reach its
+                             * reported end (the reference throws); 2 a
+                             * window above 2^30 (the reference exits) */
+    int64_t score;
+    int32_t windows;        /* ksw_extd2 calls made */
+    uint32_t *cigar;        /* the end-to-end alignment (M/I/D runs), malloc'd */
+    int32_t n_cigar;
+} ksw2_window_result_t;
+int ksw2_hip_window_batch(int n, const ksw2_window_task_t *tasks, int64_t window_size, int32_t matching_score,
+                          int32_t mismatching_penalty, int32_t open_gap1, int32_t extend_gap1, int32_t open_gap2,
+                          int32_t extend_gap2, ksw2_window_result_t *results);
+void ksw2_hip_window_free(int n, ksw2_window_result_t *results);
+
 /* Diagnostics: copies pair `index`'s raw direction bytes (stripe-kernel
  * pairs only; pass layout: stripe s = i/512 is half h = s&1 of pass p = s>>1,
  * lane k = (i%512)/8, row rho = i%8, and cell (i, j) is byte
