@@ -124,6 +124,7 @@ const char *ksw2_hip_error_string(int status)
     case KSW2_HIP_ENOMEM: return "out of memory";
     case KSW2_HIP_EDEVICE: return "HIP runtime error";
     case KSW2_HIP_ENODEV: return "no usable gfx950 device";
+    case KSW2_HIP_EAGAIN: return "deferred: device memory committed";
     default: return "unknown status";
     }
 }
@@ -299,6 +300,38 @@ int ksw2_hip_server_submit(ksw2_hip_server *server, const ksw2_pair_t *pair, ksw
 {
     if (!server || !pair || !ez) return set_err(KSW2_HIP_EINVAL);
     return set_err(server->e->submit(*pair, ez, true, ticket));
+}
+
+int ksw2_hip_server_try_submit(ksw2_hip_server *server, const ksw2_pair_t *pair, ksw_extz_t *ez, uint64_t *ticket,
+                               int *retry_ms)
+{
+    if (!server || !pair || !ez || !retry_ms) return set_err(KSW2_HIP_EINVAL);
+    *retry_ms = 0;
+    return set_err(server->e->submit(*pair, ez, true, ticket, retry_ms));
+}
+
+int ksw2_hip_server_load(ksw2_hip_server *server, ksw2_hip_load_t *load)
+{
+    if (!server || !load) return set_err(KSW2_HIP_EINVAL);
+    const kswhip::Engine::Load l = server->e->load();
+    load->budget_bytes = l.budget, load->held_bytes = l.held_bytes, load->queued_bytes = l.queued_bytes;
+    load->pending = l.pending, load->pending_cells = l.pending_cells, load->cells_per_s = l.cells_per_s;
+    return set_err(KSW2_HIP_OK);
+}
+
+uint64_t ksw2_hip_footprint(int qlen, const uint8_t *query, int tlen, const uint8_t *target, int8_t m,
+                            const int8_t *mat, int8_t q, int8_t e, int8_t q2, int8_t e2, int w, int zdrop, int flag)
+{
+    if (m <= 1 || qlen <= 0 || tlen <= 0 || !mat) return 0;
+    KswScore sc;
+    ScoreClass cls;
+    if (!normalise_scoring(m, mat, q, e, q2, e2, &sc, &cls)) return 0;  // the reference returns early
+    if (qlen > 0x7fffffff - tlen + 1) return UINT64_MAX;
+    ksw2_pair_t p;
+    p.query = query, p.target = target, p.qlen = qlen, p.tlen = tlen;
+    p.w = w, p.zdrop = zdrop, p.end_bonus = 0, p.flag = flag;
+    if (!query || !target) return 0;
+    return kswhip::pair_footprint(p, kswhip::takes_general(p, m, cls));
 }
 
 int ksw2_hip_server_poll(ksw2_hip_server *server, uint64_t *tickets, double *latency_s, int max_tickets,

@@ -147,6 +147,7 @@ struct Engine::Flight {
     int undelivered_fast = 0;
     int stream = 0;
     double cells = 0, t_start = 0;  // for the statistics
+    uint64_t bytes = 0;             // footprints admitted (held until the flight ends)
 };
 
 Engine::Engine(int8_t m, const int8_t *mat, const KswScore &sc, const ScoreClass &cls, uint64_t budget)
@@ -209,8 +210,36 @@ bool Engine::queues_empty() const
     return true;
 }
 
-int Engine::submit(const ksw2_pair_t &p, ksw_extz_t *ez, bool copy, uint64_t *ticket)
+// A task leaves the pending work (its result was delivered or it failed).
+void Engine::delivered_locked(const Task *t)
 {
+    pending_cells_ -= t->cells, since_reap_cells_ += t->cells;
+    if (pending_cells_ < 0) pending_cells_ = 0;
+}
+
+Engine::Load Engine::load()
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    Load l;
+    l.budget = budget_, l.held_bytes = held_bytes_, l.queued_bytes = queued_bytes_;
+    l.pending = pending_, l.pending_cells = pending_cells_, l.cells_per_s = cells_per_s_;
+    return l;
+}
+
+int Engine::submit(const ksw2_pair_t &p, ksw_extz_t *ez, bool copy, uint64_t *ticket, int *defer_ms)
+{
+    if (defer_ms && p.qlen > 0 && p.tlen > 0 && p.query && p.target && p.qlen <= 0x7fffffff - p.tlen + 1) {
+        const uint64_t fp = pair_footprint(p, takes_general(p, m_, cls_));
+        std::lock_guard<std::mutex> lk(mu_);
+        if (fp + kPlanSlack <= budget_ && held_bytes_ + queued_bytes_ + fp + kPlanSlack > budget_) {
+            // the GPU's memory is committed: the caller parks the task (as
+            // AlignmentTaskDeferred does) for about half the time the work
+            // ahead needs, 250 ms to 5 s (alignSlidingWindow.cpp:2500-2511)
+            const double wait_ms = cells_per_s_ > 0 ? pending_cells_ / cells_per_s_ * 1e3 : 4000.0;
+            *defer_ms = (int)std::max(250.0, std::min(5000.0, 0.5 * wait_ms));
+            return KSW2_HIP_EAGAIN;
+        }
+    }
     auto *t = new Task();
     t->pair = p;
     t->ez = ez;
@@ -257,6 +286,7 @@ int Engine::submit(const ksw2_pair_t &p, ksw_extz_t *ez, bool copy, uint64_t *ti
     } else {
         queue_[t->cls].push_back(t);
         queued_steps_ += t->steps;
+        queued_bytes_ += t->footprint, pending_cells_ += t->cells;
         ++pending_;
     }
     if (ticket) *ticket = t->ticket;
@@ -343,10 +373,11 @@ void Engine::stop_all_locked()
             t->t_done = now_s();
             done_.push_back(t->ticket);
             --pending_;
+            delivered_locked(t);
         }
         q.clear();
     }
-    queued_steps_ = 0;
+    queued_steps_ = 0, queued_bytes_ = 0;
     for (auto &f : flights_) f->plan->stop_requested = true;
     if (h_stop_) __atomic_store_n(h_stop_, 1, __ATOMIC_SEQ_CST);
     cv_done_.notify_all();
@@ -392,6 +423,7 @@ bool Engine::reap_locked(std::unique_lock<std::mutex> &lk)
                 f->delivered[k] = 1, --f->undelivered_fast;
                 f->tasks[p->order[k]] = nullptr;
                 busy_waves_ -= tk->held, inflight_steps_ -= tk->steps;
+                delivered_locked(tk);
                 tk->t_done = t;
                 done_.push_back(tk->ticket);
                 --pending_;
@@ -443,7 +475,16 @@ bool Engine::reap_locked(std::unique_lock<std::mutex> &lk)
             done_.push_back(tk->ticket);
             --pending_;
             busy_waves_ -= tk->held, inflight_steps_ -= tk->steps;
+            delivered_locked(tk);
         }
+        held_bytes_ = held_bytes_ > f->bytes ? held_bytes_ - f->bytes : 0;
+        // throughput under load: cells delivered since the last completion
+        // over the time since it
+        if (last_reap_t_ > 0 && t > last_reap_t_ && since_reap_cells_ > 0) {
+            const double rate = since_reap_cells_ / (t - last_reap_t_);
+            cells_per_s_ = cells_per_s_ > 0 ? 0.8 * cells_per_s_ + 0.2 * rate : rate;
+        }
+        last_reap_t_ = pending_ > 0 ? t : 0, since_reap_cells_ = 0;
         if (busy_waves_ < 0 || flights_.empty()) busy_waves_ = 0;
         if (inflight_steps_ < 0 || flights_.empty()) inflight_steps_ = 0;
         plan_destroy(p);
@@ -604,11 +645,13 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
                 continue;
             }
             queued_steps_ -= steps;
+            queued_bytes_ = queued_bytes_ > bytes ? queued_bytes_ - bytes : 0;
             for (Task *t : pick) {
                 t->ez->zdropped = 1;
                 t->t_done = now_s();
                 done_.push_back(t->ticket);
                 --pending_;
+                delivered_locked(t);
             }
             status_ = st;
             cv_done_.notify_all();
@@ -616,6 +659,9 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
         }
         for (Task *t : pick) t->t_start = t0;
         f->t_start = t0;
+        f->bytes = bytes;
+        held_bytes_ += bytes, queued_bytes_ = queued_bytes_ > bytes ? queued_bytes_ - bytes : 0;
+        if (last_reap_t_ == 0) last_reap_t_ = t0;
         busy_waves_ += waves, inflight_steps_ += steps, queued_steps_ -= steps;
         if (queued_steps_ < 0 || queues_empty()) queued_steps_ = 0;
         flights_.push_back(std::move(f));

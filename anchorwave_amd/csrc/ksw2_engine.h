@@ -39,7 +39,16 @@ public:
     int start();
     // Queues one pair; ez receives the result (reset now, filled when done).
     // copy: keep private copies of the sequences (the caller may free them).
-    int submit(const ksw2_pair_t &p, ksw_extz_t *ez, bool copy, uint64_t *ticket);
+    // defer_ms non-null: when the pair's footprint does not fit beside the
+    // memory already held and queued, queue nothing, return KSW2_HIP_EAGAIN
+    // and set *defer_ms to the suggested retry delay.
+    int submit(const ksw2_pair_t &p, ksw_extz_t *ez, bool copy, uint64_t *ticket, int *defer_ms = nullptr);
+    struct Load {
+        uint64_t budget, held_bytes, queued_bytes;
+        int64_t pending;
+        double pending_cells, cells_per_s;
+    };
+    Load load();
     // Batch mode: hold admission while a batch is queued, then sort the
     // queue longest first and release it.
     void hold(bool on);
@@ -100,6 +109,12 @@ private:
     // stripe-kernel steps queued and in flight (for the per-wave step budget)
     int64_t busy_waves_ = 0;
     double queued_steps_ = 0, inflight_steps_ = 0;
+    // memory and work accounting for deferral: footprints of admitted
+    // flights and of queued tasks; cells of every pending task; delivered
+    // throughput (moving average over completions under load)
+    uint64_t held_bytes_ = 0, queued_bytes_ = 0;
+    double pending_cells_ = 0, cells_per_s_ = 0, last_reap_t_ = 0, since_reap_cells_ = 0;
+    void delivered_locked(const Task *t);
     int status_ = 0;
     bool quit_ = false;
     bool hold_ = false;

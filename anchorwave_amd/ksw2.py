@@ -32,7 +32,8 @@ KSW_EZ_SEMIGLOBAL_END = 0x800
 KSW_EZ_SINGLETRACK = 0x1000  # two-track (Singletrack) traceback
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "unsupported", -3: "out of memory",
-          -4: "HIP runtime error", -5: "no usable gfx950 device"}
+          -4: "HIP runtime error", -5: "no usable gfx950 device", -6: "deferred: device memory committed"}
+EAGAIN = -6
 
 # Symbols include/ksw2_hip.h declares; tests check the library exports each.
 EXPORTS = (
@@ -59,6 +60,9 @@ EXPORTS = (
     "ksw2_hip_certify_free",
     "ksw2_hip_window_batch",
     "ksw2_hip_window_free",
+    "ksw2_hip_footprint",
+    "ksw2_hip_server_try_submit",
+    "ksw2_hip_server_load",
 )
 
 
@@ -171,6 +175,14 @@ def lib():
         L.ksw2_hip_window_batch.restype = C.c_int
         L.ksw2_hip_window_free.argtypes = [C.c_int, C.POINTER(WindowResult)]
         L.ksw2_hip_window_free.restype = None
+        L.ksw2_hip_footprint.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int8, C.c_void_p, C.c_int8,
+                                         C.c_int8, C.c_int8, C.c_int8, C.c_int, C.c_int, C.c_int]
+        L.ksw2_hip_footprint.restype = C.c_uint64
+        L.ksw2_hip_server_try_submit.argtypes = [C.c_void_p, C.POINTER(KswPair), C.POINTER(KswExtz),
+                                                 C.POINTER(C.c_uint64), C.POINTER(C.c_int)]
+        L.ksw2_hip_server_try_submit.restype = C.c_int
+        L.ksw2_hip_server_load.argtypes = [C.c_void_p, C.POINTER(HipLoad)]
+        L.ksw2_hip_server_load.restype = C.c_int
         L.ksw2_hip_plan_destroy.argtypes = [C.c_void_p]
         L.ksw2_hip_plan_destroy.restype = None
         L.ksw2_hip_set_progress_callback.argtypes = [PROGRESS_CB, C.c_void_p, C.c_int]
@@ -335,6 +347,21 @@ def certify_batch(tasks: Sequence[tuple], mismatch=-6, open1=-8, extend1=-2, ope
                              [o.best_effort_cigar[i] for i in range(o.n_best_effort)]))
     L.ksw2_hip_certify_free(n, out)
     return res
+
+
+class HipLoad(C.Structure):
+    """``ksw2_hip_load_t`` (include/ksw2_hip.h)."""
+
+    _fields_ = [("budget_bytes", C.c_uint64), ("held_bytes", C.c_uint64), ("queued_bytes", C.c_uint64),
+                ("pending", C.c_int64), ("pending_cells", C.c_double), ("cells_per_s", C.c_double)]
+
+
+def footprint(query, target, mat=None, m=5, q=8, e=2, q2=75, e2=1, w=-1, zdrop=-1, flag=0) -> int:
+    """``ksw2_hip_footprint``: device bytes the call holds while it runs."""
+    qs, ts = _u8(query), _u8(target)
+    mat = np.ascontiguousarray(np.asarray(default_matrix() if mat is None else mat, dtype=np.int8))
+    return int(lib().ksw2_hip_footprint(len(qs), qs.ctypes.data, len(ts), ts.ctypes.data, m, mat.ctypes.data,
+                                        q, e, q2, e2, w, zdrop, flag))
 
 
 class WindowTask(C.Structure):
@@ -560,6 +587,30 @@ class Server:
             raise KswError(f"ksw2_hip_server_submit: {STATUS.get(st, st)}")
         self._ez[tk.value] = ez
         return tk.value
+
+    def try_submit(self, query, target, w=-1, zdrop=-1, end_bonus=0, flag=0):
+        """``ksw2_hip_server_try_submit``: (ticket, 0) when queued, (None,
+        retry_ms) when deferred."""
+        qs, ts = _u8(query), _u8(target)
+        pr = KswPair(qs.ctypes.data, ts.ctypes.data, len(qs), len(ts), w, zdrop, end_bonus, flag)
+        ez = KswExtz()
+        tk = C.c_uint64()
+        retry = C.c_int()
+        st = self._L.ksw2_hip_server_try_submit(self._h, C.byref(pr), C.byref(ez), C.byref(tk), C.byref(retry))
+        if st == EAGAIN:
+            return None, retry.value
+        if st != 0:
+            raise KswError(f"ksw2_hip_server_try_submit: {STATUS.get(st, st)}")
+        self._ez[tk.value] = ez
+        return tk.value, 0
+
+    def load(self) -> dict:
+        """``ksw2_hip_server_load``."""
+        ld = HipLoad()
+        st = self._L.ksw2_hip_server_load(self._h, C.byref(ld))
+        if st != 0:
+            raise KswError(f"ksw2_hip_server_load: {STATUS.get(st, st)}")
+        return {k: getattr(ld, k) for k, _ in HipLoad._fields_}
 
     def poll(self, max_n: int = 1024, wait_ms: int = 0, results: bool = True):
         """Completed pairs: [(ticket, Result or None, latency_s), ...]."""

@@ -68,6 +68,7 @@ extern "C" {
 #define KSW2_HIP_ENOMEM          -3  /* device or host allocation failed */
 #define KSW2_HIP_EDEVICE         -4  /* a HIP runtime call failed */
 #define KSW2_HIP_ENODEV          -5  /* no usable gfx950 device */
+#define KSW2_HIP_EAGAIN          -6  /* deferred: the server's device memory is committed, retry later */
 
 /*
  * One alignment, same contract as ksw_extd2_sse (minimap2/ksw2.h:91-107,
@@ -191,6 +192,38 @@ int ksw2_hip_server_poll(ksw2_hip_server *server, uint64_t *tickets, double *lat
                          int wait_ms);
 /* Finishes every submitted pair, then releases the server. */
 void ksw2_hip_server_destroy(ksw2_hip_server *server);
+
+/* ---- Admission for a caller's scheduler.  These replace the CPU memory
+ * model of AnchorWave's algorithm selector (ksw2TracebackMemoryBytes,
+ * src/myImportandFunction/AlignmentAlgorithmSelector.cpp:307-358) and its
+ * memory back-pressure (AlignmentTaskDeferred,
+ * src/service/AnchorTaskExecutor.h:19-32) for the GPU path. */
+/* Device bytes a ksw_extd2_hip call with these arguments holds while it
+ * runs (direction bytes or tracks, state, off/off_end, CIGAR slots,
+ * sequences), on whichever kernel ksw2_hip_route picks; the admission
+ * footprint of the batch and serving paths.  0 when the reference returns
+ * before aligning; UINT64_MAX when qlen + tlen overflows.  Host only. */
+uint64_t ksw2_hip_footprint(int qlen, const uint8_t *query, int tlen, const uint8_t *target,
+                            int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q2, int8_t e2,
+                            int w, int zdrop, int flag);
+/* ksw2_hip_server_submit, except that a pair whose footprint does not fit
+ * beside the memory the server already holds and has queued is not queued:
+ * the call returns KSW2_HIP_EAGAIN and *retry_ms receives a retry delay
+ * (half the predicted time for the pending work, 250 ms to 5 s, as
+ * alignSlidingWindow.cpp:2500-2511 sizes AlignmentTaskDeferred).  A pair
+ * larger than the whole budget is submitted and fails as with
+ * ksw2_hip_server_submit. */
+int ksw2_hip_server_try_submit(ksw2_hip_server *server, const ksw2_pair_t *pair, ksw_extz_t *ez,
+                               uint64_t *ticket, int *retry_ms);
+typedef struct {
+    uint64_t budget_bytes;   /* device memory the server admits against */
+    uint64_t held_bytes;     /* footprints of the pairs on the device */
+    uint64_t queued_bytes;   /* footprints of the pairs waiting for admission */
+    int64_t pending;         /* pairs submitted and not yet completed */
+    double pending_cells;    /* their qlen * tlen */
+    double cells_per_s;      /* recent completion rate under load (0: none yet) */
+} ksw2_hip_load_t;
+int ksw2_hip_server_load(ksw2_hip_server *server, ksw2_hip_load_t *load);
 
 /* ---- Score-certified banded alignment, batched.  Replaces
  * alignScoreCertifiedKsw2 (src/myImportandFunction/alignSlidingWindow.cpp:

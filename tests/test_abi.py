@@ -55,3 +55,22 @@ def test_early_returns_need_no_device():
     r = ksw2.extd2(t, t, mat=ksw2.default_matrix(0, -30))  # -min_sc > 2(q+e)
     assert r.score == ksw2.KSW_NEG_INF and r.n_cigar == 0 and r.mqe == ksw2.KSW_NEG_INF
     assert r.max_t == -1 and r.max_q == -1 and r.mte_q == -1 and r.mqe_t == -1
+
+
+def test_footprint_model():
+    """ksw2_hip_footprint (host only): the admission footprint of a call,
+    at least one direction byte per cell for a full traceback, smaller with
+    a band, 0 when the reference returns before aligning."""
+    rng = np.random.default_rng(5)
+    q = rng.integers(0, 4, 3000).astype(np.uint8)
+    t = rng.integers(0, 4, 3200).astype(np.uint8)
+    full = ksw2.footprint(q, t)
+    # (the pass layout pads the target to 1024-row passes and stores the
+    # pipeline fill and drain steps)
+    assert full >= len(q) * len(t) and full < 1.5 * len(q) * len(t) + (1 << 20)
+    assert ksw2.footprint(q, t, flag=ksw2.KSW_EZ_SCORE_ONLY) < full // 50  # no direction bytes
+    banded = ksw2.footprint(q, t, w=700)
+    assert banded < full and banded >= (len(q) + len(t) - 1) * 700
+    assert ksw2.footprint(q[:1000], t[:1000]) < full
+    assert ksw2.footprint(q[:0], t) == 0
+    assert ksw2.footprint(q, t, m=1) == 0

@@ -99,3 +99,49 @@ def test_batch_larger_than_budget_streams_through_the_engine(monkeypatch):
     for k, (p, g) in enumerate(zip(pairs, got)):
         exp = _oracle(p)
         assert same(g, exp), f"pair {k}: {diff(g, exp)}"
+
+
+def test_try_submit_defers_when_memory_is_committed():
+    """A server with room for a few 2 kbp pairs: try_submit defers the rest
+    (retry 250 ms to 5 s) instead of queueing them; held + queued footprint
+    never exceeds the budget; every deferred pair, resubmitted, completes
+    exactly as it does alone."""
+    pairs = W.fixed(seed=12, n=40)
+    fp = max(ksw2.footprint(q, t) for q, t in pairs)
+    budget = 6 * fp + (1 << 20)
+    srv = ksw2.Server(device_bytes=budget)
+    waiting = list(range(len(pairs)))
+    tickets, deferred, retries, done = {}, 0, [], {}
+    for _round in range(10_000):
+        still = []
+        for k in waiting:
+            tk, retry = srv.try_submit(*pairs[k])
+            if tk is None:
+                deferred += 1
+                retries.append(retry)
+                still.append(k)
+            else:
+                tickets[tk] = k
+        waiting = still
+        ld = srv.load()
+        assert ld["held_bytes"] + ld["queued_bytes"] <= ld["budget_bytes"], ld
+        for tk, res, _lat in srv.poll(wait_ms=20):
+            done[tickets[tk]] = res
+        if not waiting and len(done) == len(pairs):
+            break
+    srv.close()
+    assert len(done) == len(pairs)
+    assert deferred > 0 and all(250 <= r <= 5000 for r in retries), (deferred, retries[:5])
+    for k, (q, t) in enumerate(pairs):
+        alone = ksw2.extd2(q, t)
+        assert same(done[k], alone), f"pair {k}: {diff(done[k], alone)}"
+
+
+def test_try_submit_pair_above_the_budget_fails_like_submit():
+    q, t = W.single(seed=1)[0]
+    srv = ksw2.Server(device_bytes=ksw2.footprint(q, t) // 2)
+    tk, retry = srv.try_submit(q, t)
+    assert tk is not None and retry == 0
+    (got,) = [r for _t, r, _l in srv.poll(wait_ms=-1)]
+    srv.close()
+    assert got.zdropped == 1
