@@ -616,7 +616,7 @@ __global__ void __launch_bounds__(kWave * kBandGroup) ksw_band_kernel(BandArgs a
                     }
                     uint32_t smask = (bits(st.MM[0]) & 1) | (bits(st.MM[0]) >> 15 & 2) |
                                      (bits(st.MM[1]) & 1) << 2 | (bits(st.MM[1]) >> 13 & 8);
-                    const uint32_t tcw[1] = {(uint32_t)bits(TB[0]) & 0xffu | ((uint32_t)bits(TB[0]) >> 8 & 0xff00u) |
+                    const uint32_t tcw[1] = {((uint32_t)bits(TB[0]) & 0xffu) | ((uint32_t)bits(TB[0]) >> 8 & 0xff00u) |
                                              ((uint32_t)bits(TB[1]) & 0xffu) << 16 | ((uint32_t)bits(TB[1]) & 0xff0000u) << 8};
                     const int cx = hi16(tx), cv = hi16(tv), cx2 = hi16(tx2);
                     // the AVX-512 lane wrap: position stB reads position stB+15

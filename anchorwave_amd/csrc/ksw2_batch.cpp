@@ -52,6 +52,12 @@ namespace {
 // process's few hardware queues for good.
 std::mutex g_band_mu;
 hipEvent_t g_band_done = nullptr;
+// Wide stripe launches likewise, on two such chains: at most two sequences
+// of wide launches run at once (each chunk at most a quarter of the device,
+// a band chunk at most half).
+std::mutex g_wide_mu;
+hipEvent_t g_wide_done[2] = {nullptr, nullptr};
+int g_wide_next = 0;
 }  // namespace
 
 int device_init(int device)
@@ -97,14 +103,22 @@ int fast_path_reject(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls)
 // the batch.  KSW2_HIP_TEAM forces a size (tuning and tests).
 static double pair_steps(const KswJob &j) { return (double)pass_count(j.tlen) * (double)pass_steps(j.qlen); }
 
+//
+// Teams above kMaxTeam waves are wide (workgroups of kWideGroup waves on
+// different CUs, up to kMaxWide waves, rounded up to whole workgroups).
+// KSW2_HIP_WIDE=k (tests) runs every pair wide on k workgroups.
 static int choose_team(const KswJob &j, double T, int forced)
 {
     const int P = pass_count(j.tlen);
-    if (forced > 0) return std::max(1, std::min(std::min(forced, kMaxTeam), P));
-    const double steps = pair_steps(j);
     int team = 1;
-    while (team < kMaxTeam && team < P && steps / team > 2 * T) team <<= 1;
-    return std::min(team, P);
+    if (forced > 0) {
+        team = std::max(1, std::min(std::min(forced, kMaxWide), P));
+    } else {
+        const double steps = pair_steps(j);
+        while (team < kMaxWide && team < P && steps / team > 2 * T) team <<= 1;
+        team = std::min(team, P);
+    }
+    return team > kMaxTeam ? (team + kWideGroup - 1) / kWideGroup * kWideGroup : team;
 }
 
 static int lane_of_team(int team) { return team >= 8 ? 0 : team >= 4 ? 1 : team >= 2 ? 2 : 3; }
@@ -124,7 +138,7 @@ uint64_t pair_footprint(const ksw2_pair_t &p, bool general)
     const uint64_t tracks = track_pair(p) ? 2 : 1;  // Singletrack: two bytes per cell
     if (!general)
         return (cigar ? tracks * dir_bytes(p.qlen, p.tlen) : 0) + 8ull * p.qlen + cig + seq +
-               16ull * stripe_count(p.tlen) + 512;
+               16ull * stripe_count(p.tlen) + 4ull * kWideCtl + 512;
     const int tlenB = (p.tlen + 63) / 64 * 64, qlenB = (p.qlen + 63) / 64 * 64;
     const uint64_t diags = (uint64_t)p.qlen + p.tlen - 1;
     const uint64_t rows = cigar ? tracks * diags * (uint64_t)gen_ncolB(p.qlen, p.tlen, norm_w(p)) + 256 : 0;
@@ -289,10 +303,14 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         const double T = std::max(1.0, total_steps / std::max(1, stripe_resident_waves(kGroupLeft)));
         int forced = env ? env->team : 0;
         if (const char *ev = getenv("KSW2_HIP_TEAM")) forced = atoi(ev);
+        int wide_forced = 0;
+        if (const char *ev = getenv("KSW2_HIP_WIDE")) wide_forced = std::max(0, std::min(kMaxWide / kWideGroup, atoi(ev)));
         std::vector<int> team(p->n_fast);
-        for (int k = 0; k < p->n_fast; ++k) team[k] = choose_team(p->jobs[k], T, forced);
+        for (int k = 0; k < p->n_fast; ++k)
+            team[k] = wide_forced ? kWideGroup * wide_forced + kMaxWide : choose_team(p->jobs[k], T, forced);
+        // keys: kMaxWide + waves for forced wide teams, then the team sizes
         for (int grp = 0; grp < kGroups; ++grp)
-            for (int t = kMaxTeam; t >= 1; --t) {
+            for (int t = 2 * kMaxWide; t >= 1; --t) {
                 std::vector<int> members;
                 for (int k = 0; k < p->n_fast; ++k)
                     if (grp_of[k] == grp && team[k] == t) members.push_back(k);
@@ -301,7 +319,11 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
                     const KswJob &a = p->jobs[x], &b = p->jobs[y];
                     return pair_steps(a) > pair_steps(b);
                 });
-                p->launches.push_back({grp, t, (int)order.size(), (int)members.size(), lane_of_team(t)});
+                const int waves = t > kMaxWide ? t - kMaxWide : t;
+                const int wgs = t > kMaxTeam ? waves / kWideGroup : 0;
+                p->launches.push_back({grp, waves, (int)order.size(), (int)members.size(),
+                                       wgs ? 0 : lane_of_team(waves), wgs, p->n_wide});
+                if (wgs) p->n_wide += (int)members.size();
                 order.insert(order.end(), members.begin(), members.end());
             }
     }
@@ -394,6 +416,7 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         o_gjobs = take(sizeof(GenJob) * p->n_gen), o_gorder = take(sizeof(int32_t) * gorder.size());
     }
     const uint64_t o_redo = p->n_band > 0 ? take(sizeof(int32_t) * p->n_gen) : 0;
+    const uint64_t o_wctl = p->n_wide > 0 ? take(4ull * kWideCtl * (p->n_wide + 1)) : 0;  // failure word, blocks
     p->block_bytes = at;
     uint8_t *base = nullptr;
     if (env && env->pool) {
@@ -407,6 +430,7 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
     p->d_seq = base + o_seq, p->d_dir = base + o_dir, p->d_bnd = (int32_t *)(base + o_bnd);
     p->d_cig = (uint32_t *)(base + o_cig), p->d_jobs = (KswJob *)(base + o_jobs), p->d_out = (KswOut *)(base + o_out);
     p->d_order = (int32_t *)(base + o_order), p->d_recs = (StripeRec *)(base + o_recs);
+    if (p->n_wide > 0) p->d_werr = (int32_t *)(base + o_wctl), p->d_wctl = p->d_werr + kWideCtl;
     if (p->n_gen > 0) {
         p->d_state = base + o_state, p->d_offs = (int32_t *)(base + o_offs);
         p->d_gjobs = (GenJob *)(base + o_gjobs), p->d_gorder = (int32_t *)(base + o_gorder);
@@ -546,7 +570,40 @@ int plan_launch(Plan *p, const Progress *prog)
     sa.seq = p->d_seq, sa.dir = p->d_dir, sa.bnd = p->d_bnd, sa.cig = p->d_cig, sa.sc = p->sc;
     sa.jobs = p->d_jobs, sa.out = p->d_out, sa.recs = p->d_recs;
     sa.host_out = p->d_mout, sa.host_cig = p->d_mcig, sa.host_flag = p->d_mflag;
+    sa.wgs = 0, sa.wctl = p->d_wctl, sa.werr = p->d_werr;
+    if (p->n_wide > 0) {
+        // wide launches (the longest pairs) first, on one of the two chains,
+        // then their epilogues outside the chain
+        hipStream_t st = lane(0);
+        if (!st) return KSW2_HIP_EDEVICE;
+        KSW_HIP_TRY(clear_wide_err(p->d_werr, st));
+        {
+            std::lock_guard<std::mutex> lk(g_wide_mu);
+            int c = -1;  // a chain whose last sequence has finished, else the next in turn
+            for (int k = 0; k < 2 && c < 0; ++k)
+                if (!g_wide_done[k] || hipEventQuery(g_wide_done[k]) == hipSuccess) c = k;
+            if (c < 0) c = g_wide_next;
+            g_wide_next = 1 - c;
+            if (!g_wide_done[c] && hipEventCreateWithFlags(&g_wide_done[c], hipEventDisableTiming) != hipSuccess) {
+                g_wide_done[c] = nullptr;
+                return KSW2_HIP_EDEVICE;
+            }
+            KSW_HIP_TRY(hipStreamWaitEvent(st, g_wide_done[c], 0));
+            for (const Plan::Launch &l : p->launches) {
+                if (!l.wgs) continue;
+                sa.order = p->d_order + l.off, sa.njobs = l.count, sa.wctl = p->d_wctl + (int64_t)l.woff * kWideCtl;
+                KSW_HIP_TRY(launch_stripe_wide(sa, l.grp, l.wgs, st));
+            }
+            KSW_HIP_TRY(hipEventRecord(g_wide_done[c], st));
+        }
+        for (const Plan::Launch &l : p->launches) {
+            if (!l.wgs) continue;
+            sa.order = p->d_order + l.off, sa.njobs = l.count, sa.wctl = p->d_wctl + (int64_t)l.woff * kWideCtl;
+            KSW_HIP_TRY(launch_stripe_wide_epilogue(sa, l.grp, st));
+        }
+    }
     for (const Plan::Launch &l : p->launches) {
+        if (l.wgs) continue;
         hipStream_t st = lane(l.lane);
         if (!st) return KSW2_HIP_EDEVICE;
         sa.order = p->d_order + l.off, sa.njobs = l.count;
@@ -643,7 +700,10 @@ int plan_fetch(Plan *p, ksw_extz_t *ez, void *km, const std::vector<char> *skip)
     std::vector<uint32_t> cig(p->cig_words);
     KSW_HIP_TRY(hipMemcpyAsync(outs.data(), p->d_out, sizeof(KswOut) * p->n, hipMemcpyDeviceToHost, p->stream));
     KSW_HIP_TRY(hipMemcpyAsync(cig.data(), p->d_cig, p->cig_words * 4, hipMemcpyDeviceToHost, p->stream));
+    int32_t werr = 0;
+    if (p->d_werr) KSW_HIP_TRY(hipMemcpyAsync(&werr, p->d_werr, 4, hipMemcpyDeviceToHost, p->stream));
     KSW_HIP_TRY(hipStreamSynchronize(p->stream));
+    if (werr) return KSW2_HIP_EDEVICE;  // a wide team gave up a wait: its results are not computed
     for (int k = 0; k < p->n; ++k) {
         if (skip && (*skip)[k]) continue;
         const int st = deliver_slot(p, k, outs[k], cig.data(), ez + p->order[k], km);

@@ -495,14 +495,16 @@ bool Engine::reap_locked(std::unique_lock<std::mutex> &lk)
 }
 
 // Team size of a queued task under the per-wave step budget T, at most
-// `cap` waves and one wave per pass; 0 for a general-kernel task (those form
+// `cap` waves and one wave per pass; teams above kMaxTeam are wide (whole
+// workgroups of kWideGroup waves); 0 for a general-kernel task (those form
 // flights of their own).
 static int team_for(int passes, double steps, bool general, double T, int cap)
 {
     if (general) return 0;
     int team = 1;
-    while (team < kMaxTeam && team < passes && 2 * team <= cap && steps / team > 2 * T) team <<= 1;
-    return std::min(team, passes);
+    while (team < kMaxWide && team < passes && 2 * team <= cap && steps / team > 2 * T) team <<= 1;
+    team = std::min(team, passes);
+    return team > kMaxTeam ? (team + kWideGroup - 1) / kWideGroup * kWideGroup : team;
 }
 
 // Forms and launches flights while streams, waves and memory allow: each free
@@ -543,7 +545,7 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
         const uint64_t avail = std::min(free_block, share);
         // per-wave step budget: work queued and in flight over the device's waves
         const double T = std::max(1.0, (queued_steps_ + inflight_steps_) / resident_waves_);
-        const int cap = (int)std::min<int64_t>(free_waves, kMaxTeam);
+        const int cap = (int)std::min<int64_t>(free_waves, kMaxWide);
         const Task *head = q.front();
         const int key = batch_ ? (head->general ? 0 : -1) : team_for(head->passes, head->steps, head->general, T, cap);
         const double span = key > 0 ? head->steps / key : 0;  // the head's steps per wave
@@ -564,7 +566,7 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
                 ++it;
                 continue;
             }
-            const int w = key > 0 ? key : key < 0 ? (t->cls == 2 ? std::min(kMaxTeam, t->passes) : 1) : t->waves;
+            const int w = key > 0 ? key : key < 0 ? (t->cls == 2 ? std::min(kMaxWide, t->passes) : 1) : t->waves;
             if (!pick.empty() && waves + w > free_waves) break;
             // a flight holds at most its class's share of the pool, except that
             // its first task may use any free block (a task larger than the share)

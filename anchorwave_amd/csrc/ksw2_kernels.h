@@ -22,6 +22,12 @@ struct StripeArgs {
     uint32_t *host_cig;
     int32_t *host_flag;
     KswScore sc;
+    // wide launches only (launch_stripe_wide): workgroups per job, the
+    // control blocks of this launch's jobs (kWideCtl dwords each, in order
+    // position), and the plan's failure word (raised when a wait gave up)
+    int wgs;
+    int32_t *wctl;
+    int32_t *werr;
 };
 
 struct GenArgs {
@@ -105,6 +111,24 @@ enum StripeGroup { kGroupLeft = 0, kGroupRight = 1, kGroupScoreOnly = 2, kGroupT
 
 // Stripe kernel: one workgroup of `team` waves (1..kMaxTeam) per job.
 hipError_t launch_stripe(const StripeArgs &a, int group, int team, hipStream_t st);
+
+// Wide stripe kernel: a job's passes spread over `wgs` workgroups of
+// kWideGroup waves on different CUs (a team of up to kMaxWide waves), handed
+// on through device-scope counters in the job's control block.  Launches go
+// in chunks of at most a quarter of the device's workgroups, so that every
+// workgroup of a chunk can be resident beside one band chunk and one other
+// wide chunk; callers run at most two wide launch sequences at once.
+// launch_stripe_wide_epilogue then runs each job's epilogue (mqe, start
+// cell, backtrack), one workgroup per job.
+constexpr int kMaxWide = 64;        // waves per wide team, at most
+constexpr int kWideGroup = 4;       // waves per workgroup: one per SIMD of a CU
+constexpr int kWideCtlStride = 16;  // dwords between two waves' progress counts
+constexpr int kWideCtlLate = kMaxWide * kWideCtlStride;  // raised when a wait gave up
+constexpr int kWideCtl = kWideCtlLate + 64;              // dwords per job
+hipError_t launch_stripe_wide(const StripeArgs &a, int group, int wgs, hipStream_t st);
+hipError_t launch_stripe_wide_epilogue(const StripeArgs &a, int group, hipStream_t st);
+// Zeroes a plan's wide failure word (kWideCtl dwords at werr).
+hipError_t clear_wide_err(int32_t *werr, hipStream_t st);
 // Waves of the stripe kernel that fit on the device at once.
 int stripe_resident_waves(int group);
 

@@ -110,8 +110,10 @@ struct Plan {
     std::vector<uint64_t> cig_off;   // plan slot -> uint32 offset in the cig arena
     // One launch per (job group, team size).  Groups: 0 left-aligned
     // traceback, 1 right-aligned, 2 score-only.  `off`/`count` index d_order.
+    // wgs > 0: a wide launch (wgs workgroups of kWideGroup waves per pair),
+    // whose control blocks start at block `woff` of d_wctl.
     struct Launch {
-        int grp, team, off, count, lane;
+        int grp, team, off, count, lane, wgs, woff;
     };
     std::vector<Launch> launches;
     struct GLaunch {
@@ -151,6 +153,9 @@ struct Plan {
     int32_t *h_stop = nullptr;       // pinned stop flag the general kernel polls
     int32_t *d_stop = nullptr;
     StripeRec *d_recs = nullptr;
+    int n_wide = 0;                  // pairs of wide launches
+    int32_t *d_wctl = nullptr;       // their control blocks (kWideCtl dwords each)
+    int32_t *d_werr = nullptr;       // raised when a wide team gave up a wait
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     // launches of different team sizes overlap on their own streams
     static constexpr int kLanes = 4;

@@ -188,6 +188,70 @@ __device__ __forceinline__ int wait_count(const int32_t *slot, int need, int kno
     return seen;
 }
 
+// Wide teams (waves on several CUs): counts and column-buffer entries are
+// device-scope accesses, coherent across the XCDs' L2s; a count is stored
+// only after the entries it covers are complete (s_waitcnt vmcnt(0)).
+constexpr uint64_t kWaitTicks = 1000000000ull;  // 10 s of the 100 MHz real-time clock
+
+__device__ __forceinline__ void publish_wide(int32_t *slot, int value)
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xF70);  // vmcnt(0)
+    asm volatile("" ::: "memory");
+    if (__lane_id() == 0) __hip_atomic_store(slot, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int load_count_wide(const int32_t *slot)
+{
+    return __builtin_amdgcn_readfirstlane(__hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// As wait_count; past kWaitTicks (a workgroup the dispatcher could not
+// place) it gives up, sets `late` and returns a count that satisfies every
+// later wait, so every wave still runs to its end.
+__device__ __forceinline__ int wait_count_wide(const int32_t *slot, int need, int known, int &late)
+{
+    if (known >= need) return known;
+    int seen = load_count_wide(slot);
+    if (seen < need) {
+        const uint64_t t_start = wall_clock64();
+        while (seen < need) {
+            __builtin_amdgcn_s_sleep(1);
+            seen = load_count_wide(slot);
+            if (seen < need && wall_clock64() - t_start > kWaitTicks) {
+                late = 1;
+                seen = INT32_MAX;
+            }
+        }
+    }
+    asm volatile("" ::: "memory");  // the entry loads stay behind the wait
+    return seen;
+}
+
+// Two column-buffer dwords (one entry): plain accesses within a workgroup,
+// device-scope ones across CUs.
+template <bool WIDE>
+__device__ __forceinline__ void load_entry(const int32_t *src, int &a, int &b)
+{
+    if (WIDE) {
+        const uint64_t v = __hip_atomic_load((const uint64_t *)src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        a = (int)(uint32_t)v, b = (int)(uint32_t)(v >> 32);
+    } else {
+        a = src[0], b = src[1];
+    }
+}
+
+template <bool WIDE>
+__device__ __forceinline__ void store_entry(int32_t *dst, int a, int b)
+{
+    if (WIDE) {
+        const uint64_t v = (uint64_t)(uint32_t)a | (uint64_t)(uint32_t)b << 32;
+        __hip_atomic_store((uint64_t *)dst, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        dst[0] = a, dst[1] = b;
+    }
+}
+
 // mqe partial of one stripe from the snapshot of U at column qlen-1 (one
 // half of Us): H(i,qlen-1) = H(-1,qlen-1) + sum_{i'<=i} u(i'), kept in the
 // -8 domain; the smallest running sum (largest H) with ties to the larger
@@ -232,9 +296,11 @@ __device__ __forceinline__ void stripe_rec(const v2s (&Us)[R], bool hi, int i0, 
 // stripe's bottom row.  Lanes compute every step; each half's left state is
 // (re)initialised at its first column and U is snapshot at its last one, so
 // the cells a half computes outside its columns never reach a result.
-template <bool RIGHT, int MODE>
+// WIDE: the team spans several CUs (progress counts in device memory,
+// kWideCtlStride dwords apart; a wait that gives up sets `late`).
+template <bool RIGHT, int MODE, bool WIDE>
 __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc, int job_id, int p, const KswJob &jb,
-                                          int qlen, int tlen, int32_t *prog, int W)
+                                          int qlen, int tlen, int32_t *prog, int W, int &late)
 {
     const KswScore &sc = a.sc;
     const int nstripes = (tlen + SR - 1) / SR, npass = (nstripes + 1) / 2;
@@ -251,9 +317,10 @@ __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc,
     const int rho_last = last ? (tlen - 1) % R : -1;
     const int lane_last = ((tlen - 1) % SR) / R;
     // progress slots: pass p-1 is wave (p-1) % W's ((p-1)/W)-th pass
-    const int32_t *prog_prev = prog + (p + W - 1) % W;
+    constexpr int PS = WIDE ? kWideCtlStride : 1;
+    const int32_t *prog_prev = prog + (p + W - 1) % W * PS;
     const int base_prev = first ? 0 : (p - 1) / W * (qlen + 1);
-    int32_t *prog_mine = prog + p % W;
+    int32_t *prog_mine = prog + p % W * PS;
     const int base_mine = p / W * (qlen + 1);
 
     const int ilo = 2 * p * SR + lane * R, ihi = ilo + SR;  // this lane's first rows
@@ -271,11 +338,15 @@ __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc,
     int bin0 = 0, bin1 = 0, bnx0 = 0, bnx1 = 0;  // lane-0 boundary queue (rotating), next block
     int out0 = 0, out1 = 0;                      // lane-63 output queue (rotating)
     int known = 0;                               // columns of pass p-1 known published
+    auto wait_prev = [&](int need) {
+        if (WIDE) known = wait_count_wide(prog_prev, need, known, late);
+        else known = wait_count(prog_prev, need, known);
+    };
     if (!first) {
-        known = wait_count(prog_prev, base_prev + min(2 * kWave, qlen), known);
+        wait_prev(base_prev + min(2 * kWave, qlen));
         const int c0 = lane, c1 = kWave + lane;
-        bin0 = c0 < qlen ? bnd[2 * c0] : 0, bin1 = c0 < qlen ? bnd[2 * c0 + 1] : 0;
-        bnx0 = c1 < qlen ? bnd[2 * c1] : 0, bnx1 = c1 < qlen ? bnd[2 * c1 + 1] : 0;
+        if (c0 < qlen) load_entry<WIDE>(bnd + 2 * c0, bin0, bin1);
+        if (c1 < qlen) load_entry<WIDE>(bnd + 2 * c1, bnx0, bnx1);
     }
     const int lim_hi = bits(kc.LIM) << 16;
     uint32_t qword = 0;
@@ -295,9 +366,10 @@ __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc,
             // rotate the queue so lane 0 sees column c+1 next step
             if ((c & (kWave - 1)) == kWave - 1) {
                 bin0 = bnx0, bin1 = bnx1;
-                known = wait_count(prog_prev, base_prev + min(c + 1 + 2 * kWave, qlen), known);
+                wait_prev(base_prev + min(c + 1 + 2 * kWave, qlen));
                 const int cn = c + 1 + kWave + lane;
-                bnx0 = cn < qlen ? bnd[2 * cn] : 0, bnx1 = cn < qlen ? bnd[2 * cn + 1] : 0;
+                bnx0 = 0, bnx1 = 0;
+                if (cn < qlen) load_entry<WIDE>(bnd + 2 * cn, bnx0, bnx1);
             } else {
                 bin0 = dpp_shl1(0, bin0), bin1 = dpp_shl1(0, bin1);
             }
@@ -387,8 +459,10 @@ __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc,
             out0 = dpp_shl1(w0, out0), out1 = dpp_shl1(Vo, out1);
             if ((c & (kWave - 1)) == kWave - 1 || c == nsteps - 1) {
                 const int col = c - (kWave - 1) - kLag - (kWave - 1) + lane;
-                if (col >= 0 && col < qlen) bnd[2 * col] = out0, bnd[2 * col + 1] = out1;
-                publish(prog_mine, base_mine + max(0, min(c - (kWave - 1) - kLag + 1, qlen)));
+                if (col >= 0 && col < qlen) store_entry<WIDE>(bnd + 2 * col, out0, out1);
+                const int count = base_mine + max(0, min(c - (kWave - 1) - kLag + 1, qlen));
+                if (WIDE) publish_wide(prog_mine, count);
+                else publish(prog_mine, count);
             }
         }
     }
@@ -601,9 +675,71 @@ __global__ void __launch_bounds__(kWave * kMaxTeam) ksw_stripe_kernel(StripeArgs
     const KswJob jb = a.jobs[job];
     const int qlen = __builtin_amdgcn_readfirstlane(jb.qlen), tlen = __builtin_amdgcn_readfirstlane(jb.tlen);
     const int npass = ((tlen + SR - 1) / SR + 1) / 2;
-    for (int p = w; p < npass; p += W) pass_body<RIGHT, MODE>(a, kc, job, p, jb, qlen, tlen, prog, W);
+    int late = 0;  // (no bounded waits within a workgroup)
+    for (int p = w; p < npass; p += W) pass_body<RIGHT, MODE, false>(a, kc, job, p, jb, qlen, tlen, prog, W, late);
     __syncthreads();  // every pass written: stripe partials, score / mte, direction bytes
     pair_epilogue(a, job, jb, tile, MODE == kTracks, &shared_n);
+}
+
+// Wide teams: `a.wgs` workgroups of kWideGroup waves per pair (consecutive
+// blocks, which the dispatcher spreads over the XCDs' CUs); wave v of the
+// team computes passes v, v+W, ...  A pair's control block holds one
+// progress count per wave; a wave whose wait gave up raises its late word.
+template <bool RIGHT, int MODE>
+__global__ void __launch_bounds__(kWave * kMaxTeam) ksw_stripe_wide_kernel(StripeArgs a)
+{
+    const int wgs = a.wgs;
+    const int W = wgs * kWideGroup;
+    const int slot = blockIdx.x / wgs;
+    const int v = __builtin_amdgcn_readfirstlane((int)(blockIdx.x % wgs) * kWideGroup + (int)threadIdx.x / kWave);
+    const int job = __builtin_amdgcn_readfirstlane(a.order[slot]);
+    int32_t *ctl = a.wctl + (int64_t)slot * kWideCtl;
+    const Consts kc = make_consts<RIGHT>(a.sc);
+    const KswJob jb = a.jobs[job];
+    const int qlen = __builtin_amdgcn_readfirstlane(jb.qlen), tlen = __builtin_amdgcn_readfirstlane(jb.tlen);
+    const int npass = ((tlen + SR - 1) / SR + 1) / 2;
+    int late = 0;
+    for (int p = v; p < npass; p += W) pass_body<RIGHT, MODE, true>(a, kc, job, p, jb, qlen, tlen, ctl, W, late);
+    if (late && threadIdx.x % kWave == 0)
+        __hip_atomic_store(ctl + kWideCtlLate, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Zeroes the control blocks of a wide launch's pairs (a fill kernel rather
+// than a memset, so a captured graph replays it correctly).
+__global__ void __launch_bounds__(256) ksw_stripe_wctl_kernel(int32_t *wctl)
+{
+    int32_t *ctl = wctl + (int64_t)blockIdx.x * kWideCtl;
+    for (int k = threadIdx.x; k < kWideCtl; k += blockDim.x) ctl[k] = 0;
+}
+
+// The epilogue of every pair of a wide launch, one workgroup per pair, after
+// all of the launch's passes.  A pair whose team gave up a wait is reported
+// as failed (zdropped, no CIGAR; the plan's failure word is raised) rather
+// than walked.
+template <int MODE>
+__global__ void __launch_bounds__(256) ksw_stripe_wide_epilogue_kernel(StripeArgs a)
+{
+    __shared__ uint64_t tile[8][64];
+    __shared__ int shared_n;
+    const int slot = blockIdx.x;
+    const int job = __builtin_amdgcn_readfirstlane(a.order[slot]);
+    const KswJob jb = a.jobs[job];
+    const int late = __builtin_amdgcn_readfirstlane(a.wctl[(int64_t)slot * kWideCtl + kWideCtlLate]);
+    if (!late) {
+        pair_epilogue(a, job, jb, tile, MODE == kTracks, &shared_n);
+        return;
+    }
+    if (threadIdx.x == 0) {
+        KswOut o = a.out[job];
+        o.score = KSW_NEG_INF_I, o.zdropped = 1, o.n_cigar = 0;
+        a.out[job] = o;
+        *a.werr = 1;
+        if (a.host_flag) {
+            a.host_out[job] = o;
+            __threadfence_system();
+            __hip_atomic_store(a.host_flag + job, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
 }
 
 template <bool RIGHT, int MODE>
@@ -624,6 +760,71 @@ hipError_t launch_stripe(const StripeArgs &a, int group, int team, hipStream_t s
     case kGroupTracks: return launch_stripe_t<false, kTracks>(a, team, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+namespace {
+
+template <bool RIGHT, int MODE>
+hipError_t launch_wide_t(const StripeArgs &a0, int wgs, hipStream_t st)
+{
+    void (*fn)(StripeArgs) = ksw_stripe_wide_kernel<RIGHT, MODE>;
+    const int threads = kWave * kWideGroup;
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, threads, 0) != hipSuccess)
+        return hipErrorInvalidConfiguration;
+    // a quarter of the device's workgroups per launch: the workgroups of a
+    // launch wait on one another, so all of them must be resident together
+    // beside a band chunk (at most half) and one other wide chunk; kernels
+    // of other streams finish on their own
+    const int cap = cus * per_cu;
+    if (cap < 4 * wgs) return hipErrorInvalidConfiguration;
+    const int chunk = cap / 4 / wgs;
+    StripeArgs a = a0;
+    a.wgs = wgs;
+    for (int off = 0; off < a0.njobs; off += chunk) {
+        a.order = a0.order + off;
+        a.wctl = a0.wctl + (int64_t)off * kWideCtl;
+        a.njobs = a0.njobs - off < chunk ? a0.njobs - off : chunk;
+        hipLaunchKernelGGL(ksw_stripe_wctl_kernel, dim3(a.njobs), dim3(256), 0, st, a.wctl);
+        hipLaunchKernelGGL(fn, dim3(a.njobs * wgs), dim3(threads), 0, st, a);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t launch_stripe_wide(const StripeArgs &a, int group, int wgs, hipStream_t st)
+{
+    if (a.njobs <= 0) return hipSuccess;
+    if (wgs < 1 || wgs * kWideGroup > kMaxWide || !a.wctl || !a.werr) return hipErrorInvalidValue;
+    switch (group) {
+    case kGroupLeft: return launch_wide_t<false, kDirs>(a, wgs, st);
+    case kGroupRight: return launch_wide_t<true, kDirs>(a, wgs, st);
+    case kGroupScoreOnly: return launch_wide_t<false, kScoreOnly>(a, wgs, st);
+    case kGroupTracks: return launch_wide_t<false, kTracks>(a, wgs, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t clear_wide_err(int32_t *werr, hipStream_t st)
+{
+    hipLaunchKernelGGL(ksw_stripe_wctl_kernel, dim3(1), dim3(256), 0, st, werr);
+    return hipGetLastError();
+}
+
+hipError_t launch_stripe_wide_epilogue(const StripeArgs &a, int group, hipStream_t st)
+{
+    if (a.njobs <= 0) return hipSuccess;
+    if (!a.wctl || !a.werr) return hipErrorInvalidValue;
+    void (*fn)(StripeArgs) = group == kGroupTracks       ? ksw_stripe_wide_epilogue_kernel<kTracks>
+                             : group == kGroupScoreOnly ? ksw_stripe_wide_epilogue_kernel<kScoreOnly>
+                                                        : ksw_stripe_wide_epilogue_kernel<kDirs>;
+    hipLaunchKernelGGL(fn, dim3(a.njobs), dim3(256), 0, st, a);
+    return hipGetLastError();
 }
 
 template <bool RIGHT, int MODE>
