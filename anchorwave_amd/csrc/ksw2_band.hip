@@ -781,8 +781,9 @@ hipError_t launch_band(const BandArgs &a0, int team, bool right, bool score_only
                            : right    ? ksw_band_kernel<true, false>
                                       : ksw_band_kernel<false, false>;
     // half the device's capacity per launch: the workgroups of a launch
-    // wait on one another, so all of them must be resident together even
-    // beside the kernels of other streams (which finish on their own)
+    // wait on one another, so all of them must be resident together, beside
+    // the kernels of other streams (which finish on their own) and one other
+    // waiting launch of at most half the device (the caller's admission)
     const int cap = band_capacity((const void *)fn, threads);
     if (cap < 2 * a.wgs) return hipErrorInvalidConfiguration;
     const int chunk = cap / 2 / a.wgs;

@@ -43,21 +43,55 @@ uint64_t mem_budget()
 }  // namespace
 
 namespace {
-// Band-kernel launches of the process run one after another.  The
-// workgroups of a band launch wait on one another, so a launch is sized to
-// be resident as a whole; two at once could each hold part of the device.
-// Each sequence of band launches waits (on its own stream) for the previous
-// sequence's completion event, then records the event again, under a lock.
-// An event and not a stream of its own: a stream would hold one of the
-// process's few hardware queues for good.
-std::mutex g_band_mu;
-hipEvent_t g_band_done = nullptr;
-// Wide stripe launches likewise, on two such chains: at most two sequences
-// of wide launches run at once (each chunk at most a quarter of the device,
-// a band chunk at most half).
-std::mutex g_wide_mu;
-hipEvent_t g_wide_done[2] = {nullptr, nullptr};
-int g_wide_next = 0;
+// Launches whose workgroups wait on one another (band kernel, wide stripe
+// teams) need every workgroup of a chunk resident at once, beside the
+// chunks of the other such launches that run at the same time; kernels that
+// do not wait finish on their own.  The device is split into four quarters,
+// each a chain of completion events.  A launch sequence takes two quarters
+// (free ones first, else those taken longest ago), waits for them on its own
+// stream, keeps every chunk within half the device and records the quarters'
+// events again, all under one lock.  So at most two such sequences run at
+// once and their chunks together fit the device.  Events rather than
+// streams of their own: a stream would hold one of the process's few
+// hardware queues for good.
+constexpr int kQuarters = 4;
+std::mutex g_wait_mu;
+hipEvent_t g_quarter[kQuarters] = {};
+uint64_t g_quarter_used[kQuarters] = {};
+uint64_t g_quarter_seq = 0;
+
+// Takes two quarters for a sequence about to be enqueued on `st` (caller
+// holds g_wait_mu); their indices go to q[0..1].
+bool wait_slots_acquire(hipStream_t st, int q[2])
+{
+    int n = 0;
+    bool taken[kQuarters] = {};
+    for (int k = 0; k < kQuarters && n < 2; ++k)
+        if (!g_quarter[k] || hipEventQuery(g_quarter[k]) == hipSuccess) q[n++] = k, taken[k] = true;
+    while (n < 2) {
+        int best = -1;
+        for (int k = 0; k < kQuarters; ++k)
+            if (!taken[k] && (best < 0 || g_quarter_used[k] < g_quarter_used[best])) best = k;
+        q[n++] = best, taken[best] = true;
+    }
+    for (int i = 0; i < 2; ++i) {
+        hipEvent_t &ev = g_quarter[q[i]];
+        if (!ev && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            ev = nullptr;
+            return false;
+        }
+        if (hipStreamWaitEvent(st, ev, 0) != hipSuccess) return false;
+        g_quarter_used[q[i]] = ++g_quarter_seq;
+    }
+    return true;
+}
+
+bool wait_slots_release(hipStream_t st, const int q[2])
+{
+    for (int i = 0; i < 2; ++i)
+        if (hipEventRecord(g_quarter[q[i]], st) != hipSuccess) return false;
+    return true;
+}
 }  // namespace
 
 int device_init(int device)
@@ -307,25 +341,38 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         if (const char *ev = getenv("KSW2_HIP_WIDE")) wide_forced = std::max(0, std::min(kMaxWide / kWideGroup, atoi(ev)));
         std::vector<int> team(p->n_fast);
         for (int k = 0; k < p->n_fast; ++k)
-            team[k] = wide_forced ? kWideGroup * wide_forced + kMaxWide : choose_team(p->jobs[k], T, forced);
-        // keys: kMaxWide + waves for forced wide teams, then the team sizes
-        for (int grp = 0; grp < kGroups; ++grp)
-            for (int t = 2 * kMaxWide; t >= 1; --t) {
-                std::vector<int> members;
-                for (int k = 0; k < p->n_fast; ++k)
-                    if (grp_of[k] == grp && team[k] == t) members.push_back(k);
-                if (members.empty()) continue;
-                std::stable_sort(members.begin(), members.end(), [&](int x, int y) {
-                    const KswJob &a = p->jobs[x], &b = p->jobs[y];
-                    return pair_steps(a) > pair_steps(b);
-                });
-                const int waves = t > kMaxWide ? t - kMaxWide : t;
-                const int wgs = t > kMaxTeam ? waves / kWideGroup : 0;
-                p->launches.push_back({grp, waves, (int)order.size(), (int)members.size(),
-                                       wgs ? 0 : lane_of_team(waves), wgs, p->n_wide});
-                if (wgs) p->n_wide += (int)members.size();
+            team[k] = wide_forced ? kWideGroup * wide_forced : choose_team(p->jobs[k], T, forced);
+        auto wide = [&](int k) { return wide_forced > 0 || team[k] > kMaxTeam; };
+        auto longest_first = [&](std::vector<int> &m) {
+            std::stable_sort(m.begin(), m.end(), [&](int x, int y) { return pair_steps(p->jobs[x]) > pair_steps(p->jobs[y]); });
+        };
+        for (int grp = 0; grp < kGroups; ++grp) {
+            // every wide pair of the group in one launch, each on its own
+            // number of workgroups
+            std::vector<int> members;
+            for (int k = 0; k < p->n_fast; ++k)
+                if (grp_of[k] == grp && wide(k)) members.push_back(k);
+            if (!members.empty()) {
+                longest_first(members);
+                const int wf = (int)p->h_wfirst.size();
+                int blocks = 0;
+                for (int k : members) p->h_wfirst.push_back(blocks), blocks += team[k] / kWideGroup;
+                p->h_wfirst.push_back(blocks);
+                p->launches.push_back({grp, 0, (int)order.size(), (int)members.size(), 0, 1, p->n_wide, wf});
+                p->n_wide += (int)members.size();
                 order.insert(order.end(), members.begin(), members.end());
             }
+            // one-workgroup teams: one launch per team size
+            for (int t = kMaxTeam; t >= 1; --t) {
+                members.clear();
+                for (int k = 0; k < p->n_fast; ++k)
+                    if (grp_of[k] == grp && !wide(k) && team[k] == t) members.push_back(k);
+                if (members.empty()) continue;
+                longest_first(members);
+                p->launches.push_back({grp, t, (int)order.size(), (int)members.size(), lane_of_team(t), 0, 0, 0});
+                order.insert(order.end(), members.begin(), members.end());
+            }
+        }
     }
     // general-kernel launches: per block size and LDS ring size, most
     // anti-diagonal cells first
@@ -417,6 +464,7 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
     }
     const uint64_t o_redo = p->n_band > 0 ? take(sizeof(int32_t) * p->n_gen) : 0;
     const uint64_t o_wctl = p->n_wide > 0 ? take(4ull * kWideCtl * (p->n_wide + 1)) : 0;  // failure word, blocks
+    const uint64_t o_wfirst = p->n_wide > 0 ? take(4ull * p->h_wfirst.size()) : 0;
     p->block_bytes = at;
     uint8_t *base = nullptr;
     if (env && env->pool) {
@@ -430,7 +478,10 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
     p->d_seq = base + o_seq, p->d_dir = base + o_dir, p->d_bnd = (int32_t *)(base + o_bnd);
     p->d_cig = (uint32_t *)(base + o_cig), p->d_jobs = (KswJob *)(base + o_jobs), p->d_out = (KswOut *)(base + o_out);
     p->d_order = (int32_t *)(base + o_order), p->d_recs = (StripeRec *)(base + o_recs);
-    if (p->n_wide > 0) p->d_werr = (int32_t *)(base + o_wctl), p->d_wctl = p->d_werr + kWideCtl;
+    if (p->n_wide > 0) {
+        p->d_werr = (int32_t *)(base + o_wctl), p->d_wctl = p->d_werr + kWideCtl;
+        p->d_wfirst = (int32_t *)(base + o_wfirst);
+    }
     if (p->n_gen > 0) {
         p->d_state = base + o_state, p->d_offs = (int32_t *)(base + o_offs);
         p->d_gjobs = (GenJob *)(base + o_gjobs), p->d_gorder = (int32_t *)(base + o_gorder);
@@ -484,6 +535,9 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         e[4] = hipMemcpyAsync(p->d_gorder, gorder.data(), sizeof(int32_t) * gorder.size(), hipMemcpyHostToDevice,
                               p->stream);
     }
+    if (p->n_wide > 0 && e[2] == hipSuccess)
+        e[2] = hipMemcpyAsync(p->d_wfirst, p->h_wfirst.data(), sizeof(int32_t) * p->h_wfirst.size(),
+                              hipMemcpyHostToDevice, p->stream);
     if (p->own_streams) {
         e[5] = hipStreamSynchronize(p->stream);
         if (p->own_seq) (void)hipHostFree(h_seq);
@@ -549,17 +603,14 @@ int plan_launch(Plan *p, const Progress *prog)
         hipStream_t st = lane(0);
         if (!st) return KSW2_HIP_EDEVICE;
         {
-            std::lock_guard<std::mutex> lk(g_band_mu);
-            if (!g_band_done && hipEventCreateWithFlags(&g_band_done, hipEventDisableTiming) != hipSuccess) {
-                g_band_done = nullptr;
-                return KSW2_HIP_EDEVICE;
-            }
-            KSW_HIP_TRY(hipStreamWaitEvent(st, g_band_done, 0));
+            std::lock_guard<std::mutex> lk(g_wait_mu);
+            int q[2];
+            if (!wait_slots_acquire(st, q)) return KSW2_HIP_EDEVICE;
             for (const Plan::BLaunch &l : p->blaunches) {
                 ba.order = p->d_gorder + l.off, ba.njobs = l.count;
-                KSW_HIP_TRY(launch_band(ba, l.team, l.grp == 1, l.grp == 2, st));
+                KSW_HIP_TRY(launch_band(ba, l.team, l.grp == 1, l.grp == 2, st));  // chunks of half the device
             }
-            KSW_HIP_TRY(hipEventRecord(g_band_done, st));
+            if (!wait_slots_release(st, q)) return KSW2_HIP_EDEVICE;
         }
         // pairs whose H went above 0 (the band kernel does not track the
         // maximum) run again on the general kernel, on the same stream
@@ -570,40 +621,34 @@ int plan_launch(Plan *p, const Progress *prog)
     sa.seq = p->d_seq, sa.dir = p->d_dir, sa.bnd = p->d_bnd, sa.cig = p->d_cig, sa.sc = p->sc;
     sa.jobs = p->d_jobs, sa.out = p->d_out, sa.recs = p->d_recs;
     sa.host_out = p->d_mout, sa.host_cig = p->d_mcig, sa.host_flag = p->d_mflag;
-    sa.wgs = 0, sa.wctl = p->d_wctl, sa.werr = p->d_werr;
+    sa.wfirst = nullptr, sa.wctl = p->d_wctl, sa.werr = p->d_werr;
     if (p->n_wide > 0) {
-        // wide launches (the longest pairs) first, on one of the two chains,
-        // then their epilogues outside the chain
+        // wide launches (the longest pairs) first, within two quarters of
+        // the device, then their epilogues, which do not wait
         hipStream_t st = lane(0);
         if (!st) return KSW2_HIP_EDEVICE;
         KSW_HIP_TRY(clear_wide_err(p->d_werr, st));
         {
-            std::lock_guard<std::mutex> lk(g_wide_mu);
-            int c = -1;  // a chain whose last sequence has finished, else the next in turn
-            for (int k = 0; k < 2 && c < 0; ++k)
-                if (!g_wide_done[k] || hipEventQuery(g_wide_done[k]) == hipSuccess) c = k;
-            if (c < 0) c = g_wide_next;
-            g_wide_next = 1 - c;
-            if (!g_wide_done[c] && hipEventCreateWithFlags(&g_wide_done[c], hipEventDisableTiming) != hipSuccess) {
-                g_wide_done[c] = nullptr;
-                return KSW2_HIP_EDEVICE;
-            }
-            KSW_HIP_TRY(hipStreamWaitEvent(st, g_wide_done[c], 0));
+            std::lock_guard<std::mutex> lk(g_wait_mu);
+            int q[2];
+            if (!wait_slots_acquire(st, q)) return KSW2_HIP_EDEVICE;
             for (const Plan::Launch &l : p->launches) {
-                if (!l.wgs) continue;
+                if (!l.wide) continue;
                 sa.order = p->d_order + l.off, sa.njobs = l.count, sa.wctl = p->d_wctl + (int64_t)l.woff * kWideCtl;
-                KSW_HIP_TRY(launch_stripe_wide(sa, l.grp, l.wgs, st));
+                sa.wfirst = p->d_wfirst + l.wf;
+                const int half = std::max(1, stripe_wide_capacity(l.grp) / 2);
+                KSW_HIP_TRY(launch_stripe_wide(sa, l.grp, p->h_wfirst.data() + l.wf, half, st));
             }
-            KSW_HIP_TRY(hipEventRecord(g_wide_done[c], st));
+            if (!wait_slots_release(st, q)) return KSW2_HIP_EDEVICE;
         }
         for (const Plan::Launch &l : p->launches) {
-            if (!l.wgs) continue;
+            if (!l.wide) continue;
             sa.order = p->d_order + l.off, sa.njobs = l.count, sa.wctl = p->d_wctl + (int64_t)l.woff * kWideCtl;
             KSW_HIP_TRY(launch_stripe_wide_epilogue(sa, l.grp, st));
         }
     }
     for (const Plan::Launch &l : p->launches) {
-        if (l.wgs) continue;
+        if (l.wide) continue;
         hipStream_t st = lane(l.lane);
         if (!st) return KSW2_HIP_EDEVICE;
         sa.order = p->d_order + l.off, sa.njobs = l.count;

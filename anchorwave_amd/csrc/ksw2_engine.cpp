@@ -129,6 +129,11 @@ struct Engine::Task {
     uint64_t footprint = 0;
     double cells = 0;
     double steps = 0;  // stripe kernel: passes x steps per pass (one wave)
+    // the shortest time the task can take, in steps of one wave: a stripe
+    // pair on its widest team (one wave per pass, at most kMaxWide) runs
+    // its rounds of passes plus the start lag of the team's last wave; a
+    // general pair its anti-diagonals
+    double span = 0;
     int passes = 1;    // stripe kernel: most useful team size
     int waves = 1;     // general kernel: waves of its workgroup
     int held = 0;      // waves held while admitted
@@ -270,9 +275,12 @@ int Engine::submit(const ksw2_pair_t &p, ksw_extz_t *ez, bool copy, uint64_t *ti
         t->cls = size_class(t->cells);
         if (t->general) {
             t->waves = gen_threads(p) / kWave;
+            t->span = (double)p.qlen + p.tlen - 1;
         } else {
             t->passes = pass_count(p.tlen);
             t->steps = (double)t->passes * (double)pass_steps(p.qlen);
+            const int team = std::min(t->passes, kMaxWide);
+            t->span = (double)((t->passes + team - 1) / team) * (double)pass_steps(p.qlen) + (team - 1) * 2.0 * kWave;
         }
     }
     std::lock_guard<std::mutex> lk(mu_);
@@ -307,8 +315,12 @@ void Engine::order_longest_first()
 {
     std::lock_guard<std::mutex> lk(mu_);
     batch_ = true;
+    // longest critical path first (then most cells): a pair whose chain is
+    // long must start early, whatever its cell count
     for (auto &q : queue_)
-        std::stable_sort(q.begin(), q.end(), [](const Task *a, const Task *b) { return a->cells > b->cells; });
+        std::stable_sort(q.begin(), q.end(), [](const Task *a, const Task *b) {
+            return a->span != b->span ? a->span > b->span : a->cells > b->cells;
+        });
 }
 
 int Engine::poll(uint64_t *tickets, double *latency, int max, int wait_ms)

@@ -22,10 +22,11 @@ struct StripeArgs {
     uint32_t *host_cig;
     int32_t *host_flag;
     KswScore sc;
-    // wide launches only (launch_stripe_wide): workgroups per job, the
-    // control blocks of this launch's jobs (kWideCtl dwords each, in order
-    // position), and the plan's failure word (raised when a wait gave up)
-    int wgs;
+    // wide launches only (launch_stripe_wide): the first workgroup of each
+    // job of this launch (njobs + 1 ascending entries, the last one past the
+    // launch), the jobs' control blocks (kWideCtl dwords each, in order
+    // position) and the plan's failure word (raised when a wait gave up)
+    const int32_t *wfirst;
     int32_t *wctl;
     int32_t *werr;
 };
@@ -83,8 +84,9 @@ constexpr int kBandCtl = 512;
 static_assert(kBandCtlRec + 8 * kBandTeam <= kBandCtl, "band control block");
 // Band kernel for a team of `team` waves per job: kBandGroup-wave
 // workgroups on different CUs that hand stripes on through device-scope
-// counters.  Launches go in chunks whose workgroups all fit the device at
-// once, so no spinning workgroup waits on one that cannot be placed.
+// counters.  Launches go in chunks of at most half the device's workgroups;
+// with at most two such waiting launches at once (the caller's admission),
+// no spinning workgroup waits on one that cannot be placed.
 hipError_t launch_band(const BandArgs &a, int team, bool right, bool score_only, hipStream_t st);
 
 // Singletrack walk of general pairs: one wavefront per job (all general
@@ -112,12 +114,13 @@ enum StripeGroup { kGroupLeft = 0, kGroupRight = 1, kGroupScoreOnly = 2, kGroupT
 // Stripe kernel: one workgroup of `team` waves (1..kMaxTeam) per job.
 hipError_t launch_stripe(const StripeArgs &a, int group, int team, hipStream_t st);
 
-// Wide stripe kernel: a job's passes spread over `wgs` workgroups of
-// kWideGroup waves on different CUs (a team of up to kMaxWide waves), handed
-// on through device-scope counters in the job's control block.  Launches go
-// in chunks of at most a quarter of the device's workgroups, so that every
-// workgroup of a chunk can be resident beside one band chunk and one other
-// wide chunk; callers run at most two wide launch sequences at once.
+// Wide stripe kernel: each job's passes spread over its own number of
+// workgroups of kWideGroup waves on different CUs (a team of up to kMaxWide
+// waves), handed on through device-scope counters in the job's control
+// block.  h_wfirst is the host copy of a.wfirst.  Launches go in chunks of
+// at most `max_wgs` workgroups, which the caller keeps within its share of
+// the device (every workgroup of a chunk must be able to be resident at
+// once beside the other waiting launches that may run with it).
 // launch_stripe_wide_epilogue then runs each job's epilogue (mqe, start
 // cell, backtrack), one workgroup per job.
 constexpr int kMaxWide = 64;        // waves per wide team, at most
@@ -125,7 +128,9 @@ constexpr int kWideGroup = 4;       // waves per workgroup: one per SIMD of a CU
 constexpr int kWideCtlStride = 16;  // dwords between two waves' progress counts
 constexpr int kWideCtlLate = kMaxWide * kWideCtlStride;  // raised when a wait gave up
 constexpr int kWideCtl = kWideCtlLate + 64;              // dwords per job
-hipError_t launch_stripe_wide(const StripeArgs &a, int group, int wgs, hipStream_t st);
+hipError_t launch_stripe_wide(const StripeArgs &a, int group, const int32_t *h_wfirst, int max_wgs, hipStream_t st);
+// Workgroups of the wide stripe kernel the device holds at once.
+int stripe_wide_capacity(int group);
 hipError_t launch_stripe_wide_epilogue(const StripeArgs &a, int group, hipStream_t st);
 // Zeroes a plan's wide failure word (kWideCtl dwords at werr).
 hipError_t clear_wide_err(int32_t *werr, hipStream_t st);

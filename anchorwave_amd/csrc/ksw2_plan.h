@@ -110,10 +110,11 @@ struct Plan {
     std::vector<uint64_t> cig_off;   // plan slot -> uint32 offset in the cig arena
     // One launch per (job group, team size).  Groups: 0 left-aligned
     // traceback, 1 right-aligned, 2 score-only.  `off`/`count` index d_order.
-    // wgs > 0: a wide launch (wgs workgroups of kWideGroup waves per pair),
-    // whose control blocks start at block `woff` of d_wctl.
+    // wide: a wide launch (each pair on its own number of kWideGroup-wave
+    // workgroups), whose control blocks start at block `woff` of d_wctl and
+    // whose first-workgroup table starts at entry `wf` of h_wfirst/d_wfirst.
     struct Launch {
-        int grp, team, off, count, lane, wgs, woff;
+        int grp, team, off, count, lane, wide, woff, wf;
     };
     std::vector<Launch> launches;
     struct GLaunch {
@@ -156,6 +157,8 @@ struct Plan {
     int n_wide = 0;                  // pairs of wide launches
     int32_t *d_wctl = nullptr;       // their control blocks (kWideCtl dwords each)
     int32_t *d_werr = nullptr;       // raised when a wide team gave up a wait
+    std::vector<int32_t> h_wfirst;   // per wide launch: each pair's first workgroup, then the end
+    int32_t *d_wfirst = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     // launches of different team sizes overlap on their own streams
     static constexpr int kLanes = 4;

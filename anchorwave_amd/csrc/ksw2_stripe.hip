@@ -681,17 +681,25 @@ __global__ void __launch_bounds__(kWave * kMaxTeam) ksw_stripe_kernel(StripeArgs
     pair_epilogue(a, job, jb, tile, MODE == kTracks, &shared_n);
 }
 
-// Wide teams: `a.wgs` workgroups of kWideGroup waves per pair (consecutive
-// blocks, which the dispatcher spreads over the XCDs' CUs); wave v of the
-// team computes passes v, v+W, ...  A pair's control block holds one
+// Wide teams: each pair has its own run of consecutive blocks (a.wfirst),
+// which the dispatcher spreads over the XCDs' CUs; wave v of a team of W
+// waves computes passes v, v+W, ...  A pair's control block holds one
 // progress count per wave; a wave whose wait gave up raises its late word.
 template <bool RIGHT, int MODE>
 __global__ void __launch_bounds__(kWave * kMaxTeam) ksw_stripe_wide_kernel(StripeArgs a)
 {
-    const int wgs = a.wgs;
-    const int W = wgs * kWideGroup;
-    const int slot = blockIdx.x / wgs;
-    const int v = __builtin_amdgcn_readfirstlane((int)(blockIdx.x % wgs) * kWideGroup + (int)threadIdx.x / kWave);
+    // the pair of this block: wfirst[slot] <= block < wfirst[slot + 1]
+    const int blk = a.wfirst[0] + (int)blockIdx.x;
+    int lo = 0, hi = a.njobs;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (a.wfirst[mid] <= blk) lo = mid;
+        else hi = mid;
+    }
+    const int slot = __builtin_amdgcn_readfirstlane(lo);
+    const int first = __builtin_amdgcn_readfirstlane(a.wfirst[slot]);
+    const int W = __builtin_amdgcn_readfirstlane(a.wfirst[slot + 1] - first) * kWideGroup;
+    const int v = __builtin_amdgcn_readfirstlane((blk - first) * kWideGroup + (int)threadIdx.x / kWave);
     const int job = __builtin_amdgcn_readfirstlane(a.order[slot]);
     int32_t *ctl = a.wctl + (int64_t)slot * kWideCtl;
     const Consts kc = make_consts<RIGHT>(a.sc);
@@ -765,49 +773,76 @@ hipError_t launch_stripe(const StripeArgs &a, int group, int team, hipStream_t s
 namespace {
 
 template <bool RIGHT, int MODE>
-hipError_t launch_wide_t(const StripeArgs &a0, int wgs, hipStream_t st)
+hipError_t launch_wide_t(const StripeArgs &a0, const int32_t *h_wfirst, int max_wgs, hipStream_t st)
 {
     void (*fn)(StripeArgs) = ksw_stripe_wide_kernel<RIGHT, MODE>;
-    const int threads = kWave * kWideGroup;
-    int dev = 0, cus = 0, per_cu = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, threads, 0) != hipSuccess)
-        return hipErrorInvalidConfiguration;
-    // a quarter of the device's workgroups per launch: the workgroups of a
-    // launch wait on one another, so all of them must be resident together
-    // beside a band chunk (at most half) and one other wide chunk; kernels
-    // of other streams finish on their own
-    const int cap = cus * per_cu;
-    if (cap < 4 * wgs) return hipErrorInvalidConfiguration;
-    const int chunk = cap / 4 / wgs;
     StripeArgs a = a0;
-    a.wgs = wgs;
-    for (int off = 0; off < a0.njobs; off += chunk) {
+    int off = 0;
+    while (off < a0.njobs) {
+        // as many pairs as fit in max_wgs workgroups (at least one)
+        int end = off + 1;
+        while (end < a0.njobs && h_wfirst[end + 1] - h_wfirst[off] <= max_wgs) ++end;
+        const int blocks = h_wfirst[end] - h_wfirst[off];
+        if (blocks > max_wgs) return hipErrorInvalidConfiguration;
         a.order = a0.order + off;
+        a.wfirst = a0.wfirst + off;
         a.wctl = a0.wctl + (int64_t)off * kWideCtl;
-        a.njobs = a0.njobs - off < chunk ? a0.njobs - off : chunk;
+        a.njobs = end - off;
         hipLaunchKernelGGL(ksw_stripe_wctl_kernel, dim3(a.njobs), dim3(256), 0, st, a.wctl);
-        hipLaunchKernelGGL(fn, dim3(a.njobs * wgs), dim3(threads), 0, st, a);
+        hipLaunchKernelGGL(fn, dim3(blocks), dim3(kWave * kWideGroup), 0, st, a);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
+        off = end;
     }
     return hipSuccess;
 }
 
+template <bool RIGHT, int MODE>
+int wide_capacity_t()
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)ksw_stripe_wide_kernel<RIGHT, MODE>,
+                                                     kWave * kWideGroup, 0) != hipSuccess)
+        return 0;
+    return cus * per_cu;
+}
+
 }  // namespace
 
-hipError_t launch_stripe_wide(const StripeArgs &a, int group, int wgs, hipStream_t st)
+hipError_t launch_stripe_wide(const StripeArgs &a, int group, const int32_t *h_wfirst, int max_wgs, hipStream_t st)
 {
     if (a.njobs <= 0) return hipSuccess;
-    if (wgs < 1 || wgs * kWideGroup > kMaxWide || !a.wctl || !a.werr) return hipErrorInvalidValue;
+    if (!a.wfirst || !h_wfirst || !a.wctl || !a.werr || max_wgs < 1) return hipErrorInvalidValue;
+    for (int k = 0; k < a.njobs; ++k) {
+        const int n = h_wfirst[k + 1] - h_wfirst[k];
+        if (n < 1 || n * kWideGroup > kMaxWide) return hipErrorInvalidValue;
+    }
     switch (group) {
-    case kGroupLeft: return launch_wide_t<false, kDirs>(a, wgs, st);
-    case kGroupRight: return launch_wide_t<true, kDirs>(a, wgs, st);
-    case kGroupScoreOnly: return launch_wide_t<false, kScoreOnly>(a, wgs, st);
-    case kGroupTracks: return launch_wide_t<false, kTracks>(a, wgs, st);
+    case kGroupLeft: return launch_wide_t<false, kDirs>(a, h_wfirst, max_wgs, st);
+    case kGroupRight: return launch_wide_t<true, kDirs>(a, h_wfirst, max_wgs, st);
+    case kGroupScoreOnly: return launch_wide_t<false, kScoreOnly>(a, h_wfirst, max_wgs, st);
+    case kGroupTracks: return launch_wide_t<false, kTracks>(a, h_wfirst, max_wgs, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+// Cached per variant: the occupancy query reads the device properties.
+int stripe_wide_capacity(int group)
+{
+    static int cache[kGroups] = {};
+    if (group < 0 || group >= kGroups) group = kGroupLeft;
+    if (cache[group] > 0) return cache[group];
+    int n = 0;
+    switch (group) {
+    case kGroupRight: n = wide_capacity_t<true, kDirs>(); break;
+    case kGroupScoreOnly: n = wide_capacity_t<false, kScoreOnly>(); break;
+    case kGroupTracks: n = wide_capacity_t<false, kTracks>(); break;
+    default: n = wide_capacity_t<false, kDirs>(); break;
+    }
+    __atomic_store_n(&cache[group], n, __ATOMIC_RELAXED);
+    return n;
 }
 
 hipError_t clear_wide_err(int32_t *werr, hipStream_t st)
