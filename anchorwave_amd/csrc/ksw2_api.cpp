@@ -156,6 +156,12 @@ void ksw2_hip_clear_progress_callback(void)
     t_cb = nullptr, t_cb_args = nullptr, t_cb_interval = 256;
 }
 
+int ksw2_hip_release_cached_memory(void)
+{
+    kswhip::Engine::release_cached_block();
+    return KSW2_HIP_OK;
+}
+
 int ksw_extd2_hip_batch(int n, const ksw2_pair_t *pairs, int8_t m, const int8_t *mat, int8_t q, int8_t e,
                         int8_t q2, int8_t e2, ksw_extz_t *ez)
 {
@@ -280,14 +286,11 @@ int ksw2_hip_server_create(ksw2_hip_server **server, int8_t m, const int8_t *mat
     if (!normalise_scoring(m, mat, q, e, q2, e2, &sc, &cls)) return set_err(KSW2_HIP_EUNSUPPORTED);
     int st = kswhip::device_init(-1);
     if (st != KSW2_HIP_OK) return set_err(st);
-    uint64_t budget = device_bytes;
-    if (budget == 0) {
-        size_t fr = 0, tot = 0;
-        if (hipMemGetInfo(&fr, &tot) != hipSuccess) return set_err(KSW2_HIP_EDEVICE);
-        budget = (uint64_t)(fr * 0.85);
-    }
+    // device_bytes == 0: 85 % of the free memory, or the library's cached
+    // engine block when that is larger (the server then holds it)
+    const uint64_t budget = device_bytes ? (uint64_t)device_bytes : kswhip::mem_budget(true);
     auto *eng = new kswhip::Engine(m, mat, sc, cls, budget);
-    st = eng->start();
+    st = eng->start(device_bytes == 0);
     if (st != KSW2_HIP_OK) {
         delete eng;
         return set_err(st);

@@ -31,14 +31,22 @@ int g_device = -1;
 
 uint64_t round_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 
-uint64_t mem_budget()
+}  // namespace
+
+// 85 % of the free device memory; with_cache: at least the cached engine
+// block (the library's own, which an engine takes).  KSW2_HIP_MEM_BUDGET
+// overrides (tests).
+uint64_t mem_budget(bool with_cache)
 {
     const char *env = getenv("KSW2_HIP_MEM_BUDGET");
     if (env) return strtoull(env, nullptr, 10);
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 8ull << 30;
-    return (uint64_t)(fr * 0.85);
+    const uint64_t own = (uint64_t)(fr * 0.85);
+    return with_cache ? std::max<uint64_t>(own, Engine::cached_block_bytes()) : own;
 }
+
+namespace {
 
 }  // namespace
 
@@ -803,10 +811,10 @@ int run_batch_host(int n, const ksw2_pair_t *pairs, int8_t m, const int8_t *mat,
         }
         todo.push_back(k);
     }
-    const uint64_t budget = mem_budget();
+    // a batch that fits the free memory runs as one plan (its own block)
     uint64_t total = Engine::kPlanSlack;
     for (int k : todo) total += pair_footprint(pairs[k], takes_general(pairs[k], m, cls));
-    if (total <= budget || todo.size() <= 1) {
+    if (total <= mem_budget(false) || todo.size() <= 1) {
         // one plan holds the whole batch
         if (todo.empty()) return status;
         Plan *plan = nullptr;
@@ -822,8 +830,8 @@ int run_batch_host(int n, const ksw2_pair_t *pairs, int8_t m, const int8_t *mat,
     }
     // larger than the budget: stream it through the engine, longest first
     // (CIGARs grown with realloc, as with km == NULL)
-    Engine eng(m, mat, sc, cls, budget);
-    int st = eng.start();
+    Engine eng(m, mat, sc, cls, mem_budget(true));
+    int st = eng.start(true);  // the cached block when it is free and large enough
     if (st != KSW2_HIP_OK) {
         for (int k : todo) ez[k].zdropped = 1;
         return st;

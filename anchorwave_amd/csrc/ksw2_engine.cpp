@@ -53,13 +53,41 @@ double now_s()
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// The device block of a batch engine, kept for the process's next batch
+// call: allocating and releasing hundreds of GB costs seconds per call,
+// and AnchorWave calls the batch aligner many times.
+struct CachedBlock {
+    std::mutex mu;
+    uint8_t *base = nullptr;
+    uint64_t bytes = 0;
+    bool busy = false;
+};
+CachedBlock g_cached;
+
 // First-fit allocator over one device allocation; blocks are 256-aligned and
 // coalesced on release.
 class FirstFitPool : public DevicePool {
 public:
-    int init(uint64_t n)
+    int init(uint64_t n, bool cached)
     {
         bytes = round_up(n, 256);
+        if (cached) {
+            std::lock_guard<std::mutex> lk(g_cached.mu);
+            if (!g_cached.busy && g_cached.base && g_cached.bytes < bytes) {
+                (void)hipFree(g_cached.base);  // too small for this budget
+                g_cached.base = nullptr, g_cached.bytes = 0;
+            }
+            if (!g_cached.busy && !g_cached.base) {
+                if (hipMalloc(&g_cached.base, bytes) != hipSuccess) g_cached.base = nullptr;
+                else g_cached.bytes = bytes;
+            }
+            if (!g_cached.busy && g_cached.base) {
+                g_cached.busy = true, cached_ = true;
+                base = g_cached.base, bytes = g_cached.bytes;
+                free_[0] = bytes;
+                return KSW2_HIP_OK;
+            }
+        }
         if (hipMalloc(&base, bytes) != hipSuccess) {
             base = nullptr;
             return KSW2_HIP_ENOMEM;
@@ -69,7 +97,12 @@ public:
     }
     ~FirstFitPool() override
     {
-        if (base) (void)hipFree(base);
+        if (cached_) {
+            std::lock_guard<std::mutex> lk(g_cached.mu);
+            g_cached.busy = false;
+        } else if (base) {
+            (void)hipFree(base);
+        }
     }
     bool alloc(uint64_t n, uint64_t *off) override
     {
@@ -110,6 +143,7 @@ public:
     }
 
 private:
+    bool cached_ = false;
     std::mutex mu_;
     std::map<uint64_t, uint64_t> free_;
     std::unordered_map<uint64_t, uint64_t> used_;
@@ -161,16 +195,34 @@ Engine::Engine(int8_t m, const int8_t *mat, const KswScore &sc, const ScoreClass
     make_gen_score(m, mat, sc, cls, &gsc_);
     const char *ev = getenv("KSW2_HIP_ENGINE_STATS");
     stats_ = ev ? atoi(ev) : 0;
+    const char *bk = getenv("KSW2_HIP_BATCH_KEYED");
+    batch_keyed_ = bk && atoi(bk) > 0;
 }
 
-int Engine::start()
+uint64_t Engine::cached_block_bytes()
+{
+    std::lock_guard<std::mutex> lk(g_cached.mu);
+    return g_cached.base && !g_cached.busy ? g_cached.bytes : 0;
+}
+
+void Engine::release_cached_block()
+{
+    std::lock_guard<std::mutex> lk(g_cached.mu);
+    if (g_cached.base && !g_cached.busy) {
+        (void)hipFree(g_cached.base);
+        g_cached.base = nullptr, g_cached.bytes = 0;
+    }
+}
+
+int Engine::start(bool cached)
 {
     int st = device_init(-1);
     if (st != KSW2_HIP_OK) return st;
     if (hipGetDevice(&device_) != hipSuccess) return KSW2_HIP_EDEVICE;
     auto *pool = new FirstFitPool();
     pool_.reset(pool);
-    st = pool->init(budget_);
+    st = pool->init(budget_, cached);
+    budget_ = pool->bytes;  // a cached block may be larger than asked
     if (st != KSW2_HIP_OK) return st;
     for (auto &s : streams_)
         if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return KSW2_HIP_EDEVICE;
@@ -559,7 +611,8 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
         const double T = std::max(1.0, (queued_steps_ + inflight_steps_) / resident_waves_);
         const int cap = (int)std::min<int64_t>(free_waves, kMaxWide);
         const Task *head = q.front();
-        const int key = batch_ ? (head->general ? 0 : -1) : team_for(head->passes, head->steps, head->general, T, cap);
+        const bool keyed = !batch_ || batch_keyed_;
+        const int key = keyed ? team_for(head->passes, head->steps, head->general, T, cap) : head->general ? 0 : -1;
         const double span = key > 0 ? head->steps / key : 0;  // the head's steps per wave
         std::vector<Task *> pick;
         uint64_t bytes = kPlanSlack, seq = 0, cigw = 0;
@@ -570,9 +623,9 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
             Task *t = *it;
             // serving: one team size (one launch) and similar durations per
             // flight; batch: general and stripe pairs in flights of their own
-            const bool same = batch_ ? (key == 0) == t->general
+            const bool same = !keyed ? (key == 0) == t->general
                                      : team_for(t->passes, t->steps, t->general, T, cap) == key &&
-                                           (key == 0 || (t->steps / key <= 2 * span && 2 * t->steps / key >= span));
+                                           (batch_ || key == 0 || (t->steps / key <= 2 * span && 2 * t->steps / key >= span));
             if (!same) {
                 if (++skipped > 64) break;  // backfill window
                 ++it;

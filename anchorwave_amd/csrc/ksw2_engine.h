@@ -36,7 +36,13 @@ public:
 
     Engine(int8_t m, const int8_t *mat, const KswScore &sc, const ScoreClass &cls, uint64_t budget);
     ~Engine();
-    int start();
+    // cached: take the process's cached device block when it is free and
+    // large enough (else allocate one and leave it cached), and give it back
+    // to the cache at destruction instead of freeing it
+    int start(bool cached = false);
+    // Bytes of the cached device block (0: none); releases it when idle.
+    static uint64_t cached_block_bytes();
+    static void release_cached_block();
     // Queues one pair; ez receives the result (reset now, filled when done).
     // copy: keep private copies of the sequences (the caller may free them).
     // defer_ms non-null: when the pair's footprint does not fit beside the
@@ -61,6 +67,7 @@ public:
     int drain(const Progress *prog);
 
 private:
+    bool batch_keyed_ = false;  // batch flights of one team size, as in serving (tuning)
     struct Task;
     struct Flight;
     void loop();

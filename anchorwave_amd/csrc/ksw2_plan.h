@@ -66,6 +66,10 @@ bool takes_band(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls);
 // matrix (qlen + tlen - 1 when it never does).
 int band_r_end(int qlen, int tlen, int w);
 
+// Device bytes an engine may hold: 85 % of the free device memory, with
+// the cached engine block counted as free when with_cache.
+uint64_t mem_budget(bool with_cache);
+
 // Device memory pool of a serving engine: one hipMalloc carved first-fit.
 struct DevicePool {
     uint8_t *base = nullptr;

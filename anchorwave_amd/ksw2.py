@@ -40,6 +40,7 @@ EXPORTS = (
     "ksw_extd2_hip",
     "ksw_extd2_singletrack_hip",
     "ksw_extd2_hip_batch",
+    "ksw2_hip_release_cached_memory",
     "ksw2_hip_last_error",
     "ksw2_hip_error_string",
     "ksw2_hip_init",
@@ -130,6 +131,14 @@ _libc = C.CDLL(None)
 _libc.free.argtypes = [C.c_void_p]
 
 
+def release_cached_memory() -> None:
+    """``ksw2_hip_release_cached_memory``: frees the device block that batch
+    calls larger than the memory budget keep for the next such call."""
+    st = lib().ksw2_hip_release_cached_memory()
+    if st != 0:
+        raise KswError(f"ksw2_hip_release_cached_memory: {STATUS.get(st, st)}")
+
+
 def lib():
     """Loads the HIP library; raises if it was not built."""
     global _lib
@@ -145,6 +154,8 @@ def lib():
         L.ksw_extd2_hip_batch.argtypes = [C.c_int, C.POINTER(KswPair), C.c_int8, C.c_void_p, C.c_int8,
                                           C.c_int8, C.c_int8, C.c_int8, C.POINTER(KswExtz)]
         L.ksw_extd2_hip_batch.restype = C.c_int
+        L.ksw2_hip_release_cached_memory.argtypes = []
+        L.ksw2_hip_release_cached_memory.restype = C.c_int
         L.ksw2_hip_last_error.restype = C.c_int
         L.ksw2_hip_error_string.restype = C.c_char_p
         L.ksw2_hip_error_string.argtypes = [C.c_int]

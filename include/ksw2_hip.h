@@ -149,6 +149,12 @@ typedef struct {
 int ksw_extd2_hip_batch(int n, const ksw2_pair_t *pairs,
                         int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q2, int8_t e2,
                         ksw_extz_t *ez);
+/* A batch larger than the device memory budget streams through a device
+ * block of the whole budget, which the library keeps for the next such call
+ * (allocating and releasing hundreds of GB costs seconds per call).  This
+ * releases it; the next such call allocates again.  No reference
+ * counterpart.  Returns KSW2_HIP_OK (also when nothing is held). */
+int ksw2_hip_release_cached_memory(void);
 
 /* ---- Device-resident batch ("plan"; no reference counterpart).  The
  * pairs' codes are uploaded to HBM once; each run aligns all of them (DP
@@ -176,7 +182,10 @@ void ksw2_hip_plan_destroy(ksw2_hip_plan *plan);
  * independently.  Each result is exactly what ksw_extd2_hip(NULL, ...)
  * returns for the same pair, alone or in any mix. */
 typedef struct ksw2_hip_server ksw2_hip_server;
-/* device_bytes == 0: 85% of the device's free memory. */
+/* device_bytes == 0: 85% of the device's free memory, or the library's
+ * cached engine block when it is free and larger (ksw_extd2_hip_batch
+ * below); the server then takes that block and hands it back to the cache
+ * when destroyed. */
 int ksw2_hip_server_create(ksw2_hip_server **server, int8_t m, const int8_t *mat, int8_t q, int8_t e,
                            int8_t q2, int8_t e2, size_t device_bytes);
 /* Queues one pair (its sequences are copied).  *ez must stay valid until the

@@ -101,6 +101,34 @@ def test_batch_larger_than_budget_streams_through_the_engine(monkeypatch):
         assert same(g, exp), f"pair {k}: {diff(g, exp)}"
 
 
+def test_engine_block_kept_between_batch_calls(monkeypatch):
+    """Batches larger than the budget, one after another: the second takes
+    the device block the first left cached, a server then takes it, and
+    after release the next batch allocates again; every result exact."""
+    monkeypatch.setenv("KSW2_HIP_MEM_BUDGET", str(40 << 20))
+    pairs = _mix(97, 60) + [(q, t, 0, -1, -1, 0) for q, t in W.fixed(seed=5, n=10)]
+    exp = [_oracle(p) for p in pairs]
+
+    def check(got):
+        for k, (g, e) in enumerate(zip(got, exp)):
+            assert same(g, e), f"pair {k}: {diff(g, e)}"
+
+    check(ksw2.extd2_batch(pairs))
+    check(ksw2.extd2_batch(pairs))
+    srv = ksw2.Server()
+    tks = {srv.submit(q, t, w=w, zdrop=z, end_bonus=eb, flag=f): k for k, (q, t, f, w, z, eb) in enumerate(pairs[:20])}
+    got = {}
+    while len(got) < len(tks):
+        for tk, res, _ in srv.poll(wait_ms=2000):
+            got[tks[tk]] = res
+    srv.close()
+    for k, g in got.items():
+        assert same(g, exp[k]), f"server pair {k}: {diff(g, exp[k])}"
+    ksw2.release_cached_memory()
+    check(ksw2.extd2_batch(pairs))
+    ksw2.release_cached_memory()
+
+
 def test_try_submit_defers_when_memory_is_committed():
     """A server with room for a few 2 kbp pairs: try_submit defers the rest
     (retry 250 ms to 5 s) instead of queueing them; held + queued footprint
