@@ -140,15 +140,14 @@ int fast_path_reject(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls)
 // Team size for one pair.  With T the batch's step count per resident wave
 // (the time the batch needs when every wave is busy), a pair whose passes
 // take P * (qlen + 127) steps on one wave gets the smallest power-of-two team
-// that brings it under 2T, capped by its pass count and kMaxTeam: small
+// that brings it under 2T, capped by its pass count and kMaxWide: small
 // pairs run on one wave with no synchronisation, long pairs do not outlast
-// the batch.  KSW2_HIP_TEAM forces a size (tuning and tests).
+// the batch.  Teams above kMaxTeam waves are wide (workgroups of kWideGroup
+// waves on different CUs, rounded up to whole workgroups).  KSW2_HIP_TEAM
+// forces a size; KSW2_HIP_WIDE=k runs every pair wide on k workgroups
+// (tuning and tests).
 static double pair_steps(const KswJob &j) { return (double)pass_count(j.tlen) * (double)pass_steps(j.qlen); }
 
-//
-// Teams above kMaxTeam waves are wide (workgroups of kWideGroup waves on
-// different CUs, up to kMaxWide waves, rounded up to whole workgroups).
-// KSW2_HIP_WIDE=k (tests) runs every pair wide on k workgroups.
 static int choose_team(const KswJob &j, double T, int forced)
 {
     const int P = pass_count(j.tlen);
@@ -163,7 +162,6 @@ static int choose_team(const KswJob &j, double T, int forced)
     return team > kMaxTeam ? (team + kWideGroup - 1) / kWideGroup * kWideGroup : team;
 }
 
-static int lane_of_team(int team) { return team >= 8 ? 0 : team >= 4 ? 1 : team >= 2 ? 2 : 3; }
 
 bool takes_general(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls)
 {
@@ -348,8 +346,12 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         int wide_forced = 0;
         if (const char *ev = getenv("KSW2_HIP_WIDE")) wide_forced = std::max(0, std::min(kMaxWide / kWideGroup, atoi(ev)));
         std::vector<int> team(p->n_fast);
-        for (int k = 0; k < p->n_fast; ++k)
-            team[k] = wide_forced ? kWideGroup * wide_forced : choose_team(p->jobs[k], T, forced);
+        for (int k = 0; k < p->n_fast; ++k) {
+            // KSW2_HIP_TEAM=-1 (tests): team sizes 1..16 in turn, one launch
+            const int f = forced < 0 ? 1 + k % kMaxTeam : forced;
+            team[k] = wide_forced ? kWideGroup * wide_forced : choose_team(p->jobs[k], T, f);
+            p->jobs[k].team = team[k];
+        }
         auto wide = [&](int k) { return wide_forced > 0 || team[k] > kMaxTeam; };
         auto longest_first = [&](std::vector<int> &m) {
             std::stable_sort(m.begin(), m.end(), [&](int x, int y) { return pair_steps(p->jobs[x]) > pair_steps(p->jobs[y]); });
@@ -370,14 +372,15 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
                 p->n_wide += (int)members.size();
                 order.insert(order.end(), members.begin(), members.end());
             }
-            // one-workgroup teams: one launch per team size
-            for (int t = kMaxTeam; t >= 1; --t) {
-                members.clear();
-                for (int k = 0; k < p->n_fast; ++k)
-                    if (grp_of[k] == grp && !wide(k) && team[k] == t) members.push_back(k);
-                if (members.empty()) continue;
+            // one-workgroup teams: one launch, workgroups of the largest
+            // team (a smaller team's extra waves end at once)
+            members.clear();
+            int tmax = 0;
+            for (int k = 0; k < p->n_fast; ++k)
+                if (grp_of[k] == grp && !wide(k)) members.push_back(k), tmax = std::max(tmax, team[k]);
+            if (!members.empty()) {
                 longest_first(members);
-                p->launches.push_back({grp, t, (int)order.size(), (int)members.size(), lane_of_team(t), 0, 0, 0});
+                p->launches.push_back({grp, tmax, (int)order.size(), (int)members.size(), grp % Plan::kLanes, 0, 0, 0});
                 order.insert(order.end(), members.begin(), members.end());
             }
         }

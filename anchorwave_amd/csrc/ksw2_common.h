@@ -42,6 +42,8 @@ struct KswJob {
     int32_t flag, end_bonus;
     int32_t stripe_off;        // first slot of this pair in the per-stripe arrays
     int32_t R;                 // rows per lane of the stripe kernel that ran it
+    int32_t team;              // waves of this pair's team (a launch's workgroups may be larger)
+    int32_t pad;
 };
 
 // Per-stripe partial of the mqe bookkeeping (column qlen-1): the sum of

@@ -195,8 +195,7 @@ Engine::Engine(int8_t m, const int8_t *mat, const KswScore &sc, const ScoreClass
     make_gen_score(m, mat, sc, cls, &gsc_);
     const char *ev = getenv("KSW2_HIP_ENGINE_STATS");
     stats_ = ev ? atoi(ev) : 0;
-    const char *bk = getenv("KSW2_HIP_BATCH_KEYED");
-    batch_keyed_ = bk && atoi(bk) > 0;
+    if (const char *mt = getenv("KSW2_HIP_MAX_TEAM")) max_team_ = std::max(1, std::min(kMaxWide, atoi(mt)));
 }
 
 uint64_t Engine::cached_block_bytes()
@@ -609,10 +608,9 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
         const uint64_t avail = std::min(free_block, share);
         // per-wave step budget: work queued and in flight over the device's waves
         const double T = std::max(1.0, (queued_steps_ + inflight_steps_) / resident_waves_);
-        const int cap = (int)std::min<int64_t>(free_waves, kMaxWide);
+        const int cap = (int)std::min<int64_t>(free_waves, max_team_);
         const Task *head = q.front();
-        const bool keyed = !batch_ || batch_keyed_;
-        const int key = keyed ? team_for(head->passes, head->steps, head->general, T, cap) : head->general ? 0 : -1;
+        const int key = batch_ ? (head->general ? 0 : -1) : team_for(head->passes, head->steps, head->general, T, cap);
         const double span = key > 0 ? head->steps / key : 0;  // the head's steps per wave
         std::vector<Task *> pick;
         uint64_t bytes = kPlanSlack, seq = 0, cigw = 0;
@@ -623,9 +621,9 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
             Task *t = *it;
             // serving: one team size (one launch) and similar durations per
             // flight; batch: general and stripe pairs in flights of their own
-            const bool same = !keyed ? (key == 0) == t->general
+            const bool same = batch_ ? (key == 0) == t->general
                                      : team_for(t->passes, t->steps, t->general, T, cap) == key &&
-                                           (batch_ || key == 0 || (t->steps / key <= 2 * span && 2 * t->steps / key >= span));
+                                           (key == 0 || (t->steps / key <= 2 * span && 2 * t->steps / key >= span));
             if (!same) {
                 if (++skipped > 64) break;  // backfill window
                 ++it;
@@ -646,7 +644,12 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
             cigw += ((uint64_t)t->pair.qlen + t->pair.tlen + 2 + 15) / 16 * 16;
             it = q.erase(it);
         }
-        if (pick.empty()) continue;  // wait for memory
+        if (pick.empty()) {  // wait for memory
+            if (stats_ > 1)
+                fprintf(stderr, "[ksw2 engine] t %.3f stream %d class %d: nothing fits (free block %.2f GB, head %.2f GB, skipped %d)\n",
+                        now_s(), s, cls, free_block / 1e9, q.front()->footprint / 1e9, skipped);
+            continue;
+        }
         // pinned upload buffer of this stream, grown to the flight's sequences
         if (staging_bytes_[s] < seq) {
             if (staging_[s]) (void)hipHostFree(staging_[s]);
@@ -727,6 +730,12 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
         for (Task *t : pick) t->t_start = t0;
         f->t_start = t0;
         f->bytes = bytes;
+        if (stats_ > 1) {
+            auto *pl = static_cast<FirstFitPool *>(pool_.get());
+            fprintf(stderr, "[ksw2 engine] t %.3f admit stream %d class %d tasks %zu bytes %.2f GB free %.2f GB held %.2f GB q %zu/%zu/%zu\n",
+                    t0, s, cls, pick.size(), bytes / 1e9, pl->largest_free() / 1e9, held_bytes_ / 1e9,
+                    queue_[0].size(), queue_[1].size(), queue_[2].size());
+        }
         held_bytes_ += bytes, queued_bytes_ = queued_bytes_ > bytes ? queued_bytes_ - bytes : 0;
         if (last_reap_t_ == 0) last_reap_t_ = t0;
         busy_waves_ += waves, inflight_steps_ += steps, queued_steps_ -= steps;

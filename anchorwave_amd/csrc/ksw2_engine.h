@@ -67,7 +67,7 @@ public:
     int drain(const Progress *prog);
 
 private:
-    bool batch_keyed_ = false;  // batch flights of one team size, as in serving (tuning)
+    int max_team_ = 64;         // largest team a flight is keyed on (kMaxWide; KSW2_HIP_MAX_TEAM, tuning)
     struct Task;
     struct Flight;
     void loop();

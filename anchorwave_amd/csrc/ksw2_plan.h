@@ -112,8 +112,10 @@ struct Plan {
     std::vector<GenJob> gjobs;       // host copy, general-kernel slots
     std::vector<int> order;          // plan slot -> caller index
     std::vector<uint64_t> cig_off;   // plan slot -> uint32 offset in the cig arena
-    // One launch per (job group, team size).  Groups: 0 left-aligned
-    // traceback, 1 right-aligned, 2 score-only.  `off`/`count` index d_order.
+    // Per job group (0 left-aligned traceback, 1 right-aligned, 2 score-only,
+    // 3 Singletrack tracks): one wide launch and one launch of one-workgroup
+    // teams (`team` = the largest; each pair's own size is in its KswJob).
+    // `off`/`count` index d_order.
     // wide: a wide launch (each pair on its own number of kWideGroup-wave
     // workgroups), whose control blocks start at block `woff` of d_wctl and
     // whose first-workgroup table starts at entry `wf` of h_wfirst/d_wfirst.

@@ -532,9 +532,9 @@ struct StripeTracks {
 };
 
 __device__ __forceinline__ void pair_epilogue(const StripeArgs &a, int job_id, const KswJob &jb,
-                                              uint64_t (*tile)[64], bool tracks, int *shared_n)
+                                              uint64_t (*tile)[64], bool tracks, int *shared_n, int nthr)
 {
-    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int tid = threadIdx.x;
     KswOut o = a.out[job_id];
     const int flag = jb.flag;
     const int qlen = jb.qlen, tlen = jb.tlen;
@@ -657,28 +657,31 @@ __device__ __forceinline__ void pair_epilogue(const StripeArgs &a, int job_id, c
     }
 }
 
-// One workgroup (a team of blockDim.x / 64 waves) per pair of this launch's
-// job list; wave w computes passes w, w+W, ...; then the workgroup runs the
-// pair's epilogue (mqe, start cell, backtrack).
+// One workgroup per pair of this launch's job list.  A pair's team is its
+// own W waves (jb.team, at most blockDim.x / 64): the waves beyond it end at
+// once (s_barrier then waits on the surviving waves only), so pairs of
+// different team sizes share one launch.  Wave w computes passes w, w+W,
+// ...; then the team runs the pair's epilogue (mqe, start cell, backtrack).
 template <bool RIGHT, int MODE>
 __global__ void __launch_bounds__(kWave * kMaxTeam) ksw_stripe_kernel(StripeArgs a)
 {
     __shared__ int32_t prog[kMaxTeam];
     __shared__ uint64_t tile[8][64];
     __shared__ int shared_n;
-    const int W = __builtin_amdgcn_readfirstlane(blockDim.x / kWave);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int job = __builtin_amdgcn_readfirstlane(a.order[blockIdx.x]);
+    const KswJob jb = a.jobs[job];
+    const int W = __builtin_amdgcn_readfirstlane(max(1, min(jb.team, (int)blockDim.x / kWave)));
+    if (w >= W) return;
     prog[w] = 0;  // every lane of wave w writes the same zero
     __syncthreads();
     const Consts kc = make_consts<RIGHT>(a.sc);
-    const KswJob jb = a.jobs[job];
     const int qlen = __builtin_amdgcn_readfirstlane(jb.qlen), tlen = __builtin_amdgcn_readfirstlane(jb.tlen);
     const int npass = ((tlen + SR - 1) / SR + 1) / 2;
     int late = 0;  // (no bounded waits within a workgroup)
     for (int p = w; p < npass; p += W) pass_body<RIGHT, MODE, false>(a, kc, job, p, jb, qlen, tlen, prog, W, late);
     __syncthreads();  // every pass written: stripe partials, score / mte, direction bytes
-    pair_epilogue(a, job, jb, tile, MODE == kTracks, &shared_n);
+    pair_epilogue(a, job, jb, tile, MODE == kTracks, &shared_n, W * kWave);
 }
 
 // Wide teams: each pair has its own run of consecutive blocks (a.wfirst),
@@ -734,7 +737,7 @@ __global__ void __launch_bounds__(256) ksw_stripe_wide_epilogue_kernel(StripeArg
     const KswJob jb = a.jobs[job];
     const int late = __builtin_amdgcn_readfirstlane(a.wctl[(int64_t)slot * kWideCtl + kWideCtlLate]);
     if (!late) {
-        pair_epilogue(a, job, jb, tile, MODE == kTracks, &shared_n);
+        pair_epilogue(a, job, jb, tile, MODE == kTracks, &shared_n, (int)blockDim.x);
         return;
     }
     if (threadIdx.x == 0) {

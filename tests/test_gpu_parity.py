@@ -122,11 +122,12 @@ def test_repeat_runs_identical():
     plan.close()
 
 
-@pytest.mark.parametrize("team", [1, 2, 3, 16])
+@pytest.mark.parametrize("team", [1, 2, 3, 16, -1])
 def test_forced_team_sizes(team, monkeypatch):
     """Every team size gives the same exact results: the stripes of a pair
     handed between the waves of one workgroup (team > 1), or run in turn on
-    one wave (team 1)."""
+    one wave (team 1); -1: sizes 1..16 in turn within one launch (workgroups
+    of 16 waves whose extra waves end at once)."""
     monkeypatch.setenv("KSW2_HIP_TEAM", str(team))
     pairs = random_pairs(300 + team, 12, 600, 5000) + \
         [p for p in W.uneven(seed=13, n=80) if len(p[0]) * len(p[1]) < 6e7][:12]

@@ -83,7 +83,7 @@ def test_singletrack_forced_general(monkeypatch):
     check_batch(random_pairs(540, 40, 1, 700))
 
 
-@pytest.mark.parametrize("team", [1, 3, 16])
+@pytest.mark.parametrize("team", [1, 3, 16, -1])
 def test_singletrack_team_sizes(team, monkeypatch):
     monkeypatch.setenv("KSW2_HIP_TEAM", str(team))
     check_batch(random_pairs(550 + team, 10, 600, 4000))
