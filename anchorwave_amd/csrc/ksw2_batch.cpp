@@ -352,7 +352,23 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
             team[k] = wide_forced ? kWideGroup * wide_forced : choose_team(p->jobs[k], T, f);
             p->jobs[k].team = team[k];
         }
-        auto wide = [&](int k) { return wide_forced > 0 || team[k] > kMaxTeam; };
+        // a plan of its own (not an engine flight) that leaves most of the
+        // device idle: teams of 5-16 waves go wide too, so that each wave
+        // has a SIMD to itself (one workgroup would put two on a SIMD); a
+        // lone 5 kbp pair runs 22 % faster
+        std::vector<char> spread(p->n_fast, 0);
+        if (!env && !wide_forced && forced == 0) {
+            int64_t waves = 0;
+            for (int k = 0; k < p->n_fast; ++k) waves += team[k];
+            if (4 * waves <= stripe_resident_waves(kGroupLeft))
+                for (int k = 0; k < p->n_fast; ++k)
+                    if (team[k] > kWideGroup && team[k] <= kMaxTeam) {
+                        team[k] = (team[k] + kWideGroup - 1) / kWideGroup * kWideGroup;
+                        p->jobs[k].team = team[k];
+                        spread[k] = 1;
+                    }
+        }
+        auto wide = [&](int k) { return wide_forced > 0 || team[k] > kMaxTeam || spread[k]; };
         auto longest_first = [&](std::vector<int> &m) {
             std::stable_sort(m.begin(), m.end(), [&](int x, int y) { return pair_steps(p->jobs[x]) > pair_steps(p->jobs[y]); });
         };
