@@ -335,7 +335,8 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         j.cig_off = co, co += round_up((uint64_t)pp.qlen + pp.tlen + 2, 16);
         p->cig_off[slot] = j.cig_off;
     }
-    // stripe-kernel launches: per group, per team size, longest pairs first
+    // stripe-kernel launches: per group, a wide launch and a launch of
+    // one-workgroup teams, longest pairs first
     std::vector<int32_t> order;
     if (p->n_fast > 0) {
         double total_steps = 0;
@@ -833,7 +834,9 @@ int run_batch_host(int n, const ksw2_pair_t *pairs, int8_t m, const int8_t *mat,
     // a batch that fits the free memory runs as one plan (its own block)
     uint64_t total = Engine::kPlanSlack;
     for (int k : todo) total += pair_footprint(pairs[k], takes_general(pairs[k], m, cls));
-    if (total <= mem_budget(false) || todo.size() <= 1) {
+    // (a batch of at most 1 GiB, a single call's usual size, skips the
+    // device memory query)
+    if (todo.size() <= 1 || (total <= (1ull << 30) && !getenv("KSW2_HIP_MEM_BUDGET")) || total <= mem_budget(false)) {
         // one plan holds the whole batch
         if (todo.empty()) return status;
         Plan *plan = nullptr;

@@ -80,7 +80,7 @@ struct DevicePool {
 };
 
 // Where a plan gets its memory and stream: defaults are its own hipMalloc'd
-// block and its own streams (one per team-size lane).
+// block and its own streams (one per kernel-group lane).
 struct PlanEnv {
     DevicePool *pool = nullptr;
     hipStream_t stream = nullptr;
@@ -166,7 +166,7 @@ struct Plan {
     std::vector<int32_t> h_wfirst;   // per wide launch: each pair's first workgroup, then the end
     int32_t *d_wfirst = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    // launches of different team sizes overlap on their own streams
+    // launches of different kernel groups overlap on their own streams
     static constexpr int kLanes = 4;
     hipStream_t lane_stream[kLanes] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t lane_done[kLanes] = {nullptr, nullptr, nullptr, nullptr};
