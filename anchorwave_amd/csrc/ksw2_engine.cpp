@@ -196,6 +196,7 @@ Engine::Engine(int8_t m, const int8_t *mat, const KswScore &sc, const ScoreClass
     const char *ev = getenv("KSW2_HIP_ENGINE_STATS");
     stats_ = ev ? atoi(ev) : 0;
     if (const char *mt = getenv("KSW2_HIP_MAX_TEAM")) max_team_ = std::max(1, std::min(kMaxWide, atoi(mt)));
+    if (const char *bf = getenv("KSW2_HIP_BACKFILL")) backfill_ = std::max(0, atoi(bf));
 }
 
 uint64_t Engine::cached_block_bytes()
@@ -625,7 +626,7 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
                                      : team_for(t->passes, t->steps, t->general, T, cap) == key &&
                                            (key == 0 || (t->steps / key <= 2 * span && 2 * t->steps / key >= span));
             if (!same) {
-                if (++skipped > 64) break;  // backfill window
+                if (++skipped > backfill_) break;  // backfill window
                 ++it;
                 continue;
             }
@@ -634,7 +635,7 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
             // a flight holds at most its class's share of the pool, except that
             // its first task may use any free block (a task larger than the share)
             if (bytes + t->footprint > (pick.empty() ? free_block : avail)) {
-                if (++skipped > 64) break;
+                if (++skipped > backfill_) break;
                 ++it;
                 continue;
             }

@@ -68,6 +68,7 @@ public:
 
 private:
     int max_team_ = 64;         // largest team a flight is keyed on (kMaxWide; KSW2_HIP_MAX_TEAM, tuning)
+    int backfill_ = 1024;       // tasks a flight may pass over in the queue (KSW2_HIP_BACKFILL, tuning)
     struct Task;
     struct Flight;
     void loop();
