@@ -197,6 +197,10 @@ Engine::Engine(int8_t m, const int8_t *mat, const KswScore &sc, const ScoreClass
     stats_ = ev ? atoi(ev) : 0;
     if (const char *mt = getenv("KSW2_HIP_MAX_TEAM")) max_team_ = std::max(1, std::min(kMaxWide, atoi(mt)));
     if (const char *bf = getenv("KSW2_HIP_BACKFILL")) backfill_ = std::max(0, atoi(bf));
+    for (int s = 0; s < kStreams; ++s) stream_class_[s] = kStreamClass[s];
+    if (const char *sc = getenv("KSW2_HIP_STREAM_CLASSES"))  // e.g. "2210" (tuning)
+        for (int s = 0; s < kStreams && sc[s]; ++s)
+            if (sc[s] >= '0' && sc[s] < '0' + kClasses) stream_class_[s] = sc[s] - '0';
 }
 
 uint64_t Engine::cached_block_bytes()
@@ -594,7 +598,7 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
         // a stream serves its own class first; the long-task streams also take
         // shorter tasks when no long ones wait, the others never take long ones
         static const int pref[kClasses][kClasses] = {{0, 1, -1}, {1, 0, -1}, {2, 1, 0}};
-        const int own = batch_ ? kBatchStreamClass[s] : kStreamClass[s];
+        const int own = batch_ ? kBatchStreamClass[s] : stream_class_[s];
         int cls = -1;
         for (int k = 0; k < kClasses && pref[own][k] >= 0; ++k)
             if (!queue_[pref[own][k]].empty()) {
