@@ -46,6 +46,12 @@ def test_error_strings_and_no_device_path():
         assert lib.ksw2_hip_init(0) == -5  # no GPU: ENODEV, no crash
 
 
+def test_release_cached_memory_with_nothing_held():
+    """With no engine block cached the release is a no-op that needs no GPU."""
+    assert ksw2.lib().ksw2_hip_release_cached_memory() == 0
+    ksw2.release_cached_memory()
+
+
 def test_early_returns_need_no_device():
     """Calls the reference answers before aligning return the reset ez
     without touching a GPU (minimap2/ksw2_extd2_sse.c:81,105)."""
