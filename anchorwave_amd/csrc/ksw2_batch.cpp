@@ -94,6 +94,15 @@ bool wait_slots_acquire(hipStream_t st, int q[2])
     return true;
 }
 
+// Quarters whose last sequence has finished (caller holds g_wait_mu).
+int wait_slots_free()
+{
+    int n = 0;
+    for (int k = 0; k < kQuarters; ++k)
+        if (!g_quarter[k] || hipEventQuery(g_quarter[k]) == hipSuccess) ++n;
+    return n;
+}
+
 bool wait_slots_release(hipStream_t st, const int q[2])
 {
     for (int i = 0; i < 2; ++i)
@@ -354,22 +363,24 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
             p->jobs[k].team = team[k];
         }
         // a plan of its own (not an engine flight) that leaves most of the
-        // device idle: teams of 5-16 waves go wide too, so that each wave
-        // has a SIMD to itself (one workgroup would put two on a SIMD); a
-        // lone 5 kbp pair runs 22 % faster
+        // device idle: teams of 5-16 waves go wide too ("spread"), so that
+        // each wave has a SIMD to itself (one workgroup would put two on a
+        // SIMD); a lone 5 kbp pair runs 22 % faster.  A spread pair keeps its
+        // team size in its KswJob: when no waiting-launch slots are free at
+        // launch (other threads' calls hold them), it runs as a one-workgroup
+        // team instead of waiting.
         std::vector<char> spread(p->n_fast, 0);
         if (!env && !wide_forced && forced == 0) {
             int64_t waves = 0;
             for (int k = 0; k < p->n_fast; ++k) waves += team[k];
             if (4 * waves <= stripe_resident_waves(kGroupLeft))
                 for (int k = 0; k < p->n_fast; ++k)
-                    if (team[k] > kWideGroup && team[k] <= kMaxTeam) {
-                        team[k] = (team[k] + kWideGroup - 1) / kWideGroup * kWideGroup;
-                        p->jobs[k].team = team[k];
-                        spread[k] = 1;
-                    }
+                    if (team[k] > kWideGroup && team[k] <= kMaxTeam) spread[k] = 1;
         }
         auto wide = [&](int k) { return wide_forced > 0 || team[k] > kMaxTeam || spread[k]; };
+        p->optional_wide = true;
+        for (int k = 0; k < p->n_fast; ++k)
+            if (wide(k) && !spread[k]) p->optional_wide = false;
         auto longest_first = [&](std::vector<int> &m) {
             std::stable_sort(m.begin(), m.end(), [&](int x, int y) { return pair_steps(p->jobs[x]) > pair_steps(p->jobs[y]); });
         };
@@ -382,10 +393,15 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
             if (!members.empty()) {
                 longest_first(members);
                 const int wf = (int)p->h_wfirst.size();
-                int blocks = 0;
-                for (int k : members) p->h_wfirst.push_back(blocks), blocks += team[k] / kWideGroup;
+                int blocks = 0, tmax = 0;
+                for (int k : members) {
+                    p->h_wfirst.push_back(blocks);
+                    blocks += (team[k] + kWideGroup - 1) / kWideGroup;
+                    tmax = std::max(tmax, team[k]);
+                }
                 p->h_wfirst.push_back(blocks);
-                p->launches.push_back({grp, 0, (int)order.size(), (int)members.size(), 0, 1, p->n_wide, wf});
+                // team: the largest, for a one-workgroup launch of spread pairs
+                p->launches.push_back({grp, tmax, (int)order.size(), (int)members.size(), 0, 1, p->n_wide, wf});
                 p->n_wide += (int)members.size();
                 order.insert(order.end(), members.begin(), members.end());
             }
@@ -656,23 +672,30 @@ int plan_launch(Plan *p, const Progress *prog)
         hipStream_t st = lane(0);
         if (!st) return KSW2_HIP_EDEVICE;
         KSW_HIP_TRY(clear_wide_err(p->d_werr, st));
+        bool wide_now = true;
         {
             std::lock_guard<std::mutex> lk(g_wait_mu);
-            int q[2];
-            if (!wait_slots_acquire(st, q)) return KSW2_HIP_EDEVICE;
-            for (const Plan::Launch &l : p->launches) {
-                if (!l.wide) continue;
-                sa.order = p->d_order + l.off, sa.njobs = l.count, sa.wctl = p->d_wctl + (int64_t)l.woff * kWideCtl;
-                sa.wfirst = p->d_wfirst + l.wf;
-                const int half = std::max(1, stripe_wide_capacity(l.grp) / 2);
-                KSW_HIP_TRY(launch_stripe_wide(sa, l.grp, p->h_wfirst.data() + l.wf, half, st));
+            // spread pairs only: run wide only if two quarters are free now
+            const bool no_spread = getenv("KSW2_HIP_SPREAD_FALLBACK") != nullptr;  // tests
+            wide_now = !p->optional_wide || (!no_spread && wait_slots_free() >= 2);
+            if (wide_now) {
+                int q[2];
+                if (!wait_slots_acquire(st, q)) return KSW2_HIP_EDEVICE;
+                for (const Plan::Launch &l : p->launches) {
+                    if (!l.wide) continue;
+                    sa.order = p->d_order + l.off, sa.njobs = l.count;
+                    sa.wctl = p->d_wctl + (int64_t)l.woff * kWideCtl, sa.wfirst = p->d_wfirst + l.wf;
+                    const int half = std::max(1, stripe_wide_capacity(l.grp) / 2);
+                    KSW_HIP_TRY(launch_stripe_wide(sa, l.grp, p->h_wfirst.data() + l.wf, half, st));
+                }
+                if (!wait_slots_release(st, q)) return KSW2_HIP_EDEVICE;
             }
-            if (!wait_slots_release(st, q)) return KSW2_HIP_EDEVICE;
         }
         for (const Plan::Launch &l : p->launches) {
             if (!l.wide) continue;
             sa.order = p->d_order + l.off, sa.njobs = l.count, sa.wctl = p->d_wctl + (int64_t)l.woff * kWideCtl;
-            KSW_HIP_TRY(launch_stripe_wide_epilogue(sa, l.grp, st));
+            if (wide_now) KSW_HIP_TRY(launch_stripe_wide_epilogue(sa, l.grp, st));
+            else KSW_HIP_TRY(launch_stripe(sa, l.grp, l.team, st));  // one-workgroup teams
         }
     }
     for (const Plan::Launch &l : p->launches) {

@@ -107,3 +107,31 @@ def test_engine_flights_with_wide_teams(monkeypatch):
     for k, ((q, t), g) in enumerate(zip(pairs, got)):
         exp = O.run_oracle(q, t)
         assert same(g, exp), f"pair {k}: {diff(g, exp)}"
+
+
+def test_spread_pairs_exact_wide_or_one_workgroup(monkeypatch):
+    """A lone call with 5-16 passes runs its team spread over two or more
+    workgroups when waiting-launch slots are free, else as one workgroup:
+    both give the oracle's results."""
+    pairs = W.single(seed=1) + random_pairs(616, 3, 4200, 9000)
+    exp = [O.run_oracle(q, t) for q, t in pairs]
+    for k, (q, t) in enumerate(pairs):
+        g = ksw2.extd2(q, t)
+        assert same(g, exp[k]), f"spread pair {k}: {diff(g, exp[k])}"
+    monkeypatch.setenv("KSW2_HIP_SPREAD_FALLBACK", "1")
+    got = ksw2.extd2_batch(pairs)  # a small batch: spread pairs, one-workgroup launch
+    for k, (g, e) in enumerate(zip(got, exp)):
+        assert same(g, e), f"fallback pair {k}: {diff(g, e)}"
+
+
+def test_concurrent_single_calls_from_threads():
+    """Calls from several host threads at once (AnchorWave's per-interval
+    pattern): some take the waiting-launch slots and run spread, the others
+    run as one workgroup; every result exact."""
+    from concurrent.futures import ThreadPoolExecutor
+    pairs = random_pairs(617, 24, 4200, 7000)
+    exp = [O.run_oracle(q, t) for q, t in pairs]
+    with ThreadPoolExecutor(8) as ex:
+        got = list(ex.map(lambda p: ksw2.extd2(p[0], p[1]), pairs))
+    for k, (g, e) in enumerate(zip(got, exp)):
+        assert same(g, e), f"pair {k}: {diff(g, e)}"
