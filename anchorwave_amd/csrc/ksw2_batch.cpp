@@ -362,15 +362,15 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
             team[k] = wide_forced ? kWideGroup * wide_forced : choose_team(p->jobs[k], T, f);
             p->jobs[k].team = team[k];
         }
-        // a plan of its own (not an engine flight) that leaves most of the
-        // device idle: teams of 5-16 waves go wide too ("spread"), so that
+        // a plan of its own or a serving flight (PlanEnv::spread) that leaves
+        // most of the device idle: teams of 5-16 waves go wide too ("spread"), so that
         // each wave has a SIMD to itself (one workgroup would put two on a
         // SIMD); a lone 5 kbp pair runs 22 % faster.  A spread pair keeps its
         // team size in its KswJob: when no waiting-launch slots are free at
         // launch (other threads' calls hold them), it runs as a one-workgroup
         // team instead of waiting.
         std::vector<char> spread(p->n_fast, 0);
-        if (!env && !wide_forced && forced == 0) {
+        if (!wide_forced && ((!env && forced == 0) || (env && env->spread && forced >= 0))) {
             int64_t waves = 0;
             for (int k = 0; k < p->n_fast; ++k) waves += team[k];
             if (4 * waves <= stripe_resident_waves(kGroupLeft))

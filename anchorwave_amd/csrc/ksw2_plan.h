@@ -95,6 +95,7 @@ struct PlanEnv {
     uint32_t *mirror_cig = nullptr, *mirror_cig_dev = nullptr;
     int32_t *mirror_flag = nullptr, *mirror_flag_dev = nullptr;
     int team = 0;                    // > 0: every stripe-kernel pair gets this team size
+    bool spread = false;             // 5-16 wave teams may run spread (as in a plan of its own)
 };
 
 // A device-resident batch: sequences uploaded, arenas sized, jobs built.
