@@ -198,6 +198,7 @@ Engine::Engine(int8_t m, const int8_t *mat, const KswScore &sc, const ScoreClass
     if (const char *mt = getenv("KSW2_HIP_MAX_TEAM")) max_team_ = std::max(1, std::min(kMaxWide, atoi(mt)));
     if (const char *bf = getenv("KSW2_HIP_BACKFILL")) backfill_ = std::max(0, atoi(bf));
     if (const char *sp = getenv("KSW2_HIP_SERVE_SPREAD")) serve_spread_ = atoi(sp) != 0;
+    if (const char *bs = getenv("KSW2_HIP_BATCH_SPREAD")) batch_spread_ = atoi(bs) != 0;
     for (int s = 0; s < kStreams; ++s) stream_class_[s] = kStreamClass[s];
     if (const char *sc = getenv("KSW2_HIP_STREAM_CLASSES"))  // e.g. "2210" (tuning)
         for (int s = 0; s < kStreams && sc[s]; ++s)
@@ -696,7 +697,7 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
         env.staging = staging_[s], env.staging_bytes = staging_bytes_[s];
         env.h_stop = h_stop_, env.d_stop = d_stop_;
         env.team = std::max(key, 0);
-        env.spread = !batch_ && serve_spread_;  // serving: a lone flight's team spread over CUs
+        env.spread = batch_ ? batch_spread_ : serve_spread_;  // a lone flight's teams spread over CUs
         if (mr.slots) {
             env.mirror_out = mr.out, env.mirror_out_dev = mr.out_dev;
             env.mirror_cig = mr.cig, env.mirror_cig_dev = mr.cig_dev;
