@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <memory>
 #include <chrono>
 #include <mutex>
 #include <thread>
@@ -793,16 +794,26 @@ int plan_fetch(Plan *p, ksw_extz_t *ez, void *km, const std::vector<char> *skip)
         return KSW2_HIP_OK;
     }
     std::vector<KswOut> outs(p->n);
-    std::vector<uint32_t> cig(p->cig_words);
+    std::unique_ptr<uint32_t[]> cig(new uint32_t[p->cig_words]);  // only [lo, hi) is filled and read
     KSW_HIP_TRY(hipMemcpyAsync(outs.data(), p->d_out, sizeof(KswOut) * p->n, hipMemcpyDeviceToHost, p->stream));
-    KSW_HIP_TRY(hipMemcpyAsync(cig.data(), p->d_cig, p->cig_words * 4, hipMemcpyDeviceToHost, p->stream));
+    // CIGAR words of the slots still to deliver only (pairs already taken
+    // from the host mirror are skipped)
+    uint64_t lo = p->cig_words, hi = 0;
+    for (int k = 0; k < p->n; ++k)
+        if (!skip || !(*skip)[k]) {
+            const uint64_t len = (uint64_t)(k < p->n_fast ? p->jobs[k].qlen + p->jobs[k].tlen
+                                                         : p->gjobs[k - p->n_fast].qlen + p->gjobs[k - p->n_fast].tlen) + 2;
+            lo = std::min(lo, p->cig_off[k]), hi = std::max(hi, std::min<uint64_t>(p->cig_off[k] + len, p->cig_words));
+        }
+    if (hi > lo)
+        KSW_HIP_TRY(hipMemcpyAsync(cig.get() + lo, p->d_cig + lo, (hi - lo) * 4, hipMemcpyDeviceToHost, p->stream));
     int32_t werr = 0;
     if (p->d_werr) KSW_HIP_TRY(hipMemcpyAsync(&werr, p->d_werr, 4, hipMemcpyDeviceToHost, p->stream));
     KSW_HIP_TRY(hipStreamSynchronize(p->stream));
     if (werr) return KSW2_HIP_EDEVICE;  // a wide team gave up a wait: its results are not computed
     for (int k = 0; k < p->n; ++k) {
         if (skip && (*skip)[k]) continue;
-        const int st = deliver_slot(p, k, outs[k], cig.data(), ez + p->order[k], km);
+        const int st = deliver_slot(p, k, outs[k], cig.get(), ez + p->order[k], km);
         if (st != KSW2_HIP_OK) return st;
     }
     return KSW2_HIP_OK;
