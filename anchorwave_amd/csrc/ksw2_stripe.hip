@@ -24,7 +24,13 @@
 //    polls that count (workgroup-scope acquire) before it reads.  All waves
 //    of a team share one CU, so no cache maintenance beyond the CU is
 //    involved.  W = 1 (no waiting at all) for pairs that are small against
-//    the batch; long pairs get up to 16 waves.
+//    the batch; long pairs get up to 16 waves.  A pair's team may be smaller
+//    than the launch's workgroups: the extra waves end at once.
+//  * Wide teams (ksw_stripe_wide_kernel).  Above 16 waves, or to give each
+//    wave a SIMD of its own, a pair's passes go to several 4-wave
+//    workgroups on different CUs; the counts and column-buffer entries are
+//    then device-scope accesses, and the epilogue runs in a launch of its
+//    own.
 //  * Scaled, negated state.  Every difference value w is kept as -8*w plus a
 //    3-bit tag, so a chain of v_pk_min_i16 yields both the maximum of the
 //    five candidates (s, a, b, a2, b2) and the index the reference's strict
