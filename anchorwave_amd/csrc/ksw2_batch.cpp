@@ -479,14 +479,8 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
     if (env && env->stream) {
         p->stream = env->stream;  // caller's stream: every launch goes there
     } else {
-        p->own_streams = true;
+        p->own_streams = true;  // lane streams are created on first use (plan_launch)
         if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) return fail(KSW2_HIP_EDEVICE);
-        for (int l = 0; l < Plan::kLanes; ++l) {
-            if (hipStreamCreateWithFlags(&p->lane_stream[l], hipStreamNonBlocking) != hipSuccess)
-                return fail(KSW2_HIP_EDEVICE);
-            if (hipEventCreateWithFlags(&p->lane_done[l], hipEventDisableTiming) != hipSuccess)
-                return fail(KSW2_HIP_EDEVICE);
-        }
     }
     for (auto &e : p->ev)
         if (hipEventCreate(&e) != hipSuccess) return fail(KSW2_HIP_EDEVICE);
@@ -623,6 +617,16 @@ int plan_launch(Plan *p, const Progress *prog)
     bool used[Plan::kLanes] = {false, false, false, false};
     auto lane = [&](int l) -> hipStream_t {
         if (!p->own_streams) return p->stream;
+        if (!p->lane_stream[l]) {
+            if (hipStreamCreateWithFlags(&p->lane_stream[l], hipStreamNonBlocking) != hipSuccess) {
+                p->lane_stream[l] = nullptr;
+                return nullptr;
+            }
+            if (hipEventCreateWithFlags(&p->lane_done[l], hipEventDisableTiming) != hipSuccess) {
+                p->lane_done[l] = nullptr;
+                return nullptr;
+            }
+        }
         if (!used[l]) {
             if (hipStreamWaitEvent(p->lane_stream[l], p->ev[0], 0) != hipSuccess) return nullptr;
             used[l] = true;
