@@ -326,10 +326,10 @@ uint64_t ksw2_hip_footprint(int qlen, const uint8_t *query, int tlen, const uint
                             const int8_t *mat, int8_t q, int8_t e, int8_t q2, int8_t e2, int w, int zdrop, int flag)
 {
     if (m <= 1 || qlen <= 0 || tlen <= 0 || !mat) return 0;
+    if (qlen > 0x7fffffff - tlen + 1) return UINT64_MAX;  // tested before the scoring (ksw2_extd2_avx.c:136-140)
     KswScore sc;
     ScoreClass cls;
     if (!normalise_scoring(m, mat, q, e, q2, e2, &sc, &cls)) return 0;  // the reference returns early
-    if (qlen > 0x7fffffff - tlen + 1) return UINT64_MAX;
     ksw2_pair_t p;
     p.query = query, p.target = target, p.qlen = qlen, p.tlen = tlen;
     p.w = w, p.zdrop = zdrop, p.end_bonus = 0, p.flag = flag;
@@ -342,7 +342,16 @@ int ksw2_hip_server_poll(ksw2_hip_server *server, uint64_t *tickets, double *lat
 {
     if (!server || max_tickets < 0 || (max_tickets > 0 && !tickets)) return set_err(KSW2_HIP_EINVAL);
     const int n = server->e->poll(tickets, latency_s, max_tickets, wait_ms);
-    set_err(KSW2_HIP_OK);
+    set_err(server->e->status());
+    return n;
+}
+
+int ksw2_hip_server_poll_status(ksw2_hip_server *server, uint64_t *tickets, double *latency_s, int32_t *status,
+                                int max_tickets, int wait_ms)
+{
+    if (!server || max_tickets < 0 || (max_tickets > 0 && (!tickets || !status))) return set_err(KSW2_HIP_EINVAL);
+    const int n = server->e->poll(tickets, latency_s, max_tickets, wait_ms, status);
+    set_err(server->e->status());
     return n;
 }
 

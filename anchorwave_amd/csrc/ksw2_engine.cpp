@@ -172,6 +172,7 @@ struct Engine::Task {
     int waves = 1;     // general kernel: waves of its workgroup
     int held = 0;      // waves held while admitted
     int cls = 0;
+    int status = 0;    // KSW2_HIP_OK, or why the device path did not run it
     bool general = false;
     double t_submit = 0, t_start = 0, t_done = 0;
 };
@@ -199,6 +200,7 @@ Engine::Engine(int8_t m, const int8_t *mat, const KswScore &sc, const ScoreClass
     if (const char *bf = getenv("KSW2_HIP_BACKFILL")) backfill_ = std::max(0, atoi(bf));
     if (const char *sp = getenv("KSW2_HIP_SERVE_SPREAD")) serve_spread_ = atoi(sp) != 0;
     if (const char *bs = getenv("KSW2_HIP_BATCH_SPREAD")) batch_spread_ = atoi(bs) != 0;
+    if (const char *ff = getenv("KSW2_HIP_FAIL_FLIGHTS")) fail_flights_ = std::max(0, atoi(ff));
     for (int s = 0; s < kStreams; ++s) stream_class_[s] = kStreamClass[s];
     if (const char *sc = getenv("KSW2_HIP_STREAM_CLASSES"))  // e.g. "2210" (tuning)
         for (int s = 0; s < kStreams && sc[s]; ++s)
@@ -346,6 +348,7 @@ int Engine::submit(const ksw2_pair_t &p, ksw_extz_t *ez, bool copy, uint64_t *ti
     tasks_[t->ticket] = t;
     if (finished) {
         t->t_done = t->t_submit;
+        t->status = why;
         done_.push_back(t->ticket);
         if (why != KSW2_HIP_OK && status_ == KSW2_HIP_OK) status_ = why;
         cv_done_.notify_all();
@@ -381,7 +384,13 @@ void Engine::order_longest_first()
         });
 }
 
-int Engine::poll(uint64_t *tickets, double *latency, int max, int wait_ms)
+int Engine::status()
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    return status_;
+}
+
+int Engine::poll(uint64_t *tickets, double *latency, int max, int wait_ms, int32_t *status)
 {
     std::unique_lock<std::mutex> lk(mu_);
     if (done_.empty() && wait_ms != 0) {
@@ -398,10 +407,12 @@ int Engine::poll(uint64_t *tickets, double *latency, int max, int wait_ms)
         auto it = tasks_.find(tk);
         if (it != tasks_.end()) {
             if (latency) latency[n - 1] = it->second->t_done - it->second->t_submit;
+            if (status) status[n - 1] = it->second->status;
             delete it->second;
             tasks_.erase(it);
-        } else if (latency) {
-            latency[n - 1] = 0;
+        } else {
+            if (latency) latency[n - 1] = 0;
+            if (status) status[n - 1] = KSW2_HIP_OK;
         }
     }
     return n;
@@ -489,7 +500,7 @@ bool Engine::reap_locked(std::unique_lock<std::mutex> &lk)
                 if (f->delivered[k] || !__atomic_load_n(p->h_mflag + k, __ATOMIC_ACQUIRE)) continue;
                 Task *tk = f->tasks[p->order[k]];
                 const int st = deliver_slot(p, k, p->h_mout[k], p->h_mcig, tk->ez, nullptr);
-                if (st != KSW2_HIP_OK) tk->ez->zdropped = 1, status_ = st;
+                if (st != KSW2_HIP_OK) tk->ez->zdropped = 1, tk->status = st, status_ = st;
                 f->delivered[k] = 1, --f->undelivered_fast;
                 f->tasks[p->order[k]] = nullptr;
                 busy_waves_ -= tk->held, inflight_steps_ -= tk->steps;
@@ -539,7 +550,7 @@ bool Engine::reap_locked(std::unique_lock<std::mutex> &lk)
             } else {
                 // keep the (possibly reallocated) CIGAR buffer with the caller
                 tk->ez->cigar = z.cigar, tk->ez->m_cigar = z.m_cigar;
-                tk->ez->zdropped = 1, status_ = st;
+                tk->ez->zdropped = 1, tk->status = st, status_ = st;
             }
             tk->t_done = t;
             done_.push_back(tk->ticket);
@@ -710,6 +721,7 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
             f->undelivered_fast = f->plan->h_mflag ? f->plan->n_fast : 0;
         }
         const double t_c1 = now_s();
+        if (st == KSW2_HIP_OK && fail_flights_ > 0) --fail_flights_, st = KSW2_HIP_EDEVICE;  // tests
         if (st == KSW2_HIP_OK) st = plan_launch(f->plan);
         const double t0 = now_s();
         lk.lock();
@@ -725,7 +737,7 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
             queued_steps_ -= steps;
             queued_bytes_ = queued_bytes_ > bytes ? queued_bytes_ - bytes : 0;
             for (Task *t : pick) {
-                t->ez->zdropped = 1;
+                t->ez->zdropped = 1, t->status = st;
                 t->t_done = now_s();
                 done_.push_back(t->ticket);
                 --pending_;

@@ -59,9 +59,12 @@ public:
     // queue longest first and release it.
     void hold(bool on);
     void order_longest_first();
-    // Completed tickets (up to max) and their submit-to-done latencies
-    // (latency may be null); waits up to wait_ms (<0: until one).
-    int poll(uint64_t *tickets, double *latency, int max, int wait_ms);
+    // Completed tickets (up to max), their submit-to-done latencies and
+    // statuses (KSW2_HIP_OK, or why the device path did not run the pair;
+    // latency and status may be null); waits up to wait_ms (<0: until one).
+    int poll(uint64_t *tickets, double *latency, int max, int wait_ms, int32_t *status = nullptr);
+    // The first failure any task of this engine has met (KSW2_HIP_OK: none).
+    int status();
     // Waits until every submitted task is done (batch mode); polls the
     // progress callback meanwhile.  Returns the first error seen, else OK.
     int drain(const Progress *prog);
@@ -72,6 +75,7 @@ private:
     bool serve_spread_ = true;  // serving flights may spread 5-16 wave teams (KSW2_HIP_SERVE_SPREAD, tuning)
     bool batch_spread_ = false; // batch flights likewise (KSW2_HIP_BATCH_SPREAD, tuning)
     int stream_class_[kStreams] = {};  // serving: each stream's own size class (KSW2_HIP_STREAM_CLASSES, tuning)
+    int fail_flights_ = 0;      // tests: the first this many flights fail before launch (KSW2_HIP_FAIL_FLIGHTS)
     struct Task;
     struct Flight;
     void loop();

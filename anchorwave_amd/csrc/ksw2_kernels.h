@@ -25,10 +25,12 @@ struct StripeArgs {
     // wide launches only (launch_stripe_wide): the first workgroup of each
     // job of this launch (njobs + 1 ascending entries, the last one past the
     // launch), the jobs' control blocks (kWideCtl dwords each, in order
-    // position) and the plan's failure word (raised when a wait gave up)
+    // position) and the plan's redo count (pairs whose team gave up a wait
+    // and were recomputed by their epilogue workgroup)
     const int32_t *wfirst;
     int32_t *wctl;
     int32_t *werr;
+    int force_late;            // tests: every wide pair is recomputed as if a wait had given up
 };
 
 struct GenArgs {
@@ -132,7 +134,7 @@ hipError_t launch_stripe_wide(const StripeArgs &a, int group, const int32_t *h_w
 // Workgroups of the wide stripe kernel the device holds at once.
 int stripe_wide_capacity(int group);
 hipError_t launch_stripe_wide_epilogue(const StripeArgs &a, int group, hipStream_t st);
-// Zeroes a plan's wide failure word (kWideCtl dwords at werr).
+// Zeroes a plan's wide redo count (kWideCtl dwords at werr).
 hipError_t clear_wide_err(int32_t *werr, hipStream_t st);
 // Waves of the stripe kernel that fit on the device at once.
 int stripe_resident_waves(int group);

@@ -164,7 +164,7 @@ struct Plan {
     int n_wide = 0;                  // pairs of wide launches
     bool optional_wide = false;      // every wide pair is a spread pair (may run as a one-workgroup team)
     int32_t *d_wctl = nullptr;       // their control blocks (kWideCtl dwords each)
-    int32_t *d_werr = nullptr;       // raised when a wide team gave up a wait
+    int32_t *d_werr = nullptr;       // count of wide pairs recomputed by their epilogue (a wait gave up)
     std::vector<int32_t> h_wfirst;   // per wide launch: each pair's first workgroup, then the end
     int32_t *d_wfirst = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};

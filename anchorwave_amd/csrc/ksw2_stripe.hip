@@ -717,7 +717,7 @@ __global__ void __launch_bounds__(kWave * kMaxTeam) ksw_stripe_wide_kernel(Strip
     const int npass = ((tlen + SR - 1) / SR + 1) / 2;
     int late = 0;
     for (int p = v; p < npass; p += W) pass_body<RIGHT, MODE, true>(a, kc, job, p, jb, qlen, tlen, ctl, W, late);
-    if (late && threadIdx.x % kWave == 0)
+    if ((late || a.force_late) && threadIdx.x % kWave == 0)
         __hip_atomic_store(ctl + kWideCtlLate, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -730,33 +730,36 @@ __global__ void __launch_bounds__(256) ksw_stripe_wctl_kernel(int32_t *wctl)
 }
 
 // The epilogue of every pair of a wide launch, one workgroup per pair, after
-// all of the launch's passes.  A pair whose team gave up a wait is reported
-// as failed (zdropped, no CIGAR; the plan's failure word is raised) rather
-// than walked.
-template <int MODE>
-__global__ void __launch_bounds__(256) ksw_stripe_wide_epilogue_kernel(StripeArgs a)
+// all of the launch's passes.  A pair whose team gave up a wait (its late
+// word is raised) is first recomputed by this workgroup as a one-workgroup
+// team of kWideGroup waves (LDS progress counts, no cross-workgroup waits),
+// into the same direction bytes, column buffer and stripe partials; the
+// plan's redo word counts such pairs.
+template <bool RIGHT, int MODE>
+__global__ void __launch_bounds__(kWave * kWideGroup) ksw_stripe_wide_epilogue_kernel(StripeArgs a)
 {
+    __shared__ int32_t prog[kWideGroup];
     __shared__ uint64_t tile[8][64];
     __shared__ int shared_n;
     const int slot = blockIdx.x;
     const int job = __builtin_amdgcn_readfirstlane(a.order[slot]);
     const KswJob jb = a.jobs[job];
-    const int late = __builtin_amdgcn_readfirstlane(a.wctl[(int64_t)slot * kWideCtl + kWideCtlLate]);
-    if (!late) {
-        pair_epilogue(a, job, jb, tile, MODE == kTracks, &shared_n, (int)blockDim.x);
-        return;
+    const int late = __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(a.wctl + (int64_t)slot * kWideCtl + kWideCtlLate, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    if (late) {
+        const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
+        prog[w] = 0;
+        __syncthreads();
+        const Consts kc = make_consts<RIGHT>(a.sc);
+        const int qlen = __builtin_amdgcn_readfirstlane(jb.qlen), tlen = __builtin_amdgcn_readfirstlane(jb.tlen);
+        const int npass = ((tlen + SR - 1) / SR + 1) / 2;
+        int late2 = 0;
+        for (int p = w; p < npass; p += kWideGroup)
+            pass_body<RIGHT, MODE, false>(a, kc, job, p, jb, qlen, tlen, prog, kWideGroup, late2);
+        __syncthreads();
+        if (threadIdx.x == 0) atomicAdd(a.werr, 1);
     }
-    if (threadIdx.x == 0) {
-        KswOut o = a.out[job];
-        o.score = KSW_NEG_INF_I, o.zdropped = 1, o.n_cigar = 0;
-        a.out[job] = o;
-        *a.werr = 1;
-        if (a.host_flag) {
-            a.host_out[job] = o;
-            __threadfence_system();
-            __hip_atomic_store(a.host_flag + job, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-        }
-    }
+    pair_epilogue(a, job, jb, tile, MODE == kTracks, &shared_n, (int)blockDim.x);
 }
 
 template <bool RIGHT, int MODE>
@@ -864,10 +867,11 @@ hipError_t launch_stripe_wide_epilogue(const StripeArgs &a, int group, hipStream
 {
     if (a.njobs <= 0) return hipSuccess;
     if (!a.wctl || !a.werr) return hipErrorInvalidValue;
-    void (*fn)(StripeArgs) = group == kGroupTracks       ? ksw_stripe_wide_epilogue_kernel<kTracks>
-                             : group == kGroupScoreOnly ? ksw_stripe_wide_epilogue_kernel<kScoreOnly>
-                                                        : ksw_stripe_wide_epilogue_kernel<kDirs>;
-    hipLaunchKernelGGL(fn, dim3(a.njobs), dim3(256), 0, st, a);
+    void (*fn)(StripeArgs) = group == kGroupTracks       ? ksw_stripe_wide_epilogue_kernel<false, kTracks>
+                             : group == kGroupScoreOnly ? ksw_stripe_wide_epilogue_kernel<false, kScoreOnly>
+                             : group == kGroupRight     ? ksw_stripe_wide_epilogue_kernel<true, kDirs>
+                                                        : ksw_stripe_wide_epilogue_kernel<false, kDirs>;
+    hipLaunchKernelGGL(fn, dim3(a.njobs), dim3(kWave * kWideGroup), 0, st, a);
     return hipGetLastError();
 }
 

@@ -50,6 +50,7 @@ EXPORTS = (
     "ksw2_hip_server_create",
     "ksw2_hip_server_submit",
     "ksw2_hip_server_poll",
+    "ksw2_hip_server_poll_status",
     "ksw2_hip_server_destroy",
     "ksw2_hip_plan_create",
     "ksw2_hip_plan_run",
@@ -209,6 +210,9 @@ def lib():
         L.ksw2_hip_server_poll.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.c_int,
                                            C.c_int]
         L.ksw2_hip_server_poll.restype = C.c_int
+        L.ksw2_hip_server_poll_status.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_double),
+                                                  C.POINTER(C.c_int32), C.c_int, C.c_int]
+        L.ksw2_hip_server_poll_status.restype = C.c_int
         L.ksw2_hip_server_destroy.argtypes = [C.c_void_p]
         L.ksw2_hip_server_destroy.restype = None
         _lib = L
@@ -587,6 +591,8 @@ class Server:
             raise KswError(f"ksw2_hip_server_create: {STATUS.get(st, st)}")
         self._h = h
         self._ez = {}   # ticket -> KswExtz (kept alive until completion)
+        self.status = {}  # ticket -> status of the completed pair (0: the device path ran it)
+        self.last_error = 0  # ksw2_hip_last_error() after the last poll: the server's first failure
 
     def submit(self, query, target, w=-1, zdrop=-1, end_bonus=0, flag=0) -> int:
         qs, ts = _u8(query), _u8(target)
@@ -628,13 +634,16 @@ class Server:
         if getattr(self, "_pbuf_n", 0) < max_n:
             self._pbuf = (C.c_uint64 * max_n)()
             self._plat = (C.c_double * max_n)()
+            self._pst = (C.c_int32 * max_n)()
             self._pbuf_n = max_n
-        buf, lat = self._pbuf, self._plat
-        n = self._L.ksw2_hip_server_poll(self._h, buf, lat, max_n, wait_ms)
+        buf, lat, pst = self._pbuf, self._plat, self._pst
+        n = self._L.ksw2_hip_server_poll_status(self._h, buf, lat, pst, max_n, wait_ms)
         if n < 0:
-            raise KswError(f"ksw2_hip_server_poll: {STATUS.get(n, n)}")
+            raise KswError(f"ksw2_hip_server_poll_status: {STATUS.get(n, n)}")
+        self.last_error = self._L.ksw2_hip_last_error()
         out = []
         for k in range(n):
+            self.status[buf[k]] = pst[k]
             ez = self._ez.pop(buf[k])
             out.append((buf[k], Result.from_ez(ez) if results else None, lat[k]))
             if ez.cigar:

@@ -97,28 +97,32 @@ def uneven_lengths(rng: np.random.Generator, n: int) -> np.ndarray:
     return np.clip(rng.lognormal(np.log(3000.0), 1.0, n), 100, 50000).astype(np.int64)
 
 
+def _uneven_pair(rng: np.random.Generator, tlen: int, ratio: float) -> Pair:
+    """One inter-anchor interval: target of ``tlen`` bases, query realised at
+    ``ratio`` times that length (one uniform segment deleted or inserted),
+    then 6% substitutions, 1.5% short indels and, for 20% of pairs, one
+    1-20 kbp random insertion."""
+    t = random_seq(rng, int(tlen))
+    want = max(1, int(round(tlen * ratio)))
+    q = t.copy()
+    if want < len(q):
+        d = len(q) - want
+        pos = int(rng.integers(0, len(q) - d + 1))
+        q = np.concatenate([q[:pos], q[pos + d:]])
+    elif want > len(q):
+        pos = int(rng.integers(0, len(q) + 1))
+        q = np.concatenate([q[:pos], random_seq(rng, want - len(q)), q[pos:]])
+    q = mutate(rng, q, 0.06, 0.015)
+    if rng.random() < 0.20:
+        q = big_indel(rng, q, 1000, 20000, ins_only=True)
+    return _nonempty(rng, q), t
+
+
 def uneven(seed: int = 3, n: int = 20000) -> List[Pair]:
     rng = np.random.default_rng(seed)
     tl = uneven_lengths(rng, n)
     ratio = rng.uniform(0.5, 2.0, n)
-    out = []
-    for k in range(n):
-        t = random_seq(rng, int(tl[k]))
-        want = max(1, int(round(tl[k] * ratio[k])))
-        q = t.copy()
-        # realise the length ratio by deleting or inserting one uniform segment
-        if want < len(q):
-            d = len(q) - want
-            pos = int(rng.integers(0, len(q) - d + 1))
-            q = np.concatenate([q[:pos], q[pos + d:]])
-        elif want > len(q):
-            pos = int(rng.integers(0, len(q) + 1))
-            q = np.concatenate([q[:pos], random_seq(rng, want - len(q)), q[pos:]])
-        q = mutate(rng, q, 0.06, 0.015)
-        if rng.random() < 0.20:
-            q = big_indel(rng, q, 1000, 20000, ins_only=True)
-        out.append((_nonempty(rng, q), t))
-    return out
+    return [_uneven_pair(rng, int(tl[k]), float(ratio[k])) for k in range(n)]
 
 
 def banded(seed: int = 4, n: int = 64, lo: int = 100_000, hi: int = 300_000) -> List[Pair]:
@@ -142,7 +146,8 @@ class Task:
 
 
 def serving(seed: int = 5, n: int = 2000, rate_per_s: float = 100.0) -> List[Task]:
-    """Poisson arrivals; sizes drawn 50/40/10 from single/fixed/uneven."""
+    """Poisson arrivals; sizes drawn 50/40/10 from single/fixed/uneven (the
+    uneven tasks with the uneven batch's 0.5-2.0 query/target length ratio)."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate_per_s, n)
     tasks = []
@@ -162,10 +167,7 @@ def serving(seed: int = 5, n: int = 2000, rate_per_s: float = 100.0) -> List[Tas
             kind = "fixed"
         else:
             tl = int(uneven_lengths(rng, 1)[0])
-            t = random_seq(rng, tl)
-            q = mutate(rng, t, 0.06, 0.015)
-            if rng.random() < 0.20:
-                q = big_indel(rng, q, 1000, 20000, ins_only=True)
+            q, t = _uneven_pair(rng, tl, float(rng.uniform(0.5, 2.0)))
             kind = "uneven"
         tasks.append(Task(tnow, _nonempty(rng, q), t, kind))
     return tasks

@@ -38,7 +38,6 @@ METRIC = "GCUPS, batched ksw_extd2 score+CIGAR (bit-exact); p50/p99 task latency
 OPS_PER_CELL = 51
 N_CU, LANES, CLK_GHZ = 256, 64, 2.4
 PEAK_PACKED16_TOPS = N_CU * LANES * CLK_GHZ * 2 / 1e3   # 78.6 T int16 lane-ops/s
-PEAK_INT32_TOPS = N_CU * LANES * CLK_GHZ / 1e3          # 39.3 T int32 lane-ops/s
 HBM_PEAK_GBS = 8000.0
 
 
@@ -46,16 +45,37 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def read_traffic(name: str):
-    """HBM bytes per dispatch of the DP kernel, from the committed rocprofv3
-    PMC summary (line 'traffic_bytes_per_dispatch: N'), or None."""
+def read_profile_value(name: str, key: str):
+    """A figure of a committed rocprofv3 summary (a line 'key: N'), or None:
+    traffic_bytes_per_dispatch (HBM bytes per dispatch of the DP kernel,
+    FETCH_SIZE and WRITE_SIZE scaled by a copy of known size) and
+    valu_issue_frac (SQ_ACTIVE_INST_VALU over the busy SIMD cycles)."""
     path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
-            m = re.search(r"traffic_bytes_per_dispatch:\s*([0-9.eE+]+)", f.read())
+            m = re.search(key + r":\s*([0-9.eE+]+)", f.read())
         return float(m.group(1)) if m else None
     except OSError:
         return None
+
+
+def host_cpu() -> dict:
+    """Model name and the widest vector ISA of the host (the reference's
+    compile-time dispatch, alignSlidingWindow.cpp:840-846, takes AVX-512 when
+    the host has AVX512BW)."""
+    model, flags = "unknown", ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("flags") and not flags:
+                    flags = line.split(":", 1)[1]
+    except OSError:
+        pass
+    fl = set(flags.split())
+    isa = "AVX-512BW" if "avx512bw" in fl else "AVX2" if "avx2" in fl else "SSE4.1" if "sse4_1" in fl else "SSE2"
+    return {"cpu_model": model, "isa": isa}
 
 
 def cpu_baseline(pairs, threads: int, flag: int = 0, w: int = -1):
@@ -354,7 +374,10 @@ def main() -> int:
     dp_avg = float(np.mean(dp_ms))
     log(f"[bench] {ms_per_step:.2f} ms/step, {gcups:.1f} GCUPS (DP kernel {dp_avg:.2f} ms, "
         f"backtrack {np.mean(bt_ms):.2f} ms)")
+    t1 = time.perf_counter()
     gpu_res = plan.fetch()
+    fetch_ms = (time.perf_counter() - t1) * 1e3
+    log(f"[bench] fetch of every score and CIGAR to host ez structs: {fetch_ms:.2f} ms")
     dir_bytes = sum(((len(t) + 511) // 512 + 1) // 2 * (len(q) + 127) * 1024 for q, t in pairs)
     plan.close()
 
@@ -381,11 +404,12 @@ def main() -> int:
         "peak": round(PEAK_PACKED16_TOPS, 1),
         "unit": "Tops/s",
         "frac": round(achieved / PEAK_PACKED16_TOPS, 4),
-        "traffic": read_traffic("stripe_kernel_fixed_4096x2000_pmc.md"),
+        "traffic": read_profile_value("stripe_kernel_fixed_4096x2000_pmc.md", "traffic_bytes_per_dispatch"),
         "kernel": "ksw_stripe_kernel",
         "ops_per_cell": OPS_PER_CELL,
         "avg_launch_ms": round(dp_avg, 3),
-        "frac_of_int32_peak": round(achieved / PEAK_INT32_TOPS, 4),
+        # measured VALU issue: busy fraction of the SIMDs' VALU issue slots (PMC)
+        "valu_issue_frac": read_profile_value("stripe_kernel_fixed_4096x2000_pmc.md", "valu_issue_frac"),
         "traceback_write_GBs": round(dir_bytes / (dp_avg * 1e-3) / 1e9, 1),
         "traceback_write_frac_hbm": round(dir_bytes / (dp_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
     }
@@ -400,12 +424,15 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int32",
+        "dtype": "int16 (packed pairs of cells; exact to the reference's int8)",
         "data": "synthetic",
         "config": {"workload": f"fixed batch: {args.pairs} pairs x 2000 bp, 5% subs, 1% indels, "
                                "10% with a 50-300 bp indel; global, full CIGAR",
                    "pairs": args.pairs, "cells": cells, "scoring": "0/-6, 8/2, 75/1",
-                   "device_ms_dp": round(dp_avg, 3), "device_ms_backtrack": round(float(np.mean(bt_ms)), 3)},
+                   "device_ms_dp": round(dp_avg, 3), "device_ms_backtrack": round(float(np.mean(bt_ms)), 3),
+                   "timed": "plan runs on resident inputs; scores and CIGARs left in HBM",
+                   "fetch_ms": round(fetch_ms, 3),
+                   "gcups_with_fetch": round(cells / ((ms_per_step + fetch_ms) * 1e-3) / 1e9, 2)},
         "roofline": roofline,
         "one_at_a_time_gcups": None if single is None else round(single, 3),
         "batched_over_one_at_a_time": None if not single else round(gcups / single, 1),
@@ -426,7 +453,7 @@ def main() -> int:
         kind, dt, cpu_res = cpu_baseline(pairs, threads)
         exact = sum(1 for g, c in zip(gpu_res, cpu_res) if g.__dict__ == c.__dict__)
         out["cpu_baseline"] = {"value": round(cells / dt / 1e9, 3), "unit": "GCUPS", "cores": threads,
-                               "kind": kind,
+                               "kind": kind, **host_cpu(),
                                "sample": f"the full fixed batch ({len(pairs)} pairs, {cells:.3g} cells), "
                                          "ksw_extd2_avx512 one pair per task, largest first"}
         out["exact"] = {"agree": exact, "of": len(pairs), "frac": exact / len(pairs)}

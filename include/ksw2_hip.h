@@ -195,10 +195,17 @@ int ksw2_hip_server_submit(ksw2_hip_server *server, const ksw2_pair_t *pair, ksw
 /* Writes up to max_tickets completed tickets, waiting up to wait_ms
  * milliseconds for the first (-1: until one completes; 0: no wait).  When
  * latency_s is not NULL it receives each ticket's time from submission to
- * completion, in seconds.  Returns how many were written, or a negative
- * status. */
+ * completion, in seconds.  Returns how many were written, or KSW2_HIP_EINVAL.
+ * ksw2_hip_last_error() then reports KSW2_HIP_OK, or the first failure the
+ * server has met (a pair the device path could not run completes with
+ * zdropped = 1 and no CIGAR, the AVX kernels' allocation-failure convention;
+ * ksw2_hip_server_poll_status tells such pairs from real z-drops). */
 int ksw2_hip_server_poll(ksw2_hip_server *server, uint64_t *tickets, double *latency_s, int max_tickets,
                          int wait_ms);
+/* ksw2_hip_server_poll, and status[i] receives ticket i's status: KSW2_HIP_OK,
+ * or the error that kept the device path from running that pair. */
+int ksw2_hip_server_poll_status(ksw2_hip_server *server, uint64_t *tickets, double *latency_s, int32_t *status,
+                                int max_tickets, int wait_ms);
 /* Finishes every submitted pair, then releases the server. */
 void ksw2_hip_server_destroy(ksw2_hip_server *server);
 

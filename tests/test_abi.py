@@ -80,3 +80,25 @@ def test_footprint_model():
     assert ksw2.footprint(q[:1000], t[:1000]) < full
     assert ksw2.footprint(q[:0], t) == 0
     assert ksw2.footprint(q, t, m=1) == 0
+
+
+def test_length_overflow_is_tested_before_the_scoring():
+    """ksw_extd2_avx512 sets zdropped for qlen > INT_MAX - tlen + 1 before it
+    swaps the gap pieces or returns early on -min_sc > 2(q+e)
+    (minimap2/ksw2_extd2_avx.c:136-140): such a call is zdropped even with a
+    matrix that would return early, and needs no device.  The lengths are
+    only compared, never read past the small buffers."""
+    import ctypes as C
+    lib = ksw2.lib()
+    buf = np.zeros(16, np.uint8)
+    mat = ksw2.default_matrix(0, -30)  # -min_sc = 30 > 2(q+e) = 20
+    ez = ksw2.KswExtz()
+    lib.ksw_extd2_hip(None, 0x7fffffff, buf.ctypes.data, 2, buf.ctypes.data, 5, mat.ctypes.data,
+                      8, 2, 75, 1, -1, -1, 0, 0, C.byref(ez))
+    assert ez.zdropped == 1 and ez.score == ksw2.KSW_NEG_INF and ez.n_cigar == 0
+    assert lib.ksw2_hip_footprint(0x7fffffff, buf.ctypes.data, 2, buf.ctypes.data, 5, mat.ctypes.data,
+                                  8, 2, 75, 1, -1, -1, 0) == 2**64 - 1
+    ez = ksw2.KswExtz()
+    lib.ksw_extd2_hip(None, 16, buf.ctypes.data, 16, buf.ctypes.data, 5, mat.ctypes.data,
+                      8, 2, 75, 1, -1, -1, 0, 0, C.byref(ez))
+    assert ez.zdropped == 0 and ez.score == ksw2.KSW_NEG_INF  # early return, no overflow

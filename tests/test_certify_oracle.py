@@ -12,6 +12,7 @@
 from __future__ import annotations
 
 import numpy as np
+import pytest
 
 import certify_py as CP
 import oracle_py as O
@@ -40,20 +41,6 @@ def _mutate(rng, s, sub=0.08, indel=0.03):
     return "".join(out)
 
 
-def _schedule(diff, initial, maximum, qlen, tlen):
-    """Bands the reference tries on a task that never certifies."""
-    max_band = min(max(qlen, tlen), maximum)
-    req = max(diff, max(1, initial))
-    band = min(max_band, max(req, 2 * req))
-    bands = [band]
-    while band != max_band:
-        band = min(max_band, max(band + 1, 2 * band))
-        bands.append(band)
-    if req < min(max_band, 2 * req):
-        bands.append(req)
-    return bands, band
-
-
 def test_certified_alignments_are_optimal():
     rng = np.random.default_rng(71)
     done = 0
@@ -73,24 +60,54 @@ def test_certified_alignments_are_optimal():
     assert done == 40
 
 
-def test_band_schedule_when_never_certified():
-    """A precomputed optimum above any alignment's score never certifies:
-    every band of the schedule is tried, then the skipped first band."""
+def _bands_tried(monkeypatch, q, t, initial, maximum):
+    """Runs a certification that never certifies (a precomputed optimum
+    above any alignment's score) and returns the bands of its banded calls,
+    in order."""
+    seen = []
+    original = CP.banded
+
+    def record(dq, dd, band, *a):
+        seen.append(band)
+        return original(dq, dd, band, *a)
+
+    monkeypatch.setattr(CP, "banded", record)
+    r = CP.certify(q, t, initial, maximum, precomputed=1, **PEN)
+    monkeypatch.setattr(CP, "banded", original)
+    return r, seen
+
+
+# (initial band, maximum band, target length, query length) -> the bands
+# alignScoreCertifiedKsw2 tries (alignSlidingWindow.cpp:1292-1394), worked
+# out by hand: cap = min(max(qlen, tlen), maximum); requested =
+# max(|qlen - tlen|, max(1, initial)); the first band is min(cap, 2 *
+# requested), then min(cap, max(b + 1, 2b)) until the cap, then the skipped
+# requested band when requested < min(cap, 2 * requested).
+SCHEDULES = [
+    ((3, 64, 120, 120), [6, 12, 24, 48, 64, 3], 64),
+    ((0, 200, 120, 120), [2, 4, 8, 16, 32, 64, 120, 1], 120),
+    ((10, 10_000, 120, 120), [20, 40, 80, 120, 10], 120),
+    ((7, 7, 120, 120), [7], 7),
+    ((50, 51, 120, 120), [51, 50], 51),
+    ((0, 1000, 100, 130), [60, 120, 130, 30], 130),   # |qlen - tlen| = 30 drives the first band
+    ((40, 45, 100, 130), [45, 40], 45),               # cap between requested and 2x: the cap, then requested
+    ((45, 45, 100, 130), [45], 45),                   # requested == cap: no skipped band
+]
+
+
+@pytest.mark.parametrize("args,bands,final", SCHEDULES)
+def test_band_schedule_literal(args, bands, final, monkeypatch):
+    initial, maximum, tl, ql = args
     rng = np.random.default_rng(72)
-    for initial, maximum in [(0, 200), (3, 64), (10, 10_000), (7, 7), (50, 51)]:
-        t = _dna(rng, 120)
-        q = _mutate(rng, t)
-        diff = abs(len(q) - len(t))
-        if maximum < diff:
-            continue
-        r = CP.certify(q, t, initial, maximum, precomputed=1, **PEN)
-        bands, last = _schedule(diff, initial, maximum, len(q), len(t))
-        assert r.status == CP.NOT_CERTIFIED
-        assert r.traceback_attempts == len(bands), (initial, maximum, r, bands)
-        assert r.final_band_width == last
-        assert r.best_effort_cigar, "a valid banded alignment is kept"
-        aq, ad = CP.rows(r.best_effort_cigar, q, t)
-        assert CP.rescore(aq, ad, **PEN) == r.score
+    t = _dna(rng, tl)
+    q = _dna(rng, ql)
+    r, seen = _bands_tried(monkeypatch, q, t, initial, maximum)
+    assert seen == bands, (args, seen)
+    assert r.status == CP.NOT_CERTIFIED and r.traceback_attempts == len(bands)
+    assert r.final_band_width == final
+    assert r.best_effort_cigar, "a valid banded alignment is kept"
+    aq, ad = CP.rows(r.best_effort_cigar, q, t)
+    assert CP.rescore(aq, ad, **PEN) == r.score
 
 
 def test_edge_cases():
@@ -136,5 +153,11 @@ def test_planted_mistakes_are_caught(monkeypatch):
     assert CP.certify(q, t, 1, 10_000, **PEN).status == CP.COMPLETED
     monkeypatch.setattr(CP, "rescore", one_piece)
     assert CP.certify(q, t, 1, 10_000, **PEN).status == CP.NOT_CERTIFIED
-    bands, _ = _schedule(0, 3, 64, 120, 120)
-    assert bands[-1] == 3 and len(bands[:-1]) == len(bands) - 1
+    monkeypatch.setattr(CP, "rescore", original)
+    # a schedule that stops at the cap without the skipped first band, or
+    # doubles from the requested band instead of starting at twice it, is
+    # not the literal list
+    rng2 = np.random.default_rng(72)
+    t2, q2 = _dna(rng2, 120), _dna(rng2, 120)
+    _r, seen = _bands_tried(monkeypatch, q2, t2, 3, 64)
+    assert seen[:-1] != [3, 6, 12, 24, 48, 64] and seen != [6, 12, 24, 48, 64]

@@ -84,6 +84,7 @@ def test_server_budget_too_small_for_a_pair():
     while len(got) < 6:
         for tk, res, _ in srv.poll(wait_ms=2000):
             got[tk] = res
+    assert srv.status[tk_big] == -3 and all(srv.status[tk] == 0 for tk in tks)  # ENOMEM for that ticket only
     srv.close()
     assert got[tk_big].zdropped == 1 and got[tk_big].n_cigar == 0
     for tk, (q, t) in tks.items():
@@ -173,3 +174,27 @@ def test_try_submit_pair_above_the_budget_fails_like_submit():
     (got,) = [r for _t, r, _l in srv.poll(wait_ms=-1)]
     srv.close()
     assert got.zdropped == 1
+
+
+def test_failed_flight_is_reported_per_ticket(monkeypatch):
+    """A flight that fails before launch (forced from the host for the first
+    flight) completes its pairs with zdropped = 1 and no CIGAR, and poll
+    reports the failure: per ticket (status EDEVICE, the others OK) and as
+    the server's first failure; later flights are exact."""
+    monkeypatch.setenv("KSW2_HIP_FAIL_FLIGHTS", "1")
+    pairs = random_pairs(94, 30, 200, 1500)
+    srv = ksw2.Server()
+    tks = {srv.submit(q, t): k for k, (q, t) in enumerate(pairs)}
+    got = {}
+    while len(got) < len(pairs):
+        for tk, res, _ in srv.poll(wait_ms=2000):
+            got[tks[tk]] = (res, srv.status[tk])
+    assert srv.last_error == -4
+    srv.close()
+    failed = [k for k, (_r, st) in got.items() if st != 0]
+    assert failed and all(got[k][1] == -4 for k in failed)
+    for k in failed:
+        assert got[k][0].zdropped == 1 and got[k][0].n_cigar == 0
+    for k, (res, st) in got.items():
+        if st == 0:
+            assert same(res, O.run_oracle(*pairs[k])), k

@@ -135,3 +135,15 @@ def test_concurrent_single_calls_from_threads():
         got = list(ex.map(lambda p: ksw2.extd2(p[0], p[1]), pairs))
     for k, (g, e) in enumerate(zip(got, exp)):
         assert same(g, e), f"pair {k}: {diff(g, e)}"
+
+
+def test_wide_give_up_recomputes_the_pair(monkeypatch):
+    """A wide pair whose team gave up a wait is recomputed by its epilogue
+    workgroup as a one-workgroup team (forced here from the host for every
+    wide pair, after its team ran normally): results stay exact and the
+    batch succeeds, for every flag."""
+    monkeypatch.setenv("KSW2_HIP_WIDE", "2")
+    monkeypatch.setenv("KSW2_HIP_WIDE_REDO", "1")
+    pairs = _pairs(611)
+    for flag in FLAGS:
+        _check(pairs, flag=flag)

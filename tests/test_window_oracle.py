@@ -81,3 +81,20 @@ def test_edge_cases():
     assert r.cigar == [] and r.score == 0
     assert WP.subsequence("ACGTA", 4, 9) == "TA" and WP.subsequence("ACGTA", 7, 9) == "A"
     assert WP.subsequence("ACGTA", 3, 1) == "ACG"
+
+
+def test_tied_ends_take_the_target_end():
+    """alignmentMinimap2WithArena continues from the query's end only when
+    mqe > mte (alignSlidingWindow.cpp:392-398).  Worked by hand for the
+    query TCG against the target TTACGGGG with windows of 3: the first
+    window (TCG against TTAC) ties mqe = mte = -12 (query end at target row
+    2, target end at query column 1), so it ends at the target's end (CIGAR
+    1M2D1M, rows T--C / TTAC); the second window (G against GGGG) has mqe 0
+    > mte -14 and ends at the query's end (1M); the target tail GGG is one
+    deletion (-14).  Score -12 + 0 - 14 = -26."""
+    r = WP._call("TCG", "TTAC", -6, -8, -2, -75, -1, WP.SEMIGLOBAL_END)
+    assert (r.mqe, r.mqe_t, r.mte, r.mte_q) == (-12, 2, -12, 1)
+    res = WP.align("TCG", "TTACGGGG", 3)
+    assert (res.status, res.score, res.windows) == (0, -26, 2)
+    assert res.rows == ("T--CG---", "TTACGGGG")
+    assert res.cigar == [1 << 4 | 0, 2 << 4 | 2, 2 << 4 | 0, 3 << 4 | 2]

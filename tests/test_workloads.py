@@ -38,3 +38,19 @@ def test_banded_cells_matches_reference_band():
     assert W.banded_cells(300, 200, -1) == 300 * 200
     assert W.banded_cells(300, 200, 400) == 300 * 200
     assert W.banded_cells(1000, 1000, 10) < 1000 * 25
+
+
+def test_serving_mix_and_uneven_ratio():
+    """Serving tasks: 50/40/10 single/fixed/uneven, Poisson arrivals, and the
+    uneven tasks drawn like the uneven batch, query/target length ratio
+    0.5-2.0 before mutation (so some queries are much shorter than their
+    targets)."""
+    ts = W.serving(seed=5, n=3000, rate_per_s=1000.0)
+    kinds = [t.kind for t in ts]
+    frac = {k: kinds.count(k) / len(ts) for k in ("single", "fixed", "uneven")}
+    assert abs(frac["single"] - 0.5) < 0.04 and abs(frac["fixed"] - 0.4) < 0.04 and abs(frac["uneven"] - 0.1) < 0.03
+    gaps = np.diff([t.arrival_s for t in ts])
+    assert (gaps >= 0).all() and abs(gaps.mean() - 1e-3) < 1e-4
+    ratios = np.array([len(t.query) / len(t.target) for t in ts if t.kind == "uneven"])
+    assert ratios.min() < 0.7 and (ratios < 1.0).mean() > 0.2  # shortened queries occur
+    assert np.median(ratios) > 0.8
