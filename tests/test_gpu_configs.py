@@ -57,7 +57,8 @@ def test_banded_long_pairs_at_scale():
     _compare(pairs, got, exp, "banded")
     exits = [k for k, r in zip(idx, exp) if r.zdropped]
     assert exits == [0, 3, 4], exits
-    assert all(r.n_cigar > 0 for r in exp)
+    # the pairs that stay in the band have a full path
+    assert all(r.n_cigar > 0 for k, r in zip(idx, exp) if k not in exits)
 
 
 def test_uneven_tail_on_planner_teams():
