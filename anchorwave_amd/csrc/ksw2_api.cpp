@@ -125,11 +125,17 @@ const char *ksw2_hip_error_string(int status)
     case KSW2_HIP_EDEVICE: return "HIP runtime error";
     case KSW2_HIP_ENODEV: return "no usable gfx950 device";
     case KSW2_HIP_EAGAIN: return "deferred: device memory committed";
+    case KSW2_HIP_ETOOBIG: return "larger than the no-allocation scratch (KSW2_HIP_NOALLOC_SCRATCH_BYTES)";
     default: return "unknown status";
     }
 }
 
-int ksw2_hip_init(int device) { return set_err(kswhip::device_init(device)); }
+int ksw2_hip_init(int device)
+{
+    int st = kswhip::device_init(device);
+    if (st == KSW2_HIP_OK) st = kswhip::noalloc_prepare();  // the no-allocation entry point's set-up
+    return set_err(st);
+}
 
 int ksw2_hip_route(int qlen, const uint8_t *query, int tlen, const uint8_t *target, int8_t m, const int8_t *mat,
                    int8_t q, int8_t e, int8_t q2, int8_t e2, int w, int zdrop, int flag)
@@ -182,6 +188,50 @@ void ksw_extd2_hip(void *km, int qlen, const uint8_t *query, int tlen, const uin
     const kswhip::Progress prog = current_progress();
     int st = kswhip::run_batch_host(1, &p, m, mat, q, e, q2, e2, ez, km, &prog);
     set_err(st);
+}
+
+int ksw_extd2_hip_noalloc(int qlen, const uint8_t *query, int tlen, const uint8_t *target, int8_t m,
+                          const int8_t *mat, int8_t q, int8_t e, int8_t q2, int8_t e2, int w, int zdrop,
+                          int end_bonus, int flag, ksw_extz_t *ez)
+{
+    if (!ez) return set_err(KSW2_HIP_EINVAL);
+    reset_extz(ez);
+    // the early returns and the length test in run_batch_host's order
+    if (m <= 1) return set_err(KSW2_HIP_OK);
+    const bool overflow = qlen > 0 && tlen > 0 && qlen > 0x7fffffff - tlen + 1;
+    if (overflow) ez->zdropped = 1;
+    if (!mat) return set_err(KSW2_HIP_EINVAL);
+    KswScore sc;
+    ScoreClass cls;
+    if (!normalise_scoring(m, mat, q, e, q2, e2, &sc, &cls)) return set_err(KSW2_HIP_OK);
+    if (m > kMaxM) {
+        ez->zdropped = 1;
+        return set_err(KSW2_HIP_EUNSUPPORTED);
+    }
+    if (qlen <= 0 || tlen <= 0 || overflow) return set_err(KSW2_HIP_OK);
+    if (!query || !target) return set_err(KSW2_HIP_EINVAL);
+    if (!(flag & KSW_EZ_SCORE_ONLY) && (!ez->cigar || (int64_t)ez->m_cigar < (int64_t)qlen + tlen))
+        return set_err(KSW2_HIP_EINVAL);  // the caller's CIGAR buffer is never grown
+    ksw2_pair_t p;
+    p.query = query, p.target = target, p.qlen = qlen, p.tlen = tlen;
+    p.w = w, p.zdrop = zdrop, p.end_bonus = end_bonus, p.flag = flag;
+    if (!kswhip::noalloc_fits(p, m, cls)) {  // rejected before any device work
+        ez->zdropped = 1;
+        return set_err(KSW2_HIP_ETOOBIG);
+    }
+    int st = kswhip::device_init(-1);
+    if (st != KSW2_HIP_OK) {
+        ez->zdropped = 1;
+        return set_err(st);
+    }
+    GenScore gsc;
+    kswhip::make_gen_score(m, mat, sc, cls, &gsc);
+    st = kswhip::noalloc_run(p, m, cls, gsc, ez);
+    if (st != KSW2_HIP_OK) {
+        reset_extz(ez);
+        ez->zdropped = 1;  // the device path did not run
+    }
+    return set_err(st);
 }
 
 void ksw_extd2_singletrack_hip(void *km, int qlen, const uint8_t *query, int tlen, const uint8_t *target,

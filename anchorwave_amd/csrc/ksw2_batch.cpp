@@ -112,6 +112,23 @@ bool wait_slots_release(hipStream_t st, const int q[2])
 }
 }  // namespace
 
+bool wait_slots_prepare()
+{
+    std::lock_guard<std::mutex> lk(g_wait_mu);
+    for (hipEvent_t &ev : g_quarter)
+        if (!ev && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            ev = nullptr;
+            return false;
+        }
+    return true;
+}
+
+int library_device()
+{
+    std::lock_guard<std::mutex> lk(g_init_mu);
+    return g_device;
+}
+
 int device_init(int device)
 {
     std::lock_guard<std::mutex> lk(g_init_mu);
@@ -482,8 +499,13 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         p->own_streams = true;  // lane streams are created on first use (plan_launch)
         if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) return fail(KSW2_HIP_EDEVICE);
     }
-    for (auto &e : p->ev)
-        if (hipEventCreate(&e) != hipSuccess) return fail(KSW2_HIP_EDEVICE);
+    if (env && env->events) {
+        p->own_events = false;
+        for (int k = 0; k < 4; ++k) p->ev[k] = env->events[k];
+    } else {
+        for (auto &e : p->ev)
+            if (hipEventCreate(&e) != hipSuccess) return fail(KSW2_HIP_EDEVICE);
+    }
     // one device block holds every arena
     uint64_t at = 0;
     auto take = [&](uint64_t bytes) {
@@ -541,28 +563,51 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         p->d_mout = env->mirror_out_dev, p->d_mcig = env->mirror_cig_dev, p->d_mflag = env->mirror_flag_dev;
         memset(p->h_mflag, 0, sizeof(int32_t) * p->n_fast);
     }
-    // stage sequences through pinned memory (kept until the plan is
-    // destroyed when the copy is left in flight on the caller's stream)
-    uint8_t *h_seq = nullptr;
-    if (env && env->staging && env->staging_bytes >= p->seq_bytes) {
-        h_seq = env->staging, p->own_seq = false;
-    } else if (hipHostMalloc((void **)&h_seq, p->seq_bytes, hipHostMallocDefault) != hipSuccess) {
-        return fail(KSW2_HIP_ENOMEM);
-    }
-    p->h_seq = h_seq;
-    memset(h_seq, 0, p->seq_bytes);
-    for (int k = 0; k < p->n_fast; ++k) {
-        const ksw2_pair_t &pp = pairs[p->order[k]];
-        memcpy(h_seq + p->jobs[k].q_off, pp.query, (size_t)pp.qlen);
-        memcpy(h_seq + p->jobs[k].t_off, pp.target, (size_t)pp.tlen);
-    }
-    for (int gk = 0; gk < p->n_gen; ++gk) {
-        const ksw2_pair_t &pp = pairs[p->order[p->n_fast + gk]];
-        memcpy(h_seq + p->gjobs[gk].q_off, pp.query, (size_t)pp.qlen);
-        memcpy(h_seq + p->gjobs[gk].t_off, pp.target, (size_t)pp.tlen);
-    }
     hipError_t e[6] = {hipSuccess, hipSuccess, hipSuccess, hipSuccess, hipSuccess, hipSuccess};
-    e[0] = hipMemcpyAsync(p->d_seq, h_seq, p->seq_bytes, hipMemcpyHostToDevice, p->stream);
+    if (env && env->direct_upload) {
+        // straight from the caller's buffers; the arena's padding is zero
+        // as with the staged copy
+        p->own_seq = false;
+        e[0] = launch_fill_zero(p->d_seq, p->seq_bytes, p->stream);
+        for (int k = 0; k < p->n_fast && e[0] == hipSuccess; ++k) {
+            const ksw2_pair_t &pp = pairs[p->order[k]];
+            e[0] = hipMemcpyAsync(p->d_seq + p->jobs[k].q_off, pp.query, (size_t)pp.qlen, hipMemcpyHostToDevice,
+                                  p->stream);
+            if (e[0] == hipSuccess)
+                e[0] = hipMemcpyAsync(p->d_seq + p->jobs[k].t_off, pp.target, (size_t)pp.tlen,
+                                      hipMemcpyHostToDevice, p->stream);
+        }
+        for (int gk = 0; gk < p->n_gen && e[0] == hipSuccess; ++gk) {
+            const ksw2_pair_t &pp = pairs[p->order[p->n_fast + gk]];
+            e[0] = hipMemcpyAsync(p->d_seq + p->gjobs[gk].q_off, pp.query, (size_t)pp.qlen, hipMemcpyHostToDevice,
+                                  p->stream);
+            if (e[0] == hipSuccess)
+                e[0] = hipMemcpyAsync(p->d_seq + p->gjobs[gk].t_off, pp.target, (size_t)pp.tlen,
+                                      hipMemcpyHostToDevice, p->stream);
+        }
+    } else {
+        // stage sequences through pinned memory (kept until the plan is
+        // destroyed when the copy is left in flight on the caller's stream)
+        uint8_t *h_seq = nullptr;
+        if (env && env->staging && env->staging_bytes >= p->seq_bytes) {
+            h_seq = env->staging, p->own_seq = false;
+        } else if (hipHostMalloc((void **)&h_seq, p->seq_bytes, hipHostMallocDefault) != hipSuccess) {
+            return fail(KSW2_HIP_ENOMEM);
+        }
+        p->h_seq = h_seq;
+        memset(h_seq, 0, p->seq_bytes);
+        for (int k = 0; k < p->n_fast; ++k) {
+            const ksw2_pair_t &pp = pairs[p->order[k]];
+            memcpy(h_seq + p->jobs[k].q_off, pp.query, (size_t)pp.qlen);
+            memcpy(h_seq + p->jobs[k].t_off, pp.target, (size_t)pp.tlen);
+        }
+        for (int gk = 0; gk < p->n_gen; ++gk) {
+            const ksw2_pair_t &pp = pairs[p->order[p->n_fast + gk]];
+            memcpy(h_seq + p->gjobs[gk].q_off, pp.query, (size_t)pp.qlen);
+            memcpy(h_seq + p->gjobs[gk].t_off, pp.target, (size_t)pp.tlen);
+        }
+        e[0] = hipMemcpyAsync(p->d_seq, h_seq, p->seq_bytes, hipMemcpyHostToDevice, p->stream);
+    }
     if (p->n_fast > 0)
         e[1] = hipMemcpyAsync(p->d_jobs, p->jobs.data(), sizeof(KswJob) * p->n_fast, hipMemcpyHostToDevice, p->stream);
     if (!order.empty())
@@ -579,7 +624,7 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
                               hipMemcpyHostToDevice, p->stream);
     if (p->own_streams) {
         e[5] = hipStreamSynchronize(p->stream);
-        if (p->own_seq) (void)hipHostFree(h_seq);
+        if (p->own_seq && p->h_seq) (void)hipHostFree(p->h_seq);
         p->h_seq = nullptr;
     }
     for (hipError_t x : e)
@@ -836,9 +881,9 @@ void plan_destroy(Plan *p)
     if (p->block) (void)hipFree(p->block);
     if (p->h_stop && p->own_stop) (void)hipHostFree(p->h_stop);
     if (p->h_seq && p->own_seq) (void)hipHostFree(p->h_seq);
-    for (auto &e : p->ev)
-        if (e) (void)hipEventDestroy(e);
-
+    if (p->own_events)
+        for (auto &e : p->ev)
+            if (e) (void)hipEventDestroy(e);
     if (p->own_streams) {
         for (int l = 0; l < Plan::kLanes; ++l) {
             if (p->lane_done[l]) (void)hipEventDestroy(p->lane_done[l]);

@@ -139,4 +139,8 @@ hipError_t clear_wide_err(int32_t *werr, hipStream_t st);
 // Waves of the stripe kernel that fit on the device at once.
 int stripe_resident_waves(int group);
 
+// Zero-fills `bytes` bytes at `p` (16-byte aligned) with a kernel on `st`
+// (a fill kernel, not a memset, so that a captured graph replays it).
+hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t st);
+
 }  // namespace kswhip

@@ -96,6 +96,12 @@ struct PlanEnv {
     int32_t *mirror_flag = nullptr, *mirror_flag_dev = nullptr;
     int team = 0;                    // > 0: every stripe-kernel pair gets this team size
     bool spread = false;             // 5-16 wave teams may run spread (as in a plan of its own)
+    // The plan's four timing events, created by the caller (the plan then
+    // creates none and destroys none); nullptr: the plan creates its own.
+    hipEvent_t *events = nullptr;
+    // Sequences go straight from the caller's buffers to the device (no
+    // pinned staging); the seq arena is zero-filled by a kernel first.
+    bool direct_upload = false;
 };
 
 // A device-resident batch: sequences uploaded, arenas sized, jobs built.
@@ -168,6 +174,7 @@ struct Plan {
     std::vector<int32_t> h_wfirst;   // per wide launch: each pair's first workgroup, then the end
     int32_t *d_wfirst = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool own_events = true;          // false: ev[] belong to the caller (PlanEnv::events)
     // launches of different kernel groups overlap on their own streams
     static constexpr int kLanes = 4;
     hipStream_t lane_stream[kLanes] = {nullptr, nullptr, nullptr, nullptr};
@@ -180,6 +187,20 @@ struct Plan {
 
 int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, int8_t m, const ScoreClass &cls,
                 const GenScore &gsc, const PlanEnv *env = nullptr);
+// The device the library runs on (device_init), -1 before it.
+int library_device();
+// Creates the process-wide events of the waiting-launch slots now, so that
+// later launches create none.  Returns false when a HIP call fails.
+bool wait_slots_prepare();
+// Set-up of the no-allocation entry point on the library's device (address
+// of its scratch array, its stream and events); done by ksw2_hip_init and
+// otherwise by the entry point's first call.  Returns a status code.
+int noalloc_prepare();
+// One pair through the no-allocation path (the C entry point has checked
+// the arguments); the CIGAR goes into ez->cigar.  Returns a status code.
+// Host only: whether a pair's plan fits the no-allocation scratch.
+bool noalloc_fits(const ksw2_pair_t &pair, int8_t m, const ScoreClass &cls);
+int noalloc_run(const ksw2_pair_t &pair, int8_t m, const ScoreClass &cls, const GenScore &gsc, ksw_extz_t *ez);
 // launch (asynchronous) / completed? / wait (polls the progress callback) / both
 int plan_launch(Plan *p, const Progress *prog = nullptr);
 bool plan_done(Plan *p);

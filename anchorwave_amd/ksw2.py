@@ -32,14 +32,18 @@ KSW_EZ_SEMIGLOBAL_END = 0x800
 KSW_EZ_SINGLETRACK = 0x1000  # two-track (Singletrack) traceback
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "unsupported", -3: "out of memory",
-          -4: "HIP runtime error", -5: "no usable gfx950 device", -6: "deferred: device memory committed"}
+          -4: "HIP runtime error", -5: "no usable gfx950 device", -6: "deferred: device memory committed",
+          -7: "larger than the no-allocation scratch"}
 EAGAIN = -6
+ETOOBIG = -7
+NOALLOC_SCRATCH_BYTES = 64 << 20  # KSW2_HIP_NOALLOC_SCRATCH_BYTES
 
 # Symbols include/ksw2_hip.h declares; tests check the library exports each.
 EXPORTS = (
     "ksw_extd2_hip",
     "ksw_extd2_singletrack_hip",
     "ksw_extd2_hip_batch",
+    "ksw_extd2_hip_noalloc",
     "ksw2_hip_release_cached_memory",
     "ksw2_hip_last_error",
     "ksw2_hip_error_string",
@@ -152,6 +156,8 @@ def lib():
                                     C.c_int, C.c_int, C.c_int, C.POINTER(KswExtz)]
         L.ksw_extd2_singletrack_hip.argtypes = L.ksw_extd2_hip.argtypes
         L.ksw_extd2_hip.restype = None
+        L.ksw_extd2_hip_noalloc.argtypes = L.ksw_extd2_hip.argtypes[1:]
+        L.ksw_extd2_hip_noalloc.restype = C.c_int
         L.ksw_extd2_hip_batch.argtypes = [C.c_int, C.POINTER(KswPair), C.c_int8, C.c_void_p, C.c_int8,
                                           C.c_int8, C.c_int8, C.c_int8, C.POINTER(KswExtz)]
         L.ksw_extd2_hip_batch.restype = C.c_int
@@ -274,6 +280,26 @@ def extd2(query, target, mat=None, m=5, q=8, e=2, q2=75, e2=1, w=-1, zdrop=-1, e
     if check and st != 0:
         raise KswError(f"ksw_extd2_hip: {STATUS.get(st, st)}")
     return res
+
+
+def extd2_noalloc(query, target, mat=None, m=5, q=8, e=2, q2=75, e2=1, w=-1, zdrop=-1, end_bonus=0,
+                  flag=0, cigar_capacity=None):
+    """One call of ``ksw_extd2_hip_noalloc``: ``extd2`` with every device
+    buffer in the library's fixed scratch array and the CIGAR written into a
+    caller buffer of ``cigar_capacity`` words (default qlen + tlen).
+    Returns ``(status, Result)``."""
+    L = lib()
+    qs, ts = _u8(query), _u8(target)
+    mat = np.ascontiguousarray(np.asarray(default_matrix() if mat is None else mat, dtype=np.int8))
+    cap = len(qs) + len(ts) if cigar_capacity is None else cigar_capacity
+    buf = (C.c_uint32 * max(cap, 1))()
+    ez = KswExtz()
+    ez.cigar = C.cast(buf, C.POINTER(C.c_uint32))
+    ez.m_cigar = cap
+    st = L.ksw_extd2_hip_noalloc(len(qs), qs.ctypes.data, len(ts), ts.ctypes.data, m, mat.ctypes.data,
+                                 q, e, q2, e2, w, zdrop, end_bonus, flag, C.byref(ez))
+    assert C.addressof(ez.cigar.contents) == C.addressof(buf) and ez.m_cigar == cap, "CIGAR buffer replaced"
+    return st, Result.from_ez(ez)
 
 
 def extd2_singletrack(query, target, mat=None, m=5, q=8, e=2, q2=75, e2=1, w=-1, zdrop=-1, end_bonus=0,

@@ -69,6 +69,7 @@ extern "C" {
 #define KSW2_HIP_EDEVICE         -4  /* a HIP runtime call failed */
 #define KSW2_HIP_ENODEV          -5  /* no usable gfx950 device */
 #define KSW2_HIP_EAGAIN          -6  /* deferred: the server's device memory is committed, retry later */
+#define KSW2_HIP_ETOOBIG         -7  /* larger than the no-allocation entry point's fixed scratch */
 
 /*
  * One alignment, same contract as ksw_extd2_sse (minimap2/ksw2.h:91-107,
@@ -101,6 +102,36 @@ void ksw_extd2_hip(void *km, int qlen, const uint8_t *query, int tlen, const uin
 void ksw_extd2_singletrack_hip(void *km, int qlen, const uint8_t *query, int tlen, const uint8_t *target,
                                int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q2, int8_t e2,
                                int w, int zdrop, int end_bonus, int flag, ksw_extz_t *ez);
+
+/*
+ * No-allocation variant of ksw_extd2_hip.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and tie-breaking as ksw_extd2_hip(NULL, ...), except:
+ *   - every device buffer of the call (direction bytes, column buffers,
+ *     state, CIGAR slots, job tables) lives in a fixed scratch array of the
+ *     library, KSW2_HIP_NOALLOC_SCRATCH_BYTES on each device it runs on,
+ *     held until the process ends; the call allocates no device, pinned or
+ *     managed memory and creates no stream or event (its set-up, done by
+ *     ksw2_hip_init or else by its first call on a device, creates one
+ *     stream and four events);
+ *   - a call that needs more than the scratch is rejected before any device
+ *     work: KSW2_HIP_ETOOBIG, ez->zdropped = 1;
+ *   - the CIGAR is written to the caller's ez->cigar, which must hold at
+ *     least qlen + tlen words (ez->m_cigar), else KSW2_HIP_EINVAL; it is
+ *     never grown, freed or replaced (a Singletrack walk that finds no path
+ *     leaves n_cigar = 0 with zdropped = 1);
+ *   - no progress callback is polled;
+ *   - calls on one device share its scratch, which keeps the last call's
+ *     contents (each call clears what it needs zeroed): the library
+ *     serialises them with a lock, since calls that overlapped on the array
+ *     would race.
+ * Returns the status (also in ksw2_hip_last_error()).
+ */
+#define KSW2_HIP_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int ksw_extd2_hip_noalloc(int qlen, const uint8_t *query, int tlen, const uint8_t *target,
+                          int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q2, int8_t e2,
+                          int w, int zdrop, int end_bonus, int flag, ksw_extz_t *ez);
 
 /* Status of the last entry point called on this host thread, and its text. */
 int ksw2_hip_last_error(void);

@@ -102,3 +102,20 @@ def test_length_overflow_is_tested_before_the_scoring():
     lib.ksw_extd2_hip(None, 16, buf.ctypes.data, 16, buf.ctypes.data, 5, mat.ctypes.data,
                       8, 2, 75, 1, -1, -1, 0, 0, C.byref(ez))
     assert ez.zdropped == 0 and ez.score == ksw2.KSW_NEG_INF  # early return, no overflow
+
+
+def test_noalloc_rejections_need_no_device():
+    """ksw_extd2_hip_noalloc refuses, before any device work, a CIGAR buffer
+    smaller than qlen + tlen words (it never grows the caller's buffer) and a
+    call larger than its fixed scratch; early returns reset ez as
+    ksw_extd2_hip does."""
+    rng = np.random.default_rng(3)
+    t = rng.integers(0, 4, 12000).astype(np.uint8)
+    st, r = ksw2.extd2_noalloc(t[:600], t[:600], cigar_capacity=1199)
+    assert st == -1 and r.n_cigar == 0
+    st, r = ksw2.extd2_noalloc(t, t)  # ~150 MB of direction bytes > 64 MiB
+    assert st == ksw2.ETOOBIG and r.zdropped == 1 and r.n_cigar == 0
+    st, r = ksw2.extd2_noalloc(t[:0], t)  # qlen == 0: the reference returns at once
+    assert st == 0 and r.score == ksw2.KSW_NEG_INF and r.zdropped == 0
+    st, r = ksw2.extd2_noalloc(t[:10], t[:10], flag=ksw2.KSW_EZ_SCORE_ONLY, cigar_capacity=0, m=1)
+    assert st == 0 and r.score == ksw2.KSW_NEG_INF
