@@ -297,20 +297,22 @@ int ksw2_hip_plan_timing(const ksw2_hip_plan *plan, float *dp_ms, float *backtra
     return set_err(KSW2_HIP_OK);
 }
 
-int ksw2_hip_plan_dirs(const ksw2_hip_plan *plan, int index, uint8_t *out, size_t bytes, int *R)
+int ksw2_hip_plan_dirs(const ksw2_hip_plan *plan, int index, uint8_t *out, size_t bytes, int *R, int *transposed)
 {
-    if (!plan || !out || index < 0 || index >= plan->p->n) return set_err(KSW2_HIP_EINVAL);
+    if (!plan || index < 0 || index >= plan->p->n) return set_err(KSW2_HIP_EINVAL);
     const kswhip::Plan *p = plan->p;
     int k = -1;
     for (int x = 0; x < p->n; ++x)
         if (p->order[x] == index) k = x;
     if (k < 0 || k >= p->n_fast) return set_err(KSW2_HIP_EINVAL);  // general-kernel pair: other layout
     const kswhip::KswJob &j = p->jobs[k];
+    if (R) *R = j.R;
+    if (transposed) *transposed = j.transposed;
+    if (!out) return set_err(KSW2_HIP_OK);  // the geometry only
     const uint64_t need = (track_pair(j.flag) ? 2 : 1) * kswhip::dir_bytes(j.qlen, j.tlen);
     if (bytes < need || (j.flag & KSW_EZ_SCORE_ONLY)) return set_err(KSW2_HIP_EINVAL);
     if (hipMemcpy(out, p->d_dir + j.dir_off, need, hipMemcpyDeviceToHost) != hipSuccess)
         return set_err(KSW2_HIP_EDEVICE);
-    if (R) *R = j.R;
     return set_err(KSW2_HIP_OK);
 }
 

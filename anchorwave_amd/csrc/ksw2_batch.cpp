@@ -190,10 +190,33 @@ static int choose_team(const KswJob &j, double T, int forced)
 }
 
 
+// The stripe kernel's byte-table score (one v_perm gives two cells' score
+// candidates) needs every code below 4 and both candidates, 8|mis| plus a
+// tag of at most 7, in a byte.  KSW2_HIP_LUT=0 turns it off (tests).
+static bool lut_ok(const ksw2_pair_t &p, const KswScore &sc)
+{
+    const char *ev = getenv("KSW2_HIP_LUT");
+    if ((ev && atoi(ev) == 0) || sc.mch != 0 || -8 * sc.mis + 7 > 255) return false;
+    uint8_t acc = 0;
+    for (int i = 0; i < p.qlen; ++i) acc |= p.query[i];
+    for (int i = 0; i < p.tlen; ++i) acc |= p.target[i];
+    return (acc & 0xfc) == 0;
+}
+
 bool takes_general(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls)
 {
     const char *ev = getenv("KSW2_HIP_FORCE_GENERAL");
     return (ev && atoi(ev) > 0) || fast_path_reject(p, m, cls) != kFastOK;
+}
+
+StripeGeom stripe_geom(const ksw2_pair_t &p)
+{
+    bool tr = pass_count(p.qlen) > pass_count(p.tlen);
+    if (const char *ev = getenv("KSW2_HIP_TRANSPOSE")) {
+        if (ev[0] == '0') tr = false;
+        else if (ev[0] == '1') tr = true;
+    }
+    return StripeGeom{tr, tr ? p.tlen : p.qlen, tr ? p.qlen : p.tlen};
 }
 
 static int norm_w(const ksw2_pair_t &p) { return p.w < 0 ? std::max(p.qlen, p.tlen) : p.w; }
@@ -203,9 +226,11 @@ uint64_t pair_footprint(const ksw2_pair_t &p, bool general)
     const uint64_t cig = 4ull * ((uint64_t)p.qlen + p.tlen + 2) + 64, seq = (uint64_t)p.qlen + p.tlen + 32;
     const bool cigar = !(p.flag & KSW_EZ_SCORE_ONLY);
     const uint64_t tracks = track_pair(p) ? 2 : 1;  // Singletrack: two bytes per cell
-    if (!general)
-        return (cigar ? tracks * dir_bytes(p.qlen, p.tlen) : 0) + 8ull * p.qlen + cig + seq +
-               16ull * stripe_count(p.tlen) + 4ull * kWideCtl + 512;
+    if (!general) {
+        const StripeGeom g = stripe_geom(p);
+        return (cigar ? tracks * dir_bytes(g.kq, g.kt) : 0) + 8ull * g.kq + cig + seq +
+               16ull * stripe_count(g.kt) + 4ull * kWideCtl + 512;
+    }
     const int tlenB = (p.tlen + 63) / 64 * 64, qlenB = (p.qlen + 63) / 64 * 64;
     const uint64_t diags = (uint64_t)p.qlen + p.tlen - 1;
     const uint64_t rows = cigar ? tracks * diags * (uint64_t)gen_ncolB(p.qlen, p.tlen, norm_w(p)) + 256 : 0;
@@ -319,16 +344,20 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
     for (int k = 0; k < p->n_fast; ++k) {
         const ksw2_pair_t &pp = pairs[p->order[k]];
         KswJob &j = p->jobs[k];
-        j.qlen = pp.qlen, j.tlen = pp.tlen, j.flag = pp.flag, j.end_bonus = pp.end_bonus;
+        // the kernel's lengths: swept (qlen) and striped (tlen)
+        const StripeGeom g = stripe_geom(pp);
+        j.transposed = g.tr ? 1 : 0;
+        j.qlen = g.kq, j.tlen = g.kt, j.flag = pp.flag, j.end_bonus = pp.end_bonus;
         j.R = R;
-        j.q_off = so, so = round_up(so + pp.qlen, 16);
-        j.t_off = so, so = round_up(so + pp.tlen, 16);
+        j.lut = lut_ok(pp, gsc.k) ? 1 : 0;
+        j.q_off = so, so = round_up(so + g.kq, 16);
+        j.t_off = so, so = round_up(so + g.kt, 16);
         j.dir_off = dof;
         if (!(pp.flag & KSW_EZ_SCORE_ONLY))
-            dof = round_up(dof + (track_pair(pp) ? 2 : 1) * dir_bytes(pp.qlen, pp.tlen), 256);
-        j.bnd_off = bo, bo += round_up(2ull * pp.qlen, 64);
+            dof = round_up(dof + (track_pair(pp) ? 2 : 1) * dir_bytes(g.kq, g.kt), 256);
+        j.bnd_off = bo, bo += round_up(2ull * g.kq, 64);
         j.cig_off = co, co += round_up((uint64_t)pp.qlen + pp.tlen + 2, 16);
-        j.stripe_off = stripes, stripes += (pp.tlen + SR - 1) / SR;
+        j.stripe_off = stripes, stripes += (g.kt + SR - 1) / SR;
         p->cig_off[k] = j.cig_off;
     }
     p->nstripes_total = stripes;
@@ -571,10 +600,11 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         e[0] = launch_fill_zero(p->d_seq, p->seq_bytes, p->stream);
         for (int k = 0; k < p->n_fast && e[0] == hipSuccess; ++k) {
             const ksw2_pair_t &pp = pairs[p->order[k]];
-            e[0] = hipMemcpyAsync(p->d_seq + p->jobs[k].q_off, pp.query, (size_t)pp.qlen, hipMemcpyHostToDevice,
-                                  p->stream);
+            const KswJob &j = p->jobs[k];
+            e[0] = hipMemcpyAsync(p->d_seq + j.q_off, j.transposed ? pp.target : pp.query, (size_t)j.qlen,
+                                  hipMemcpyHostToDevice, p->stream);
             if (e[0] == hipSuccess)
-                e[0] = hipMemcpyAsync(p->d_seq + p->jobs[k].t_off, pp.target, (size_t)pp.tlen,
+                e[0] = hipMemcpyAsync(p->d_seq + j.t_off, j.transposed ? pp.query : pp.target, (size_t)j.tlen,
                                       hipMemcpyHostToDevice, p->stream);
         }
         for (int gk = 0; gk < p->n_gen && e[0] == hipSuccess; ++gk) {
@@ -598,8 +628,9 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         memset(h_seq, 0, p->seq_bytes);
         for (int k = 0; k < p->n_fast; ++k) {
             const ksw2_pair_t &pp = pairs[p->order[k]];
-            memcpy(h_seq + p->jobs[k].q_off, pp.query, (size_t)pp.qlen);
-            memcpy(h_seq + p->jobs[k].t_off, pp.target, (size_t)pp.tlen);
+            const KswJob &j = p->jobs[k];
+            memcpy(h_seq + j.q_off, j.transposed ? pp.target : pp.query, (size_t)j.qlen);
+            memcpy(h_seq + j.t_off, j.transposed ? pp.query : pp.target, (size_t)j.tlen);
         }
         for (int gk = 0; gk < p->n_gen; ++gk) {
             const ksw2_pair_t &pp = pairs[p->order[p->n_fast + gk]];

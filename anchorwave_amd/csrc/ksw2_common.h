@@ -43,6 +43,10 @@ struct KswJob {
     int32_t stripe_off;        // first slot of this pair in the per-stripe arrays
     int32_t R;                 // rows per lane of the stripe kernel that ran it
     int32_t team;              // waves of this pair's team (a launch's workgroups may be larger)
+    int32_t lut;               // 1: codes below 4, score candidates from a per-step byte table
+    // 1: the reference's query is in the stripes and its target is swept
+    // (q_off/qlen then describe the reference's target, t_off/tlen its query)
+    int32_t transposed;
     int32_t pad;
 };
 

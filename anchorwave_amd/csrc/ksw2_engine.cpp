@@ -337,10 +337,11 @@ int Engine::submit(const ksw2_pair_t &p, ksw_extz_t *ez, bool copy, uint64_t *ti
             t->waves = gen_threads(p) / kWave;
             t->span = (double)p.qlen + p.tlen - 1;
         } else {
-            t->passes = pass_count(p.tlen);
-            t->steps = (double)t->passes * (double)pass_steps(p.qlen);
+            const StripeGeom g = stripe_geom(p);
+            t->passes = pass_count(g.kt);
+            t->steps = (double)t->passes * (double)pass_steps(g.kq);
             const int team = std::min(t->passes, kMaxWide);
-            t->span = (double)((t->passes + team - 1) / team) * (double)pass_steps(p.qlen) + (team - 1) * 2.0 * kWave;
+            t->span = (double)((t->passes + team - 1) / team) * (double)pass_steps(g.kq) + (team - 1) * 2.0 * kWave;
         }
     }
     std::lock_guard<std::mutex> lk(mu_);

@@ -52,6 +52,16 @@ int fast_path_reject(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls);
 // True when the pair runs on the general kernel (not fast-path eligible, or
 // KSW2_HIP_FORCE_GENERAL set).
 bool takes_general(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls);
+// Orientation of a stripe-kernel pair: the reference's query is put in the
+// stripes and its target swept (transposed) when that gives the pair more
+// passes, so a long query no longer sets a long sweep; kq and kt are the
+// swept and striped lengths.  KSW2_HIP_TRANSPOSE=0 / 1 forces either
+// orientation (tests).
+struct StripeGeom {
+    bool tr;
+    int kq, kt;
+};
+StripeGeom stripe_geom(const ksw2_pair_t &p);
 // Device bytes a pair needs while its plan is alive.
 uint64_t pair_footprint(const ksw2_pair_t &p, bool general);
 // Threads of the general kernel's workgroup for one pair (64..1024).

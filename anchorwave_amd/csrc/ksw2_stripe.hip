@@ -80,6 +80,17 @@ __device__ __forceinline__ v2s vmin(v2s a, v2s b) { return __builtin_elementwise
 __device__ __forceinline__ int lo16(int x) { return (int)(int16_t)(x & 0xffff); }
 __device__ __forceinline__ int hi16(int x) { return x >> 16; }
 
+// (a & m) | (b & ~m): one v_bfi_b32
+__device__ __forceinline__ int bfi(int m, int a, int b) { return (a & m) | (b & ~m); }
+
+// (a & m) | b as one v_and_or_b32
+__device__ __forceinline__ int and_or(int a, int m, int b)
+{
+    int r;
+    asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(m), "v"(b));
+    return r;
+}
+
 __device__ __forceinline__ int dpp_shl1(int old, int src)
 {
     return __builtin_amdgcn_update_dpp(old, src, kDppWaveShl1, 0xF, 0xF, false);
@@ -109,25 +120,41 @@ template <bool RIGHT> struct Tags {
 //           Y, Y2 likewise; BIAS = 0 (left) or 8 (right)
 //   cands   A = (X+V) + CA = -8a + tagA, ..., S = -8s + tagS
 //   update  T = (X+V) + 8e - (Z & ~7) = -8(a-z+q) - BIAS, X' = min(T, -BIAS)
+// A transposed pair (query in the stripes, target swept) has the cell above
+// and the cell to the left exchanged: its A and A2 candidates are the
+// reference's b and b2 (insertions), its B and B2 the reference's a and a2.
+// They take those candidates' tags and direction-byte flag bits, so the min
+// chain keeps the reference's tie order and the byte is the reference's byte.
 struct Consts {
     v2s CA, CB, CA2, CB2, E1, E2, LIM, SM, SD;
+    v2u shA, shB, shA2, shB2;  // logical shift taking a T's sign (bit 15) to its flag bit
+    int mA, mB, mA2, mB2;      // that flag bit in both halves
 };
 
 template <bool RIGHT>
-__device__ __forceinline__ Consts make_consts(const KswScore &sc)
+__device__ __forceinline__ Consts make_consts(const KswScore &sc, bool transposed)
 {
     using T = Tags<RIGHT>;
     const int bias = RIGHT ? 8 : 0;
+    const int tA = transposed ? T::B : T::A, tB = transposed ? T::A : T::B;
+    const int tA2 = transposed ? T::B2 : T::A2, tB2 = transposed ? T::A2 : T::B2;
+    // flag bits of the reference's byte: a 3, b 4, a2 5, b2 6
+    const int fA = transposed ? 4 : 3, fB = transposed ? 3 : 4, fA2 = transposed ? 6 : 5, fB2 = transposed ? 5 : 6;
     Consts k;
-    k.CA = splat(8 * sc.G1 + T::A + bias);
-    k.CB = splat(8 * sc.G1 + T::B + bias);
-    k.CA2 = splat(8 * sc.G2 + T::A2 + bias);
-    k.CB2 = splat(8 * sc.G2 + T::B2 + bias);
+    k.CA = splat(8 * sc.G1 + tA + bias);
+    k.CB = splat(8 * sc.G1 + tB + bias);
+    k.CA2 = splat(8 * sc.G2 + tA2 + bias);
+    k.CB2 = splat(8 * sc.G2 + tB2 + bias);
     k.E1 = splat(8 * sc.e);
     k.E2 = splat(8 * sc.e2);
     k.LIM = splat(-bias);
     k.SM = splat(-8 * sc.mch + T::S);
     k.SD = splat(-8 * (sc.mis - sc.mch));
+    k.shA = (v2u){(unsigned short)(15 - fA), (unsigned short)(15 - fA)};
+    k.shB = (v2u){(unsigned short)(15 - fB), (unsigned short)(15 - fB)};
+    k.shA2 = (v2u){(unsigned short)(15 - fA2), (unsigned short)(15 - fA2)};
+    k.shB2 = (v2u){(unsigned short)(15 - fB2), (unsigned short)(15 - fB2)};
+    k.mA = 0x10001 << fA, k.mB = 0x10001 << fB, k.mA2 = 0x10001 << fA2, k.mB2 = 0x10001 << fB2;
     return k;
 }
 
@@ -137,14 +164,8 @@ __device__ __forceinline__ Consts make_consts(const KswScore &sc)
 // codes.  Returns the two direction bytes in the low byte of each half (0
 // when SCORE_ONLY).
 template <bool RIGHT, bool SCORE_ONLY>
-__device__ __forceinline__ int cell2(const Consts &k, v2u tb, v2u qb, v2s &X, v2s &X2, v2s &V, v2s &U,
-                                     v2s &Y, v2s &Y2)
+__device__ __forceinline__ int cell2(const Consts &k, v2s S, v2s &X, v2s &X2, v2s &V, v2s &U, v2s &Y, v2s &Y2)
 {
-    v2u mm = __builtin_elementwise_min(tb ^ qb, (v2u){1, 1});  // 0 match, 1 mismatch
-    // keep the multiply-add form (one v_pk_mad) rather than a per-half
-    // compare-and-select the optimiser would otherwise derive from mm <= 1
-    asm volatile("" : "+v"(mm));
-    const v2s S = k.SM + __builtin_bit_cast(v2s, mm) * k.SD;
     const v2s XV = X + V, X2V = X2 + V, YU = Y + U, Y2U = Y2 + U;
     const v2s A = XV + k.CA, B = YU + k.CB, A2 = X2V + k.CA2, B2 = Y2U + k.CB2;
     const v2s Z = vmin(vmin(vmin(vmin(S, A), B), A2), B2);
@@ -155,13 +176,23 @@ __device__ __forceinline__ int cell2(const Consts &k, v2u tb, v2u qb, v2s &X, v2
     X = vmin(Ta, k.LIM), Y = vmin(Tb, k.LIM), X2 = vmin(Ta2, k.LIM), Y2 = vmin(Tb2, k.LIM);
     U = Un, V = Vn;
     if (SCORE_ONLY) return 0;
-    // state index (tags) and the four continuation flags (sign bits of T)
+    // state index (tags) and the four continuation flags (sign bits of T,
+    // each shifted to its bit of the byte)
     int d = RIGHT ? (~bits(Z) & 0x00070007) : (bits(Z) & 0x00070007);
-    d |= bits(__builtin_bit_cast(v2s, __builtin_bit_cast(v2u, Ta) >> (v2u){12, 12})) & 0x00080008;
-    d |= bits(__builtin_bit_cast(v2s, __builtin_bit_cast(v2u, Tb) >> (v2u){11, 11})) & 0x00100010;
-    d |= bits(__builtin_bit_cast(v2s, __builtin_bit_cast(v2u, Ta2) >> (v2u){10, 10})) & 0x00200020;
-    d |= bits(__builtin_bit_cast(v2s, __builtin_bit_cast(v2u, Tb2) >> (v2u){9, 9})) & 0x00400040;
+    d = and_or(bits(__builtin_bit_cast(v2s, __builtin_bit_cast(v2u, Ta) >> k.shA)), k.mA, d);
+    d = and_or(bits(__builtin_bit_cast(v2s, __builtin_bit_cast(v2u, Tb) >> k.shB)), k.mB, d);
+    d = and_or(bits(__builtin_bit_cast(v2s, __builtin_bit_cast(v2u, Ta2) >> k.shA2)), k.mA2, d);
+    d = and_or(bits(__builtin_bit_cast(v2s, __builtin_bit_cast(v2u, Tb2) >> k.shB2)), k.mB2, d);
     return d;
+}
+
+// Score candidate S of two cells (match/mismatch) from their codes: with
+// ONE = (1, 1) opaque to the optimiser, min(tb ^ qb, ONE) stays one
+// v_pk_min_u16 (0 match, 1 mismatch) and the score one v_pk_mad_u16.
+__device__ __forceinline__ v2s score_mm(const Consts &k, v2u tb, v2u qb, int one)
+{
+    const v2u mm = __builtin_elementwise_min(tb ^ qb, __builtin_bit_cast(v2u, one));
+    return k.SM + __builtin_bit_cast(v2s, mm) * k.SD;
 }
 
 // Sets one half of a packed register.
@@ -304,7 +335,7 @@ __device__ __forceinline__ void stripe_rec(const v2s (&Us)[R], bool hi, int i0, 
 // the cells a half computes outside its columns never reach a result.
 // WIDE: the team spans several CUs (progress counts in device memory,
 // kWideCtlStride dwords apart; a wait that gives up sets `late`).
-template <bool RIGHT, int MODE, bool WIDE>
+template <bool RIGHT, int MODE, bool WIDE, bool LUT>
 __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc, int job_id, int p, const KswJob &jb,
                                           int qlen, int tlen, int32_t *prog, int W, int &late)
 {
@@ -335,7 +366,11 @@ __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc,
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int tl = ilo + r < tlen ? (int)ts[ilo + r] : 0, th = ihi + r < tlen ? (int)ts[ihi + r] : 0;
-        TB[r] = (v2u){(unsigned short)tl, (unsigned short)th};
+        // LUT: the v_perm selector of row r's two score bytes (table byte
+        // tl of the low half's table, byte th of the high half's, zero high
+        // bytes); else the two target codes
+        TB[r] = LUT ? __builtin_bit_cast(v2u, (uint32_t)tl | 12u << 8 | (uint32_t)(th + 4) << 16 | 12u << 24)
+                    : (v2u){(unsigned short)tl, (unsigned short)th};
         U[r] = (v2s){(short)(-8 * edge_diff(sc, ilo + r)), (short)(-8 * edge_diff(sc, ihi + r))};
         Y[r] = kc.LIM, Y2[r] = kc.LIM, Us[r] = U[r];
     }
@@ -355,6 +390,13 @@ __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc,
         if (c1 < qlen) load_entry<WIDE>(bnd + 2 * c1, bnx0, bnx1);
     }
     const int lim_hi = bits(kc.LIM) << 16;
+    // LUT: per step, byte k of a half's table is the score candidate of
+    // target code k against that half's query code (SM when equal, SX
+    // otherwise); the query codes travel as 8 * code
+    const uint32_t sx_rep = (uint32_t)(lo16(bits(kc.SM + kc.SD)) & 0xff) * 0x01010101u;
+    const uint32_t s_flip = (uint32_t)((lo16(bits(kc.SM)) ^ lo16(bits(kc.SM + kc.SD))) & 0xff);
+    int one = 0x00010001;
+    asm volatile("" : "+s"(one));
     uint32_t qword = 0;
     for (int c = 0; c < nsteps; ++c) {
         // lane 0's low-stripe inputs for column c, each in a high half
@@ -365,7 +407,7 @@ __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc,
             bxh = bin0, bx2h = bin0 << 16, bvh = bin1;
         }
         if ((c & 3) == 0) qword = c < qlen ? *(const uint32_t *)(qs + c) : 0u;
-        const int qh = (int)((qword >> (8 * (c & 3))) & 0xffu) << 16;
+        const int qh = (int)((qword >> (8 * (c & 3))) & 0xffu) << (LUT ? 19 : 16);
         v2s X = vec(carry(Xo, bxh, lane0)), X2 = vec(carry(X2o, bx2h, lane0)), V = vec(carry(Vo, bvh, lane0));
         const int Qw = carry(Qo, qh, lane0);
         if (!first) {
@@ -380,25 +422,34 @@ __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc,
                 bin0 = dpp_shl1(0, bin0), bin1 = dpp_shl1(0, bin1);
             }
         }
-        // a half starts at its first column: initial left state (only the
-        // first 128 steps can hold a start; the guard is wave-uniform)
+        // a half starts at its first column: the lanes there take the
+        // half's initial left state (U from Us, which holds it until the
+        // half's last column; only the first 128 steps can hold a start, the
+        // guard is wave-uniform)
         const int jl = c - lane;
         if (c < kWave + kLag) {
-            if (jl == 0 || jl == kLag) {
-                const bool hi = jl == kLag;
+            const int m = jl == 0 ? 0x0000ffff : jl == kLag ? (int)0xffff0000u : 0;
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    U[r] = set_half(U[r], -8 * edge_diff(sc, (hi ? ihi : ilo) + r), hi);
-                    Y[r] = set_half(Y[r], -(RIGHT ? 8 : 0), hi), Y2[r] = set_half(Y2[r], -(RIGHT ? 8 : 0), hi);
-                }
+            for (int r = 0; r < R; ++r) {
+                U[r] = vec(bfi(m, bits(Us[r]), bits(U[r])));
+                Y[r] = vec(bfi(m, bits(kc.LIM), bits(Y[r]))), Y2[r] = vec(bfi(m, bits(kc.LIM), bits(Y2[r])));
             }
         }
         const v2u qb = __builtin_bit_cast(v2u, Qw);
+        uint32_t tab_lo = 0, tab_hi = 0;
+        if (LUT) {
+            tab_lo = sx_rep ^ (s_flip << (Qw & 31));
+            tab_hi = sx_rep ^ (s_flip << ((Qw >> 16) & 31));
+        }
+        auto score = [&](int r) -> v2s {
+            return LUT ? vec((int)__builtin_amdgcn_perm(tab_hi, tab_lo, bits(__builtin_bit_cast(v2s, TB[r]))))
+                       : score_mm(kc, TB[r], qb, one);
+        };
         int d[R];
         if (!last) {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                d[r] = cell2<RIGHT, MODE != kDirs>(kc, TB[r], qb, X, X2, V, U[r], Y[r], Y2[r]);
+                d[r] = cell2<RIGHT, MODE != kDirs>(kc, score(r), X, X2, V, U[r], Y[r], Y2[r]);
                 if (MODE == kTracks) d[r] = bits(V);
             }
         } else {
@@ -406,7 +457,7 @@ __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc,
             v2s vl = V;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                d[r] = cell2<RIGHT, MODE != kDirs>(kc, TB[r], qb, X, X2, V, U[r], Y[r], Y2[r]);
+                d[r] = cell2<RIGHT, MODE != kDirs>(kc, score(r), X, X2, V, U[r], Y[r], Y2[r]);
                 if (MODE == kTracks) d[r] = bits(V);
                 if (r == rho_last) vl = V;
             }
@@ -419,11 +470,10 @@ __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc,
         }
         // a half ends at column qlen-1: keep U for the mqe partials
         if (c >= qlen - 1 && c < qlen - 1 + kWave + kLag) {
-            if (jl == qlen - 1 || jl == qlen - 1 + kLag) {
-                const bool hi = jl != qlen - 1;
+            const int je = jl - (qlen - 1);
+            const int m = je == 0 ? 0x0000ffff : je == kLag ? (int)0xffff0000u : 0;
 #pragma unroll
-                for (int r = 0; r < R; ++r) Us[r] = set_half(Us[r], hi ? hi16(bits(U[r])) : lo16(bits(U[r])), hi);
-            }
+            for (int r = 0; r < R; ++r) Us[r] = vec(bfi(m, bits(U[r]), bits(Us[r])));
         }
         Xo = bits(X), X2o = bits(X2), Vo = bits(V), Qo = Qw;
         if (MODE == kDirs) {
@@ -519,14 +569,22 @@ __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc,
 // path (minimap2/ksw2_extd2_avx.c:752-760), and a failed walk reports
 // n_cigar = -1 (no CIGAR, zdropped).
 
-// Singletrack tracks of one stripe-kernel pair (layout of the tracks mode).
+// Singletrack tracks of one stripe-kernel pair (layout of the tracks mode),
+// read in the reference's coordinates (i target, j query).  A transposed
+// pair stores cell (i, j) at (j, i) and its horizontal and vertical
+// differences exchanged.
 struct StripeTracks {
     const uint8_t *dir;
-    const uint8_t *qs, *ts;
+    const uint8_t *qs, *ts;   // the reference's query and target
     uint64_t nsteps;
     int32_t mch, mis;
+    bool tr;
     __device__ int32_t get(int track, int i, int j) const
     {
+        if (tr) {
+            const int t = i;
+            i = j, j = t, track = 1 - track;
+        }
         const int s = i / SR, pp = s >> 1, h = s & 1, k = (i % SR) / R, rho = i % R;
         const uint64_t c = (uint64_t)(j + k + kLag * h);
         const uint8_t b =
@@ -543,9 +601,13 @@ __device__ __forceinline__ void pair_epilogue(const StripeArgs &a, int job_id, c
     const int tid = threadIdx.x;
     KswOut o = a.out[job_id];
     const int flag = jb.flag;
-    const int qlen = jb.qlen, tlen = jb.tlen;
+    // kq, kt: the swept and striped lengths; qlen, tlen: the reference's
+    const bool tr = jb.transposed != 0;
+    const int kq = jb.qlen, kt = jb.tlen;
+    const int qlen = tr ? kt : kq, tlen = tr ? kq : kt;
     {
-        const int nstripes = (tlen + SR - 1) / SR;
+        // the stripes' partials at the last swept column, in row order
+        const int nstripes = (kt + SR - 1) / SR;
         int carry_ = 0, best = 0x7fffffff, best_i = -1;
         for (int s = 0; s < nstripes; ++s) {
             const StripeRec rec = a.recs[jb.stripe_off + s];
@@ -553,9 +615,16 @@ __device__ __forceinline__ void pair_epilogue(const StripeArgs &a, int job_id, c
             carry_ += rec.total;
         }
         const int64_t h_anchor = (int64_t)a.sc.G1 - a.sc.qe_orig;
-        const int h_top_last_col = (int)(h_anchor + edge_prefix(a.sc, qlen - 1));  // H(-1, qlen-1)
-        o.mqe = h_top_last_col - (best >> 3);
-        o.mqe_t = best_i;
+        const int h_top_last_col = (int)(h_anchor + edge_prefix(a.sc, kq - 1));  // H(-1, kq-1)
+        const int col_max = h_top_last_col - (best >> 3);
+        if (!tr) {
+            o.mqe = col_max, o.mqe_t = best_i;
+        } else {
+            // the last swept column is the reference's last row (mte) and the
+            // last striped row, tracked by the last pass, its last column (mqe)
+            o.mqe = o.mte, o.mqe_t = o.mte_q;
+            o.mte = col_max, o.mte_q = best_i;
+        }
     }
     int i0 = -1, j0 = -1;
     int n = 0;
@@ -564,9 +633,11 @@ __device__ __forceinline__ void pair_epilogue(const StripeArgs &a, int job_id, c
         if (!o.stopped && !o.zdropped && !(flag & (0x40 | 0x800))) {
             if (tid < kWave) {
                 StripeTracks A;
-                A.dir = a.dir + jb.dir_off, A.qs = a.seq + jb.q_off, A.ts = a.seq + jb.t_off;
-                A.nsteps = (uint64_t)qlen + kWave - 1 + kLag;
+                A.dir = a.dir + jb.dir_off;
+                A.qs = a.seq + (tr ? jb.t_off : jb.q_off), A.ts = a.seq + (tr ? jb.q_off : jb.t_off);
+                A.nsteps = (uint64_t)kq + kWave - 1 + kLag;
                 A.mch = a.sc.mch, A.mis = a.sc.mis;
+                A.tr = tr;
                 const WalkGaps g = {-a.sc.q, -a.sc.e, -a.sc.q2, -a.sc.e2};
                 const int nn = track_walk(A, qlen, tlen, g, cig);
                 if (tid == 0) *shared_n = nn;
@@ -591,9 +662,10 @@ __device__ __forceinline__ void pair_epilogue(const StripeArgs &a, int job_id, c
     }
     if (i0 >= 0 || j0 >= 0) {
         const uint8_t *dir = a.dir + jb.dir_off;
-        const uint64_t nsteps = (uint64_t)qlen + kWave - 1 + kLag;
+        const uint64_t nsteps = (uint64_t)kq + kWave - 1 + kLag;
         int i = i0, j = j0, state = 0;
-        int t_i_lo = 1, t_i_hi = 0, t_j_lo = 1, t_j_hi = 0;  // staged tile (empty)
+        // the staged tile, in stored coordinates (striped row, swept column)
+        int t_r_lo = 1, t_r_hi = 0, t_c_lo = 1, t_c_hi = 0;
         int run_op = -1, run_len = 0;
         auto push = [&](int op, int len) {
             if (op == run_op) {
@@ -605,32 +677,33 @@ __device__ __forceinline__ void pair_epilogue(const StripeArgs &a, int job_id, c
             }
         };
         while (i >= 0 && j >= 0) {
-            if (i < t_i_lo || i > t_i_hi || j < t_j_lo || j > t_j_hi) {
-                // stage lanes kb..k of stripe s (rows up to i) x columns j-63..j
-                const int s = i / SR, k = (i - s * SR) / R, kb = k >= 7 ? k - 7 : 0;
+            const int r = tr ? j : i, c = tr ? i : j;  // stored coordinates of cell (i, j)
+            if (r < t_r_lo || r > t_r_hi || c < t_c_lo || c > t_c_hi) {
+                // stage lanes kb..k of stripe s (rows up to r) x columns c-63..c
+                const int s = r / SR, k = (r - s * SR) / R, kb = k >= 7 ? k - 7 : 0;
                 const int pp = s >> 1, h = s & 1;
-                t_i_lo = s * SR + kb * R, t_i_hi = s * SR + k * R + R - 1;
-                t_j_hi = j, t_j_lo = j >= 63 ? j - 63 : 0;
-                const int c_lo = t_j_lo + kb + kLag * h, c_hi = t_j_hi + k + kLag * h;
+                t_r_lo = s * SR + kb * R, t_r_hi = s * SR + k * R + R - 1;
+                t_c_hi = c, t_c_lo = c >= 63 ? c - 63 : 0;
+                const int c_lo = t_c_lo + kb + kLag * h, c_hi = t_c_hi + k + kLag * h;
                 __syncthreads();  // the previous tile is no longer read
-                for (int c = c_lo + tid; c <= c_hi; c += nthr) {
-                    const uint4 *src = (const uint4 *)(dir + (((uint64_t)pp * nsteps + (uint64_t)c) * kWave + kb) * 16);
+                for (int cc = c_lo + tid; cc <= c_hi; cc += nthr) {
+                    const uint4 *src = (const uint4 *)(dir + (((uint64_t)pp * nsteps + (uint64_t)cc) * kWave + kb) * 16);
                     for (int kr = 0; kr <= k - kb; ++kr) {
-                        const int jj = c - kLag * h - (kb + kr);
-                        if (jj >= t_j_lo && jj <= t_j_hi) {
+                        const int jj = cc - kLag * h - (kb + kr);
+                        if (jj >= t_c_lo && jj <= t_c_hi) {
                             const uint4 w = src[kr];
                             // the 8 row bytes of half h: bytes 2h, 2h+1 of each dword
                             const unsigned sel = h ? 0x07060302u : 0x05040100u;
                             const uint32_t lo = __builtin_amdgcn_perm(w.y, w.x, sel);
                             const uint32_t hi = __builtin_amdgcn_perm(w.w, w.z, sel);
-                            tile[kr][jj - t_j_lo] = (uint64_t)hi << 32 | lo;
+                            tile[kr][jj - t_c_lo] = (uint64_t)hi << 32 | lo;
                         }
                     }
                 }
                 __syncthreads();
             }
-            const int rel = i - t_i_lo;
-            const int tmp = (int)(tile[rel / R][j - t_j_lo] >> (8 * (rel % R))) & 0xff;
+            const int rel = r - t_r_lo;
+            const int tmp = (int)(tile[rel / R][c - t_c_lo] >> (8 * (rel % R))) & 0xff;
             if (state == 0) state = tmp & 7;
             else if (!(tmp >> (state + 2) & 1)) state = 0;
             if (state == 0) state = tmp & 7;
@@ -681,11 +754,14 @@ __global__ void __launch_bounds__(kWave * kMaxTeam) ksw_stripe_kernel(StripeArgs
     if (w >= W) return;
     prog[w] = 0;  // every lane of wave w writes the same zero
     __syncthreads();
-    const Consts kc = make_consts<RIGHT>(a.sc);
+    const Consts kc = make_consts<RIGHT>(a.sc, jb.transposed != 0);
     const int qlen = __builtin_amdgcn_readfirstlane(jb.qlen), tlen = __builtin_amdgcn_readfirstlane(jb.tlen);
     const int npass = ((tlen + SR - 1) / SR + 1) / 2;
     int late = 0;  // (no bounded waits within a workgroup)
-    for (int p = w; p < npass; p += W) pass_body<RIGHT, MODE, false>(a, kc, job, p, jb, qlen, tlen, prog, W, late);
+    if (jb.lut)
+        for (int p = w; p < npass; p += W) pass_body<RIGHT, MODE, false, true>(a, kc, job, p, jb, qlen, tlen, prog, W, late);
+    else
+        for (int p = w; p < npass; p += W) pass_body<RIGHT, MODE, false, false>(a, kc, job, p, jb, qlen, tlen, prog, W, late);
     __syncthreads();  // every pass written: stripe partials, score / mte, direction bytes
     pair_epilogue(a, job, jb, tile, MODE == kTracks, &shared_n, W * kWave);
 }
@@ -711,12 +787,15 @@ __global__ void __launch_bounds__(kWave * kMaxTeam) ksw_stripe_wide_kernel(Strip
     const int v = __builtin_amdgcn_readfirstlane((blk - first) * kWideGroup + (int)threadIdx.x / kWave);
     const int job = __builtin_amdgcn_readfirstlane(a.order[slot]);
     int32_t *ctl = a.wctl + (int64_t)slot * kWideCtl;
-    const Consts kc = make_consts<RIGHT>(a.sc);
     const KswJob jb = a.jobs[job];
+    const Consts kc = make_consts<RIGHT>(a.sc, jb.transposed != 0);
     const int qlen = __builtin_amdgcn_readfirstlane(jb.qlen), tlen = __builtin_amdgcn_readfirstlane(jb.tlen);
     const int npass = ((tlen + SR - 1) / SR + 1) / 2;
     int late = 0;
-    for (int p = v; p < npass; p += W) pass_body<RIGHT, MODE, true>(a, kc, job, p, jb, qlen, tlen, ctl, W, late);
+    if (jb.lut)
+        for (int p = v; p < npass; p += W) pass_body<RIGHT, MODE, true, true>(a, kc, job, p, jb, qlen, tlen, ctl, W, late);
+    else
+        for (int p = v; p < npass; p += W) pass_body<RIGHT, MODE, true, false>(a, kc, job, p, jb, qlen, tlen, ctl, W, late);
     if ((late || a.force_late) && threadIdx.x % kWave == 0)
         __hip_atomic_store(ctl + kWideCtlLate, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -750,12 +829,14 @@ __global__ void __launch_bounds__(kWave * kWideGroup) ksw_stripe_wide_epilogue_k
         const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
         prog[w] = 0;
         __syncthreads();
-        const Consts kc = make_consts<RIGHT>(a.sc);
+        const Consts kc = make_consts<RIGHT>(a.sc, jb.transposed != 0);
         const int qlen = __builtin_amdgcn_readfirstlane(jb.qlen), tlen = __builtin_amdgcn_readfirstlane(jb.tlen);
         const int npass = ((tlen + SR - 1) / SR + 1) / 2;
         int late2 = 0;
-        for (int p = w; p < npass; p += kWideGroup)
-            pass_body<RIGHT, MODE, false>(a, kc, job, p, jb, qlen, tlen, prog, kWideGroup, late2);
+        for (int p = w; p < npass; p += kWideGroup) {
+            if (jb.lut) pass_body<RIGHT, MODE, false, true>(a, kc, job, p, jb, qlen, tlen, prog, kWideGroup, late2);
+            else pass_body<RIGHT, MODE, false, false>(a, kc, job, p, jb, qlen, tlen, prog, kWideGroup, late2);
+        }
         __syncthreads();
         if (threadIdx.x == 0) atomicAdd(a.werr, 1);
     }

@@ -181,7 +181,8 @@ def lib():
         L.ksw2_hip_plan_timing.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                            C.POINTER(C.c_double)]
         L.ksw2_hip_plan_timing.restype = C.c_int
-        L.ksw2_hip_plan_dirs.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_int)]
+        L.ksw2_hip_plan_dirs.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int)]
         L.ksw2_hip_plan_dirs.restype = C.c_int
         L.ksw2_hip_certify_batch.argtypes = [C.c_int, C.POINTER(CertifyTask), C.c_int32, C.c_int32, C.c_int32,
                                              C.c_int32, C.c_int32, C.c_double, C.POINTER(CertifyResult)]
@@ -542,54 +543,63 @@ class Plan:
             raise KswError(f"ksw2_hip_plan_fetch: {STATUS.get(st, st)}")
         return out
 
-    def dirs(self, index: int, qlen: int, tlen: int, raw: bool = False) -> np.ndarray:
-        """Direction bytes of pair ``index`` as a (tlen, qlen) matrix (or the
-        raw pass-layout buffer when ``raw``).  Layout: stripe s = i // 512 is
-        half h = s & 1 of pass p = s >> 1; lane k = (i % 512) // 8, row
-        rho = i % 8; column j of that half is written at step c = j + k + 64h;
-        byte ((p * (qlen + 127) + c) * 64 + k) * 16 + 4 * (rho // 2) + 2h +
-        rho % 2."""
-        npass = ((tlen + 511) // 512 + 1) // 2
-        nsteps = qlen + 127
-        nbytes = npass * nsteps * 64 * 16
-        buf = np.zeros(nbytes, dtype=np.uint8)
-        r = C.c_int()
-        st = lib().ksw2_hip_plan_dirs(self.h, index, buf.ctypes.data, nbytes, C.byref(r))
+    def stored_transposed(self, index: int) -> bool:
+        """Whether pair ``index`` is stored transposed (its query in the
+        stripes and its target swept), from ``ksw2_hip_plan_dirs``."""
+        r, tr = C.c_int(), C.c_int()
+        st = lib().ksw2_hip_plan_dirs(self.h, index, None, 0, C.byref(r), C.byref(tr))
         if st != 0:
             raise KswError(f"ksw2_hip_plan_dirs: {STATUS.get(st, st)}")
-        if raw:
-            return buf
-        i = np.arange(tlen)[:, None]
-        j = np.arange(qlen)[None, :]
+        return bool(tr.value)
+
+    def _stored(self, index: int, qlen: int, tlen: int, per_lane: int):
+        """Raw pass-layout bytes of a pair and the byte index of every
+        stored cell (striped row i, swept column j) as a (kt, kq) array."""
+        tr = self.stored_transposed(index)
+        kq, kt = (tlen, qlen) if tr else (qlen, tlen)
+        npass = ((kt + 511) // 512 + 1) // 2
+        nsteps = kq + 127
+        nbytes = npass * nsteps * 64 * per_lane
+        buf = np.zeros(nbytes, dtype=np.uint8)
+        r, t2 = C.c_int(), C.c_int()
+        st = lib().ksw2_hip_plan_dirs(self.h, index, buf.ctypes.data, nbytes, C.byref(r), C.byref(t2))
+        if st != 0:
+            raise KswError(f"ksw2_hip_plan_dirs: {STATUS.get(st, st)}")
+        i = np.arange(kt)[:, None]
+        j = np.arange(kq)[None, :]
         s, rem = i // 512, i % 512
         p, h = s // 2, s % 2
         k, rho = rem // 8, rem % 8
         c = j + k + 64 * h
-        idx = ((p * nsteps + c) * 64 + k) * 16 + 4 * (rho // 2) + 2 * h + rho % 2
-        return buf[idx]
+        idx = ((p * nsteps + c) * 64 + k) * per_lane + 4 * (rho // 2) + 2 * h + rho % 2
+        return tr, buf, idx
+
+    def dirs(self, index: int, qlen: int, tlen: int, raw: bool = False) -> np.ndarray:
+        """Direction bytes of pair ``index`` as a (tlen, qlen) matrix (or the
+        raw pass-layout buffer when ``raw``).  Stored layout, with i the
+        striped index and j the swept one (the target and the query, or the
+        query and the target for a transposed pair): stripe s = i // 512 is
+        half h = s & 1 of pass p = s >> 1; lane k = (i % 512) // 8, row
+        rho = i % 8; column j of that half is written at step c = j + k + 64h;
+        byte ((p * (swept + 127) + c) * 64 + k) * 16 + 4 * (rho // 2) + 2h +
+        rho % 2.  The bytes are the reference's direction bytes either way."""
+        tr, buf, idx = self._stored(index, qlen, tlen, 16)
+        if raw:
+            return buf
+        m = buf[idx]
+        return m.T if tr else m
 
     def tracks(self, index: int, qlen: int, tlen: int) -> np.ndarray:
         """Singletrack tracks of pair ``index`` (flag KSW_EZ_SINGLETRACK) as a
         (2, tlen, qlen) int8 array: p = H(i,j)-H(i,j-1) and p2 =
         H(i,j)-H(i-1,j).  Same cell order as ``dirs``, 32 bytes per lane and
-        step (the negated p track, then the negated p2 track)."""
-        npass = ((tlen + 511) // 512 + 1) // 2
-        nsteps = qlen + 127
-        nbytes = npass * nsteps * 64 * 32
-        buf = np.zeros(nbytes, dtype=np.uint8)
-        r = C.c_int()
-        st = lib().ksw2_hip_plan_dirs(self.h, index, buf.ctypes.data, nbytes, C.byref(r))
-        if st != 0:
-            raise KswError(f"ksw2_hip_plan_dirs: {STATUS.get(st, st)}")
-        i = np.arange(tlen)[:, None]
-        j = np.arange(qlen)[None, :]
-        s, rem = i // 512, i % 512
-        p, h = s // 2, s % 2
-        k, rho = rem // 8, rem % 8
-        c = j + k + 64 * h
-        idx = ((p * nsteps + c) * 64 + k) * 32 + 4 * (rho // 2) + 2 * h + rho % 2
+        step (the negated stored-horizontal track, then the negated
+        stored-vertical one; a transposed pair's are the reference's p2 and
+        p)."""
+        tr, buf, idx = self._stored(index, qlen, tlen, 32)
         raw = buf.view(np.int8)
-        return -np.stack([raw[idx], raw[idx + 16]]).astype(np.int16)
+        t0, t1 = -raw[idx].astype(np.int16), -raw[idx + 16].astype(np.int16)
+        return np.stack([t1.T, t0.T]) if tr else np.stack([t0, t1])
 
     def close(self):
         if self.h:

@@ -357,16 +357,22 @@ int ksw2_hip_window_batch(int n, const ksw2_window_task_t *tasks, int64_t window
 void ksw2_hip_window_free(int n, ksw2_window_result_t *results);
 
 /* Diagnostics: copies pair `index`'s raw direction bytes (stripe-kernel
- * pairs only; pass layout: stripe s = i/512 is half h = s&1 of pass p = s>>1,
- * lane k = (i%512)/8, row rho = i%8, and cell (i, j) is byte
- * ((p*(qlen+127) + j+k+64h)*64 + k)*16 + 4*(rho/2) + 2h + rho%2) into `out`
- * and reports the rows per lane (8); `bytes` must be at least
- * ceil(ceil(tlen/512)/2)*(qlen+127)*1024.  A KSW_EZ_SINGLETRACK pair has
- * twice that: per lane and step 32 bytes, the negated p track
- * (H(i,j)-H(i,j-1)) in the first 16 and the negated p2 track
- * (H(i,j)-H(i-1,j)) in the next 16, each in the byte order above.
- * Returns a status code. */
-int ksw2_hip_plan_dirs(const ksw2_hip_plan *plan, int index, uint8_t *out, size_t bytes, int *R);
+ * pairs only) into `out` and reports the rows per lane (8) and whether the
+ * pair is stored transposed; with out == NULL it reports those two only.
+ * Stored layout, with i the striped index and j the swept one (target and
+ * query, or query and target when *transposed is 1, which the library picks
+ * when the query gives more 1024-row passes than the target): stripe
+ * s = i/512 is half h = s&1 of pass p = s>>1, lane k = (i%512)/8, row
+ * rho = i%8, and cell (i, j) is byte
+ * ((p*(swept+127) + j+k+64h)*64 + k)*16 + 4*(rho/2) + 2h + rho%2;
+ * `bytes` must be at least ceil(ceil(striped/512)/2)*(swept+127)*1024.  The
+ * bytes are the reference's direction bytes in either orientation.  A
+ * KSW_EZ_SINGLETRACK pair has twice that: per lane and step 32 bytes, the
+ * negated horizontal difference (along the swept sequence) in the first 16
+ * and the negated vertical one in the next 16, each in the byte order
+ * above.  Returns a status code. */
+int ksw2_hip_plan_dirs(const ksw2_hip_plan *plan, int index, uint8_t *out, size_t bytes, int *R,
+                       int *transposed);
 
 #ifdef __cplusplus
 }

@@ -148,3 +148,47 @@ def test_direction_bytes_team_of_three(monkeypatch):
         assert int((gd != rd).sum()) == 0, k
         assert same(got[k], ref)
     plan.close()
+
+
+@pytest.mark.parametrize("flag", [0, ksw2.KSW_EZ_RIGHT, ksw2.KSW_EZ_SCORE_ONLY, ksw2.KSW_EZ_SINGLETRACK])
+def test_generic_score_path(flag, monkeypatch):
+    """The stripe kernel's two score forms give the same bytes: the per-step
+    byte table (codes below 4, the default here) and, with it turned off,
+    the xor/min/multiply-add form."""
+    pairs = random_pairs(300 + flag, 40, 1, 1500)
+    ref = ksw2.extd2_batch(pairs, flag=flag)
+    monkeypatch.setenv("KSW2_HIP_LUT", "0")
+    got = ksw2.extd2_batch(pairs, flag=flag)
+    for k, (g, r) in enumerate(zip(got, ref)):
+        assert same(g, r), f"pair {k} flag {flag:#x}: {diff(g, r)}"
+    if flag != ksw2.KSW_EZ_SINGLETRACK:
+        check_batch(pairs, flag=flag)
+
+
+def test_fast_path_outside_the_byte_table():
+    """Fast-path calls the byte table cannot serve take the generic form:
+    code 4 (N) as an ordinary symbol under KSW_EZ_GENERIC_SC with a
+    diagonal-constant matrix, and a mismatch score below -31."""
+    rng = np.random.default_rng(41)
+    pairs = []
+    for n in (300, 1100, 2600):
+        t = W.random_seq(rng, n)
+        t[rng.integers(0, n, n // 20)] = 4
+        q = W.mutate(rng, t, 0.05, 0.01)
+        pairs.append((q, t))
+    mat = ksw2.default_matrix(0, -6)  # N vs N scores as a match
+    for p in pairs:
+        assert ksw2.route(*p, mat=mat, flag=ksw2.KSW_EZ_GENERIC_SC) == 0
+    got = ksw2.extd2_batch(pairs, mat=mat, flag=ksw2.KSW_EZ_GENERIC_SC)
+    for k, ((q, t), g) in enumerate(zip(pairs, got)):
+        r = O.run_oracle(q, t, mat=mat, flag=ksw2.KSW_EZ_GENERIC_SC)
+        assert same(g, r), f"N pair {k}: {diff(g, r)}"
+    # (a gap open of 20 keeps -40 within the reference's 2(q+e) bound)
+    mat40 = ksw2.default_matrix(0, -40)
+    plain = [(W.mutate(rng, t, 0.05, 0.01), t) for t in (W.random_seq(rng, n) for n in (700, 2100))]
+    for p in plain:
+        assert ksw2.route(*p, mat=mat40, q=20) == 0
+    got = ksw2.extd2_batch(plain, mat=mat40, q=20)
+    for k, ((q, t), g) in enumerate(zip(plain, got)):
+        r = O.run_oracle(q, t, mat=mat40, q=20)
+        assert same(g, r), f"mismatch -40 pair {k}: {diff(g, r)}"
