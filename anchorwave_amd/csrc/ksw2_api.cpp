@@ -165,6 +165,7 @@ void ksw2_hip_clear_progress_callback(void)
 int ksw2_hip_release_cached_memory(void)
 {
     kswhip::Engine::release_cached_block();
+    kswhip::release_call_context();
     return KSW2_HIP_OK;
 }
 

@@ -474,8 +474,8 @@ __global__ void __launch_bounds__(kWave * kBandGroup) ksw_band_kernel(BandArgs a
     // per-lane results over this wave's stripes
     int mqe_h = KSW_NEG_INF_I, mqe_t = -1, mte_h = KSW_NEG_INF_I, mte_q = -1, score = KSW_NEG_INF_I, pos = 0;
     const int nstripes = (tlenB + SR - 1) / SR;
-    int known = 0, late = 0;
-    for (int s = wv, kk = 0; s < nstripes; s += W, ++kk) {
+    int known = 0, late = 0, stop = 0;
+    for (int s = wv, kk = 0; s < nstripes && !stop; s += W, ++kk) {
         const int t0 = s * SR, tl = t0 + lane * R;
         int r_lo, r_hi, p_lo = 0, p_hi = -1;
         stripe_range(t0, bw, r_end, r_lo, r_hi);
@@ -529,6 +529,13 @@ __global__ void __launch_bounds__(kWave * kBandGroup) ksw_band_kernel(BandArgs a
         int hmax = KSW_NEG_INF_I;  // largest in-band H (checked against 0 at the end)
         uint8_t *prow = p + (uint64_t)r_lo * ncolB;  // direction row of anti-diagonal r
         for (int rb = r_lo; rb <= r_hi; rb += kBlock) {
+            if (a.stop && ((rb - r_lo) & 255) < kBlock) {
+                // the host's stop flag, every 256 anti-diagonals: end the
+                // wave (nothing of this pair is delivered)
+                stop = __builtin_amdgcn_readfirstlane(
+                    __hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
+                if (stop) break;
+            }
             if (s > 0) known = wait_count(prog + pw * kBandCtlStride, pbase + min(rb + kLead, p_hi) + 1, known, late);
             {
                 const int j = rb + lane - t0;
@@ -705,7 +712,8 @@ __global__ void __launch_bounds__(kWave * kBandGroup) ksw_band_kernel(BandArgs a
                     publish(prog + wv * kBandCtlStride, kk * (nr + 2) + r + 1);
             }
         }
-        publish(prog + wv * kBandCtlStride, kk * (nr + 2) + nr + 1);  // past every need of the next stripe
+        // past every need of the next stripe (of every later one when stopped)
+        publish(prog + wv * kBandCtlStride, stop ? 0x3fffffff : kk * (nr + 2) + nr + 1);
         if (hmax > 0) pos = 1;
     }
 

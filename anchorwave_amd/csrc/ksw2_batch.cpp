@@ -576,15 +576,18 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         p->d_state = base + o_state, p->d_offs = (int32_t *)(base + o_offs);
         p->d_gjobs = (GenJob *)(base + o_gjobs), p->d_gorder = (int32_t *)(base + o_gorder);
         if (p->n_band > 0) p->d_redo = (int32_t *)(base + o_redo);
-        if (env && env->h_stop) {
-            p->h_stop = env->h_stop, p->d_stop = env->d_stop, p->own_stop = false;
-        } else {
-            if (hipHostMalloc((void **)&p->h_stop, sizeof(int32_t), hipHostMallocMapped) != hipSuccess)
-                return fail(KSW2_HIP_ENOMEM);
-            *p->h_stop = 0;
-            if (hipHostGetDevicePointer((void **)&p->d_stop, p->h_stop, 0) != hipSuccess)
-                return fail(KSW2_HIP_EDEVICE);
-        }
+    }
+    // the stop flag the kernels sample (the general kernel always; the
+    // stripe and band kernels when a progress callback runs or the owner of
+    // a shared flag asks for it)
+    if (env && env->h_stop) {
+        p->h_stop = env->h_stop, p->d_stop = env->d_stop, p->own_stop = false;
+        p->poll_stop = env->poll_stop;
+    } else if (p->n > 0) {
+        if (hipHostMalloc((void **)&p->h_stop, sizeof(int32_t), hipHostMallocMapped) != hipSuccess)
+            return fail(KSW2_HIP_ENOMEM);
+        *p->h_stop = 0;
+        if (hipHostGetDevicePointer((void **)&p->d_stop, p->h_stop, 0) != hipSuccess) return fail(KSW2_HIP_EDEVICE);
     }
     // host mirror of the stripe-kernel results (engine flights)
     if (env && env->mirror_flag && p->n_fast > 0) {
@@ -722,6 +725,7 @@ int plan_launch(Plan *p, const Progress *prog)
     }
     if (!p->blaunches.empty()) {
         BandArgs ba;
+        ba.stop = (p->poll_stop || (prog && prog->cb)) ? p->d_stop : nullptr;
         ba.seq = p->d_seq, ba.dir = p->d_dir, ba.offs = p->d_offs, ba.bnd = p->d_bnd, ba.jobs = p->d_gjobs;
         ba.out = p->d_out, ba.redo = p->d_redo, ba.sc = p->gsc;
         const char *fr = getenv("KSW2_HIP_BAND_REDO");
@@ -743,7 +747,11 @@ int plan_launch(Plan *p, const Progress *prog)
         ga.order = p->d_gorder + p->band_off, ga.njobs = p->n_band, ga.ring = 0, ga.only_if = p->d_redo;
         KSW_HIP_TRY(launch_general(ga, 1024, st));
     }
+    // the stripe and band kernels sample the stop flag only when it may be
+    // raised: a progress callback runs, or the flag's owner raises it
+    const int32_t *poll = (p->poll_stop || (prog && prog->cb)) ? p->d_stop : nullptr;
     StripeArgs sa;
+    sa.stop = poll;
     sa.seq = p->d_seq, sa.dir = p->d_dir, sa.bnd = p->d_bnd, sa.cig = p->d_cig, sa.sc = p->sc;
     sa.jobs = p->d_jobs, sa.out = p->d_out, sa.recs = p->d_recs;
     sa.host_out = p->d_mout, sa.host_cig = p->d_mcig, sa.host_flag = p->d_mflag;
@@ -857,9 +865,10 @@ int deliver_slot(const Plan *p, int k, const KswOut &o, const uint32_t *cig_aren
     z->mqe = o.mqe, z->mqe_t = o.mqe_t, z->mte = o.mte, z->mte_q = o.mte_q;
     z->score = o.score, z->reach_end = o.reach_end, z->stopped = o.stopped;
     if (p->stop_requested && (k < p->n_fast || p->gjobs[k - p->n_fast].r_end > 0)) {
-        // the stripe and band kernels have no stop point: a pair whose deadline passed
-        // while it ran is reported stopped with no path (callers discard
-        // stopped results, alignSlidingWindow.cpp:957-963)
+        // a stripe- or band-kernel pair whose deadline passed while it ran
+        // (the kernels end at their next sample of the flag, at most 256
+        // columns or anti-diagonals later) is reported stopped with no path;
+        // callers discard stopped results (alignSlidingWindow.cpp:957-963)
         z->stopped = 1, z->score = KSW_NEG_INF;
         z->n_cigar = 0;
         return KSW2_HIP_OK;
@@ -925,6 +934,91 @@ void plan_destroy(Plan *p)
     delete p;
 }
 
+namespace {
+
+// What a host thread's single calls reuse from one call to the next: a
+// stream, the plan's timing events, pinned staging for the sequences, a
+// pinned stop flag and a device block grown to the largest call so far (up
+// to kCallBlockMax).  A call then allocates and frees nothing, which matters
+// most when many executor threads call at once: hipFree and hipHostFree
+// wait for the device.  Freed when the thread ends, or by
+// ksw2_hip_release_cached_memory on the thread.
+constexpr uint64_t kCallBlockMax = 1ull << 30;
+
+struct GrowPool : DevicePool {
+    // one plan at a time; a larger one replaces the block (the previous
+    // call's work has finished: every call waits for its plan)
+    bool alloc(uint64_t n, uint64_t *off) override
+    {
+        if (n > bytes) {
+            if (base) (void)hipFree(base);
+            base = nullptr, bytes = 0;
+            if (hipMalloc(&base, n) != hipSuccess) {
+                base = nullptr;
+                return false;
+            }
+            bytes = n;
+        }
+        *off = 0;
+        return true;
+    }
+    void free(uint64_t) override {}
+    void drop()
+    {
+        if (base) (void)hipFree(base);
+        base = nullptr, bytes = 0;
+    }
+};
+
+struct CallContext {
+    bool ready = false;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint8_t *staging = nullptr;
+    uint64_t staging_bytes = 0;
+    int32_t *h_stop = nullptr, *d_stop = nullptr;
+    GrowPool pool;
+
+    bool prepare(uint64_t seq_bytes)
+    {
+        if (!ready) {
+            if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return false;
+            for (auto &e : ev)
+                if (hipEventCreate(&e) != hipSuccess) return false;
+            if (hipHostMalloc((void **)&h_stop, sizeof(int32_t), hipHostMallocMapped) != hipSuccess) return false;
+            if (hipHostGetDevicePointer((void **)&d_stop, h_stop, 0) != hipSuccess) return false;
+            ready = true;
+        }
+        if (seq_bytes > staging_bytes) {
+            if (staging) (void)hipHostFree(staging);
+            staging = nullptr, staging_bytes = 0;
+            const uint64_t want = std::max<uint64_t>(seq_bytes + seq_bytes / 4, 1u << 16);
+            if (hipHostMalloc((void **)&staging, want, hipHostMallocDefault) != hipSuccess) {
+                staging = nullptr;
+                return false;
+            }
+            staging_bytes = want;
+        }
+        *(volatile int32_t *)h_stop = 0;
+        return true;
+    }
+    ~CallContext()
+    {
+        pool.drop();
+        if (staging) (void)hipHostFree(staging);
+        if (h_stop) (void)hipHostFree(h_stop);
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+thread_local CallContext t_call;
+
+}  // namespace
+
+void release_call_context() { t_call.pool.drop(); }
+
 int run_batch_host(int n, const ksw2_pair_t *pairs, int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q2,
                    int8_t e2, ksw_extz_t *ez, void *km, const Progress *prog)
 {
@@ -961,10 +1055,25 @@ int run_batch_host(int n, const ksw2_pair_t *pairs, int8_t m, const int8_t *mat,
     // (a batch of at most 1 GiB, a single call's usual size, skips the
     // device memory query)
     if (todo.size() <= 1 || (total <= (1ull << 30) && !getenv("KSW2_HIP_MEM_BUDGET")) || total <= mem_budget(false)) {
-        // one plan holds the whole batch
+        // one plan holds the whole batch; a single pair up to kCallBlockMax
+        // runs in the thread's cached context
         if (todo.empty()) return status;
         Plan *plan = nullptr;
-        int st = plan_create(&plan, pairs, todo.data(), (int)todo.size(), m, cls, gsc);
+        PlanEnv env;
+        const PlanEnv *use = nullptr;
+        if (todo.size() == 1 && total <= kCallBlockMax) {
+            const ksw2_pair_t &pp = pairs[todo[0]];
+            if (t_call.prepare((uint64_t)pp.qlen + pp.tlen + 64)) {
+                env.pool = &t_call.pool;
+                env.stream = t_call.stream;
+                env.events = t_call.ev;
+                env.staging = t_call.staging, env.staging_bytes = t_call.staging_bytes;
+                env.h_stop = t_call.h_stop, env.d_stop = t_call.d_stop;
+                env.spread = true;  // a plan of its own: 5-16 wave teams may spread over CUs
+                use = &env;
+            }
+        }
+        int st = plan_create(&plan, pairs, todo.data(), (int)todo.size(), m, cls, gsc, use);
         if (st == KSW2_HIP_OK) st = plan_run(plan, prog);
         if (st == KSW2_HIP_OK) st = plan_fetch(plan, ez, km);
         plan_destroy(plan);

@@ -1,7 +1,10 @@
 // ksw2_certify.cpp -- score-certified banded alignment over batched calls:
 // alignScoreCertifiedKsw2 (src/myImportandFunction/alignSlidingWindow.cpp:
-// 1197-1400, selectionPlan == NULL) for many intervals, with every task's
-// next ksw_extd2 call in one batched launch per round.
+// 1197-1400) for many intervals, with every task's next ksw_extd2 calls in
+// one batched launch per round: the score-only calls, then the chained
+// traceback's gap alignments of every task with a selection plan
+// (tryChainedKsw2Traceback, :1056-1195), then the widening bands.
+#include <ctype.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -55,6 +58,29 @@ Rescored rescore(const char *dq, int64_t ql, const char *dd, int64_t tl, const u
     r.valid = ok && kswhip::reconstructs(aq, dq, ql) && kswhip::reconstructs(ad, dd, tl);
     return r;
 }
+
+// Runs of aligned (M), query-only (I) and reference-only (D) columns of two
+// gapped rows.
+std::vector<uint32_t> cigar_of_rows(const std::string &aq, const std::string &ad)
+{
+    std::vector<uint32_t> out;
+    for (size_t c = 0; c < aq.size(); ++c) {
+        const uint32_t op = ad[c] == '-' ? 1u : aq[c] == '-' ? 2u : 0u;
+        if (!out.empty() && (out.back() & 0xf) == op) out.back() += 1u << 4;
+        else out.push_back(1u << 4 | op);
+    }
+    return out;
+}
+
+// One piece of a chained alignment (tryChainedKsw2Traceback): an anchor's
+// matched columns, a gap with one side empty, or a gap aligned by a banded
+// ksw_extd2 call (alignSlidingWindow_minimap2 at band min(max(lengths),
+// maximum band), :1098-1121).
+struct Piece {
+    int kind;                    // 0 anchor, 1 one-sided gap, 2 aligned gap
+    int64_t qs, qe, rs, re;
+    int call = -1;               // aligned gap: index of its call in the round
+};
 
 uint32_t *copy_cigar(const uint32_t *src, int n)
 {
@@ -210,6 +236,134 @@ extern "C" int ksw2_hip_certify_batch(int n, const ksw2_certify_task_t *tasks, i
         w.fallback = !w.active;
         r.status = KSW2_CERT_NOT_CERTIFIED;
         loop.push_back((int)i);
+    }
+
+    // the chained traceback through the selection plan's anchors
+    // (:1260-1289, tryChainedKsw2Traceback :1056-1195): every task's aligned
+    // gaps in one batched call, then each chain assembled in order
+    {
+        std::vector<std::vector<Piece>> chains(work.size());
+        std::vector<char> has_chain(work.size(), 0);
+        std::vector<ksw2_pair_t> cp;  // the aligned gaps (codes point into the tasks)
+        for (int i : loop) {
+            Task &w = work[(size_t)i];
+            const ksw2_certify_task_t &t = tasks[w.index];
+            if (!t.anchors) continue;
+            ++res[w.index].traceback_attempts;
+            if (t.n_anchors < 2 || w.max_band == 0) continue;  // :1071-1074
+            const int64_t ql = (int64_t)w.q.size(), tl = (int64_t)w.d.size();
+            std::vector<Piece> pieces;
+            bool ok = true;
+            auto gap = [&](int64_t qs, int64_t qe, int64_t rs, int64_t re) {
+                const int64_t a = qe - qs, b = re - rs;
+                if (a == 0 || b == 0) {
+                    if (a || b) pieces.push_back(Piece{1, qs, qe, rs, re});
+                    return true;
+                }
+                const uint64_t d = (uint64_t)(a > b ? a - b : b - a);
+                if (d > w.max_band || a > 0x7fffffff || b > 0x7fffffff) return false;
+                Piece pc{2, qs, qe, rs, re};
+                pc.call = (int)cp.size();
+                ksw2_pair_t pr;
+                pr.query = w.q.data() + qs, pr.target = w.d.data() + rs;
+                pr.qlen = (int32_t)a, pr.tlen = (int32_t)b;
+                pr.w = (int32_t)std::min<uint64_t>((uint64_t)std::max(a, b), w.max_band);
+                pr.zdrop = -1, pr.end_bonus = 0, pr.flag = 0;
+                for (int64_t k = 0; k < a && !pr.flag; ++k)
+                    if (pr.query[k] == 4) pr.flag = KSW_EZ_GENERIC_SC;
+                for (int64_t k = 0; k < b && !pr.flag; ++k)
+                    if (pr.target[k] == 4) pr.flag = KSW_EZ_GENERIC_SC;
+                cp.push_back(pr);
+                pieces.push_back(pc);
+                return true;
+            };
+            int64_t qpos = 0, rpos = 0, used = 0;
+            for (int64_t k = 0; k < t.n_anchors && ok; ++k) {  // :1133-1170
+                const ksw2_chain_anchor_t &an = t.anchors[k];
+                const int64_t aq = an.query_position, ar = an.reference_position, al = an.length;
+                if (al == 0 || aq < qpos || ar < rpos || aq + al > ql || ar + al > tl) continue;
+                bool match = true;
+                for (int64_t o = 0; o < al && match; ++o)
+                    match = toupper((unsigned char)t.query[aq + o]) == toupper((unsigned char)t.target[ar + o]);
+                if (!match) continue;
+                if (!gap(qpos, aq, rpos, ar)) ok = false;
+                else pieces.push_back(Piece{0, aq, aq + al, ar, ar + al}), qpos = aq + al, rpos = ar + al, ++used;
+            }
+            if (ok && used >= 2 && gap(qpos, ql, rpos, tl)) {  // :1171-1179
+                chains[(size_t)i] = std::move(pieces);
+                has_chain[(size_t)i] = 1;
+            } else {
+                // unavailable: none of its calls run (they are the last ones)
+                cp.resize(cp.size() - std::count_if(pieces.begin(), pieces.end(),
+                                                     [](const Piece &pc) { return pc.kind == 2; }));
+            }
+        }
+        std::vector<ksw_extz_t> cez(cp.size(), ksw_extz_t{});
+        if (!cp.empty()) {
+            const int st = kswhip::run_batch_host((int)cp.size(), cp.data(), 5, mat, q, e, q2, e2q, cez.data(), nullptr,
+                                                  dl.on ? &prog : nullptr);
+            if (st != KSW2_HIP_OK) {
+                for (auto &z : cez) free(z.cigar);
+                release();
+                return st;
+            }
+        }
+        const bool late = dl.passed();
+        for (int i : loop) {
+            if (!has_chain[(size_t)i]) continue;
+            Task &w = work[(size_t)i];
+            const ksw2_certify_task_t &t = tasks[w.index];
+            ksw2_certify_result_t &r = res[w.index];
+            std::string aq, ad;
+            bool ok = true, stopped = false;
+            for (const Piece &pc : chains[(size_t)i]) {
+                if (pc.kind == 0 || pc.kind == 1) {
+                    if (pc.qe > pc.qs) aq.append(t.query + pc.qs, (size_t)(pc.qe - pc.qs));
+                    else aq.append((size_t)(pc.re - pc.rs), '-');
+                    if (pc.re > pc.rs) ad.append(t.target + pc.rs, (size_t)(pc.re - pc.rs));
+                    else ad.append((size_t)(pc.qe - pc.qs), '-');
+                    continue;
+                }
+                const ksw_extz_t &z = cez[(size_t)pc.call];
+                if (z.stopped || late) {  // :1122-1127 (the deadline)
+                    stopped = true;
+                    break;
+                }
+                std::string sq, sd;
+                int64_t qp = 0, tp = 0;
+                const int64_t a = pc.qe - pc.qs, b = pc.re - pc.rs;
+                kswhip::append_rows(sq, sd, t.query + pc.qs, a, qp, t.target + pc.rs, b, tp, z.cigar, z.n_cigar);
+                if (sq.size() != sd.size() || !kswhip::reconstructs(sq, t.query + pc.qs, a) ||
+                    !kswhip::reconstructs(sd, t.target + pc.rs, b)) {
+                    ok = false;
+                    break;
+                }
+                aq += sq, ad += sd;
+            }
+            if (stopped) {
+                r.status = KSW2_CERT_TIME_LIMIT;
+                w.active = w.fallback = false;
+                continue;
+            }
+            if (!ok || !kswhip::reconstructs(aq, t.query, (int64_t)w.q.size()) ||
+                !kswhip::reconstructs(ad, t.target, (int64_t)w.d.size()))
+                continue;  // unavailable (:1180-1185)
+            bool rok = false;
+            const int64_t score = kswhip::rescore_rows(aq, ad, mis, o1, e1, o2, e2, &rok);
+            if (!rok) continue;
+            const std::vector<uint32_t> cig = cigar_of_rows(aq, ad);
+            r.score = score;
+            if (score == r.optimal_score) {
+                r.final_band_width = -2;
+                r.cigar = copy_cigar(cig.data(), (int)cig.size()), r.n_cigar = (int)cig.size();
+                r.status = KSW2_CERT_COMPLETED;
+                w.active = w.fallback = false;
+            } else {
+                w.best = cig;
+                w.have_best = true;
+            }
+        }
+        for (auto &z : cez) free(z.cigar);
     }
 
     // one attempt for each task in `idx` at its band; returns the status

@@ -708,6 +708,7 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
         env.stream = streams_[s];
         env.staging = staging_[s], env.staging_bytes = staging_bytes_[s];
         env.h_stop = h_stop_, env.d_stop = d_stop_;
+        env.poll_stop = true;  // stop_all_locked raises it
         env.team = std::max(key, 0);
         env.spread = batch_ ? batch_spread_ : serve_spread_;  // a lone flight's teams spread over CUs
         if (mr.slots) {

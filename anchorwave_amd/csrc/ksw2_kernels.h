@@ -31,6 +31,10 @@ struct StripeArgs {
     int32_t *wctl;
     int32_t *werr;
     int force_late;            // tests: every wide pair is recomputed as if a wait had given up
+    // host-raised stop flag (pinned, mapped) sampled every 256 columns, or
+    // null: a stopped pass publishes a count past every wait and ends, and
+    // the epilogue skips the walk (the host reports such pairs stopped)
+    const int32_t *stop;
 };
 
 struct GenArgs {
@@ -73,6 +77,10 @@ struct BandArgs {
     int32_t *redo;             // per job: 1 when H > 0 was seen (the general kernel must run it)
     GenScore sc;
     int force_redo;            // tests: flag every job for the general kernel
+    // host-raised stop flag (pinned, mapped) sampled every 256 anti-diagonals
+    // of a stripe, or null: a stopped wave publishes a count past every wait
+    // and ends (the host reports such pairs stopped)
+    const int32_t *stop;
 };
 constexpr int kBandTeam = 16;       // waves per pair, at most
 constexpr int kBandGroup = 4;       // waves per workgroup: one per SIMD of a CU

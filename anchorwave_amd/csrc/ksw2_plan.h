@@ -32,6 +32,9 @@ struct Progress {
 };
 int run_batch_host(int n, const ksw2_pair_t *pairs, int8_t m, const int8_t *mat, int8_t q, int8_t e, int8_t q2,
                    int8_t e2, ksw_extz_t *ez, void *km, const Progress *prog = nullptr);
+// Frees the device block the calling thread's single calls keep between
+// calls (run_batch_host).
+void release_call_context();
 int deliver_cigar(ksw_extz_t *ez, const uint32_t *src, int n, void *km);
 int drop_cigar(ksw_extz_t *ez, void *km);
 // A pair whose CIGAR comes from the Singletrack walk over two stored tracks.
@@ -112,6 +115,9 @@ struct PlanEnv {
     // Sequences go straight from the caller's buffers to the device (no
     // pinned staging); the seq arena is zero-filled by a kernel first.
     bool direct_upload = false;
+    // The stripe and band kernels sample h_stop even with no progress
+    // callback (its owner raises it: an engine stopping its flights).
+    bool poll_stop = false;
 };
 
 // A device-resident batch: sequences uploaded, arenas sized, jobs built.
@@ -159,6 +165,7 @@ struct Plan {
     bool own_streams = false;
     uint8_t *h_seq = nullptr;        // pinned staging kept while an upload may be in flight
     bool own_seq = true, own_stop = true;
+    bool poll_stop = false;          // PlanEnv::poll_stop
     int progress_calls = 0;
     KswOut *h_mout = nullptr, *d_mout = nullptr;   // host mirror (not owned)
     uint32_t *h_mcig = nullptr, *d_mcig = nullptr;

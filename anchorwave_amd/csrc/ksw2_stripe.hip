@@ -229,6 +229,7 @@ __device__ __forceinline__ int wait_count(const int32_t *slot, int need, int kno
 // device-scope accesses, coherent across the XCDs' L2s; a count is stored
 // only after the entries it covers are complete (s_waitcnt vmcnt(0)).
 constexpr uint64_t kWaitTicks = 1000000000ull;  // 10 s of the 100 MHz real-time clock
+constexpr int kStopCount = 0x3fffffff;          // published by a stopped wave: past every need
 
 __device__ __forceinline__ void publish_wide(int32_t *slot, int value)
 {
@@ -337,7 +338,7 @@ __device__ __forceinline__ void stripe_rec(const v2s (&Us)[R], bool hi, int i0, 
 // kWideCtlStride dwords apart; a wait that gives up sets `late`).
 template <bool RIGHT, int MODE, bool WIDE, bool LUT>
 __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc, int job_id, int p, const KswJob &jb,
-                                          int qlen, int tlen, int32_t *prog, int W, int &late)
+                                          int qlen, int tlen, int32_t *prog, int W, int &late, int &stop)
 {
     const KswScore &sc = a.sc;
     const int nstripes = (tlen + SR - 1) / SR, npass = (nstripes + 1) / 2;
@@ -399,6 +400,16 @@ __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc,
     asm volatile("" : "+s"(one));
     uint32_t qword = 0;
     for (int c = 0; c < nsteps; ++c) {
+        if (a.stop && (c & 255) == 0) {
+            // the host's stop flag: publish a count past every wait of the
+            // team and end (nothing of this pair is delivered)
+            stop = __builtin_amdgcn_readfirstlane(__hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
+            if (stop) {
+                if (WIDE) publish_wide(prog_mine, kStopCount);
+                else publish(prog_mine, kStopCount);
+                return;
+            }
+        }
         // lane 0's low-stripe inputs for column c, each in a high half
         int bxh, bx2h, bvh;
         if (first) {
@@ -599,7 +610,17 @@ __device__ __forceinline__ void pair_epilogue(const StripeArgs &a, int job_id, c
                                               uint64_t (*tile)[64], bool tracks, int *shared_n, int nthr)
 {
     const int tid = threadIdx.x;
+    // the host's stop flag, read once for the whole workgroup: a stopped
+    // pair's passes may have ended early, so it takes no walk
+    int stopped = 0;
+    if (a.stop) {
+        if (tid == 0) *shared_n = __hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __syncthreads();
+        stopped = *shared_n;
+        __syncthreads();
+    }
     KswOut o = a.out[job_id];
+    if (stopped) o.stopped = 1;
     const int flag = jb.flag;
     // kq, kt: the swept and striped lengths; qlen, tlen: the reference's
     const bool tr = jb.transposed != 0;
@@ -629,7 +650,10 @@ __device__ __forceinline__ void pair_epilogue(const StripeArgs &a, int job_id, c
     int i0 = -1, j0 = -1;
     int n = 0;
     uint32_t *cig = a.cig + jb.cig_off;
-    if (tracks) {
+    if (stopped) {
+        // no walk: the passes, the stripe partials and this pair's result
+        // slot may be incomplete
+    } else if (tracks) {
         if (!o.stopped && !o.zdropped && !(flag & (0x40 | 0x800))) {
             if (tid < kWave) {
                 StripeTracks A;
@@ -757,11 +781,13 @@ __global__ void __launch_bounds__(kWave * kMaxTeam) ksw_stripe_kernel(StripeArgs
     const Consts kc = make_consts<RIGHT>(a.sc, jb.transposed != 0);
     const int qlen = __builtin_amdgcn_readfirstlane(jb.qlen), tlen = __builtin_amdgcn_readfirstlane(jb.tlen);
     const int npass = ((tlen + SR - 1) / SR + 1) / 2;
-    int late = 0;  // (no bounded waits within a workgroup)
+    int late = 0, stop = 0;  // (no bounded waits within a workgroup)
     if (jb.lut)
-        for (int p = w; p < npass; p += W) pass_body<RIGHT, MODE, false, true>(a, kc, job, p, jb, qlen, tlen, prog, W, late);
+        for (int p = w; p < npass && !stop; p += W)
+            pass_body<RIGHT, MODE, false, true>(a, kc, job, p, jb, qlen, tlen, prog, W, late, stop);
     else
-        for (int p = w; p < npass; p += W) pass_body<RIGHT, MODE, false, false>(a, kc, job, p, jb, qlen, tlen, prog, W, late);
+        for (int p = w; p < npass && !stop; p += W)
+            pass_body<RIGHT, MODE, false, false>(a, kc, job, p, jb, qlen, tlen, prog, W, late, stop);
     __syncthreads();  // every pass written: stripe partials, score / mte, direction bytes
     pair_epilogue(a, job, jb, tile, MODE == kTracks, &shared_n, W * kWave);
 }
@@ -791,11 +817,13 @@ __global__ void __launch_bounds__(kWave * kMaxTeam) ksw_stripe_wide_kernel(Strip
     const Consts kc = make_consts<RIGHT>(a.sc, jb.transposed != 0);
     const int qlen = __builtin_amdgcn_readfirstlane(jb.qlen), tlen = __builtin_amdgcn_readfirstlane(jb.tlen);
     const int npass = ((tlen + SR - 1) / SR + 1) / 2;
-    int late = 0;
+    int late = 0, stop = 0;
     if (jb.lut)
-        for (int p = v; p < npass; p += W) pass_body<RIGHT, MODE, true, true>(a, kc, job, p, jb, qlen, tlen, ctl, W, late);
+        for (int p = v; p < npass && !stop; p += W)
+            pass_body<RIGHT, MODE, true, true>(a, kc, job, p, jb, qlen, tlen, ctl, W, late, stop);
     else
-        for (int p = v; p < npass; p += W) pass_body<RIGHT, MODE, true, false>(a, kc, job, p, jb, qlen, tlen, ctl, W, late);
+        for (int p = v; p < npass && !stop; p += W)
+            pass_body<RIGHT, MODE, true, false>(a, kc, job, p, jb, qlen, tlen, ctl, W, late, stop);
     if ((late || a.force_late) && threadIdx.x % kWave == 0)
         __hip_atomic_store(ctl + kWideCtlLate, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -832,10 +860,10 @@ __global__ void __launch_bounds__(kWave * kWideGroup) ksw_stripe_wide_epilogue_k
         const Consts kc = make_consts<RIGHT>(a.sc, jb.transposed != 0);
         const int qlen = __builtin_amdgcn_readfirstlane(jb.qlen), tlen = __builtin_amdgcn_readfirstlane(jb.tlen);
         const int npass = ((tlen + SR - 1) / SR + 1) / 2;
-        int late2 = 0;
-        for (int p = w; p < npass; p += kWideGroup) {
-            if (jb.lut) pass_body<RIGHT, MODE, false, true>(a, kc, job, p, jb, qlen, tlen, prog, kWideGroup, late2);
-            else pass_body<RIGHT, MODE, false, false>(a, kc, job, p, jb, qlen, tlen, prog, kWideGroup, late2);
+        int late2 = 0, stop = 0;
+        for (int p = w; p < npass && !stop; p += kWideGroup) {
+            if (jb.lut) pass_body<RIGHT, MODE, false, true>(a, kc, job, p, jb, qlen, tlen, prog, kWideGroup, late2, stop);
+            else pass_body<RIGHT, MODE, false, false>(a, kc, job, p, jb, qlen, tlen, prog, kWideGroup, late2, stop);
         }
         __syncthreads();
         if (threadIdx.x == 0) atomicAdd(a.werr, 1);

@@ -322,12 +322,19 @@ def extd2_singletrack(query, target, mat=None, m=5, q=8, e=2, q2=75, e2=1, w=-1,
     return res
 
 
+class ChainAnchor(C.Structure):
+    """``ksw2_chain_anchor_t`` (AlignmentChainAnchor)."""
+
+    _fields_ = [("query_position", C.c_uint32), ("reference_position", C.c_uint32), ("length", C.c_uint16)]
+
+
 class CertifyTask(C.Structure):
     """``ksw2_certify_task_t`` (include/ksw2_hip.h)."""
 
     _fields_ = [("query", C.c_char_p), ("target", C.c_char_p), ("qlen", C.c_int64), ("tlen", C.c_int64),
                 ("initial_band", C.c_int64), ("maximum_band", C.c_int64),
-                ("precomputed_optimal_score", C.POINTER(C.c_int64))]
+                ("precomputed_optimal_score", C.POINTER(C.c_int64)),
+                ("anchors", C.POINTER(ChainAnchor)), ("n_anchors", C.c_int64)]
 
 
 class CertifyResult(C.Structure):
@@ -359,9 +366,10 @@ class Certified:
 def certify_batch(tasks: Sequence[tuple], mismatch=-6, open1=-8, extend1=-2, open2=-75, extend2=-1,
                   max_seconds: float = 0.0) -> List[Certified]:
     """``ksw2_hip_certify_batch``: tasks are (query, target, initial_band,
-    maximum_band[, precomputed_optimal_score]) with str or bytes sequences;
-    the scoring is AnchorWave's penalties (negative), as
-    alignScoreCertifiedKsw2 takes them."""
+    maximum_band[, precomputed_optimal_score[, anchors]]) with str or bytes
+    sequences; anchors (None: no selection plan) are (query position,
+    reference position, length) triples; the scoring is AnchorWave's
+    penalties (negative), as alignScoreCertifiedKsw2 takes them."""
     L = lib()
     n = len(tasks)
     arr = (CertifyTask * max(n, 1))()
@@ -377,6 +385,12 @@ def certify_batch(tasks: Sequence[tuple], mismatch=-6, open1=-8, extend1=-2, ope
             v = C.c_int64(int(t[4]))
             keep.append(v)
             arr[k].precomputed_optimal_score = C.pointer(v)
+        if len(t) > 5 and t[5] is not None:
+            an = (ChainAnchor * max(len(t[5]), 1))()
+            for i, (aq, ar, al) in enumerate(t[5]):
+                an[i].query_position, an[i].reference_position, an[i].length = aq, ar, al
+            keep.append(an)
+            arr[k].anchors, arr[k].n_anchors = C.cast(an, C.POINTER(ChainAnchor)), len(t[5])
     out = (CertifyResult * max(n, 1))()
     st = L.ksw2_hip_certify_batch(n, arr, mismatch, open1, extend1, open2, extend2, float(max_seconds), out)
     if st != 0:

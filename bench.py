@@ -161,6 +161,38 @@ def single_pair_leg(args) -> dict:
     return out
 
 
+def threads_leg(args, pairs) -> dict:
+    """AnchorWave's per-interval pattern: -t executor threads each calling the
+    drop-in ksw_extd2_hip on their own pairs at once (fixed-batch pairs,
+    2 kbp; 16 threads), against the reference ksw_extd2_avx512 on the same
+    pairs and threads."""
+    threads = 16
+    sub = pairs[: args.threads_leg]
+    cells = W.cells(sub)
+    ksw2.extd2(*sub[0], mat=W_MAT, **SC)  # device initialised
+    res = [None] * len(sub)
+
+    def work(w):
+        for k in range(w, len(sub), threads):
+            res[k] = ksw2.extd2(*sub[k], mat=W_MAT, **SC)
+
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(work, range(threads)))  # every thread's first calls set up its context
+        t0 = time.perf_counter()
+        list(ex.map(work, range(threads)))
+        dt = time.perf_counter() - t0
+    out = {"pairs": len(sub), "threads": threads, "cells": cells, "seconds": round(dt, 3),
+           "gcups": round(cells / dt / 1e9, 2), "calls_per_s": round(len(sub) / dt, 1),
+           "timing": "wall clock of all calls, 16 host threads each calling ksw_extd2_hip one pair at a time"}
+    if not args.no_cpu:
+        kind, cdt, cres = cpu_baseline(sub, threads)
+        out["cpu"] = {"kind": kind, "gcups": round(cells / cdt / 1e9, 3), "cores": threads}
+        exact = sum(1 for g, c in zip(res, cres) if g.__dict__ == c.__dict__)
+        out["exact"] = {"agree": exact, "of": len(sub), "frac": exact / len(sub)}
+    log(f"[bench] {threads} threads calling ksw_extd2_hip: {out}")
+    return out
+
+
 def banded_leg(args) -> dict:
     """64 long pairs (100-300 kbp; 10% substitutions, 2% indels, 2-5
     structural indels of 5-50 kbp), w = 4096, zdrop off, full traceback,
@@ -343,6 +375,8 @@ def main() -> int:
                     help="pairs in the banded long-pair leg, w = 4096 (0: skip it)")
     ap.add_argument("--singletrack", type=int, default=1,
                     help="also time the fixed batch with KSW_EZ_SINGLETRACK (0: skip)")
+    ap.add_argument("--threads-leg", type=int, default=1024,
+                    help="pairs of the 16-thread ksw_extd2_hip leg (0: skip it)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (PMC byte-counter scale)")
     args = ap.parse_args()
@@ -441,6 +475,8 @@ def main() -> int:
         out["single_pair"] = single_pair_leg(args)
     if args.singletrack:
         out["singletrack"] = singletrack_leg(args, pairs, cells)
+    if args.threads_leg > 0:
+        out["threads"] = threads_leg(args, pairs)
     if args.banded > 0:
         out["banded"] = banded_leg(args)
     if args.uneven > 0:

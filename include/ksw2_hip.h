@@ -182,9 +182,13 @@ int ksw_extd2_hip_batch(int n, const ksw2_pair_t *pairs,
                         ksw_extz_t *ez);
 /* A batch larger than the device memory budget streams through a device
  * block of the whole budget, which the library keeps for the next such call
- * (allocating and releasing hundreds of GB costs seconds per call).  This
- * releases it; the next such call allocates again.  No reference
- * counterpart.  Returns KSW2_HIP_OK (also when nothing is held). */
+ * (allocating and releasing hundreds of GB costs seconds per call).  Single
+ * calls (ksw_extd2_hip) likewise keep, per host thread, a stream, pinned
+ * staging and a device block of up to 1 GiB grown to the thread's largest
+ * call, released when the thread ends.  This releases the batch block and
+ * the calling thread's device block; the next such call allocates again.
+ * No reference counterpart.  Returns KSW2_HIP_OK (also when nothing is
+ * held). */
 int ksw2_hip_release_cached_memory(void);
 
 /* ---- Device-resident batch ("plan"; no reference counterpart).  The
@@ -274,18 +278,22 @@ int ksw2_hip_server_load(ksw2_hip_server *server, ksw2_hip_load_t *load);
 
 /* ---- Score-certified banded alignment, batched.  Replaces
  * alignScoreCertifiedKsw2 (src/myImportandFunction/alignSlidingWindow.cpp:
- * 1197-1400) for many intervals at once, without its chained-traceback
- * selection plan (selectionPlan == NULL there).  Per task, as the reference:
+ * 1197-1400) for many intervals at once.  Per task, as the reference:
  * the optimal score from a full score-only ksw_extd2 call (ksw2ScoreOnly,
- * :723-796) unless precomputed; then banded ksw_extd2 tracebacks
+ * :723-796) unless precomputed; with a selection plan (anchors != NULL), the
+ * chained traceback (tryChainedKsw2Traceback, :1056-1195): the alignment
+ * through the plan's usable anchors, each gap between them aligned by a
+ * banded ksw_extd2 call, which certifies the task with final band -2 when
+ * it rescores to the optimum; then banded ksw_extd2 tracebacks
  * (alignSlidingWindow_minimap2, :800-890) at bands starting from twice
  * max(|qlen - tlen|, initial_band, 1) and doubling up to
  * min(max(qlen, tlen), maximum_band), until the rescored alignment
  * (scoreGappedTwoPieceAffineAlignment, :677-721) equals the optimal score
  * and both rows reconstruct their sequences; on failure the skipped first
  * band is tried last.  Every task's next call goes into one batched launch
- * per round, so a batch of n tasks costs one score-only launch and about
- * log2(maximum/initial) + 2 banded launches.
+ * per round, so a batch of n tasks costs one score-only launch, one launch
+ * of every chain's gaps and about log2(maximum/initial) + 2 banded
+ * launches.
  * Sequences are characters (A/C/G/T in either case; anything else is the
  * wildcard code 4 with KSW_EZ_GENERIC_SC, :644-660); the scoring is
  * AnchorWave's penalties (mismatch and the two gap pieces, negative), used
@@ -298,12 +306,24 @@ enum {
     KSW2_CERT_NOT_CERTIFIED = 2,
     KSW2_CERT_FAILED = 3
 };
+/* One certification anchor of AnchorWave's selection plan: an exact match
+ * of `length` bases at these positions (AlignmentChainAnchor,
+ * src/myImportandFunction/AlignmentAlgorithmSelector.h:129-133). */
+typedef struct {
+    uint32_t query_position;
+    uint32_t reference_position;
+    uint16_t length;
+} ksw2_chain_anchor_t;
 typedef struct {
     const char *query;      /* dna_q */
     const char *target;     /* dna_d */
     int64_t qlen, tlen;
     int64_t initial_band, maximum_band;
     const int64_t *precomputed_optimal_score;  /* NULL: computed by a score-only call */
+    /* the selection plan's certification anchors
+     * (selectionPlan->certificationAnchors); NULL: no selection plan */
+    const ksw2_chain_anchor_t *anchors;
+    int64_t n_anchors;
 } ksw2_certify_task_t;
 typedef struct {
     int32_t status;             /* KSW2_CERT_* (Ksw2CertifiedStatus order) */

@@ -10,8 +10,10 @@ src/myImportandFunction/alignSlidingWindow.cpp:
   * scoreGappedTwoPieceAffineAlignment :677-721
   * ksw2ScoreOnly :723-796
   * alignSlidingWindow_minimap2 :800-890 (CIGAR -> rows, rescoring)
-  * alignScoreCertifiedKsw2 :1197-1400 with selectionPlan == NULL
-    (no chained traceback) and no deadline.
+  * tryChainedKsw2Traceback :1056-1195 (the selection plan's
+    certification anchors)
+  * alignScoreCertifiedKsw2 :1197-1400, with or without a selection plan,
+    and no deadline.
 Pinned by tests/test_certify_oracle.py: a certified alignment's rescored
 score equals an independent Gotoh optimum, the band schedule matches its
 closed form, and the rows rebuild both sequences.
@@ -106,6 +108,64 @@ def banded(dq: str, dd: str, band: int, mis, o1, e1, o2, e2):
     return rescore(aq, ad, mis, o1, e1, o2, e2), list(r.cigar), valid
 
 
+def cigar_of_rows(aq: str, ad: str) -> List[int]:
+    """Runs of aligned (M), query-only (I) and reference-only (D) columns."""
+    out: List[int] = []
+    for a, b in zip(aq, ad):
+        op = 1 if b == "-" else 2 if a == "-" else 0
+        if out and (out[-1] & 0xF) == op:
+            out[-1] += 1 << 4
+        else:
+            out.append(1 << 4 | op)
+    return out
+
+
+def chained(dq: str, dd: str, anchors, maximum_band: int, mis, o1, e1, o2, e2):
+    """tryChainedKsw2Traceback (:1056-1195) without a deadline: the
+    alignment through the plan's certification anchors (query position,
+    reference position, length), each gap between them aligned by
+    alignSlidingWindow_minimap2 at band min(max(lengths), maximum_band).
+    Returns (completed, rows, score)."""
+    if len(anchors) < 2 or maximum_band == 0:  # :1071-1074
+        return False, None, 0
+    aq, ad = [], []
+
+    def segment(qs, qe, rs, re_):  # appendSegment :1081-1131
+        ql, rl = qe - qs, re_ - rs
+        if ql == 0 or rl == 0:
+            if ql:
+                aq.append(dq[qs:qe]), ad.append("-" * ql)
+            elif rl:
+                aq.append("-" * rl), ad.append(dd[rs:re_])
+            return True
+        if abs(ql - rl) > maximum_band or ql > 0x7FFFFFFF or rl > 0x7FFFFFFF:
+            return False
+        band = min(max(ql, rl), maximum_band)
+        _score, cigar, valid = banded(dq[qs:qe], dd[rs:re_], band, mis, o1, e1, o2, e2)
+        if not valid:
+            return False
+        a, b = rows(cigar, dq[qs:qe], dd[rs:re_])
+        aq.append(a), ad.append(b)
+        return True
+
+    qpos = rpos = used = 0
+    for aqp, arp, aln in anchors:  # :1133-1170
+        if aln == 0 or aqp < qpos or arp < rpos or aqp + aln > len(dq) or arp + aln > len(dd):
+            continue
+        if dq[aqp:aqp + aln].upper() != dd[arp:arp + aln].upper():
+            continue
+        if not segment(qpos, aqp, rpos, arp):
+            return False, None, 0
+        aq.append(dq[aqp:aqp + aln]), ad.append(dd[arp:arp + aln])
+        qpos, rpos, used = aqp + aln, arp + aln, used + 1
+    if used < 2 or not segment(qpos, len(dq), rpos, len(dd)):  # :1171-1179
+        return False, None, 0
+    a, b = "".join(aq), "".join(ad)
+    if not reconstructs(a, dq) or not reconstructs(b, dd):  # :1180-1185
+        return False, None, 0
+    return True, (a, b), rescore(a, b, mis, o1, e1, o2, e2)
+
+
 @dataclass
 class Certified:
     status: int = FAILED
@@ -118,9 +178,10 @@ class Certified:
 
 
 def certify(dq: str, dd: str, initial_band: int, maximum_band: int, mis=-6, o1=-8, e1=-2, o2=-75, e2=-1,
-            precomputed: Optional[int] = None) -> Certified:
-    """alignScoreCertifiedKsw2 (:1197-1400), selectionPlan == NULL, no
-    deadline; alignments as CIGARs."""
+            precomputed: Optional[int] = None, anchors=None) -> Certified:
+    """alignScoreCertifiedKsw2 (:1197-1400) without a deadline; ``anchors``
+    (None: no selection plan) are the plan's certification anchors;
+    alignments as CIGARs."""
     res = Certified()
     minlen = min(len(dq), len(dd))
     diff = abs(len(dq) - len(dd))
@@ -140,11 +201,21 @@ def certify(dq: str, dd: str, initial_band: int, maximum_band: int, mis=-6, o1=-
         res.status = FAILED
         return res
     max_band = min(max(len(dq), len(dd)), maximum_band)  # :1257-1259
+    have_best = False
+    best: List[int] = []
+    if anchors is not None:  # :1260-1289
+        res.traceback_attempts += 1
+        ok, rws, score = chained(dq, dd, anchors, max_band, mis, o1, e1, o2, e2)
+        if ok:
+            cig = cigar_of_rows(*rws)
+            res.score = score
+            if score == res.optimal_score:
+                res.final_band_width, res.cigar, res.status = -2, cig, COMPLETED
+                return res
+            best, have_best = cig, True
     requested = max(diff, max(1, initial_band))  # :1292-1300
     doubled = requested * 2
     band = min(max_band, max(requested, doubled))
-    have_best = False
-    best: List[int] = []
 
     def attempt(b):
         nonlocal have_best, best

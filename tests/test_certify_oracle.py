@@ -161,3 +161,89 @@ def test_planted_mistakes_are_caught(monkeypatch):
     t2, q2 = _dna(rng2, 120), _dna(rng2, 120)
     _r, seen = _bands_tried(monkeypatch, q2, t2, 3, 64)
     assert seen[:-1] != [3, 6, 12, 24, 48, 64] and seen != [6, 12, 24, 48, 64]
+
+
+def _runs_of_matches(cigar, q, t, min_len):
+    """Exact-match runs of an alignment, as (query position, reference
+    position, length) anchors."""
+    out, qp, tp = [], 0, 0
+    for c in cigar:
+        ln, op = c >> 4, c & 0xF
+        if op == 0:
+            k = 0
+            while k < ln:
+                if q[qp + k] == t[tp + k]:
+                    s = k
+                    while k < ln and q[qp + k] == t[tp + k]:
+                        k += 1
+                    if k - s >= min_len:
+                        out.append((qp + s, tp + s, k - s))
+                else:
+                    k += 1
+            qp, tp = qp + ln, tp + ln
+        elif op == 1:
+            qp += ln
+        else:
+            tp += ln
+    return out
+
+
+def test_chained_traceback_through_optimal_anchors_is_optimal():
+    """Anchors taken from an optimal alignment's exact-match runs lie on an
+    optimal path and no gap crosses them, so the chained alignment (each gap
+    between anchors aligned with a band covering it) scores the independent
+    Gotoh optimum: the task completes at the chained attempt (band -2, one
+    traceback), and its rows rebuild both sequences."""
+    rng = np.random.default_rng(81)
+    done = 0
+    for k in range(24):
+        t = _dna(rng, int(rng.integers(200, 700)))
+        q = _mutate(rng, t)
+        base = CP.certify(q, t, 0, 10_000, **PEN)
+        assert base.status == CP.COMPLETED
+        anchors = _runs_of_matches(base.cigar, q, t, 12)
+        if len(anchors) < 2:
+            continue
+        qs, _ = CP.encode(q)
+        ts, _ = CP.encode(t)
+        opt = O.gotoh2_global(qs, ts, **KS)
+        r = CP.certify(q, t, 0, 10_000, anchors=anchors, **PEN)
+        assert r.status == CP.COMPLETED and r.final_band_width == -2 and r.traceback_attempts == 1, (k, r)
+        assert r.score == opt == r.optimal_score
+        aq, ad = CP.rows(r.cigar, q, t)
+        assert CP.reconstructs(aq, q) and CP.reconstructs(ad, t) and CP.rescore(aq, ad, **PEN) == opt
+        done += 1
+    assert done >= 15
+
+
+def test_chained_traceback_literal_cases():
+    """Hand-checked cases of tryChainedKsw2Traceback (alignSlidingWindow.cpp:
+    1056-1195): the anchors' runs merge with the gaps' M runs; an anchor that
+    does not match (case-insensitively) or starts before the previous
+    anchor's end is skipped; fewer than two usable anchors, or a gap whose
+    length difference exceeds the band, makes the chain unavailable and the
+    banded loop runs (one more traceback attempt)."""
+    q = t = "AAAACCCCGGGGTTTT"
+    r = CP.certify(q, t, 2, 64, anchors=[(0, 0, 4), (8, 8, 4)], **PEN)
+    assert (r.status, r.final_band_width, r.cigar, r.traceback_attempts, r.score) == \
+        (CP.COMPLETED, -2, [16 << 4 | 0], 1, 0)
+    # the overlapping (2, 2, 4) and the mismatching (9, 10, 3) are skipped
+    r = CP.certify(q, t, 2, 64, anchors=[(0, 0, 4), (2, 2, 4), (9, 10, 3), (12, 12, 4)], **PEN)
+    assert (r.status, r.final_band_width, r.traceback_attempts) == (CP.COMPLETED, -2, 1)
+    # anchors match case-insensitively, but the rescoring compares
+    # characters: 16 mismatches against a lower-case copy
+    ok, rws, sc = CP.chained(q, t.lower(), [(0, 0, 4), (12, 12, 4)], 64, **PEN)
+    assert ok and sc == 16 * -6 and rws == (q, t.lower())
+    # one usable anchor: unavailable, then the banded loop certifies at band 4
+    r = CP.certify(q, t, 2, 64, anchors=[(0, 0, 4), (2, 3, 4)], **PEN)
+    assert (r.status, r.final_band_width, r.traceback_attempts) == (CP.COMPLETED, 4, 2)
+    # a gap of query 8 against reference 0 after (4, 4): difference 8 > maximum band 6
+    q2 = "AAAACCCCGGGGTTTT"
+    t2 = "AAAATTTT"
+    ok, _rows, _s = CP.chained(q2, t2, [(0, 0, 4), (12, 4, 4)], 6, **PEN)
+    assert ok  # one side empty: a pure gap, no DP
+    ok, _rows, _s = CP.chained(q2 + "A", t2 + "CC", [(0, 0, 4), (12, 4, 4)], 6, **PEN)
+    assert ok
+    ok, _rows, _s = CP.chained("AAAA" + "CGCGCGCGCGCG" + "TTTT", "AAAA" + "CG" + "TTTT", [(0, 0, 4), (16, 6, 4)],
+                               6, **PEN)
+    assert not ok  # 12 against 2: difference 10 > 6

@@ -69,8 +69,9 @@ def _tasks(seed, n):
 def _compare(tasks, got):
     bad = []
     for k, (task, g) in enumerate(zip(tasks, got)):
-        q, t, a, b, pre = task
-        e = CP.certify(q, t, a, b, precomputed=pre, **PEN)
+        q, t, a, b, pre = task[:5]
+        anchors = task[5] if len(task) > 5 else None
+        e = CP.certify(q, t, a, b, precomputed=pre, anchors=anchors, **PEN)
         gd = (g.status, g.score, g.optimal_score, g.final_band_width, g.traceback_attempts, list(g.cigar),
               list(g.best_effort_cigar))
         ed = (e.status, e.score, e.optimal_score, e.final_band_width, e.traceback_attempts, list(e.cigar),
@@ -120,3 +121,64 @@ def test_certify_batch_deadline():
     got = ksw2.certify_batch(tasks, max_seconds=1e-9, **GPU_PEN)
     assert got[0].status == CP.TIME_LIMIT
     assert got[1].status == CP.COMPLETED
+
+
+def _exact_runs(cigar, q, t, min_len):
+    out, qp, tp = [], 0, 0
+    for c in cigar:
+        ln, op = c >> 4, c & 0xF
+        if op == 0:
+            k = 0
+            while k < ln:
+                if q[qp + k] == t[tp + k]:
+                    s = k
+                    while k < ln and q[qp + k] == t[tp + k]:
+                        k += 1
+                    if k - s >= min_len:
+                        out.append((qp + s, tp + s, min(k - s, 65535)))
+                else:
+                    k += 1
+            qp, tp = qp + ln, tp + ln
+        elif op == 1:
+            qp += ln
+        else:
+            tp += ln
+    return out
+
+
+def test_certify_batch_with_selection_plans():
+    """Tasks with a selection plan take the chained traceback first
+    (tryChainedKsw2Traceback): anchors from an optimal alignment certify at
+    band -2; perturbed plans (anchors shifted off the path, mismatching,
+    overlapping, too few, gaps wider than the band) fall through to the
+    banded loop; every field equals the restatement's."""
+    rng = np.random.default_rng(84)
+    tasks = []
+    for k in range(36):
+        t = _dna(rng, int(rng.integers(300, 4000)))
+        q = _mutate(rng, t, 0.05, 0.02)
+        if k % 4 == 1:
+            a = int(rng.integers(0, len(t) // 2))
+            q = q[:a] + q[a + int(rng.integers(50, 300)):]
+        if k % 6 == 2:
+            q = "".join("N" if rng.random() < 0.02 else c for c in q)
+        base = CP.certify(q, t, 0, 1 << 20, **PEN)
+        anchors = _exact_runs(base.cigar, q, t, 10)
+        kind = k % 5
+        if kind == 1 and len(anchors) > 3:  # one anchor moved off the path
+            aq, ar, al = anchors[len(anchors) // 2]
+            anchors[len(anchors) // 2] = (aq + 1, ar, al)
+        elif kind == 2:
+            anchors = anchors[:1]
+        elif kind == 3:  # narrow maximum band: some gaps may exceed it
+            tasks.append((q, t, 3, max(abs(len(q) - len(t)), 40), None, anchors))
+            continue
+        elif kind == 4:  # overlapping duplicates are skipped
+            anchors = sorted(anchors + anchors[::3])
+        tasks.append((q, t, int(rng.choice([0, 5, 40])), 1 << 20, None, anchors))
+    tasks.append(("ACGTACGTACGT", "ACGTACGTACGT", 2, 64, None, []))  # a plan with no anchors
+    got = ksw2.certify_batch(tasks, **GPU_PEN)
+    bad = _compare(tasks, got)
+    chained = sum(1 for g in got if g.final_band_width == -2)
+    assert chained >= 10, chained
+    assert not bad, "\n".join(bad[:6]) + f"\n({len(bad)} of {len(tasks)} differ)"

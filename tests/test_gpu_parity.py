@@ -192,3 +192,29 @@ def test_fast_path_outside_the_byte_table():
     for k, ((q, t), g) in enumerate(zip(plain, got)):
         r = O.run_oracle(q, t, mat=mat40, q=20)
         assert same(g, r), f"mismatch -40 pair {k}: {diff(g, r)}"
+
+
+def test_single_calls_reuse_the_thread_context():
+    """Single calls on one thread reuse its stream, staging and device block:
+    sizes growing and shrinking, both kernels, a release of the cached block
+    in between, and a deadline stop followed by a normal call all give the
+    reference's results."""
+    rng = np.random.default_rng(44)
+    sizes = [300, 2500, 700, 6000, 120, 3000]
+    for k, n in enumerate(sizes):
+        t = W.random_seq(rng, n)
+        q = W.mutate(rng, t, 0.06, 0.015)
+        w = 150 if k % 3 == 2 else -1  # every third call on the general kernel
+        g = ksw2.extd2(q, t, w=w)
+        r = O.run_oracle(q, t, w=w)
+        assert same(g, r), f"call {k} n={n} w={w}: {diff(g, r)}"
+        if k == 3:
+            ksw2.release_cached_memory()
+    t = W.random_seq(rng, 20000)
+    q = W.mutate(rng, t, 0.06, 0.015)
+    with ksw2.progress_callback(lambda r, total: r == 0):
+        g = ksw2.extd2(q, t, w=400)
+    assert g.stopped == 1
+    q2, t2 = q[:900], t[:900]
+    g = ksw2.extd2(q2, t2, w=400)
+    assert same(g, O.run_oracle(q2, t2, w=400)) and g.stopped == 0
