@@ -310,7 +310,7 @@ int ksw2_hip_plan_dirs(const ksw2_hip_plan *plan, int index, uint8_t *out, size_
     if (R) *R = j.R;
     if (transposed) *transposed = j.transposed;
     if (!out) return set_err(KSW2_HIP_OK);  // the geometry only
-    const uint64_t need = (track_pair(j.flag) ? 2 : 1) * kswhip::dir_bytes(j.qlen, j.tlen);
+    const uint64_t need = (track_pair(j.flag) ? 2 : 1) * kswhip::dir_bytes(j.qlen, j.tlen, j.R);
     if (bytes < need || (j.flag & KSW_EZ_SCORE_ONLY)) return set_err(KSW2_HIP_EINVAL);
     if (hipMemcpy(out, p->d_dir + j.dir_off, need, hipMemcpyDeviceToHost) != hipSuccess)
         return set_err(KSW2_HIP_EDEVICE);

@@ -173,11 +173,11 @@ int fast_path_reject(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls)
 // waves on different CUs, rounded up to whole workgroups).  KSW2_HIP_TEAM
 // forces a size; KSW2_HIP_WIDE=k runs every pair wide on k workgroups
 // (tuning and tests).
-static double pair_steps(const KswJob &j) { return (double)pass_count(j.tlen) * (double)pass_steps(j.qlen); }
+static double pair_steps(const KswJob &j) { return (double)pass_count(j.tlen, j.R) * (double)pass_steps(j.qlen); }
 
 static int choose_team(const KswJob &j, double T, int forced)
 {
-    const int P = pass_count(j.tlen);
+    const int P = pass_count(j.tlen, j.R);
     int team = 1;
     if (forced > 0) {
         team = std::max(1, std::min(std::min(forced, kMaxWide), P));
@@ -221,6 +221,28 @@ StripeGeom stripe_geom(const ksw2_pair_t &p)
 
 static int norm_w(const ksw2_pair_t &p) { return p.w < 0 ? std::max(p.qlen, p.tlen) : p.w; }
 
+// Rows per lane of a plan's stripe kernel.  Eight rows per lane take the
+// fewest VALU operations per cell; four rows give a pair twice the passes,
+// so twice the waves can share its sweep.  A plan whose pairs would leave
+// most of the device idle at eight rows (a lone pair, a few short ones: the
+// sum over pairs of passes, each a wave, is below an eighth of the
+// device's resident waves) takes four, which shortens the pair's time when
+// its team was capped by its pass count.  A caller that knows the device's
+// load (the engine) may choose instead (PlanEnv::rows); KSW2_HIP_ROWS=4 or
+// 8 forces it.
+static int choose_rows(const ksw2_pair_t *pairs, const std::vector<int> &fast, const PlanEnv *env)
+{
+    if (const char *ev = getenv("KSW2_HIP_ROWS")) {
+        const int r = atoi(ev);
+        if (r == 4 || r == 8) return r;
+    }
+    if (env && (env->rows == 4 || env->rows == 8)) return env->rows;
+    if (fast.empty()) return kRowsPerLane;
+    int64_t waves = 0;
+    for (int id : fast) waves += pass_count(stripe_geom(pairs[id]).kt, 8);
+    return 8 * waves <= stripe_resident_waves(kGroupLeft, 4) ? 4 : 8;
+}
+
 uint64_t pair_footprint(const ksw2_pair_t &p, bool general)
 {
     const uint64_t cig = 4ull * ((uint64_t)p.qlen + p.tlen + 2) + 64, seq = (uint64_t)p.qlen + p.tlen + 32;
@@ -228,8 +250,10 @@ uint64_t pair_footprint(const ksw2_pair_t &p, bool general)
     const uint64_t tracks = track_pair(p) ? 2 : 1;  // Singletrack: two bytes per cell
     if (!general) {
         const StripeGeom g = stripe_geom(p);
-        return (cigar ? tracks * dir_bytes(g.kq, g.kt) : 0) + 8ull * g.kq + cig + seq +
-               16ull * stripe_count(g.kt) + 4ull * kWideCtl + 512;
+        // the larger of both rows-per-lane layouts: direction bytes at eight
+        // rows (rows padded to 1024), stripe partials at four (twice the stripes)
+        return (cigar ? tracks * dir_bytes(g.kq, g.kt, 8) : 0) + 8ull * g.kq + cig + seq +
+               16ull * stripe_count(g.kt, 4) + 4ull * kWideCtl + 512;
     }
     const int tlenB = (p.tlen + 63) / 64 * 64, qlenB = (p.qlen + 63) / 64 * 64;
     const uint64_t diags = (uint64_t)p.qlen + p.tlen - 1;
@@ -310,23 +334,24 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
     p->device = g_device;
     p->sc = gsc.k;
     p->gsc = gsc;
-    p->R = kRowsPerLane;
-    const int R = p->R, SR = kWave * R;
     // slots: stripe-kernel pairs by group (left, right, score-only, tracks),
     // then general-kernel pairs
-    std::vector<int> g[kGroups], gen;
+    std::vector<int> g[kGroups], gen, fast;
     for (int k = 0; k < nidx; ++k) {
         const ksw2_pair_t &pp = pairs[idx[k]];
         if (takes_general(pp, m, cls)) {
             gen.push_back(idx[k]);
             continue;
         }
+        fast.push_back(idx[k]);
         const int grp = (pp.flag & KSW_EZ_SCORE_ONLY) ? kGroupScoreOnly
                         : track_pair(pp)               ? kGroupTracks
                         : (pp.flag & KSW_EZ_RIGHT)     ? kGroupRight
                                                        : kGroupLeft;
         g[grp].push_back(idx[k]);
     }
+    p->R = choose_rows(pairs, fast, env);
+    const int R = p->R, SR = kWave * R;
     std::vector<int> grp_of;
     for (int grp = 0; grp < kGroups; ++grp)
         for (int id : g[grp]) p->order.push_back(id), grp_of.push_back(grp);
@@ -354,7 +379,7 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         j.t_off = so, so = round_up(so + g.kt, 16);
         j.dir_off = dof;
         if (!(pp.flag & KSW_EZ_SCORE_ONLY))
-            dof = round_up(dof + (track_pair(pp) ? 2 : 1) * dir_bytes(g.kq, g.kt), 256);
+            dof = round_up(dof + (track_pair(pp) ? 2 : 1) * dir_bytes(g.kq, g.kt, R), 256);
         j.bnd_off = bo, bo += round_up(2ull * g.kq, 64);
         j.cig_off = co, co += round_up((uint64_t)pp.qlen + pp.tlen + 2, 16);
         j.stripe_off = stripes, stripes += (g.kt + SR - 1) / SR;
@@ -397,7 +422,7 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
     if (p->n_fast > 0) {
         double total_steps = 0;
         for (const KswJob &j : p->jobs) total_steps += pair_steps(j);
-        const double T = std::max(1.0, total_steps / std::max(1, stripe_resident_waves(kGroupLeft)));
+        const double T = std::max(1.0, total_steps / std::max(1, stripe_resident_waves(kGroupLeft, R)));
         int forced = env ? env->team : 0;
         if (const char *ev = getenv("KSW2_HIP_TEAM")) forced = atoi(ev);
         int wide_forced = 0;
@@ -420,7 +445,7 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         if (!wide_forced && ((!env && forced == 0) || (env && env->spread && forced >= 0))) {
             int64_t waves = 0;
             for (int k = 0; k < p->n_fast; ++k) waves += team[k];
-            if (4 * waves <= stripe_resident_waves(kGroupLeft))
+            if (4 * waves <= stripe_resident_waves(kGroupLeft, R))
                 for (int k = 0; k < p->n_fast; ++k)
                     if (team[k] > kWideGroup && team[k] <= kMaxTeam) spread[k] = 1;
         }
@@ -777,8 +802,8 @@ int plan_launch(Plan *p, const Progress *prog)
                     if (!l.wide) continue;
                     sa.order = p->d_order + l.off, sa.njobs = l.count;
                     sa.wctl = p->d_wctl + (int64_t)l.woff * kWideCtl, sa.wfirst = p->d_wfirst + l.wf;
-                    const int half = std::max(1, stripe_wide_capacity(l.grp) / 2);
-                    KSW_HIP_TRY(launch_stripe_wide(sa, l.grp, p->h_wfirst.data() + l.wf, half, st));
+                    const int half = std::max(1, stripe_wide_capacity(l.grp, p->R) / 2);
+                    KSW_HIP_TRY(launch_stripe_wide(sa, l.grp, p->R, p->h_wfirst.data() + l.wf, half, st));
                 }
                 if (!wait_slots_release(st, q)) return KSW2_HIP_EDEVICE;
             }
@@ -786,8 +811,8 @@ int plan_launch(Plan *p, const Progress *prog)
         for (const Plan::Launch &l : p->launches) {
             if (!l.wide) continue;
             sa.order = p->d_order + l.off, sa.njobs = l.count, sa.wctl = p->d_wctl + (int64_t)l.woff * kWideCtl;
-            if (wide_now) KSW_HIP_TRY(launch_stripe_wide_epilogue(sa, l.grp, st));
-            else KSW_HIP_TRY(launch_stripe(sa, l.grp, l.team, st));  // one-workgroup teams
+            if (wide_now) KSW_HIP_TRY(launch_stripe_wide_epilogue(sa, l.grp, p->R, st));
+            else KSW_HIP_TRY(launch_stripe(sa, l.grp, l.team, p->R, st));  // one-workgroup teams
         }
     }
     for (const Plan::Launch &l : p->launches) {
@@ -795,7 +820,7 @@ int plan_launch(Plan *p, const Progress *prog)
         hipStream_t st = lane(l.lane);
         if (!st) return KSW2_HIP_EDEVICE;
         sa.order = p->d_order + l.off, sa.njobs = l.count;
-        KSW_HIP_TRY(launch_stripe(sa, l.grp, l.team, st));
+        KSW_HIP_TRY(launch_stripe(sa, l.grp, l.team, p->R, st));
     }
     for (int k = 0; k < Plan::kLanes; ++k)
         if (used[k]) {
@@ -885,6 +910,9 @@ int plan_fetch(Plan *p, ksw_extz_t *ez, void *km, const std::vector<char> *skip)
             if (!skip || !(*skip)[k]) ez[p->order[k]].stopped = 1;  // stopped at r = 0
         return KSW2_HIP_OK;
     }
+    int todo = 0;
+    for (int k = 0; k < p->n; ++k) todo += !skip || !(*skip)[k];
+    if (todo == 0) return KSW2_HIP_OK;  // every pair taken from the host mirror
     std::vector<KswOut> outs(p->n);
     std::unique_ptr<uint32_t[]> cig(new uint32_t[p->cig_words]);  // only [lo, hi) is filled and read
     KSW_HIP_TRY(hipMemcpyAsync(outs.data(), p->d_out, sizeof(KswOut) * p->n, hipMemcpyDeviceToHost, p->stream));

@@ -154,42 +154,45 @@ KSW_HD inline int64_t edge_prefix(const KswScore &sc, int n)
     return s;
 }
 
-// Rows (target positions) handled by one lane of the stripe kernel; a
-// stripe is 64 lanes x kRowsPerLane rows, a pass is two stripes computed by
-// one wave (the second in the high halves, kWave steps behind).
+// Rows (target positions) handled by one lane of the stripe kernel: R = 8
+// (the throughput form) or 4 (half the rows per pass, for pairs whose
+// latency matters more than the device's throughput).  A stripe is 64 lanes
+// x R rows, a pass is two stripes computed by one wave (the second in the
+// high halves, kWave steps behind).
 constexpr int kRowsPerLane = 8;
 constexpr int kStripeRows = kWave * kRowsPerLane;
 
-KSW_HD inline int stripe_count(int tlen) { return (tlen + kStripeRows - 1) / kStripeRows; }
-KSW_HD inline int pass_count(int tlen) { return (stripe_count(tlen) + 1) / 2; }
+KSW_HD inline int stripe_count(int tlen, int R = kRowsPerLane) { return (tlen + kWave * R - 1) / (kWave * R); }
+KSW_HD inline int pass_count(int tlen, int R = kRowsPerLane) { return (stripe_count(tlen, R) + 1) / 2; }
 KSW_HD inline uint64_t pass_steps(int qlen) { return (uint64_t)qlen + 2 * kWave - 1; }
 
-// Direction-byte address of cell (i = target, j = query) in the pass layout:
-// stripe s = i / 512 is half h = s & 1 of pass p = s >> 1; lane k, row rho;
-// the lane visits column j of half h at step c = j + k + 64h; each step the
-// wave writes 64 lanes x 16 bytes, byte 4*(rho/2) + 2h + (rho&1) of a lane's
-// 16 being row rho of half h.
-KSW_HD inline uint64_t dir_index(int i, int j, int qlen)
+// Direction-byte address of cell (i = striped, j = swept) in the pass
+// layout: stripe s = i / 64R is half h = s & 1 of pass p = s >> 1; lane k,
+// row rho; the lane visits column j of half h at step c = j + k + 64h; each
+// step the wave writes 64 lanes x 2R bytes, byte 4*(rho/2) + 2h + (rho&1)
+// of a lane's 2R being row rho of half h.
+KSW_HD inline uint64_t dir_index(int i, int j, int qlen, int R = kRowsPerLane)
 {
-    const int s = i / kStripeRows, rem = i - s * kStripeRows, k = rem / kRowsPerLane, rho = rem % kRowsPerLane;
+    const int SR = kWave * R;
+    const int s = i / SR, rem = i - s * SR, k = rem / R, rho = rem % R;
     const int p = s >> 1, h = s & 1;
     const uint64_t c = (uint64_t)j + k + (uint64_t)kWave * h;
-    return (((uint64_t)p * pass_steps(qlen) + c) * kWave + (uint64_t)k) * 16 + 4 * (rho >> 1) + 2 * h + (rho & 1);
+    return (((uint64_t)p * pass_steps(qlen) + c) * kWave + (uint64_t)k) * (2 * R) + 4 * (rho >> 1) + 2 * h + (rho & 1);
 }
 
-KSW_HD inline uint64_t dir_bytes(int qlen, int tlen)
+KSW_HD inline uint64_t dir_bytes(int qlen, int tlen, int R = kRowsPerLane)
 {
-    return (uint64_t)pass_count(tlen) * pass_steps(qlen) * kWave * 16;
+    return (uint64_t)pass_count(tlen, R) * pass_steps(qlen) * kWave * (2 * R);
 }
 
-// Singletrack track byte of cell (i, j): 32 bytes per lane per step, the p
-// track (H(i,j) - H(i,j-1)) in the first 16 and the p2 track
-// (H(i,j) - H(i-1,j)) in the next 16, each in the direction-byte order;
+// Singletrack track byte of cell (i, j): 4R bytes per lane per step, the p
+// track (H(i,j) - H(i,j-1)) in the first 2R and the p2 track
+// (H(i,j) - H(i-1,j)) in the next 2R, each in the direction-byte order;
 // the stored byte is the negated value.  Twice dir_bytes() per pair.
-KSW_HD inline uint64_t track_index(int track, int i, int j, int qlen)
+KSW_HD inline uint64_t track_index(int track, int i, int j, int qlen, int R = kRowsPerLane)
 {
-    const uint64_t d = dir_index(i, j, qlen);
-    return (d >> 4) * 32 + (uint64_t)track * 16 + (d & 15);
+    const uint64_t d = dir_index(i, j, qlen, R);
+    return (d / (2 * R)) * (4 * R) + (uint64_t)track * (2 * R) + (d % (2 * R));
 }
 
 }  // namespace kswhip

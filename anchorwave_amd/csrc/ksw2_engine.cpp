@@ -201,9 +201,12 @@ Engine::Engine(int8_t m, const int8_t *mat, const KswScore &sc, const ScoreClass
     if (const char *sp = getenv("KSW2_HIP_SERVE_SPREAD")) serve_spread_ = atoi(sp) != 0;
     if (const char *bs = getenv("KSW2_HIP_BATCH_SPREAD")) batch_spread_ = atoi(bs) != 0;
     if (const char *ff = getenv("KSW2_HIP_FAIL_FLIGHTS")) fail_flights_ = std::max(0, atoi(ff));
-    for (int s = 0; s < kStreams; ++s) stream_class_[s] = kStreamClass[s];
+    const char *hq = getenv("GPU_MAX_HW_QUEUES");
+    nstreams_ = std::max(1, std::min(kMaxStreams, hq ? atoi(hq) : 4));
+    if (const char *ns = getenv("KSW2_HIP_STREAMS")) nstreams_ = std::max(1, std::min(kMaxStreams, atoi(ns)));
+    for (int s = 0; s < kMaxStreams; ++s) stream_class_[s] = kStreamClass[s % 4];
     if (const char *sc = getenv("KSW2_HIP_STREAM_CLASSES"))  // e.g. "2210" (tuning)
-        for (int s = 0; s < kStreams && sc[s]; ++s)
+        for (int s = 0; s < nstreams_ && sc[s]; ++s)
             if (sc[s] >= '0' && sc[s] < '0' + kClasses) stream_class_[s] = sc[s] - '0';
 }
 
@@ -232,8 +235,8 @@ int Engine::start(bool cached)
     st = pool->init(budget_, cached);
     budget_ = pool->bytes;  // a cached block may be larger than asked
     if (st != KSW2_HIP_OK) return st;
-    for (auto &s : streams_)
-        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return KSW2_HIP_EDEVICE;
+    for (int s = 0; s < nstreams_; ++s)
+        if (hipStreamCreateWithFlags(&streams_[s], hipStreamNonBlocking) != hipSuccess) return KSW2_HIP_EDEVICE;
     resident_waves_ = std::max(1, stripe_resident_waves(kGroupLeft));
     if (hipHostMalloc((void **)&h_stop_, sizeof(int32_t), hipHostMallocMapped) != hipSuccess) return KSW2_HIP_ENOMEM;
     *h_stop_ = 0;
@@ -266,6 +269,14 @@ Engine::~Engine()
                 "[ksw2 engine] flights %lld tasks %lld | host s: create %.3f launch %.3f fetch+deliver %.3f | "
                 "device s (sum over flights): %.3f\n",
                 (long long)st_flights_, (long long)st_tasks_, st_create_, st_launch_, st_fetch_, st_device_);
+    if (stats_ && !st_wait_.empty()) {
+        auto med = [](std::vector<float> v) {
+            std::nth_element(v.begin(), v.begin() + (long)(v.size() / 2), v.end());
+            return v[v.size() / 2] * 1e3;
+        };
+        fprintf(stderr, "[ksw2 engine] per task p50 ms: queued %.2f, launch to delivery %.2f (%zu tasks)\n",
+                med(st_wait_), med(st_serv_), st_wait_.size());
+    }
 }
 
 bool Engine::queues_empty() const
@@ -408,6 +419,10 @@ int Engine::poll(uint64_t *tickets, double *latency, int max, int wait_ms, int32
         auto it = tasks_.find(tk);
         if (it != tasks_.end()) {
             if (latency) latency[n - 1] = it->second->t_done - it->second->t_submit;
+            if (stats_ && it->second->t_start > 0) {
+                st_wait_.push_back((float)(it->second->t_start - it->second->t_submit));
+                st_serv_.push_back((float)(it->second->t_done - it->second->t_start));
+            }
             if (status) status[n - 1] = it->second->status;
             delete it->second;
             tasks_.erase(it);
@@ -605,14 +620,14 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
 {
     bool any = false;
     if (hold_) return false;
-    for (int s = 0; s < kStreams; ++s) {
+    for (int s = 0; s < nstreams_; ++s) {
         if (busy_[s]) continue;
         const int64_t free_waves = batch_ ? resident_waves_ : resident_waves_ - busy_waves_;
         if (free_waves <= 0) break;  // the device is full: wait for pairs to complete
         // a stream serves its own class first; the long-task streams also take
         // shorter tasks when no long ones wait, the others never take long ones
         static const int pref[kClasses][kClasses] = {{0, 1, -1}, {1, 0, -1}, {2, 1, 0}};
-        const int own = batch_ ? kBatchStreamClass[s] : stream_class_[s];
+        const int own = batch_ ? kBatchStreamClass[s % 4] : stream_class_[s];
         int cls = -1;
         for (int k = 0; k < kClasses && pref[own][k] >= 0; ++k)
             if (!queue_[pref[own][k]].empty()) {
@@ -669,6 +684,17 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
                         now_s(), s, cls, free_block / 1e9, q.front()->footprint / 1e9, skipped);
             continue;
         }
+        // Serving flights whose teams are capped by the pairs' pass count
+        // (the device has waves to spare) run four rows per lane: twice the
+        // passes, so twice the waves share each pair's sweep, while the team
+        // stays within one workgroup (kMaxTeam waves) and the flight within
+        // half the free waves.  Team keys and passes are in eight-row passes,
+        // so a flight with a forced team runs eight rows otherwise.
+        int rows = key > 0 ? 8 : 0, team = std::max(key, 0);
+        if (!batch_ && key > 0 && key == head->passes && 2 * key <= kMaxTeam && 4 * waves <= free_waves) {
+            rows = 4, team = 2 * key, waves *= 2;
+            for (Task *t : pick) t->held *= 2;
+        }
         // pinned upload buffer of this stream, grown to the flight's sequences
         if (staging_bytes_[s] < seq) {
             if (staging_[s]) (void)hipHostFree(staging_[s]);
@@ -709,7 +735,7 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
         env.staging = staging_[s], env.staging_bytes = staging_bytes_[s];
         env.h_stop = h_stop_, env.d_stop = d_stop_;
         env.poll_stop = true;  // stop_all_locked raises it
-        env.team = std::max(key, 0);
+        env.team = team, env.rows = rows;
         env.spread = batch_ ? batch_spread_ : serve_spread_;  // a lone flight's teams spread over CUs
         if (mr.slots) {
             env.mirror_out = mr.out, env.mirror_out_dev = mr.out_dev;

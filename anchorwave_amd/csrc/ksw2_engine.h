@@ -19,14 +19,18 @@ namespace kswhip {
 
 class Engine {
 public:
-    static constexpr int kStreams = 4;             // flights in flight at once
+    // Flights in flight at once: one stream each, as many as the process's
+    // hardware queues (HIP's GPU_MAX_HW_QUEUES, 4 by default; streams beyond
+    // them would share a queue and run one after another), at most
+    // kMaxStreams; KSW2_HIP_STREAMS overrides (tuning).
+    static constexpr int kMaxStreams = 16;
     static constexpr int kClasses = 3;             // task size classes
-    // stream -> size class it serves first: one for the long tasks, one for
-    // the medium and two for the short ones (most tasks), so short tasks
-    // never queue behind long ones
-    static constexpr int kStreamClass[kStreams] = {2, 1, 0, 0};
-    // batch mode: the long tasks hold most of the work, so two streams
-    static constexpr int kBatchStreamClass[kStreams] = {2, 2, 1, 0};
+    // stream s -> size class it serves first, kStreamClass[s % 4]: per four
+    // streams one for the long tasks, one for the medium and two for the
+    // short ones (most tasks), so short tasks never queue behind long ones
+    static constexpr int kStreamClass[4] = {2, 1, 0, 0};
+    // batch mode: the long tasks hold most of the work, so two streams in four
+    static constexpr int kBatchStreamClass[4] = {2, 2, 1, 0};
     // share of the pool one flight of a class may hold: long pairs hold
     // memory longest per byte (their speed is capped by one workgroup), so
     // they never take the whole pool and shorter pairs keep the device full
@@ -74,7 +78,8 @@ private:
     int backfill_ = 1024;       // tasks a flight may pass over in the queue (KSW2_HIP_BACKFILL, tuning)
     bool serve_spread_ = true;  // serving flights may spread 5-16 wave teams (KSW2_HIP_SERVE_SPREAD, tuning)
     bool batch_spread_ = false; // batch flights likewise (KSW2_HIP_BATCH_SPREAD, tuning)
-    int stream_class_[kStreams] = {};  // serving: each stream's own size class (KSW2_HIP_STREAM_CLASSES, tuning)
+    int nstreams_ = 4;          // streams in use (hardware queues; KSW2_HIP_STREAMS, tuning)
+    int stream_class_[kMaxStreams] = {};  // serving: each stream's own size class (KSW2_HIP_STREAM_CLASSES, tuning)
     int fail_flights_ = 0;      // tests: the first this many flights fail before launch (KSW2_HIP_FAIL_FLIGHTS)
     struct Task;
     struct Flight;
@@ -91,8 +96,8 @@ private:
     int device_ = 0;
     int resident_waves_ = 4096;
     std::unique_ptr<DevicePool> pool_;
-    hipStream_t streams_[kStreams] = {};
-    bool busy_[kStreams] = {};
+    hipStream_t streams_[kMaxStreams] = {};
+    bool busy_[kMaxStreams] = {};
     // per-stream host mirror (pinned, mapped, coherent) of flight results
     struct Mirror {
         KswOut *out = nullptr, *out_dev = nullptr;
@@ -107,9 +112,9 @@ private:
             *this = Mirror();
         }
     };
-    Mirror mirror_[kStreams];
-    uint8_t *staging_[kStreams] = {};              // pinned upload buffers, per stream
-    uint64_t staging_bytes_[kStreams] = {};
+    Mirror mirror_[kMaxStreams];
+    uint8_t *staging_[kMaxStreams] = {};           // pinned upload buffers, per stream
+    uint64_t staging_bytes_[kMaxStreams] = {};
     int32_t *h_stop_ = nullptr, *d_stop_ = nullptr;  // stop flag shared by all flights
     std::thread worker_;
     std::mutex mu_;
@@ -137,6 +142,7 @@ private:
     // KSW2_HIP_ENGINE_STATS: per-phase host time, printed when the engine ends
     int stats_ = 0;  // 1: totals, 2: also one line per flight
     double st_create_ = 0, st_launch_ = 0, st_fetch_ = 0, st_device_ = 0;
+    std::vector<float> st_wait_, st_serv_;  // per delivered task: queued, launched-to-delivered (s)
     int64_t st_flights_ = 0, st_tasks_ = 0;
 };
 

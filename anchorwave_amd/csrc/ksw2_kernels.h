@@ -121,8 +121,9 @@ hipError_t launch_general_walk(const GenWalkArgs &a, hipStream_t st);
 // right gap alignment, score only, or the two Singletrack tracks.
 enum StripeGroup { kGroupLeft = 0, kGroupRight = 1, kGroupScoreOnly = 2, kGroupTracks = 3, kGroups = 4 };
 
-// Stripe kernel: one workgroup of `team` waves (1..kMaxTeam) per job.
-hipError_t launch_stripe(const StripeArgs &a, int group, int team, hipStream_t st);
+// Stripe kernel: one workgroup of `team` waves (1..kMaxTeam) per job, R
+// rows per lane (8 or 4; the jobs' layouts must use the same R).
+hipError_t launch_stripe(const StripeArgs &a, int group, int team, int R, hipStream_t st);
 
 // Wide stripe kernel: each job's passes spread over its own number of
 // workgroups of kWideGroup waves on different CUs (a team of up to kMaxWide
@@ -138,14 +139,15 @@ constexpr int kWideGroup = 4;       // waves per workgroup: one per SIMD of a CU
 constexpr int kWideCtlStride = 16;  // dwords between two waves' progress counts
 constexpr int kWideCtlLate = kMaxWide * kWideCtlStride;  // raised when a wait gave up
 constexpr int kWideCtl = kWideCtlLate + 64;              // dwords per job
-hipError_t launch_stripe_wide(const StripeArgs &a, int group, const int32_t *h_wfirst, int max_wgs, hipStream_t st);
+hipError_t launch_stripe_wide(const StripeArgs &a, int group, int R, const int32_t *h_wfirst, int max_wgs,
+                              hipStream_t st);
 // Workgroups of the wide stripe kernel the device holds at once.
-int stripe_wide_capacity(int group);
-hipError_t launch_stripe_wide_epilogue(const StripeArgs &a, int group, hipStream_t st);
+int stripe_wide_capacity(int group, int R);
+hipError_t launch_stripe_wide_epilogue(const StripeArgs &a, int group, int R, hipStream_t st);
 // Zeroes a plan's wide redo count (kWideCtl dwords at werr).
 hipError_t clear_wide_err(int32_t *werr, hipStream_t st);
 // Waves of the stripe kernel that fit on the device at once.
-int stripe_resident_waves(int group);
+int stripe_resident_waves(int group, int R = kRowsPerLane);
 
 // Zero-fills `bytes` bytes at `p` (16-byte aligned) with a kernel on `st`
 // (a fill kernel, not a memset, so that a captured graph replays it).

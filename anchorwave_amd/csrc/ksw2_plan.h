@@ -108,6 +108,7 @@ struct PlanEnv {
     uint32_t *mirror_cig = nullptr, *mirror_cig_dev = nullptr;
     int32_t *mirror_flag = nullptr, *mirror_flag_dev = nullptr;
     int team = 0;                    // > 0: every stripe-kernel pair gets this team size
+    int rows = 0;                    // 4 or 8: the stripe kernel's rows per lane (0: the plan picks)
     bool spread = false;             // 5-16 wave teams may run spread (as in a plan of its own)
     // The plan's four timing events, created by the caller (the plan then
     // creates none and destroys none); nullptr: the plan creates its own.

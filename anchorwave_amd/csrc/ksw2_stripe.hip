@@ -63,8 +63,6 @@ namespace {
 
 constexpr int kDppWaveShl1 = 0x130;
 constexpr int kDppWaveRor1 = 0x13C;
-constexpr int R = kRowsPerLane;  // rows per lane
-constexpr int SR = kWave * R;    // rows per stripe
 constexpr int kLag = kWave;      // steps the high stripe trails the low one
 
 // What a pass stores per cell.
@@ -294,6 +292,7 @@ __device__ __forceinline__ void store_entry(int32_t *dst, int a, int b)
 // half of Us): H(i,qlen-1) = H(-1,qlen-1) + sum_{i'<=i} u(i'), kept in the
 // -8 domain; the smallest running sum (largest H) with ties to the larger
 // row, and the stripe total.
+template <int R>
 __device__ __forceinline__ void stripe_rec(const v2s (&Us)[R], bool hi, int i0, int tlen, int lane, StripeRec *out)
 {
     int u[R], lsum = 0;
@@ -336,10 +335,11 @@ __device__ __forceinline__ void stripe_rec(const v2s (&Us)[R], bool hi, int i0, 
 // the cells a half computes outside its columns never reach a result.
 // WIDE: the team spans several CUs (progress counts in device memory,
 // kWideCtlStride dwords apart; a wait that gives up sets `late`).
-template <bool RIGHT, int MODE, bool WIDE, bool LUT>
+template <bool RIGHT, int MODE, bool WIDE, bool LUT, int R>
 __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc, int job_id, int p, const KswJob &jb,
                                           int qlen, int tlen, int32_t *prog, int W, int &late, int &stop)
 {
+    constexpr int SR = kWave * R;  // rows per stripe
     const KswScore &sc = a.sc;
     const int nstripes = (tlen + SR - 1) / SR, npass = (nstripes + 1) / 2;
     const bool first = p == 0, last = p == npass - 1, has_hi = 2 * p + 1 < nstripes;
@@ -488,16 +488,24 @@ __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc,
         }
         Xo = bits(X), X2o = bits(X2), Vo = bits(V), Qo = Qw;
         if (MODE == kDirs) {
-            // 16 bytes per lane: byte 4g + 2h + (rho & 1) is row 2g + (rho & 1)
+            // 2R bytes per lane: byte 4g + 2h + (rho & 1) is row 2g + (rho & 1)
             // of half h
-            uint4 w;
-            w.x = __builtin_amdgcn_perm((unsigned)d[1], (unsigned)d[0], 0x06020400u);
-            w.y = __builtin_amdgcn_perm((unsigned)d[3], (unsigned)d[2], 0x06020400u);
-            w.z = __builtin_amdgcn_perm((unsigned)d[5], (unsigned)d[4], 0x06020400u);
-            w.w = __builtin_amdgcn_perm((unsigned)d[7], (unsigned)d[6], 0x06020400u);
-            *(uint4 *)(dir + (((uint64_t)p * nsteps + c) * kWave + lane) * 16) = w;
+            uint8_t *dst = dir + (((uint64_t)p * nsteps + c) * kWave + lane) * (2 * R);
+            if (R == 8) {
+                uint4 w;
+                w.x = __builtin_amdgcn_perm((unsigned)d[1], (unsigned)d[0], 0x06020400u);
+                w.y = __builtin_amdgcn_perm((unsigned)d[3], (unsigned)d[2], 0x06020400u);
+                w.z = __builtin_amdgcn_perm((unsigned)d[5 % R], (unsigned)d[4 % R], 0x06020400u);
+                w.w = __builtin_amdgcn_perm((unsigned)d[7 % R], (unsigned)d[6 % R], 0x06020400u);
+                *(uint4 *)dst = w;
+            } else {
+                uint2 w;
+                w.x = __builtin_amdgcn_perm((unsigned)d[1], (unsigned)d[0], 0x06020400u);
+                w.y = __builtin_amdgcn_perm((unsigned)d[3], (unsigned)d[2], 0x06020400u);
+                *(uint2 *)dst = w;
+            }
         } else if (MODE == kTracks) {
-            // 32 bytes per lane: the p track (new V) then the p2 track (new
+            // 4R bytes per lane: the p track (new V) then the p2 track (new
             // U), each in the byte order of the direction bytes; a value is
             // stored as -8w >> 3 = -w (exact: both are multiples of 8)
             int vt[R], ut[R];
@@ -506,17 +514,26 @@ __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc,
                 vt[r] = bits(vec(d[r]) >> (v2s){3, 3});
                 ut[r] = bits(U[r] >> (v2s){3, 3});
             }
-            uint4 wv, wu;
-            wv.x = __builtin_amdgcn_perm((unsigned)vt[1], (unsigned)vt[0], 0x06020400u);
-            wv.y = __builtin_amdgcn_perm((unsigned)vt[3], (unsigned)vt[2], 0x06020400u);
-            wv.z = __builtin_amdgcn_perm((unsigned)vt[5], (unsigned)vt[4], 0x06020400u);
-            wv.w = __builtin_amdgcn_perm((unsigned)vt[7], (unsigned)vt[6], 0x06020400u);
-            wu.x = __builtin_amdgcn_perm((unsigned)ut[1], (unsigned)ut[0], 0x06020400u);
-            wu.y = __builtin_amdgcn_perm((unsigned)ut[3], (unsigned)ut[2], 0x06020400u);
-            wu.z = __builtin_amdgcn_perm((unsigned)ut[5], (unsigned)ut[4], 0x06020400u);
-            wu.w = __builtin_amdgcn_perm((unsigned)ut[7], (unsigned)ut[6], 0x06020400u);
-            uint4 *dst = (uint4 *)(dir + (((uint64_t)p * nsteps + c) * kWave + lane) * 32);
-            dst[0] = wv, dst[1] = wu;
+            uint8_t *dst = dir + (((uint64_t)p * nsteps + c) * kWave + lane) * (4 * R);
+            if (R == 8) {
+                uint4 wv, wu;
+                wv.x = __builtin_amdgcn_perm((unsigned)vt[1], (unsigned)vt[0], 0x06020400u);
+                wv.y = __builtin_amdgcn_perm((unsigned)vt[3], (unsigned)vt[2], 0x06020400u);
+                wv.z = __builtin_amdgcn_perm((unsigned)vt[5 % R], (unsigned)vt[4 % R], 0x06020400u);
+                wv.w = __builtin_amdgcn_perm((unsigned)vt[7 % R], (unsigned)vt[6 % R], 0x06020400u);
+                wu.x = __builtin_amdgcn_perm((unsigned)ut[1], (unsigned)ut[0], 0x06020400u);
+                wu.y = __builtin_amdgcn_perm((unsigned)ut[3], (unsigned)ut[2], 0x06020400u);
+                wu.z = __builtin_amdgcn_perm((unsigned)ut[5 % R], (unsigned)ut[4 % R], 0x06020400u);
+                wu.w = __builtin_amdgcn_perm((unsigned)ut[7 % R], (unsigned)ut[6 % R], 0x06020400u);
+                ((uint4 *)dst)[0] = wv, ((uint4 *)dst)[1] = wu;
+            } else {
+                uint2 wv, wu;
+                wv.x = __builtin_amdgcn_perm((unsigned)vt[1], (unsigned)vt[0], 0x06020400u);
+                wv.y = __builtin_amdgcn_perm((unsigned)vt[3], (unsigned)vt[2], 0x06020400u);
+                wu.x = __builtin_amdgcn_perm((unsigned)ut[1], (unsigned)ut[0], 0x06020400u);
+                wu.y = __builtin_amdgcn_perm((unsigned)ut[3], (unsigned)ut[2], 0x06020400u);
+                ((uint2 *)dst)[0] = wv, ((uint2 *)dst)[1] = wu;
+            }
         }
         if (!last) {
             // lane 63 appends the high stripe's bottom-row output (column
@@ -533,8 +550,8 @@ __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc,
             }
         }
     }
-    stripe_rec(Us, false, ilo, tlen, lane, a.recs + jb.stripe_off + 2 * p);
-    if (has_hi) stripe_rec(Us, true, ihi, tlen, lane, a.recs + jb.stripe_off + 2 * p + 1);
+    stripe_rec<R>(Us, false, ilo, tlen, lane, a.recs + jb.stripe_off + 2 * p);
+    if (has_hi) stripe_rec<R>(Us, true, ihi, tlen, lane, a.recs + jb.stripe_off + 2 * p + 1);
     if (last) {
         // score / mte from the lane that owns row tlen-1
         const int sc_row = __builtin_amdgcn_readlane(row_acc, lane_last);
@@ -584,7 +601,9 @@ __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc,
 // read in the reference's coordinates (i target, j query).  A transposed
 // pair stores cell (i, j) at (j, i) and its horizontal and vertical
 // differences exchanged.
+template <int R>
 struct StripeTracks {
+    static constexpr int SR = kWave * R;
     const uint8_t *dir;
     const uint8_t *qs, *ts;   // the reference's query and target
     uint64_t nsteps;
@@ -599,16 +618,18 @@ struct StripeTracks {
         const int s = i / SR, pp = s >> 1, h = s & 1, k = (i % SR) / R, rho = i % R;
         const uint64_t c = (uint64_t)(j + k + kLag * h);
         const uint8_t b =
-            dir[((pp * nsteps + c) * kWave + k) * 32 + track * 16 + 4 * (rho >> 1) + 2 * h + (rho & 1)];
+            dir[((pp * nsteps + c) * kWave + k) * (4 * R) + track * (2 * R) + 4 * (rho >> 1) + 2 * h + (rho & 1)];
         return -(int32_t)(int8_t)b;
     }
     __device__ int forced(int, int) const { return 0; }
     __device__ int32_t cost(int i, int j) const { return ts[i] == qs[j] ? mch : mis; }
 };
 
+template <int R>
 __device__ __forceinline__ void pair_epilogue(const StripeArgs &a, int job_id, const KswJob &jb,
                                               uint64_t (*tile)[64], bool tracks, int *shared_n, int nthr)
 {
+    constexpr int SR = kWave * R;  // rows per stripe
     const int tid = threadIdx.x;
     // the host's stop flag, read once for the whole workgroup: a stopped
     // pair's passes may have ended early, so it takes no walk
@@ -656,7 +677,7 @@ __device__ __forceinline__ void pair_epilogue(const StripeArgs &a, int job_id, c
     } else if (tracks) {
         if (!o.stopped && !o.zdropped && !(flag & (0x40 | 0x800))) {
             if (tid < kWave) {
-                StripeTracks A;
+                StripeTracks<R> A;
                 A.dir = a.dir + jb.dir_off;
                 A.qs = a.seq + (tr ? jb.t_off : jb.q_off), A.ts = a.seq + (tr ? jb.q_off : jb.t_off);
                 A.nsteps = (uint64_t)kq + kWave - 1 + kLag;
@@ -711,16 +732,21 @@ __device__ __forceinline__ void pair_epilogue(const StripeArgs &a, int job_id, c
                 const int c_lo = t_c_lo + kb + kLag * h, c_hi = t_c_hi + k + kLag * h;
                 __syncthreads();  // the previous tile is no longer read
                 for (int cc = c_lo + tid; cc <= c_hi; cc += nthr) {
-                    const uint4 *src = (const uint4 *)(dir + (((uint64_t)pp * nsteps + (uint64_t)cc) * kWave + kb) * 16);
+                    const uint8_t *src = dir + (((uint64_t)pp * nsteps + (uint64_t)cc) * kWave + kb) * (2 * R);
                     for (int kr = 0; kr <= k - kb; ++kr) {
                         const int jj = cc - kLag * h - (kb + kr);
                         if (jj >= t_c_lo && jj <= t_c_hi) {
-                            const uint4 w = src[kr];
-                            // the 8 row bytes of half h: bytes 2h, 2h+1 of each dword
+                            // the R row bytes of half h: bytes 2h, 2h+1 of each dword
                             const unsigned sel = h ? 0x07060302u : 0x05040100u;
-                            const uint32_t lo = __builtin_amdgcn_perm(w.y, w.x, sel);
-                            const uint32_t hi = __builtin_amdgcn_perm(w.w, w.z, sel);
-                            tile[kr][jj - t_c_lo] = (uint64_t)hi << 32 | lo;
+                            if (R == 8) {
+                                const uint4 w = ((const uint4 *)src)[kr];
+                                const uint32_t lo = __builtin_amdgcn_perm(w.y, w.x, sel);
+                                const uint32_t hi = __builtin_amdgcn_perm(w.w, w.z, sel);
+                                tile[kr][jj - t_c_lo] = (uint64_t)hi << 32 | lo;
+                            } else {
+                                const uint2 w = ((const uint2 *)src)[kr];
+                                tile[kr][jj - t_c_lo] = __builtin_amdgcn_perm(w.y, w.x, sel);
+                            }
                         }
                     }
                 }
@@ -765,9 +791,10 @@ __device__ __forceinline__ void pair_epilogue(const StripeArgs &a, int job_id, c
 // once (s_barrier then waits on the surviving waves only), so pairs of
 // different team sizes share one launch.  Wave w computes passes w, w+W,
 // ...; then the team runs the pair's epilogue (mqe, start cell, backtrack).
-template <bool RIGHT, int MODE>
+template <bool RIGHT, int MODE, int R>
 __global__ void __launch_bounds__(kWave * kMaxTeam) ksw_stripe_kernel(StripeArgs a)
 {
+    constexpr int SR = kWave * R;
     __shared__ int32_t prog[kMaxTeam];
     __shared__ uint64_t tile[8][64];
     __shared__ int shared_n;
@@ -784,21 +811,22 @@ __global__ void __launch_bounds__(kWave * kMaxTeam) ksw_stripe_kernel(StripeArgs
     int late = 0, stop = 0;  // (no bounded waits within a workgroup)
     if (jb.lut)
         for (int p = w; p < npass && !stop; p += W)
-            pass_body<RIGHT, MODE, false, true>(a, kc, job, p, jb, qlen, tlen, prog, W, late, stop);
+            pass_body<RIGHT, MODE, false, true, R>(a, kc, job, p, jb, qlen, tlen, prog, W, late, stop);
     else
         for (int p = w; p < npass && !stop; p += W)
-            pass_body<RIGHT, MODE, false, false>(a, kc, job, p, jb, qlen, tlen, prog, W, late, stop);
+            pass_body<RIGHT, MODE, false, false, R>(a, kc, job, p, jb, qlen, tlen, prog, W, late, stop);
     __syncthreads();  // every pass written: stripe partials, score / mte, direction bytes
-    pair_epilogue(a, job, jb, tile, MODE == kTracks, &shared_n, W * kWave);
+    pair_epilogue<R>(a, job, jb, tile, MODE == kTracks, &shared_n, W * kWave);
 }
 
 // Wide teams: each pair has its own run of consecutive blocks (a.wfirst),
 // which the dispatcher spreads over the XCDs' CUs; wave v of a team of W
 // waves computes passes v, v+W, ...  A pair's control block holds one
 // progress count per wave; a wave whose wait gave up raises its late word.
-template <bool RIGHT, int MODE>
+template <bool RIGHT, int MODE, int R>
 __global__ void __launch_bounds__(kWave * kMaxTeam) ksw_stripe_wide_kernel(StripeArgs a)
 {
+    constexpr int SR = kWave * R;
     // the pair of this block: wfirst[slot] <= block < wfirst[slot + 1]
     const int blk = a.wfirst[0] + (int)blockIdx.x;
     int lo = 0, hi = a.njobs;
@@ -820,10 +848,10 @@ __global__ void __launch_bounds__(kWave * kMaxTeam) ksw_stripe_wide_kernel(Strip
     int late = 0, stop = 0;
     if (jb.lut)
         for (int p = v; p < npass && !stop; p += W)
-            pass_body<RIGHT, MODE, true, true>(a, kc, job, p, jb, qlen, tlen, ctl, W, late, stop);
+            pass_body<RIGHT, MODE, true, true, R>(a, kc, job, p, jb, qlen, tlen, ctl, W, late, stop);
     else
         for (int p = v; p < npass && !stop; p += W)
-            pass_body<RIGHT, MODE, true, false>(a, kc, job, p, jb, qlen, tlen, ctl, W, late, stop);
+            pass_body<RIGHT, MODE, true, false, R>(a, kc, job, p, jb, qlen, tlen, ctl, W, late, stop);
     if ((late || a.force_late) && threadIdx.x % kWave == 0)
         __hip_atomic_store(ctl + kWideCtlLate, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -842,9 +870,10 @@ __global__ void __launch_bounds__(256) ksw_stripe_wctl_kernel(int32_t *wctl)
 // team of kWideGroup waves (LDS progress counts, no cross-workgroup waits),
 // into the same direction bytes, column buffer and stripe partials; the
 // plan's redo word counts such pairs.
-template <bool RIGHT, int MODE>
+template <bool RIGHT, int MODE, int R>
 __global__ void __launch_bounds__(kWave * kWideGroup) ksw_stripe_wide_epilogue_kernel(StripeArgs a)
 {
+    constexpr int SR = kWave * R;
     __shared__ int32_t prog[kWideGroup];
     __shared__ uint64_t tile[8][64];
     __shared__ int shared_n;
@@ -862,107 +891,141 @@ __global__ void __launch_bounds__(kWave * kWideGroup) ksw_stripe_wide_epilogue_k
         const int npass = ((tlen + SR - 1) / SR + 1) / 2;
         int late2 = 0, stop = 0;
         for (int p = w; p < npass && !stop; p += kWideGroup) {
-            if (jb.lut) pass_body<RIGHT, MODE, false, true>(a, kc, job, p, jb, qlen, tlen, prog, kWideGroup, late2, stop);
-            else pass_body<RIGHT, MODE, false, false>(a, kc, job, p, jb, qlen, tlen, prog, kWideGroup, late2, stop);
+            if (jb.lut) pass_body<RIGHT, MODE, false, true, R>(a, kc, job, p, jb, qlen, tlen, prog, kWideGroup, late2, stop);
+            else pass_body<RIGHT, MODE, false, false, R>(a, kc, job, p, jb, qlen, tlen, prog, kWideGroup, late2, stop);
         }
         __syncthreads();
         if (threadIdx.x == 0) atomicAdd(a.werr, 1);
     }
-    pair_epilogue(a, job, jb, tile, MODE == kTracks, &shared_n, (int)blockDim.x);
+    pair_epilogue<R>(a, job, jb, tile, MODE == kTracks, &shared_n, (int)blockDim.x);
 }
 
-template <bool RIGHT, int MODE>
-static hipError_t launch_stripe_t(const StripeArgs &a, int team, hipStream_t st)
+// Dispatch over the kernel variants: group (left, right, score-only,
+// tracks) and rows per lane (8 or 4).
+template <template <bool, int, int> class F, typename... Args>
+static auto by_variant(int group, int R, Args &&...args)
 {
-    hipLaunchKernelGGL((ksw_stripe_kernel<RIGHT, MODE>), dim3(a.njobs), dim3(kWave * team), 0, st, a);
-    return hipGetLastError();
+    const bool r8 = R != 4;
+    switch (group) {
+    case kGroupRight:
+        return r8 ? F<true, kDirs, 8>::run(args...) : F<true, kDirs, 4>::run(args...);
+    case kGroupScoreOnly:
+        return r8 ? F<false, kScoreOnly, 8>::run(args...) : F<false, kScoreOnly, 4>::run(args...);
+    case kGroupTracks:
+        return r8 ? F<false, kTracks, 8>::run(args...) : F<false, kTracks, 4>::run(args...);
+    default:
+        return r8 ? F<false, kDirs, 8>::run(args...) : F<false, kDirs, 4>::run(args...);
+    }
 }
 
-hipError_t launch_stripe(const StripeArgs &a, int group, int team, hipStream_t st)
+template <bool RIGHT, int MODE, int R>
+struct LaunchStripe {
+    static hipError_t run(const StripeArgs &a, int team, hipStream_t st)
+    {
+        hipLaunchKernelGGL((ksw_stripe_kernel<RIGHT, MODE, R>), dim3(a.njobs), dim3(kWave * team), 0, st, a);
+        return hipGetLastError();
+    }
+};
+
+hipError_t launch_stripe(const StripeArgs &a, int group, int team, int R, hipStream_t st)
 {
     if (a.njobs <= 0) return hipSuccess;
-    if (team < 1 || team > kMaxTeam) return hipErrorInvalidValue;
-    switch (group) {
-    case kGroupLeft: return launch_stripe_t<false, kDirs>(a, team, st);
-    case kGroupRight: return launch_stripe_t<true, kDirs>(a, team, st);
-    case kGroupScoreOnly: return launch_stripe_t<false, kScoreOnly>(a, team, st);
-    case kGroupTracks: return launch_stripe_t<false, kTracks>(a, team, st);
-    default: return hipErrorInvalidValue;
-    }
+    if (team < 1 || team > kMaxTeam || (R != 4 && R != 8) || group < 0 || group >= kGroups) return hipErrorInvalidValue;
+    return by_variant<LaunchStripe>(group, R, a, team, st);
 }
 
 namespace {
 
-template <bool RIGHT, int MODE>
-hipError_t launch_wide_t(const StripeArgs &a0, const int32_t *h_wfirst, int max_wgs, hipStream_t st)
-{
-    void (*fn)(StripeArgs) = ksw_stripe_wide_kernel<RIGHT, MODE>;
-    StripeArgs a = a0;
-    int off = 0;
-    while (off < a0.njobs) {
-        // as many pairs as fit in max_wgs workgroups (at least one)
-        int end = off + 1;
-        while (end < a0.njobs && h_wfirst[end + 1] - h_wfirst[off] <= max_wgs) ++end;
-        const int blocks = h_wfirst[end] - h_wfirst[off];
-        if (blocks > max_wgs) return hipErrorInvalidConfiguration;
-        a.order = a0.order + off;
-        a.wfirst = a0.wfirst + off;
-        a.wctl = a0.wctl + (int64_t)off * kWideCtl;
-        a.njobs = end - off;
-        hipLaunchKernelGGL(ksw_stripe_wctl_kernel, dim3(a.njobs), dim3(256), 0, st, a.wctl);
-        hipLaunchKernelGGL(fn, dim3(blocks), dim3(kWave * kWideGroup), 0, st, a);
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        off = end;
+template <bool RIGHT, int MODE, int R>
+struct LaunchWide {
+    static hipError_t run(const StripeArgs &a0, const int32_t *h_wfirst, int max_wgs, hipStream_t st)
+    {
+        void (*fn)(StripeArgs) = ksw_stripe_wide_kernel<RIGHT, MODE, R>;
+        StripeArgs a = a0;
+        int off = 0;
+        while (off < a0.njobs) {
+            // as many pairs as fit in max_wgs workgroups (at least one)
+            int end = off + 1;
+            while (end < a0.njobs && h_wfirst[end + 1] - h_wfirst[off] <= max_wgs) ++end;
+            const int blocks = h_wfirst[end] - h_wfirst[off];
+            if (blocks > max_wgs) return hipErrorInvalidConfiguration;
+            a.order = a0.order + off;
+            a.wfirst = a0.wfirst + off;
+            a.wctl = a0.wctl + (int64_t)off * kWideCtl;
+            a.njobs = end - off;
+            hipLaunchKernelGGL(ksw_stripe_wctl_kernel, dim3(a.njobs), dim3(256), 0, st, a.wctl);
+            hipLaunchKernelGGL(fn, dim3(blocks), dim3(kWave * kWideGroup), 0, st, a);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+            off = end;
+        }
+        return hipSuccess;
     }
-    return hipSuccess;
-}
+};
 
-template <bool RIGHT, int MODE>
-int wide_capacity_t()
-{
-    int dev = 0, cus = 0, per_cu = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)ksw_stripe_wide_kernel<RIGHT, MODE>,
-                                                     kWave * kWideGroup, 0) != hipSuccess)
-        return 0;
-    return cus * per_cu;
-}
+template <bool RIGHT, int MODE, int R>
+struct WideCapacity {
+    static int run()
+    {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)ksw_stripe_wide_kernel<RIGHT, MODE, R>,
+                                                         kWave * kWideGroup, 0) != hipSuccess)
+            return 0;
+        return cus * per_cu;
+    }
+};
+
+template <bool RIGHT, int MODE, int R>
+struct LaunchWideEpilogue {
+    static hipError_t run(const StripeArgs &a, hipStream_t st)
+    {
+        hipLaunchKernelGGL((ksw_stripe_wide_epilogue_kernel<RIGHT, MODE, R>), dim3(a.njobs), dim3(kWave * kWideGroup),
+                           0, st, a);
+        return hipGetLastError();
+    }
+};
+
+template <bool RIGHT, int MODE, int R>
+struct Resident {
+    static int run()
+    {
+        int per_cu = 0, dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 1024;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ksw_stripe_kernel<RIGHT, MODE, R>, kWave, 0) !=
+                hipSuccess ||
+            per_cu <= 0)
+            return prop.multiProcessorCount * 4;
+        return per_cu * prop.multiProcessorCount;
+    }
+};
 
 }  // namespace
 
-hipError_t launch_stripe_wide(const StripeArgs &a, int group, const int32_t *h_wfirst, int max_wgs, hipStream_t st)
+hipError_t launch_stripe_wide(const StripeArgs &a, int group, int R, const int32_t *h_wfirst, int max_wgs,
+                              hipStream_t st)
 {
     if (a.njobs <= 0) return hipSuccess;
     if (!a.wfirst || !h_wfirst || !a.wctl || !a.werr || max_wgs < 1) return hipErrorInvalidValue;
+    if ((R != 4 && R != 8) || group < 0 || group >= kGroups) return hipErrorInvalidValue;
     for (int k = 0; k < a.njobs; ++k) {
         const int n = h_wfirst[k + 1] - h_wfirst[k];
         if (n < 1 || n * kWideGroup > kMaxWide) return hipErrorInvalidValue;
     }
-    switch (group) {
-    case kGroupLeft: return launch_wide_t<false, kDirs>(a, h_wfirst, max_wgs, st);
-    case kGroupRight: return launch_wide_t<true, kDirs>(a, h_wfirst, max_wgs, st);
-    case kGroupScoreOnly: return launch_wide_t<false, kScoreOnly>(a, h_wfirst, max_wgs, st);
-    case kGroupTracks: return launch_wide_t<false, kTracks>(a, h_wfirst, max_wgs, st);
-    default: return hipErrorInvalidValue;
-    }
+    return by_variant<LaunchWide>(group, R, a, h_wfirst, max_wgs, st);
 }
 
 // Cached per variant: the occupancy query reads the device properties.
-int stripe_wide_capacity(int group)
+int stripe_wide_capacity(int group, int R)
 {
-    static int cache[kGroups] = {};
+    static int cache[kGroups][2] = {};
     if (group < 0 || group >= kGroups) group = kGroupLeft;
-    if (cache[group] > 0) return cache[group];
-    int n = 0;
-    switch (group) {
-    case kGroupRight: n = wide_capacity_t<true, kDirs>(); break;
-    case kGroupScoreOnly: n = wide_capacity_t<false, kScoreOnly>(); break;
-    case kGroupTracks: n = wide_capacity_t<false, kTracks>(); break;
-    default: n = wide_capacity_t<false, kDirs>(); break;
-    }
-    __atomic_store_n(&cache[group], n, __ATOMIC_RELAXED);
+    const int ri = R == 4 ? 1 : 0;
+    if (cache[group][ri] > 0) return cache[group][ri];
+    const int n = by_variant<WideCapacity>(group, R);
+    __atomic_store_n(&cache[group][ri], n, __ATOMIC_RELAXED);
     return n;
 }
 
@@ -972,45 +1035,23 @@ hipError_t clear_wide_err(int32_t *werr, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_stripe_wide_epilogue(const StripeArgs &a, int group, hipStream_t st)
+hipError_t launch_stripe_wide_epilogue(const StripeArgs &a, int group, int R, hipStream_t st)
 {
     if (a.njobs <= 0) return hipSuccess;
     if (!a.wctl || !a.werr) return hipErrorInvalidValue;
-    void (*fn)(StripeArgs) = group == kGroupTracks       ? ksw_stripe_wide_epilogue_kernel<false, kTracks>
-                             : group == kGroupScoreOnly ? ksw_stripe_wide_epilogue_kernel<false, kScoreOnly>
-                             : group == kGroupRight     ? ksw_stripe_wide_epilogue_kernel<true, kDirs>
-                                                        : ksw_stripe_wide_epilogue_kernel<false, kDirs>;
-    hipLaunchKernelGGL(fn, dim3(a.njobs), dim3(kWave * kWideGroup), 0, st, a);
-    return hipGetLastError();
-}
-
-template <bool RIGHT, int MODE>
-static int resident_t()
-{
-    int per_cu = 0, dev = 0;
-    hipDeviceProp_t prop;
-    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 1024;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ksw_stripe_kernel<RIGHT, MODE>, kWave, 0) !=
-            hipSuccess ||
-        per_cu <= 0)
-        return prop.multiProcessorCount * 4;
-    return per_cu * prop.multiProcessorCount;
+    if ((R != 4 && R != 8) || group < 0 || group >= kGroups) return hipErrorInvalidValue;
+    return by_variant<LaunchWideEpilogue>(group, R, a, st);
 }
 
 // Cached per variant: the occupancy query reads the device properties.
-int stripe_resident_waves(int group)
+int stripe_resident_waves(int group, int R)
 {
-    static int cache[kGroups] = {};
+    static int cache[kGroups][2] = {};
     if (group < 0 || group >= kGroups) group = kGroupLeft;
-    if (cache[group] > 0) return cache[group];
-    int n = 0;
-    switch (group) {
-    case kGroupRight: n = resident_t<true, kDirs>(); break;
-    case kGroupScoreOnly: n = resident_t<false, kScoreOnly>(); break;
-    case kGroupTracks: n = resident_t<false, kTracks>(); break;
-    default: n = resident_t<false, kDirs>(); break;
-    }
-    __atomic_store_n(&cache[group], n, __ATOMIC_RELAXED);
+    const int ri = R == 4 ? 1 : 0;
+    if (cache[group][ri] > 0) return cache[group][ri];
+    const int n = by_variant<Resident>(group, R);
+    __atomic_store_n(&cache[group][ri], n, __ATOMIC_RELAXED);
     return n;
 }
 

@@ -557,22 +557,29 @@ class Plan:
             raise KswError(f"ksw2_hip_plan_fetch: {STATUS.get(st, st)}")
         return out
 
-    def stored_transposed(self, index: int) -> bool:
-        """Whether pair ``index`` is stored transposed (its query in the
-        stripes and its target swept), from ``ksw2_hip_plan_dirs``."""
+    def stored_geometry(self, index: int) -> tuple[int, bool]:
+        """Rows per lane of the stripe kernel that ran pair ``index`` and
+        whether the pair is stored transposed (its query in the stripes and
+        its target swept), from ``ksw2_hip_plan_dirs``."""
         r, tr = C.c_int(), C.c_int()
         st = lib().ksw2_hip_plan_dirs(self.h, index, None, 0, C.byref(r), C.byref(tr))
         if st != 0:
             raise KswError(f"ksw2_hip_plan_dirs: {STATUS.get(st, st)}")
-        return bool(tr.value)
+        return r.value, bool(tr.value)
 
-    def _stored(self, index: int, qlen: int, tlen: int, per_lane: int):
-        """Raw pass-layout bytes of a pair and the byte index of every
-        stored cell (striped row i, swept column j) as a (kt, kq) array."""
-        tr = self.stored_transposed(index)
+    def stored_transposed(self, index: int) -> bool:
+        return self.stored_geometry(index)[1]
+
+    def _stored(self, index: int, qlen: int, tlen: int, tracks: int):
+        """Raw pass-layout bytes of a pair, the byte index of every stored
+        cell (striped row i, swept column j) as a (kt, kq) array, and the
+        bytes per lane of one track."""
+        R, tr = self.stored_geometry(index)
         kq, kt = (tlen, qlen) if tr else (qlen, tlen)
-        npass = ((kt + 511) // 512 + 1) // 2
+        SR = 64 * R
+        npass = ((kt + SR - 1) // SR + 1) // 2
         nsteps = kq + 127
+        per_lane = 2 * R * tracks
         nbytes = npass * nsteps * 64 * per_lane
         buf = np.zeros(nbytes, dtype=np.uint8)
         r, t2 = C.c_int(), C.c_int()
@@ -581,23 +588,24 @@ class Plan:
             raise KswError(f"ksw2_hip_plan_dirs: {STATUS.get(st, st)}")
         i = np.arange(kt)[:, None]
         j = np.arange(kq)[None, :]
-        s, rem = i // 512, i % 512
+        s, rem = i // SR, i % SR
         p, h = s // 2, s % 2
-        k, rho = rem // 8, rem % 8
+        k, rho = rem // R, rem % R
         c = j + k + 64 * h
         idx = ((p * nsteps + c) * 64 + k) * per_lane + 4 * (rho // 2) + 2 * h + rho % 2
-        return tr, buf, idx
+        return tr, buf, idx, 2 * R
 
     def dirs(self, index: int, qlen: int, tlen: int, raw: bool = False) -> np.ndarray:
         """Direction bytes of pair ``index`` as a (tlen, qlen) matrix (or the
         raw pass-layout buffer when ``raw``).  Stored layout, with i the
-        striped index and j the swept one (the target and the query, or the
-        query and the target for a transposed pair): stripe s = i // 512 is
-        half h = s & 1 of pass p = s >> 1; lane k = (i % 512) // 8, row
-        rho = i % 8; column j of that half is written at step c = j + k + 64h;
-        byte ((p * (swept + 127) + c) * 64 + k) * 16 + 4 * (rho // 2) + 2h +
+        striped index, j the swept one (the target and the query, or the
+        query and the target for a transposed pair) and R the rows per lane
+        (8 or 4; a stripe is 64R rows): stripe s = i // 64R is half h = s & 1
+        of pass p = s >> 1; lane k = (i % 64R) // R, row rho = i % R; column
+        j of that half is written at step c = j + k + 64h; byte
+        ((p * (swept + 127) + c) * 64 + k) * 2R + 4 * (rho // 2) + 2h +
         rho % 2.  The bytes are the reference's direction bytes either way."""
-        tr, buf, idx = self._stored(index, qlen, tlen, 16)
+        tr, buf, idx, _ = self._stored(index, qlen, tlen, 1)
         if raw:
             return buf
         m = buf[idx]
@@ -606,13 +614,13 @@ class Plan:
     def tracks(self, index: int, qlen: int, tlen: int) -> np.ndarray:
         """Singletrack tracks of pair ``index`` (flag KSW_EZ_SINGLETRACK) as a
         (2, tlen, qlen) int8 array: p = H(i,j)-H(i,j-1) and p2 =
-        H(i,j)-H(i-1,j).  Same cell order as ``dirs``, 32 bytes per lane and
+        H(i,j)-H(i-1,j).  Same cell order as ``dirs``, 4R bytes per lane and
         step (the negated stored-horizontal track, then the negated
-        stored-vertical one; a transposed pair's are the reference's p2 and
-        p)."""
-        tr, buf, idx = self._stored(index, qlen, tlen, 32)
+        stored-vertical one, 2R bytes each; a transposed pair's are the
+        reference's p2 and p)."""
+        tr, buf, idx, half = self._stored(index, qlen, tlen, 2)
         raw = buf.view(np.int8)
-        t0, t1 = -raw[idx].astype(np.int16), -raw[idx + 16].astype(np.int16)
+        t0, t1 = -raw[idx].astype(np.int16), -raw[idx + half].astype(np.int16)
         return np.stack([t1.T, t0.T]) if tr else np.stack([t0, t1])
 
     def close(self):

@@ -27,6 +27,11 @@ from concurrent.futures import ThreadPoolExecutor
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# Hardware queues of this process (read by the HIP runtime at its start, so
+# set before the library loads).  The library runs one stream per concurrent
+# flight (serving) or calling thread (ksw_extd2_hip); HIP's default of 4
+# queues makes the fifth such stream wait behind another's kernels.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 import numpy as np  # noqa: E402
 
@@ -466,7 +471,8 @@ def main() -> int:
                    "device_ms_dp": round(dp_avg, 3), "device_ms_backtrack": round(float(np.mean(bt_ms)), 3),
                    "timed": "plan runs on resident inputs; scores and CIGARs left in HBM",
                    "fetch_ms": round(fetch_ms, 3),
-                   "gcups_with_fetch": round(cells / ((ms_per_step + fetch_ms) * 1e-3) / 1e9, 2)},
+                   "gcups_with_fetch": round(cells / ((ms_per_step + fetch_ms) * 1e-3) / 1e9, 2),
+                   "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
         "roofline": roofline,
         "one_at_a_time_gcups": None if single is None else round(single, 3),
         "batched_over_one_at_a_time": None if not single else round(gcups / single, 1),
