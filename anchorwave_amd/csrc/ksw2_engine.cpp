@@ -198,7 +198,7 @@ Engine::Engine(int8_t m, const int8_t *mat, const KswScore &sc, const ScoreClass
     stats_ = ev ? atoi(ev) : 0;
     if (const char *mt = getenv("KSW2_HIP_MAX_TEAM")) max_team_ = std::max(1, std::min(kMaxWide, atoi(mt)));
     if (const char *bf = getenv("KSW2_HIP_BACKFILL")) backfill_ = std::max(0, atoi(bf));
-    if (const char *sp = getenv("KSW2_HIP_SERVE_SPREAD")) serve_spread_ = atoi(sp) != 0;
+    if (const char *sp = getenv("KSW2_HIP_SERVE_SPREAD")) serve_spread_ = atoi(sp);
     if (const char *bs = getenv("KSW2_HIP_BATCH_SPREAD")) batch_spread_ = atoi(bs) != 0;
     if (const char *ff = getenv("KSW2_HIP_FAIL_FLIGHTS")) fail_flights_ = std::max(0, atoi(ff));
     const char *hq = getenv("GPU_MAX_HW_QUEUES");
@@ -238,6 +238,7 @@ int Engine::start(bool cached)
     for (int s = 0; s < nstreams_; ++s)
         if (hipStreamCreateWithFlags(&streams_[s], hipStreamNonBlocking) != hipSuccess) return KSW2_HIP_EDEVICE;
     resident_waves_ = std::max(1, stripe_resident_waves(kGroupLeft));
+    if (stats_) fprintf(stderr, "[ksw2 engine] streams %d resident waves %d\n", nstreams_, resident_waves_);
     if (hipHostMalloc((void **)&h_stop_, sizeof(int32_t), hipHostMallocMapped) != hipSuccess) return KSW2_HIP_ENOMEM;
     *h_stop_ = 0;
     if (hipHostGetDevicePointer((void **)&d_stop_, h_stop_, 0) != hipSuccess) return KSW2_HIP_EDEVICE;
@@ -552,8 +553,8 @@ bool Engine::reap_locked(std::unique_lock<std::mutex> &lk)
                 int maxteam = 0;
                 for (const auto &l : p->launches) maxteam = std::max(maxteam, l.team);
                 fprintf(stderr, "[ksw2 engine] flight stream %d tasks %zu cells %.3g dp %.1f ms bt %.1f ms "
-                        "fast %d gen %d maxteam %d wall %.1f ms\n", f->stream, f->tasks.size(), f->cells,
-                        p->last_dp_ms, p->last_bt_ms, p->n_fast, p->n_gen, maxteam,
+                        "fast %d gen %d maxteam %d rows %d wide %d wall %.1f ms\n", f->stream, f->tasks.size(),
+                        f->cells, p->last_dp_ms, p->last_bt_ms, p->n_fast, p->n_gen, maxteam, p->R, p->n_wide,
                         (t - f->t_start) * 1e3);
             }
         }
@@ -687,9 +688,10 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
         // Serving flights whose teams are capped by the pairs' pass count
         // (the device has waves to spare) run four rows per lane: twice the
         // passes, so twice the waves share each pair's sweep, while the team
-        // stays within one workgroup (kMaxTeam waves) and the flight within
-        // half the free waves.  Team keys and passes are in eight-row passes,
-        // so a flight with a forced team runs eight rows otherwise.
+        // stays within one workgroup (kMaxTeam waves) and the flight within a
+        // quarter of the free waves.  Team keys and passes are in eight-row
+        // passes, so a flight with a forced team runs eight rows otherwise.
+        const bool light = queues_empty() && 8 * (busy_waves_ + waves) <= resident_waves_;
         int rows = key > 0 ? 8 : 0, team = std::max(key, 0);
         if (!batch_ && key > 0 && key == head->passes && 2 * key <= kMaxTeam && 4 * waves <= free_waves) {
             rows = 4, team = 2 * key, waves *= 2;
@@ -736,7 +738,13 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
         env.h_stop = h_stop_, env.d_stop = d_stop_;
         env.poll_stop = true;  // stop_all_locked raises it
         env.team = team, env.rows = rows;
-        env.spread = batch_ ? batch_spread_ : serve_spread_;  // a lone flight's teams spread over CUs
+        // a flight's teams may spread over CUs (tuning switch; off by
+        // default: with sixteen streams a spread team gains nothing at 20 %
+        // load, and under more load it holds waiting-launch slots that the
+        // flights needing wide teams then queue behind); 1 spreads only while
+        // the device is lightly loaded (nothing left queued, the flights in
+        // the device holding under an eighth of its waves)
+        env.spread = batch_ ? batch_spread_ : serve_spread_ == 2 || (serve_spread_ == 1 && light);
         if (mr.slots) {
             env.mirror_out = mr.out, env.mirror_out_dev = mr.out_dev;
             env.mirror_cig = mr.cig, env.mirror_cig_dev = mr.cig_dev;

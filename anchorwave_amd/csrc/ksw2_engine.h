@@ -76,7 +76,7 @@ public:
 private:
     int max_team_ = 64;         // largest team a flight is keyed on (kMaxWide; KSW2_HIP_MAX_TEAM, tuning)
     int backfill_ = 1024;       // tasks a flight may pass over in the queue (KSW2_HIP_BACKFILL, tuning)
-    bool serve_spread_ = true;  // serving flights may spread 5-16 wave teams (KSW2_HIP_SERVE_SPREAD, tuning)
+    int serve_spread_ = 0;      // serving flights spread 5-16 wave teams: 0 never, 1 under light load, 2 always (KSW2_HIP_SERVE_SPREAD, tuning)
     bool batch_spread_ = false; // batch flights likewise (KSW2_HIP_BATCH_SPREAD, tuning)
     int nstreams_ = 4;          // streams in use (hardware queues; KSW2_HIP_STREAMS, tuning)
     int stream_class_[kMaxStreams] = {};  // serving: each stream's own size class (KSW2_HIP_STREAM_CLASSES, tuning)
