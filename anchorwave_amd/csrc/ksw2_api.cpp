@@ -298,6 +298,14 @@ int ksw2_hip_plan_timing(const ksw2_hip_plan *plan, float *dp_ms, float *backtra
     return set_err(KSW2_HIP_OK);
 }
 
+int ksw2_hip_plan_graph(const ksw2_hip_plan *plan, int *runs, int *replays)
+{
+    if (!plan) return set_err(KSW2_HIP_EINVAL);
+    if (runs) *runs = plan->p->runs;
+    if (replays) *replays = plan->p->graph_replays;
+    return set_err(KSW2_HIP_OK);
+}
+
 int ksw2_hip_plan_dirs(const ksw2_hip_plan *plan, int index, uint8_t *out, size_t bytes, int *R, int *transposed)
 {
     if (!plan || index < 0 || index >= plan->p->n) return set_err(KSW2_HIP_EINVAL);

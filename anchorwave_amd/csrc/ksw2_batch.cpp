@@ -703,6 +703,73 @@ bool progress_continue(const Progress *prog, int &calls, int total)
     return prog->cb(prog->args, (int)r, total) != 0;
 }
 
+// Plans run again and again (a caller's resident batch, ksw2_hip_plan_run)
+// replay their launches as one captured graph from the second run on.
+// Eligible: stripe-kernel pairs only (general, band and wide launches are
+// left out: the waiting-launch slots are events shared between plans, which
+// a graph cannot wait on), no progress callback and no host mirror (both
+// change per run).  KSW2_HIP_GRAPH=0 turns it off.
+static bool graph_eligible(const Plan *p, const Progress *prog)
+{
+    const char *ev = getenv("KSW2_HIP_GRAPH");
+    if (ev && atoi(ev) == 0) return false;
+    return !(prog && prog->cb) && p->n_gen == 0 && p->n_wide == 0 && p->blaunches.empty() && !p->h_mflag &&
+           p->n_fast > 0;
+}
+
+static void stripe_args(const Plan *p, const int32_t *poll, StripeArgs &sa)
+{
+    sa.stop = poll;
+    sa.seq = p->d_seq, sa.dir = p->d_dir, sa.bnd = p->d_bnd, sa.cig = p->d_cig, sa.sc = p->sc;
+    sa.jobs = p->d_jobs, sa.out = p->d_out, sa.recs = p->d_recs;
+    sa.host_out = p->d_mout, sa.host_cig = p->d_mcig, sa.host_flag = p->d_mflag;
+    sa.wfirst = nullptr, sa.wctl = p->d_wctl, sa.werr = p->d_werr;
+    const char *wr = getenv("KSW2_HIP_WIDE_REDO");  // tests: every wide pair takes the redo path
+    sa.force_late = wr && atoi(wr) > 0 ? 1 : 0;
+}
+
+// Captures an eligible plan's stripe launches on its stream (forking to the
+// lane streams the first run created, joining back) and instantiates them.
+// A failed capture leaves the plan on direct launches for good.
+static void plan_capture(Plan *p)
+{
+    hipStream_t s = p->stream;
+    bool ok = p->fork_ev || hipEventCreateWithFlags(&p->fork_ev, hipEventDisableTiming) == hipSuccess;
+    if (ok) ok = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (!ok) {
+        p->graph_failed = true;
+        (void)hipGetLastError();
+        return;
+    }
+    StripeArgs sa;
+    stripe_args(p, p->poll_stop ? p->d_stop : nullptr, sa);
+    ok = hipEventRecord(p->fork_ev, s) == hipSuccess;
+    bool used[Plan::kLanes] = {false, false, false, false};
+    for (const Plan::Launch &l : p->launches) {
+        hipStream_t st = s;
+        if (p->own_streams) {
+            st = p->lane_stream[l.lane];
+            if (!st) ok = false;
+            if (ok && !used[l.lane]) ok = hipStreamWaitEvent(st, p->fork_ev, 0) == hipSuccess, used[l.lane] = true;
+        }
+        sa.order = p->d_order + l.off, sa.njobs = l.count;
+        if (ok) ok = launch_stripe(sa, l.grp, l.team, p->R, st) == hipSuccess;
+    }
+    for (int k = 0; k < Plan::kLanes; ++k)
+        if (used[k] && ok)
+            ok = hipEventRecord(p->lane_done[k], p->lane_stream[k]) == hipSuccess &&
+                 hipStreamWaitEvent(s, p->lane_done[k], 0) == hipSuccess;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    ok = ok && e == hipSuccess && g;
+    if (ok) ok = hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0) == hipSuccess;
+    if (g) (void)hipGraphDestroy(g);
+    if (!ok) {
+        p->graph = nullptr, p->graph_failed = true;
+        (void)hipGetLastError();
+    }
+}
+
 int plan_launch(Plan *p, const Progress *prog)
 {
     p->stop_requested = false;
@@ -717,7 +784,23 @@ int plan_launch(Plan *p, const Progress *prog)
     }
     if (p->h_stop && p->own_stop) *(volatile int32_t *)p->h_stop = 0;
     p->started = true;
+    const bool graphed = graph_eligible(p, prog);
+    if (graphed && !p->graph && !p->graph_failed && p->runs >= 1) plan_capture(p);
+    ++p->runs;
     KSW_HIP_TRY(hipEventRecord(p->ev[0], p->stream));
+    if (graphed && p->graph) {
+        if (getenv("KSW2_HIP_GRAPH_CHECK")) {
+            // tests: clear the results first (fill kernels, outside the
+            // graph), so a replay that wrote nothing cannot pass for one
+            KSW_HIP_TRY(launch_fill_zero(p->d_out, sizeof(KswOut) * (uint64_t)p->n, p->stream));
+            KSW_HIP_TRY(launch_fill_zero(p->d_cig, 4 * p->cig_words, p->stream));
+        }
+        KSW_HIP_TRY(hipGraphLaunch(p->graph, p->stream));
+        ++p->graph_replays;
+        KSW_HIP_TRY(hipEventRecord(p->ev[1], p->stream));
+        KSW_HIP_TRY(hipEventRecord(p->ev[2], p->stream));
+        return KSW2_HIP_OK;
+    }
     bool used[Plan::kLanes] = {false, false, false, false};
     auto lane = [&](int l) -> hipStream_t {
         if (!p->own_streams) return p->stream;
@@ -776,13 +859,7 @@ int plan_launch(Plan *p, const Progress *prog)
     // raised: a progress callback runs, or the flag's owner raises it
     const int32_t *poll = (p->poll_stop || (prog && prog->cb)) ? p->d_stop : nullptr;
     StripeArgs sa;
-    sa.stop = poll;
-    sa.seq = p->d_seq, sa.dir = p->d_dir, sa.bnd = p->d_bnd, sa.cig = p->d_cig, sa.sc = p->sc;
-    sa.jobs = p->d_jobs, sa.out = p->d_out, sa.recs = p->d_recs;
-    sa.host_out = p->d_mout, sa.host_cig = p->d_mcig, sa.host_flag = p->d_mflag;
-    sa.wfirst = nullptr, sa.wctl = p->d_wctl, sa.werr = p->d_werr;
-    const char *wr = getenv("KSW2_HIP_WIDE_REDO");  // tests: every wide pair takes the redo path
-    sa.force_late = wr && atoi(wr) > 0 ? 1 : 0;
+    stripe_args(p, poll, sa);
     if (p->n_wide > 0) {
         // wide launches (the longest pairs) first, within two quarters of
         // the device, then their epilogues, which do not wait
@@ -949,6 +1026,8 @@ void plan_destroy(Plan *p)
     if (p->block) (void)hipFree(p->block);
     if (p->h_stop && p->own_stop) (void)hipHostFree(p->h_stop);
     if (p->h_seq && p->own_seq) (void)hipHostFree(p->h_seq);
+    if (p->graph) (void)hipGraphExecDestroy(p->graph);
+    if (p->fork_ev) (void)hipEventDestroy(p->fork_ev);
     if (p->own_events)
         for (auto &e : p->ev)
             if (e) (void)hipEventDestroy(e);

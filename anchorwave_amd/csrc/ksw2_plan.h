@@ -198,6 +198,12 @@ struct Plan {
     hipStream_t lane_stream[kLanes] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t lane_done[kLanes] = {nullptr, nullptr, nullptr, nullptr};
     float last_dp_ms = 0.f, last_bt_ms = 0.f;
+    // the plan's launches captured as one graph (plan_launch, from the
+    // second run of an eligible plan), the capture's fork event, and counts
+    hipGraphExec_t graph = nullptr;
+    hipEvent_t fork_ev = nullptr;
+    bool graph_failed = false;
+    int runs = 0, graph_replays = 0;
     bool stop_requested = false;     // the progress callback stopped the last run
     bool started = false;            // the last run launched (not stopped at r = 0)
     int max_diags = 0;               // largest qlen + tlen - 1 in the plan

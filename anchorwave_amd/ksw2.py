@@ -60,6 +60,7 @@ EXPORTS = (
     "ksw2_hip_plan_run",
     "ksw2_hip_plan_fetch",
     "ksw2_hip_plan_timing",
+    "ksw2_hip_plan_graph",
     "ksw2_hip_plan_destroy",
     "ksw2_hip_plan_dirs",
     "ksw2_hip_certify_batch",
@@ -181,6 +182,8 @@ def lib():
         L.ksw2_hip_plan_timing.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                            C.POINTER(C.c_double)]
         L.ksw2_hip_plan_timing.restype = C.c_int
+        L.ksw2_hip_plan_graph.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.ksw2_hip_plan_graph.restype = C.c_int
         L.ksw2_hip_plan_dirs.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_int),
                                          C.POINTER(C.c_int)]
         L.ksw2_hip_plan_dirs.restype = C.c_int
@@ -544,6 +547,14 @@ class Plan:
         dp, bt, cells = C.c_float(), C.c_float(), C.c_double()
         lib().ksw2_hip_plan_timing(self.h, C.byref(dp), C.byref(bt), C.byref(cells))
         return dp.value, bt.value, cells.value
+
+    def graph_stats(self) -> tuple[int, int]:
+        """(runs so far, runs that replayed the captured hipGraph)."""
+        runs, rep = C.c_int(), C.c_int()
+        st = lib().ksw2_hip_plan_graph(self.h, C.byref(runs), C.byref(rep))
+        if st != 0:
+            raise KswError(f"ksw2_hip_plan_graph: {STATUS.get(st, st)}")
+        return runs.value, rep.value
 
     def fetch(self) -> List[Result]:
         ezs = (KswExtz * max(self.n, 1))()

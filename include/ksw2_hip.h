@@ -206,6 +206,11 @@ int ksw2_hip_plan_fetch(ksw2_hip_plan *plan, ksw_extz_t *ez);
 /* device time of the last run: DP sweep kernels and backtrack kernel, from
  * HIP events on the plan's stream; also the sum of qlen*tlen */
 int ksw2_hip_plan_timing(const ksw2_hip_plan *plan, float *dp_ms, float *backtrack_ms, double *cells);
+/* runs of the plan so far, and how many of them replayed the plan's
+ * launches as one captured hipGraph (from the second run of a plan of
+ * stripe-kernel pairs only, no progress callback; KSW2_HIP_GRAPH=0 turns
+ * capture off) */
+int ksw2_hip_plan_graph(const ksw2_hip_plan *plan, int *runs, int *replays);
 void ksw2_hip_plan_destroy(ksw2_hip_plan *plan);
 /* ---- Serving (no reference counterpart: the request loop that keeps the
  * device busy while interval tasks arrive, replacing AnchorWave's
