@@ -169,13 +169,18 @@ int fast_path_reject(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls)
 // take P * (qlen + 127) steps on one wave gets the smallest power-of-two team
 // that brings it under 2T, capped by its pass count and kMaxWide: small
 // pairs run on one wave with no synchronisation, long pairs do not outlast
-// the batch.  Teams above kMaxTeam waves are wide (workgroups of kWideGroup
+// the batch.  With a per-wave byte budget (an engine streaming a batch larger
+// than device memory), the team also grows until each wave holds at most that
+// many direction bytes: a pair holds its bytes from its first pass to its
+// walk, so memory held times the time it is held shrinks with the team, and
+// it is that product, not the waves, that limits such a batch.  Teams above
+// kMaxTeam waves are wide (workgroups of kWideGroup
 // waves on different CUs, rounded up to whole workgroups).  KSW2_HIP_TEAM
 // forces a size; KSW2_HIP_WIDE=k runs every pair wide on k workgroups
 // (tuning and tests).
 static double pair_steps(const KswJob &j) { return (double)pass_count(j.tlen, j.R) * (double)pass_steps(j.qlen); }
 
-static int choose_team(const KswJob &j, double T, int forced)
+static int choose_team(const KswJob &j, double T, int forced, uint64_t wave_bytes = 0)
 {
     const int P = pass_count(j.tlen, j.R);
     int team = 1;
@@ -183,7 +188,12 @@ static int choose_team(const KswJob &j, double T, int forced)
         team = std::max(1, std::min(std::min(forced, kMaxWide), P));
     } else {
         const double steps = pair_steps(j);
-        while (team < kMaxWide && team < P && steps / team > 2 * T) team <<= 1;
+        // memory: at most wave_bytes of direction bytes per wave
+        const double bytes = (j.flag & KSW_EZ_SCORE_ONLY) ? 0.0
+                             : (double)(track_pair(j.flag) ? 2 : 1) * (double)dir_bytes(j.qlen, j.tlen, j.R);
+        while (team < kMaxWide && team < P &&
+               (steps / team > 2 * T || (wave_bytes > 0 && bytes > (double)wave_bytes * team)))
+            team <<= 1;
         team = std::min(team, P);
     }
     return team > kMaxTeam ? (team + kWideGroup - 1) / kWideGroup * kWideGroup : team;
@@ -422,7 +432,10 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
     if (p->n_fast > 0) {
         double total_steps = 0;
         for (const KswJob &j : p->jobs) total_steps += pair_steps(j);
-        const double T = std::max(1.0, total_steps / std::max(1, stripe_resident_waves(kGroupLeft, R)));
+        const double T = env && env->team_T > 0
+                             ? env->team_T
+                             : std::max(1.0, total_steps / std::max(1, stripe_resident_waves(kGroupLeft, R)));
+        const uint64_t wave_bytes = env ? env->wave_bytes : 0;
         int forced = env ? env->team : 0;
         if (const char *ev = getenv("KSW2_HIP_TEAM")) forced = atoi(ev);
         int wide_forced = 0;
@@ -431,7 +444,7 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         for (int k = 0; k < p->n_fast; ++k) {
             // KSW2_HIP_TEAM=-1 (tests): team sizes 1..16 in turn, one launch
             const int f = forced < 0 ? 1 + k % kMaxTeam : forced;
-            team[k] = wide_forced ? kWideGroup * wide_forced : choose_team(p->jobs[k], T, f);
+            team[k] = wide_forced ? kWideGroup * wide_forced : choose_team(p->jobs[k], T, f, wave_bytes);
             p->jobs[k].team = team[k];
         }
         // a plan of its own or a serving flight (PlanEnv::spread) that leaves
@@ -960,6 +973,16 @@ int plan_run(Plan *p, const Progress *prog)
 }
 
 // One slot's result into the caller's ez (CIGAR grown with krealloc/realloc).
+bool plan_dir_span(const Plan *p, int k, uint64_t *off, uint64_t *len)
+{
+    if (!p->pool || k < 0 || k >= p->n_fast) return false;
+    const KswJob &j = p->jobs[k];
+    if (j.flag & KSW_EZ_SCORE_ONLY) return false;
+    *off = (uint64_t)(p->d_dir - p->pool->base) + j.dir_off;
+    *len = (track_pair(j.flag) ? 2 : 1) * dir_bytes(j.qlen, j.tlen, j.R);
+    return true;
+}
+
 int deliver_slot(const Plan *p, int k, const KswOut &o, const uint32_t *cig_arena, ksw_extz_t *z, void *km)
 {
     z->max = (uint32_t)o.max, z->zdropped = o.zdropped;

@@ -37,6 +37,7 @@
 #include <vector>
 
 #include "../../include/ksw2_hip.h"
+#include "ksw2_carve.h"
 #include "ksw2_common.h"
 #include "ksw2_engine.h"
 #include "ksw2_kernels.h"
@@ -64,8 +65,9 @@ struct CachedBlock {
 };
 CachedBlock g_cached;
 
-// First-fit allocator over one device allocation; blocks are 256-aligned and
-// coalesced on release.
+// First-fit allocator over one device allocation (RangeCarver's bookkeeping
+// under a lock): blocks are 256-aligned, a block may give back parts early,
+// and freed runs coalesce.
 class FirstFitPool : public DevicePool {
 public:
     int init(uint64_t n, bool cached)
@@ -84,7 +86,7 @@ public:
             if (!g_cached.busy && g_cached.base) {
                 g_cached.busy = true, cached_ = true;
                 base = g_cached.base, bytes = g_cached.bytes;
-                free_[0] = bytes;
+                carve_.reset(bytes);
                 return KSW2_HIP_OK;
             }
         }
@@ -92,7 +94,7 @@ public:
             base = nullptr;
             return KSW2_HIP_ENOMEM;
         }
-        free_[0] = bytes;
+        carve_.reset(bytes);
         return KSW2_HIP_OK;
     }
     ~FirstFitPool() override
@@ -107,46 +109,28 @@ public:
     bool alloc(uint64_t n, uint64_t *off) override
     {
         std::lock_guard<std::mutex> lk(mu_);
-        n = round_up(std::max<uint64_t>(n, 1), 256);
-        for (auto it = free_.begin(); it != free_.end(); ++it) {
-            if (it->second < n) continue;
-            *off = it->first;
-            const uint64_t rest = it->second - n, at = it->first + n;
-            free_.erase(it);
-            if (rest) free_[at] = rest;
-            used_[*off] = n;
-            return true;
-        }
-        return false;
+        return carve_.alloc(n, off);
     }
     void free(uint64_t off) override
     {
         std::lock_guard<std::mutex> lk(mu_);
-        auto u = used_.find(off);
-        if (u == used_.end()) return;
-        uint64_t start = off, len = u->second;
-        used_.erase(u);
-        auto next = free_.lower_bound(start);
-        if (next != free_.end() && next->first == start + len) len += next->second, next = free_.erase(next);
-        if (next != free_.begin()) {
-            auto prev = std::prev(next);
-            if (prev->first + prev->second == start) start = prev->first, len += prev->second, free_.erase(prev);
-        }
-        free_[start] = len;
+        carve_.free(off);
+    }
+    void release(uint64_t off, uint64_t n) override
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        (void)carve_.release(off, n);
     }
     uint64_t largest_free()
     {
         std::lock_guard<std::mutex> lk(mu_);
-        uint64_t m = 0;
-        for (auto &f : free_) m = std::max(m, f.second);
-        return m;
+        return carve_.largest_free();
     }
 
 private:
     bool cached_ = false;
     std::mutex mu_;
-    std::map<uint64_t, uint64_t> free_;
-    std::unordered_map<uint64_t, uint64_t> used_;
+    RangeCarver carve_;
 };
 
 }  // namespace
@@ -187,7 +171,7 @@ struct Engine::Flight {
     int undelivered_fast = 0;
     int stream = 0;
     double cells = 0, t_start = 0;  // for the statistics
-    uint64_t bytes = 0;             // footprints admitted (held until the flight ends)
+    uint64_t bytes = 0;             // footprints admitted, less direction bytes given back at delivery
 };
 
 Engine::Engine(int8_t m, const int8_t *mat, const KswScore &sc, const ScoreClass &cls, uint64_t budget)
@@ -201,6 +185,9 @@ Engine::Engine(int8_t m, const int8_t *mat, const KswScore &sc, const ScoreClass
     if (const char *sp = getenv("KSW2_HIP_SERVE_SPREAD")) serve_spread_ = atoi(sp);
     if (const char *bs = getenv("KSW2_HIP_BATCH_SPREAD")) batch_spread_ = atoi(bs) != 0;
     if (const char *ff = getenv("KSW2_HIP_FAIL_FLIGHTS")) fail_flights_ = std::max(0, atoi(ff));
+    if (const char *pr = getenv("KSW2_HIP_PAIR_RELEASE")) pair_release_ = atoi(pr) != 0;
+    if (const char *wb = getenv("KSW2_HIP_WAVE_BYTES")) wave_bytes_scale_ = atof(wb);
+    if (const char *et = getenv("KSW2_HIP_ENGINE_T")) engine_team_T_ = atoi(et) != 0;
     const char *hq = getenv("GPU_MAX_HW_QUEUES");
     nstreams_ = std::max(1, std::min(kMaxStreams, hq ? atoi(hq) : 4));
     if (const char *ns = getenv("KSW2_HIP_STREAMS")) nstreams_ = std::max(1, std::min(kMaxStreams, atoi(ns)));
@@ -516,6 +503,15 @@ bool Engine::reap_locked(std::unique_lock<std::mutex> &lk)
             for (int k = 0; k < p->n_fast; ++k) {
                 if (f->delivered[k] || !__atomic_load_n(p->h_mflag + k, __ATOMIC_ACQUIRE)) continue;
                 Task *tk = f->tasks[p->order[k]];
+                // the pair's kernels are done with its direction bytes: give
+                // them back now rather than when the whole flight ends
+                uint64_t doff, dlen;
+                if (pair_release_ && plan_dir_span(p, k, &doff, &dlen)) {
+                    pool_->release(doff, dlen);
+                    dlen = dlen / 256 * 256;
+                    f->bytes = f->bytes > dlen ? f->bytes - dlen : 0;
+                    held_bytes_ = held_bytes_ > dlen ? held_bytes_ - dlen : 0;
+                }
                 const int st = deliver_slot(p, k, p->h_mout[k], p->h_mcig, tk->ez, nullptr);
                 if (st != KSW2_HIP_OK) tk->ez->zdropped = 1, tk->status = st, status_ = st;
                 f->delivered[k] = 1, --f->undelivered_fast;
@@ -738,6 +734,12 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
         env.h_stop = h_stop_, env.d_stop = d_stop_;
         env.poll_stop = true;  // stop_all_locked raises it
         env.team = team, env.rows = rows;
+        if (batch_) {
+            // teams by memory: each wave holds at most its share of the pool
+            // (the step budget stays the flight's own unless asked otherwise)
+            if (engine_team_T_) env.team_T = T;
+            if (wave_bytes_scale_ > 0) env.wave_bytes = (uint64_t)(wave_bytes_scale_ * (double)budget_ / resident_waves_);
+        }
         // a flight's teams may spread over CUs (tuning switch; off by
         // default: with sixteen streams a spread team gains nothing at 20 %
         // load, and under more load it holds waiting-launch slots that the

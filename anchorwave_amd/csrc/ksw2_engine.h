@@ -81,6 +81,16 @@ private:
     int nstreams_ = 4;          // streams in use (hardware queues; KSW2_HIP_STREAMS, tuning)
     int stream_class_[kMaxStreams] = {};  // serving: each stream's own size class (KSW2_HIP_STREAM_CLASSES, tuning)
     int fail_flights_ = 0;      // tests: the first this many flights fail before launch (KSW2_HIP_FAIL_FLIGHTS)
+    // Tuning switches, off by default (uneven batch in one process, 2.07-2.28 s
+    // with neither): a delivered pair's direction bytes go back to the pool at
+    // once (KSW2_HIP_PAIR_RELEASE=1: 2.22-2.50 s); batch teams keep each
+    // wave's direction bytes under this times budget / resident waves
+    // (KSW2_HIP_WAVE_BYTES=1: 2.07 s); batch teams from the engine's per-wave
+    // step budget instead of each flight's own (KSW2_HIP_ENGINE_T=1: 2.6 s
+    // and slower)
+    bool pair_release_ = false;
+    double wave_bytes_scale_ = 0.0;
+    bool engine_team_T_ = false;
     struct Task;
     struct Flight;
     void loop();

@@ -89,6 +89,8 @@ struct DevicePool {
     uint64_t bytes = 0;
     virtual bool alloc(uint64_t n, uint64_t *off) = 0;
     virtual void free(uint64_t off) = 0;
+    // gives back part of an allocated block before the block is freed
+    virtual void release(uint64_t off, uint64_t n) { (void)off, (void)n; }
     virtual ~DevicePool() {}
 };
 
@@ -119,6 +121,13 @@ struct PlanEnv {
     // The stripe and band kernels sample h_stop even with no progress
     // callback (its owner raises it: an engine stopping its flights).
     bool poll_stop = false;
+    // Team sizing by a caller that sees more than this plan (the engine):
+    // team_T > 0 is the per-wave step budget in place of the plan's own, and
+    // wave_bytes > 0 the direction bytes one wave may hold, so a pair gets
+    // at least bytes / wave_bytes waves (a long pair then holds its memory
+    // for a shorter time).
+    double team_T = 0;
+    uint64_t wave_bytes = 0;
 };
 
 // A device-resident batch: sequences uploaded, arenas sized, jobs built.
@@ -235,6 +244,10 @@ bool progress_continue(const Progress *prog, int &calls, int total);
 // skip[k] set are left alone).
 int plan_fetch(Plan *p, ksw_extz_t *ez, void *km, const std::vector<char> *skip = nullptr);
 int deliver_slot(const Plan *p, int k, const KswOut &o, const uint32_t *cig_arena, ksw_extz_t *z, void *km);
+// The pool range [*off, *off + *len) of stripe slot k's direction bytes (a
+// plan carved from a pool); false when the pair stores none.  Once the slot's
+// completion flag is set no kernel of the plan reads or writes that range.
+bool plan_dir_span(const Plan *p, int k, uint64_t *off, uint64_t *len);
 void plan_destroy(Plan *p);
 
 }  // namespace kswhip

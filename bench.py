@@ -229,6 +229,35 @@ def banded_leg(args) -> dict:
     return res
 
 
+def general_leg(args, pairs) -> dict:
+    """The general kernel's workload in AnchorWave: narrow bands, as the
+    score-certified path asks for (alignSlidingWindow.cpp:1297-1310 widens
+    the band from twice the length difference).  The fixed batch's pairs at
+    w = 256, below the band kernel's minimum width, through
+    ksw_extd2_hip_batch; GCUPS over in-band cells up to each band exit."""
+    w = args.general_w
+    sub = pairs[: args.general]
+    cells = sum(band_cells(len(q), len(t), w) for q, t in sub)
+    ksw2.extd2_batch(sub[:8], mat=W_MAT, w=w, **SC)
+    tm = {}
+    gres = ksw2.extd2_batch(sub, mat=W_MAT, w=w, **SC, timing=tm)
+    dt = tm["seconds"]
+    exits = sum(1 for g in gres if g.zdropped)
+    res = {"pairs": len(sub), "w": w, "in_band_cells": cells, "seconds": round(dt, 4),
+           "gcups": round(cells / dt / 1e9, 2), "band_exits": exits, "kernel": "ksw_general_kernel",
+           "timing": "wall clock of ksw_extd2_hip_batch: upload, DP, backtrack, CIGAR download"}
+    log(f"[bench] general kernel (w={w}): {cells:.3g} in-band cells in {dt * 1e3:.1f} ms = "
+        f"{cells / dt / 1e9:.1f} GCUPS ({exits} band exits)")
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        kind, cdt, cres = cpu_baseline(sub, threads, w=w)
+        exact = sum(1 for g, c in zip(gres, cres) if g.__dict__ == c.__dict__)
+        res["exact"] = {"agree": exact, "of": len(sub), "frac": exact / len(sub)}
+        res["cpu"] = {"kind": kind, "gcups": round(cells / cdt / 1e9, 3), "cores": threads}
+        log(f"[bench] general-kernel CPU {cells / cdt / 1e9:.2f} GCUPS; exact {exact}/{len(sub)}")
+    return res
+
+
 def uneven_leg(args) -> dict:
     """The uneven inter-anchor batch through ksw_extd2_hip_batch (host
     buffers in and out, chunked by the device memory budget), against the
@@ -400,6 +429,9 @@ def main() -> int:
                     help="also time the fixed batch with KSW_EZ_SINGLETRACK (0: skip)")
     ap.add_argument("--threads-leg", type=int, default=1024,
                     help="pairs of the 16-thread ksw_extd2_hip leg (0: skip it)")
+    ap.add_argument("--general", type=int, default=4096,
+                    help="fixed-batch pairs of the narrow-band (general kernel) leg (0: skip it)")
+    ap.add_argument("--general-w", type=int, default=256, help="band of the general-kernel leg")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (PMC byte-counter scale)")
     args = ap.parse_args()
@@ -501,6 +533,8 @@ def main() -> int:
         out["singletrack"] = singletrack_leg(args, pairs, cells)
     if args.threads_leg > 0:
         out["threads"] = threads_leg(args, pairs)
+    if args.general > 0:
+        out["general"] = general_leg(args, pairs)
     if args.banded > 0:
         out["banded"] = banded_leg(args)
     if args.uneven > 0:
