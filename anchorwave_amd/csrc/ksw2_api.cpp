@@ -351,7 +351,7 @@ int ksw2_hip_server_create(ksw2_hip_server **server, int8_t m, const int8_t *mat
     // engine block when that is larger (the server then holds it)
     const uint64_t budget = device_bytes ? (uint64_t)device_bytes : kswhip::mem_budget(true);
     auto *eng = new kswhip::Engine(m, mat, sc, cls, budget);
-    st = eng->start(device_bytes == 0);
+    st = eng->start(device_bytes == 0, true);
     if (st != KSW2_HIP_OK) {
         delete eng;
         return set_err(st);
@@ -380,6 +380,8 @@ int ksw2_hip_server_load(ksw2_hip_server *server, ksw2_hip_load_t *load)
     const kswhip::Engine::Load l = server->e->load();
     load->budget_bytes = l.budget, load->held_bytes = l.held_bytes, load->queued_bytes = l.queued_bytes;
     load->pending = l.pending, load->pending_cells = l.pending_cells, load->cells_per_s = l.cells_per_s;
+    load->seq_arena_bytes = l.seq_arena_bytes, load->resident_tasks = l.resident_tasks;
+    load->staged_flights = l.staged_flights;
     return set_err(KSW2_HIP_OK);
 }
 

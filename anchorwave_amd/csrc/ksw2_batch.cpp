@@ -376,6 +376,9 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
     p->cig_off.resize(p->n);
     uint64_t so = 0, dof = 0, bo = 0, co = 0, sto = 0, oo = 0;
     int stripes = 0;
+    // sequences already on the device (an engine's resident arena): the
+    // jobs address them there and the plan uploads none
+    const bool resident = env && env->seq_dev && env->seq_offs;
     for (int k = 0; k < p->n_fast; ++k) {
         const ksw2_pair_t &pp = pairs[p->order[k]];
         KswJob &j = p->jobs[k];
@@ -385,8 +388,13 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         j.qlen = g.kq, j.tlen = g.kt, j.flag = pp.flag, j.end_bonus = pp.end_bonus;
         j.R = R;
         j.lut = lut_ok(pp, gsc.k) ? 1 : 0;
-        j.q_off = so, so = round_up(so + g.kq, 16);
-        j.t_off = so, so = round_up(so + g.kt, 16);
+        if (resident) {
+            const uint64_t oq = env->seq_offs[2 * p->order[k]], ot = env->seq_offs[2 * p->order[k] + 1];
+            j.q_off = g.tr ? ot : oq, j.t_off = g.tr ? oq : ot;
+        } else {
+            j.q_off = so, so = round_up(so + g.kq, 16);
+            j.t_off = so, so = round_up(so + g.kt, 16);
+        }
         j.dir_off = dof;
         if (!(pp.flag & KSW_EZ_SCORE_ONLY))
             dof = round_up(dof + (track_pair(pp) ? 2 : 1) * dir_bytes(g.kq, g.kt, R), 256);
@@ -413,8 +421,12 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
             j.r_end = band_r_end(pp.qlen, pp.tlen, j.w);
             j.bnd_off = bo, bo += kBandCtl + round_up(6ull * ((uint64_t)pp.qlen + pp.tlen - 1), 64);
         }
-        j.q_off = so, so = round_up(so + pp.qlen, 16);
-        j.t_off = so, so = round_up(so + pp.tlen, 16);
+        if (resident) {
+            j.q_off = env->seq_offs[2 * p->order[slot]], j.t_off = env->seq_offs[2 * p->order[slot] + 1];
+        } else {
+            j.q_off = so, so = round_up(so + pp.qlen, 16);
+            j.t_off = so, so = round_up(so + pp.tlen, 16);
+        }
         j.st_off = sto, sto += gen_layout(j.tlenB, j.qlenB).bytes;
         const uint64_t diags = (uint64_t)pp.qlen + pp.tlen - 1;
         j.dir_off = dof;
@@ -603,7 +615,8 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         if (hipMalloc(&p->block, p->block_bytes) != hipSuccess) return fail(KSW2_HIP_ENOMEM);
         base = p->block;
     }
-    p->d_seq = base + o_seq, p->d_dir = base + o_dir, p->d_bnd = (int32_t *)(base + o_bnd);
+    p->d_seq = resident ? env->seq_dev : base + o_seq;
+    p->d_dir = base + o_dir, p->d_bnd = (int32_t *)(base + o_bnd);
     p->d_cig = (uint32_t *)(base + o_cig), p->d_jobs = (KswJob *)(base + o_jobs), p->d_out = (KswOut *)(base + o_out);
     p->d_order = (int32_t *)(base + o_order), p->d_recs = (StripeRec *)(base + o_recs);
     if (p->n_wide > 0) {
@@ -634,7 +647,9 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         memset(p->h_mflag, 0, sizeof(int32_t) * p->n_fast);
     }
     hipError_t e[6] = {hipSuccess, hipSuccess, hipSuccess, hipSuccess, hipSuccess, hipSuccess};
-    if (env && env->direct_upload) {
+    if (resident) {
+        p->own_seq = false;  // nothing to upload
+    } else if (env && env->direct_upload) {
         // straight from the caller's buffers; the arena's padding is zero
         // as with the staged copy
         p->own_seq = false;

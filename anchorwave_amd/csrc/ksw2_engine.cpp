@@ -158,6 +158,8 @@ struct Engine::Task {
     int cls = 0;
     int status = 0;    // KSW2_HIP_OK, or why the device path did not run it
     bool general = false;
+    bool resident = false;  // sequences in the engine's device arena at seq_off
+    uint64_t seq_off = 0;
     double t_submit = 0, t_start = 0, t_done = 0;
 };
 
@@ -212,7 +214,7 @@ void Engine::release_cached_block()
     }
 }
 
-int Engine::start(bool cached)
+int Engine::start(bool cached, bool resident_seq)
 {
     int st = device_init(-1);
     if (st != KSW2_HIP_OK) return st;
@@ -222,6 +224,29 @@ int Engine::start(bool cached)
     st = pool->init(budget_, cached);
     budget_ = pool->bytes;  // a cached block may be larger than asked
     if (st != KSW2_HIP_OK) return st;
+    if (resident_seq) {
+        // a sixteenth of the pool, at most 4 GiB (about 400k tasks of 5 kbp);
+        // a pool under 16 MiB keeps none (flights then upload their own)
+        SeqArena &A = arena_;
+        A.bytes = std::min<uint64_t>(budget_ / 16, 4ull << 30) / 256 * 256;
+        A.ring_bytes = std::min<uint64_t>(64ull << 20, A.bytes);
+        if (A.bytes >= (1u << 20)) {
+            if (!pool->alloc(A.bytes, &A.pool_off)) return KSW2_HIP_ENOMEM;
+            if (hipHostMalloc((void **)&A.ring, A.ring_bytes, hipHostMallocDefault) != hipSuccess) return KSW2_HIP_ENOMEM;
+            if (hipStreamCreateWithFlags(&A.up, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&A.ev_up, hipEventDisableTiming) != hipSuccess)
+                return KSW2_HIP_EDEVICE;
+            A.dev = pool->base + A.pool_off;
+            A.carve.reset(A.bytes);
+            budget_ -= A.bytes;
+        } else {
+            A.bytes = 0, A.ring_bytes = 0;
+        }
+    }
+    // the upload stream (created first) takes a hardware queue of its own:
+    // flights wait for its copies, so it must not queue behind a flight's
+    // kernels on a shared one
+    if (arena_.up && nstreams_ > 1) --nstreams_;
     for (int s = 0; s < nstreams_; ++s)
         if (hipStreamCreateWithFlags(&streams_[s], hipStreamNonBlocking) != hipSuccess) return KSW2_HIP_EDEVICE;
     resident_waves_ = std::max(1, stripe_resident_waves(kGroupLeft));
@@ -250,6 +275,9 @@ Engine::~Engine()
     for (auto *b : staging_)
         if (b) (void)hipHostFree(b);
     for (auto &m : mirror_) m.release();
+    if (arena_.up) (void)hipStreamSynchronize(arena_.up), (void)hipStreamDestroy(arena_.up);
+    if (arena_.ev_up) (void)hipEventDestroy(arena_.ev_up);
+    if (arena_.ring) (void)hipHostFree(arena_.ring);
     if (h_stop_) (void)hipHostFree(h_stop_);
     for (auto &kv : tasks_) delete kv.second;
     if (stats_)
@@ -275,8 +303,54 @@ bool Engine::queues_empty() const
 }
 
 // A task leaves the pending work (its result was delivered or it failed).
+// Sequence bytes of a task in the arena: the query, zero padding to a
+// 16-byte boundary plus 16, then the target likewise.
+static uint64_t seq_span(int qlen, int tlen) { return (uint64_t)(qlen + 31) / 16 * 16 + (uint64_t)(tlen + 31) / 16 * 16; }
+
+bool Engine::upload_seq_locked(Task *t)
+{
+    SeqArena &A = arena_;
+    const int ql = t->pair.qlen, tl = t->pair.tlen;
+    const uint64_t n = seq_span(ql, tl), tq = (uint64_t)(ql + 31) / 16 * 16;
+    if (n > A.ring_bytes) return false;
+    uint64_t off;
+    if (!A.carve.alloc(n, &off)) return false;
+    if (A.ring_w + n > A.ring_bytes) {
+        // the ring is reused from its start once the copies out of it are done
+        if (hipStreamSynchronize(A.up) != hipSuccess) {
+            A.carve.free(off);
+            return false;
+        }
+        A.ring_w = 0;
+    }
+    uint8_t *h = A.ring + A.ring_w;
+    memcpy(h, t->pair.query, (size_t)ql);
+    memset(h + ql, 0, (size_t)(tq - ql));
+    memcpy(h + tq, t->pair.target, (size_t)tl);
+    memset(h + tq + tl, 0, (size_t)(n - tq - tl));
+    if (hipMemcpyAsync(A.dev + off, h, n, hipMemcpyHostToDevice, A.up) != hipSuccess ||
+        hipEventRecord(A.ev_up, A.up) != hipSuccess) {
+        // the copy may be queued: keep the range (it returns with the engine)
+        return false;
+    }
+    A.ring_w += n;
+    A.uploads = true;
+    ++resident_tasks_;
+    t->seq_off = off, t->resident = true;
+    return true;
+}
+
+void Engine::free_seq(Task *t)
+{
+    if (!t->resident) return;
+    std::lock_guard<std::mutex> lk(seq_mu_);
+    arena_.carve.free(t->seq_off);
+    t->resident = false;
+}
+
 void Engine::delivered_locked(const Task *t)
 {
+    free_seq(const_cast<Task *>(t));
     pending_cells_ -= t->cells, since_reap_cells_ += t->cells;
     if (pending_cells_ < 0) pending_cells_ = 0;
 }
@@ -287,6 +361,11 @@ Engine::Load Engine::load()
     Load l;
     l.budget = budget_, l.held_bytes = held_bytes_, l.queued_bytes = queued_bytes_;
     l.pending = pending_, l.pending_cells = pending_cells_, l.cells_per_s = cells_per_s_;
+    l.staged_flights = staged_flights_;
+    {
+        std::lock_guard<std::mutex> sl(seq_mu_);
+        l.seq_arena_bytes = arena_.bytes, l.resident_tasks = resident_tasks_;
+    }
     return l;
 }
 
@@ -329,6 +408,10 @@ int Engine::submit(const ksw2_pair_t &p, ksw_extz_t *ez, bool copy, uint64_t *ti
             t->qcopy.assign(p.query, p.query + p.qlen);
             t->tcopy.assign(p.target, p.target + p.tlen);
             t->pair.query = t->qcopy.data(), t->pair.target = t->tcopy.data();
+            if (arena_.dev) {
+                std::lock_guard<std::mutex> lk(seq_mu_);
+                (void)upload_seq_locked(t);  // else the flight uploads it
+            }
         }
         t->cells = (double)p.qlen * p.tlen;
         t->cls = size_class(t->cells);
@@ -752,6 +835,23 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
             env.mirror_cig = mr.cig, env.mirror_cig_dev = mr.cig_dev;
             env.mirror_flag = mr.flag, env.mirror_flag_dev = mr.flag_dev;
         }
+        // every task's sequences in the arena: the plan reads them there,
+        // after the uploads so far (the flight's stream waits for them)
+        std::vector<uint64_t> seq_offs;
+        bool resident = arena_.dev != nullptr;
+        for (const Task *tk : pick) resident = resident && tk->resident;
+        if (resident) {
+            seq_offs.resize(2 * pick.size());
+            for (size_t k = 0; k < pick.size(); ++k) {
+                seq_offs[2 * k] = pick[k]->seq_off;
+                seq_offs[2 * k + 1] = pick[k]->seq_off + (uint64_t)(pick[k]->pair.qlen + 31) / 16 * 16;
+            }
+            env.seq_dev = arena_.dev, env.seq_offs = seq_offs.data();
+            std::lock_guard<std::mutex> sl(seq_mu_);
+            if (arena_.uploads && hipStreamWaitEvent(streams_[s], arena_.ev_up, 0) != hipSuccess)
+                env.seq_dev = nullptr, env.seq_offs = nullptr;  // upload them with the flight
+        }
+        const bool staged = !env.seq_dev;
         const double t_c0 = now_s();
         int st = plan_create(&f->plan, f->pairs.data(), idx.data(), (int)idx.size(), m_, cls_, gsc_, &env);
         if (st == KSW2_HIP_OK) {
@@ -786,6 +886,7 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
             continue;
         }
         for (Task *t : pick) t->t_start = t0;
+        if (staged) ++staged_flights_;
         f->t_start = t0;
         f->bytes = bytes;
         if (stats_ > 1) {

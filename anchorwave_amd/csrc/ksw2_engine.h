@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/ksw2_hip.h"
+#include "ksw2_carve.h"
 #include "ksw2_common.h"
 #include "ksw2_plan.h"
 
@@ -43,7 +44,9 @@ public:
     // cached: take the process's cached device block when it is free and
     // large enough (else allocate one and leave it cached), and give it back
     // to the cache at destruction instead of freeing it
-    int start(bool cached = false);
+    // resident_seq: submissions copy their sequences into a device arena at
+    // once (serving), so flights upload none
+    int start(bool cached = false, bool resident_seq = false);
     // Bytes of the cached device block (0: none); releases it when idle.
     static uint64_t cached_block_bytes();
     static void release_cached_block();
@@ -57,6 +60,8 @@ public:
         uint64_t budget, held_bytes, queued_bytes;
         int64_t pending;
         double pending_cells, cells_per_s;
+        uint64_t seq_arena_bytes;
+        int64_t resident_tasks, staged_flights;
     };
     Load load();
     // Batch mode: hold admission while a batch is queued, then sort the
@@ -126,6 +131,26 @@ private:
     uint8_t *staging_[kMaxStreams] = {};           // pinned upload buffers, per stream
     uint64_t staging_bytes_[kMaxStreams] = {};
     int32_t *h_stop_ = nullptr, *d_stop_ = nullptr;  // stop flag shared by all flights
+    // Device-resident sequences (serving): an arena carved from the pool,
+    // filled at submission from a pinned ring through one upload stream;
+    // ev_up_ is recorded after every upload, and a flight's stream waits on
+    // it before its kernels.  A task's range returns when it is delivered.
+    struct SeqArena {
+        uint8_t *dev = nullptr;       // arena start (inside the pool)
+        uint64_t pool_off = 0, bytes = 0;
+        RangeCarver carve;
+        uint8_t *ring = nullptr;      // pinned staging, filled linearly
+        uint64_t ring_bytes = 0, ring_w = 0;
+        hipStream_t up = nullptr;
+        hipEvent_t ev_up = nullptr;
+        bool uploads = false;         // an upload was recorded since the last wait
+    };
+    SeqArena arena_;
+    std::mutex seq_mu_;
+    int64_t resident_tasks_ = 0;   // under seq_mu_
+    int64_t staged_flights_ = 0;   // under mu_
+    bool upload_seq_locked(Task *t);     // holds seq_mu_
+    void free_seq(Task *t);
     std::thread worker_;
     std::mutex mu_;
     std::condition_variable cv_work_, cv_done_;

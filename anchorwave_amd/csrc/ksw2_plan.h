@@ -128,6 +128,12 @@ struct PlanEnv {
     // for a shorter time).
     double team_T = 0;
     uint64_t wave_bytes = 0;
+    // Sequences resident on the device: pair k's query and target start at
+    // seq_dev + seq_offs[2k] and seq_dev + seq_offs[2k + 1] (k indexing the
+    // pairs array of plan_create; 16-byte aligned, zero-padded to 16 bytes
+    // past the end); the plan then uploads none.
+    uint8_t *seq_dev = nullptr;
+    const uint64_t *seq_offs = nullptr;
 };
 
 // A device-resident batch: sequences uploaded, arenas sized, jobs built.

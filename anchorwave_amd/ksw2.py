@@ -412,7 +412,8 @@ class HipLoad(C.Structure):
     """``ksw2_hip_load_t`` (include/ksw2_hip.h)."""
 
     _fields_ = [("budget_bytes", C.c_uint64), ("held_bytes", C.c_uint64), ("queued_bytes", C.c_uint64),
-                ("pending", C.c_int64), ("pending_cells", C.c_double), ("cells_per_s", C.c_double)]
+                ("pending", C.c_int64), ("pending_cells", C.c_double), ("cells_per_s", C.c_double),
+                ("seq_arena_bytes", C.c_uint64), ("resident_tasks", C.c_int64), ("staged_flights", C.c_int64)]
 
 
 def footprint(query, target, mat=None, m=5, q=8, e=2, q2=75, e2=1, w=-1, zdrop=-1, flag=0) -> int:

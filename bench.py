@@ -340,28 +340,40 @@ def serving_leg(args) -> dict:
     cap = len(tasks) / wall
     log(f"[bench] serving batch capacity (every task submitted at once): {cap:.0f} tasks/s, "
         f"{cells.sum() / wall / 1e9:.1f} GCUPS")
-    levels = []
     rng = np.random.default_rng(11)
-    for load in [float(x) for x in args.loads.split(",") if x]:
-        arr = np.cumsum(rng.exponential(1.0 / (load * cap), len(tasks)))
+
+    def level(rate, load):
+        arr = np.cumsum(rng.exponential(1.0 / rate, len(tasks)))
         wall, lat = run(arr)
-        levels.append({"load": load, "offered_tasks_per_s": round(load * cap, 1),
-                       "achieved_tasks_per_s": round(len(tasks) / wall, 1),
-                       "gcups": round(float(cells.sum() / wall / 1e9), 2),
-                       "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
-                       "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
-                       "by_kind_p50_p99_ms": {k: [round(float(np.percentile(lat[kinds == k], 50)) * 1e3, 2),
-                                                  round(float(np.percentile(lat[kinds == k], 99)) * 1e3, 2)]
-                                              for k in ("single", "fixed", "uneven")}})
+        lv = {"load": load, "offered_tasks_per_s": round(rate, 1),
+              "achieved_tasks_per_s": round(len(tasks) / wall, 1),
+              "gcups": round(float(cells.sum() / wall / 1e9), 2),
+              "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+              "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+              "by_kind_p50_p99_ms": {k: [round(float(np.percentile(lat[kinds == k], 50)) * 1e3, 2),
+                                         round(float(np.percentile(lat[kinds == k], 99)) * 1e3, 2)]
+                                     for k in ("single", "fixed", "uneven")}}
+        return lv
+
+    # sustained capacity: the highest offered Poisson rate (fractions of the
+    # batch capacity) that the server keeps up with, achieved >= 0.98 offered
+    search = []
+    for f in [float(x) for x in args.search.split(",") if x]:
+        lv = level(f * cap, None)
+        search.append({k: lv[k] for k in ("offered_tasks_per_s", "achieved_tasks_per_s", "p50_ms", "p99_ms")})
+        log(f"[bench] serving capacity search {f:.2f} of batch capacity: {search[-1]}")
+        if lv["achieved_tasks_per_s"] < 0.98 * lv["offered_tasks_per_s"]:
+            break
+    kept = [lv for lv in search if lv["achieved_tasks_per_s"] >= 0.98 * lv["offered_tasks_per_s"]]
+    sustained = max(lv["offered_tasks_per_s"] for lv in kept) if kept else search[0]["achieved_tasks_per_s"]
+    log(f"[bench] serving sustained capacity: {sustained:.0f} tasks/s ({sustained / cap:.2f} of batch capacity)")
+    # the sweep: fractions of the sustained capacity
+    levels = []
+    for load in [float(x) for x in args.loads.split(",") if x]:
+        levels.append(level(load * sustained, load))
         log(f"[bench] serving load {load:.2f}: {levels[-1]}")
+    ld = srv.load()
     srv.close()
-    # sustained capacity: the highest offered Poisson rate of the sweep that
-    # the server kept up with (achieved at least 0.98 of offered)
-    kept = [lv for lv in levels if lv["achieved_tasks_per_s"] >= 0.98 * lv["offered_tasks_per_s"]]
-    sustained = max(kept, key=lambda lv: lv["offered_tasks_per_s"]) if kept else None
-    if sustained:
-        log(f"[bench] serving sustained capacity: {sustained['achieved_tasks_per_s']:.0f} tasks/s "
-            f"({sustained['load']:.2f} of batch capacity), {sustained['gcups']} GCUPS")
     # the same tasks one call at a time (seeded sample per kind)
     single = {}
     for kind in ("single", "fixed", "uneven"):
@@ -375,12 +387,15 @@ def serving_leg(args) -> dict:
                         "gcups": round(float(cells[idx].sum() / sum(ts) / 1e9), 3), "sample": len(idx)}
     log(f"[bench] serving one-at-a-time: {single}")
     return {"tasks_per_level": len(tasks), "mix": "50% 5 kbp pair, 40% 2 kbp pair, 10% uneven-batch pair",
-            "capacity_tasks_per_s": round(cap, 1), "capacity_gcups": round(cells.sum() * cap / len(tasks) / 1e9, 2),
-            "capacity": "batch capacity: every task submitted at once; the sweep's loads are fractions of it",
-            "sustained_tasks_per_s": sustained["achieved_tasks_per_s"] if sustained else None,
-            "sustained_load": sustained["load"] if sustained else None,
-            "sustained": "highest offered Poisson rate of the sweep with achieved >= 0.98 offered",
+            "batch_capacity_tasks_per_s": round(cap, 1),
+            "batch_capacity_gcups": round(cells.sum() * cap / len(tasks) / 1e9, 2),
+            "capacity_search": search,
+            "sustained_tasks_per_s": round(sustained, 1),
+            "sustained": "highest offered Poisson rate (fractions of the batch capacity, every task submitted "
+                         "at once) with achieved >= 0.98 offered; the sweep's loads are fractions of it",
             "admission_budget_bytes": SERVE_BUDGET,
+            "sequences": {"where": "device arena, uploaded at submission", "arena_bytes": ld["seq_arena_bytes"],
+                          "resident_tasks": ld["resident_tasks"], "staged_flights": ld["staged_flights"]},
             "levels": levels, "one_at_a_time": single,
             "latency": "submission to completion inside the library (queueing + device + CIGAR delivery)"}
 
@@ -419,8 +434,11 @@ def main() -> int:
                     help="pairs in the uneven-batch leg (0: skip it)")
     ap.add_argument("--serve", type=int, default=12000,
                     help="tasks per offered-load level in the serving leg (0: skip it)")
-    ap.add_argument("--loads", default="0.2,0.3,0.4,0.5,0.6,0.8,1.0,1.2",
-                    help="offered loads of the serving leg, as fractions of measured capacity")
+    ap.add_argument("--search", default="0.15,0.2,0.25,0.3,0.35,0.4,0.5,0.6,0.8",
+                    help="offered rates tried for the serving leg's sustained capacity, as fractions of "
+                         "its batch capacity (stops at the first one not kept up with)")
+    ap.add_argument("--loads", default="0.2,0.4,0.6,0.8,1.0,1.2",
+                    help="offered loads of the serving leg, as fractions of its sustained capacity")
     ap.add_argument("--single-pair", type=int, default=1,
                     help="also time one 5 kbp pair on the GPU and on one host core (0: skip)")
     ap.add_argument("--banded", type=int, default=64,

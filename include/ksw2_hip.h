@@ -278,6 +278,9 @@ typedef struct {
     int64_t pending;         /* pairs submitted and not yet completed */
     double pending_cells;    /* their qlen * tlen */
     double cells_per_s;      /* recent completion rate under load (0: none yet) */
+    uint64_t seq_arena_bytes;  /* device arena holding submitted sequences */
+    int64_t resident_tasks;    /* pairs whose sequences went to the arena at submission */
+    int64_t staged_flights;    /* flights that uploaded their own sequences (no arena, or it was full) */
 } ksw2_hip_load_t;
 int ksw2_hip_server_load(ksw2_hip_server *server, ksw2_hip_load_t *load);
 

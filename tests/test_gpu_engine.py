@@ -43,7 +43,11 @@ def test_server_results_alone_and_in_the_mix():
         for tk, res, lat in srv.poll(wait_ms=2000):
             got[tickets[tk]] = res
             assert lat >= 0
+    ld = srv.load()
     srv.close()
+    # every pair's sequences went to the device arena at submission and no
+    # flight uploaded its own
+    assert ld["seq_arena_bytes"] > 0 and ld["resident_tasks"] == len(pairs) and ld["staged_flights"] == 0, ld
     for k, p in enumerate(pairs):
         alone = ksw2.extd2(p[0], p[1], flag=p[2], w=p[3], zdrop=p[4], end_bonus=p[5])
         exp = _oracle(p)
@@ -198,3 +202,20 @@ def test_failed_flight_is_reported_per_ticket(monkeypatch):
     for k, (res, st) in got.items():
         if st == 0:
             assert same(res, O.run_oracle(*pairs[k])), k
+
+
+def test_server_without_sequence_arena_stages_per_flight():
+    """A server too small for a sequence arena (under 16 MiB) uploads each
+    flight's sequences itself, with the same results."""
+    pairs = random_pairs(95, 12, 50, 400)
+    srv = ksw2.Server(device_bytes=12 << 20)
+    tickets = {srv.submit(q, t): k for k, (q, t) in enumerate(pairs)}
+    got = {}
+    while len(got) < len(pairs):
+        for tk, res, _ in srv.poll(wait_ms=2000):
+            got[tickets[tk]] = res
+    ld = srv.load()
+    srv.close()
+    assert ld["seq_arena_bytes"] == 0 and ld["resident_tasks"] == 0 and ld["staged_flights"] > 0, ld
+    for k, (q, t) in enumerate(pairs):
+        assert same(got[k], O.run_oracle(q, t)), k
