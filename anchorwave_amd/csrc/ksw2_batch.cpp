@@ -605,6 +605,7 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
     const uint64_t o_redo = p->n_band > 0 ? take(sizeof(int32_t) * p->n_gen) : 0;
     const uint64_t o_wctl = p->n_wide > 0 ? take(4ull * kWideCtl * (p->n_wide + 1)) : 0;  // failure word, blocks
     const uint64_t o_wfirst = p->n_wide > 0 ? take(4ull * p->h_wfirst.size()) : 0;
+    const uint64_t o_qhead = take(4ull * kMaxQueueHeads);
     p->block_bytes = at;
     uint8_t *base = nullptr;
     if (env && env->pool) {
@@ -619,6 +620,7 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
     p->d_dir = base + o_dir, p->d_bnd = (int32_t *)(base + o_bnd);
     p->d_cig = (uint32_t *)(base + o_cig), p->d_jobs = (KswJob *)(base + o_jobs), p->d_out = (KswOut *)(base + o_out);
     p->d_order = (int32_t *)(base + o_order), p->d_recs = (StripeRec *)(base + o_recs);
+    p->d_qhead = (int32_t *)(base + o_qhead);
     if (p->n_wide > 0) {
         p->d_werr = (int32_t *)(base + o_wctl), p->d_wctl = p->d_werr + kWideCtl;
         p->d_wfirst = (int32_t *)(base + o_wfirst);
@@ -920,12 +922,26 @@ int plan_launch(Plan *p, const Progress *prog)
             else KSW_HIP_TRY(launch_stripe(sa, l.grp, l.team, p->R, st));  // one-workgroup teams
         }
     }
-    for (const Plan::Launch &l : p->launches) {
+    // one-wave teams run as persistent workers over the launch's job list (a
+    // device queue, longest first); KSW2_HIP_QUEUE=0 launches one workgroup
+    // per pair instead (tests)
+    const char *qv = getenv("KSW2_HIP_QUEUE");
+    const bool queue = !(qv && atoi(qv) == 0);
+    int nq = 0;
+    for (size_t k = 0; k < p->launches.size(); ++k) {
+        const Plan::Launch &l = p->launches[k];
         if (l.wide) continue;
         hipStream_t st = lane(l.lane);
         if (!st) return KSW2_HIP_EDEVICE;
         sa.order = p->d_order + l.off, sa.njobs = l.count;
-        KSW_HIP_TRY(launch_stripe(sa, l.grp, l.team, p->R, st));
+        if (queue && l.team == 1 && nq < kMaxQueueHeads) {
+            // its head zeroed on the launch's own stream, before the workers
+            KSW_HIP_TRY(clear_queue_heads(p->d_qhead + nq, 1, st));
+            KSW_HIP_TRY(launch_stripe_queue(sa, l.grp, p->R, p->d_qhead + nq, st));
+            ++nq;
+        } else {
+            KSW_HIP_TRY(launch_stripe(sa, l.grp, l.team, p->R, st));
+        }
     }
     for (int k = 0; k < Plan::kLanes; ++k)
         if (used[k]) {

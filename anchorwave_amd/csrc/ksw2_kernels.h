@@ -124,6 +124,13 @@ enum StripeGroup { kGroupLeft = 0, kGroupRight = 1, kGroupScoreOnly = 2, kGroupT
 // Stripe kernel: one workgroup of `team` waves (1..kMaxTeam) per job, R
 // rows per lane (8 or 4; the jobs' layouts must use the same R).
 hipError_t launch_stripe(const StripeArgs &a, int group, int team, int R, hipStream_t st);
+// Persistent one-wave workers (as many as fit on the device, at most one per
+// job) claiming the launch's jobs in list order from a device queue head
+// (*qhead, zero at launch: clear_queue_heads); every job runs as a one-wave
+// team.  A drop-in for launch_stripe with team 1.
+hipError_t launch_stripe_queue(const StripeArgs &a, int group, int R, int32_t *qhead, hipStream_t st);
+hipError_t clear_queue_heads(int32_t *qhead, int n, hipStream_t st);
+int stripe_queue_workers(int group, int R);
 
 // Wide stripe kernel: each job's passes spread over its own number of
 // workgroups of kWideGroup waves on different CUs (a team of up to kMaxWide

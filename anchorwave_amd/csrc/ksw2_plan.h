@@ -136,6 +136,10 @@ struct PlanEnv {
     const uint64_t *seq_offs = nullptr;
 };
 
+// Queue heads a plan keeps for its persistent one-wave launches (one per
+// launch of one-wave teams; further such launches use one workgroup per pair).
+constexpr int kMaxQueueHeads = 16;
+
 // A device-resident batch: sequences uploaded, arenas sized, jobs built.
 // Created from host pairs, run any number of times, results read back.
 struct Plan {
@@ -192,6 +196,7 @@ struct Plan {
     KswJob *d_jobs = nullptr;
     KswOut *d_out = nullptr;
     int32_t *d_order = nullptr;      // job indices, grouped by launch, longest first
+    int32_t *d_qhead = nullptr;      // queue heads of the persistent one-wave launches (kMaxQueueHeads)
     uint64_t state_bytes = 0, offs_ints = 0;
     uint8_t *d_state = nullptr;      // general kernel state blocks
     int32_t *d_offs = nullptr;       // general kernel off[]/off_end[]

@@ -53,6 +53,9 @@
 // mte/mte_q).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <algorithm>
+
 #include "ksw2_common.h"
 #include "ksw2_kernels.h"
 #include "ksw2_walk.h"
@@ -819,6 +822,54 @@ __global__ void __launch_bounds__(kWave * kMaxTeam) ksw_stripe_kernel(StripeArgs
     pair_epilogue<R>(a, job, jb, tile, MODE == kTracks, &shared_n, W * kWave);
 }
 
+// Persistent one-wave workers over a device work queue.  The launch's job
+// list is the queue, ordered longest first by the host (length-binned: the
+// longest pairs are claimed first, short ones fill the tail); each wave
+// claims the next entry with one device-scope atomic add on qhead, runs the
+// pair's passes and its epilogue, and claims again until the list is
+// exhausted, then ends.  No wave waits on another: a one-wave pair hands
+// each pass to the next through the wave's own LDS count.  qhead is zero at
+// launch (ksw_stripe_qhead_kernel, a fill kernel rather than a memset so a
+// captured graph replays it).
+template <bool RIGHT, int MODE, int R>
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4)))
+ksw_stripe_queue_kernel(StripeArgs a, int32_t *qhead)
+{
+    constexpr int SR = kWave * R;
+    __shared__ int32_t prog[1];
+    __shared__ uint64_t tile[8][64];
+    __shared__ int shared_n;
+    for (;;) {
+        int idx = 0;
+        if (threadIdx.x == 0) idx = __hip_atomic_fetch_add(qhead, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        idx = __builtin_amdgcn_readlane(idx, 0);
+        if (idx >= a.njobs) return;
+        const int job = __builtin_amdgcn_readfirstlane(a.order[idx]);
+        const KswJob jb = a.jobs[job];
+        prog[0] = 0;
+        __syncthreads();
+        const Consts kc = make_consts<RIGHT>(a.sc, jb.transposed != 0);
+        const int qlen = __builtin_amdgcn_readfirstlane(jb.qlen), tlen = __builtin_amdgcn_readfirstlane(jb.tlen);
+        const int npass = ((tlen + SR - 1) / SR + 1) / 2;
+        int late = 0, stop = 0;
+        if (jb.lut)
+            for (int p = 0; p < npass && !stop; ++p)
+                pass_body<RIGHT, MODE, false, true, R>(a, kc, job, p, jb, qlen, tlen, prog, 1, late, stop);
+        else
+            for (int p = 0; p < npass && !stop; ++p)
+                pass_body<RIGHT, MODE, false, false, R>(a, kc, job, p, jb, qlen, tlen, prog, 1, late, stop);
+        __syncthreads();
+        pair_epilogue<R>(a, job, jb, tile, MODE == kTracks, &shared_n, kWave);
+        __syncthreads();  // tile and prog are the next pair's
+    }
+}
+
+// Zeroes the queue heads of a plan's queue launches.
+__global__ void __launch_bounds__(64) ksw_stripe_qhead_kernel(int32_t *qhead, int n)
+{
+    if ((int)threadIdx.x < n) qhead[threadIdx.x] = 0;
+}
+
 // Wide teams: each pair has its own run of consecutive blocks (a.wfirst),
 // which the dispatcher spreads over the XCDs' CUs; wave v of a team of W
 // waves computes passes v, v+W, ...  A pair's control block holds one
@@ -926,6 +977,56 @@ struct LaunchStripe {
         return hipGetLastError();
     }
 };
+
+template <bool RIGHT, int MODE, int R>
+struct LaunchQueue {
+    static hipError_t run(const StripeArgs &a, int32_t *qhead, int workers, hipStream_t st)
+    {
+        hipLaunchKernelGGL((ksw_stripe_queue_kernel<RIGHT, MODE, R>), dim3(workers), dim3(kWave), 0, st, a, qhead);
+        return hipGetLastError();
+    }
+};
+
+template <bool RIGHT, int MODE, int R>
+struct QueueWorkers {
+    static int run()
+    {
+        int per_cu = 0, dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 1024;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ksw_stripe_queue_kernel<RIGHT, MODE, R>, kWave, 0) !=
+                hipSuccess ||
+            per_cu <= 0)
+            return prop.multiProcessorCount * 4;
+        return per_cu * prop.multiProcessorCount;
+    }
+};
+
+hipError_t launch_stripe_queue(const StripeArgs &a, int group, int R, int32_t *qhead, hipStream_t st)
+{
+    if (a.njobs <= 0) return hipSuccess;
+    if (!qhead || (R != 4 && R != 8) || group < 0 || group >= kGroups) return hipErrorInvalidValue;
+    return by_variant<LaunchQueue>(group, R, a, qhead, std::min(a.njobs, stripe_queue_workers(group, R)), st);
+}
+
+hipError_t clear_queue_heads(int32_t *qhead, int n, hipStream_t st)
+{
+    if (n < 1 || n > 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(ksw_stripe_qhead_kernel, dim3(1), dim3(64), 0, st, qhead, n);
+    return hipGetLastError();
+}
+
+// Cached per variant: the occupancy query reads the device properties.
+int stripe_queue_workers(int group, int R)
+{
+    static int cache[kGroups][2] = {};
+    if (group < 0 || group >= kGroups) group = kGroupLeft;
+    const int ri = R == 4 ? 1 : 0;
+    if (cache[group][ri] > 0) return cache[group][ri];
+    const int n = by_variant<QueueWorkers>(group, R);
+    __atomic_store_n(&cache[group][ri], n, __ATOMIC_RELAXED);
+    return n;
+}
 
 hipError_t launch_stripe(const StripeArgs &a, int group, int team, int R, hipStream_t st)
 {

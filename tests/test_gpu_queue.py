@@ -1,0 +1,62 @@
+"""Persistent one-wave workers over a device work queue: a plan's one-wave
+teams run as workers that claim pairs from the launch's longest-first list
+with a device-scope atomic add until it is exhausted.  More pairs than the
+device holds waves (each worker takes several), every flag group, repeated
+runs (graph replays), and the same pairs launched one workgroup per pair
+(KSW2_HIP_QUEUE=0) all give the reference's results."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+from anchorwave_amd import ksw2, workloads as W
+from helpers import diff, random_pairs, same
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(got, exp, tag):
+    bad = [f"{tag} pair {k}: {diff(g, r)}" for k, (g, r) in enumerate(zip(got, exp)) if not same(g, r)]
+    assert not bad, "\n".join(bad[:5]) + f"\n({len(bad)} of {len(exp)} differ)"
+
+
+def test_more_pairs_than_workers():
+    """9,000 short pairs: more than the 4,096 one-wave workers the device
+    holds, so workers claim again after their first pair."""
+    print("queue: 9000 short pairs", flush=True)
+    pairs = random_pairs(3101, 9000, 20, 600)
+    exp = [O.run_oracle(q, t) for q, t in pairs]
+    plan = ksw2.Plan(pairs)
+    for rep in range(3):
+        plan.run()
+        _check(plan.fetch(), exp, f"run {rep}")
+    plan.close()
+
+
+@pytest.mark.parametrize("flag", [0, ksw2.KSW_EZ_RIGHT, ksw2.KSW_EZ_SCORE_ONLY, ksw2.KSW_EZ_SINGLETRACK,
+                                  ksw2.KSW_EZ_SEMIGLOBAL_END, ksw2.KSW_EZ_EXTZ_ONLY | ksw2.KSW_EZ_REV_CIGAR])
+def test_queue_and_grid_agree(flag, monkeypatch):
+    """The fixed batch's kind of pairs plus short ragged ones, team 1 forced:
+    the queue workers' results equal the oracle's and those of one
+    workgroup per pair."""
+    print(f"queue vs grid flag {flag:#x}", flush=True)
+    monkeypatch.setenv("KSW2_HIP_TEAM", "1")
+    pairs = W.fixed(seed=7, n=300) + random_pairs(3200 + flag, 500, 1, 700)
+    exp = [O.run_oracle(q, t, flag=flag) for q, t in pairs]
+    got_q = ksw2.extd2_batch(pairs, flag=flag)
+    _check(got_q, exp, "queue")
+    monkeypatch.setenv("KSW2_HIP_QUEUE", "0")
+    got_g = ksw2.extd2_batch(pairs, flag=flag)
+    _check(got_g, exp, "grid")
+
+
+def test_mixed_groups_in_one_plan(monkeypatch):
+    """Four flag groups in one plan: four queue launches, each with its own
+    head."""
+    monkeypatch.setenv("KSW2_HIP_TEAM", "1")
+    flags = [0, ksw2.KSW_EZ_RIGHT, ksw2.KSW_EZ_SCORE_ONLY, ksw2.KSW_EZ_SINGLETRACK]
+    pairs = random_pairs(3300, 1600, 1, 900)
+    rng = np.random.default_rng(5)
+    fl = [int(flags[k]) for k in rng.integers(0, 4, len(pairs))]
+    exp = [O.run_oracle(q, t, flag=f) for (q, t), f in zip(pairs, fl)]
+    got = ksw2.extd2_batch([(q, t, f) for (q, t), f in zip(pairs, fl)])
+    _check(got, exp, "mixed")
