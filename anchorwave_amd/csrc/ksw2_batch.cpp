@@ -1,7 +1,9 @@
 // ksw2_batch.cpp -- batches of pairs on one device: eligibility, staging,
 // launch, read-back.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -9,6 +11,7 @@
 #include <memory>
 #include <chrono>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -200,6 +203,74 @@ static int choose_team(const KswJob &j, double T, int forced, uint64_t wave_byte
 }
 
 
+// Lone-pair layouts measured by anchorwave_amd/autotune.py on the GPU: per
+// bin of swept and striped lengths, the rows per lane, team size and layout
+// (one workgroup, or spread over CUs) that ran a pair of that size fastest.
+// Read once from KSW2_HIP_TUNING (a path; "0": none) or from
+// tuning/gfx950.txt beside the library's package; used for plans that leave
+// most of the device idle (latency_plan).
+struct TuneEntry {
+    int q_lo, q_hi, t_lo, t_hi, rows, team, passes, spread;
+    // the team for a pair of P passes at these rows: the measured team scaled
+    // to the pair's passes, at most one wave per pass
+    int team_for(int P) const
+    {
+        const int t = team >= passes ? P : (int)(((int64_t)team * P + passes / 2) / passes);
+        return std::max(1, std::min(t, P));
+    }
+};
+
+static const std::vector<TuneEntry> &tune_table()
+{
+    static std::vector<TuneEntry> table;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        std::string path;
+        if (const char *ev = getenv("KSW2_HIP_TUNING")) {
+            if (ev[0] == '0' && ev[1] == 0) return;
+            path = ev;
+        } else {
+            Dl_info info;
+            if (!dladdr((void *)&tune_table, &info) || !info.dli_fname) return;
+            path = info.dli_fname;  // .../anchorwave_amd/lib/libksw2hip.so
+            const size_t cut = path.rfind('/');
+            path = (cut == std::string::npos ? std::string(".") : path.substr(0, cut)) + "/../tuning/gfx950.txt";
+        }
+        FILE *f = fopen(path.c_str(), "r");
+        if (!f) return;
+        char line[512];
+        while (fgets(line, sizeof line, f)) {
+            TuneEntry e;
+            if (sscanf(line, "bin %d %d %d %d %d %d %d %d", &e.q_lo, &e.q_hi, &e.t_lo, &e.t_hi, &e.rows, &e.team,
+                       &e.passes, &e.spread) == 8 &&
+                (e.rows == 4 || e.rows == 8) && e.team >= 1 && e.team <= kMaxWide && e.passes >= 1)
+                table.push_back(e);
+        }
+        fclose(f);
+    });
+    return table;
+}
+
+// The measured layout for a pair of swept length kq and striped length kt, or null.
+static const TuneEntry *tune_lookup(int kq, int kt)
+{
+    for (const TuneEntry &e : tune_table())
+        if (kq >= e.q_lo && kq < e.q_hi && kt >= e.t_lo && kt < e.t_hi) return &e;
+    return nullptr;
+}
+
+// A plan of its own (or one whose owner allows spread teams) that leaves
+// most of the device idle and forces nothing: its time is its pairs' chains
+// of passes, so the measured lone-pair layouts apply.
+static bool latency_plan(const ksw2_pair_t *pairs, const std::vector<int> &fast, const PlanEnv *env)
+{
+    if (fast.empty() || (env && (!env->spread || env->team != 0 || env->rows != 0))) return false;
+    if (getenv("KSW2_HIP_ROWS") || getenv("KSW2_HIP_TEAM") || getenv("KSW2_HIP_WIDE")) return false;
+    int64_t waves = 0;
+    for (int id : fast) waves += pass_count(stripe_geom(pairs[id]).kt, 8);
+    return 8 * waves <= stripe_resident_waves(kGroupLeft, 4);
+}
+
 // The stripe kernel's byte-table score (one v_perm gives two cells' score
 // candidates) needs every code below 4 and both candidates, 8|mis| plus a
 // tag of at most 7, in a byte.  KSW2_HIP_LUT=0 turns it off (tests).
@@ -248,6 +319,14 @@ static int choose_rows(const ksw2_pair_t *pairs, const std::vector<int> &fast, c
     }
     if (env && (env->rows == 4 || env->rows == 8)) return env->rows;
     if (fast.empty()) return kRowsPerLane;
+    if (latency_plan(pairs, fast, env)) {
+        // the measured layout of the plan's largest pair
+        int big = fast[0];
+        for (int id : fast)
+            if ((double)pairs[id].qlen * pairs[id].tlen > (double)pairs[big].qlen * pairs[big].tlen) big = id;
+        const StripeGeom g = stripe_geom(pairs[big]);
+        if (const TuneEntry *e = tune_lookup(g.kq, g.kt)) return e->rows;
+    }
     int64_t waves = 0;
     for (int id : fast) waves += pass_count(stripe_geom(pairs[id]).kt, 8);
     return 8 * waves <= stripe_resident_waves(kGroupLeft, 4) ? 4 : 8;
@@ -467,12 +546,26 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         // launch (other threads' calls hold them), it runs as a one-workgroup
         // team instead of waiting.
         std::vector<char> spread(p->n_fast, 0);
+        std::vector<char> tuned(p->n_fast, 0);
+        if (!wide_forced && forced == 0 && latency_plan(pairs, fast, env)) {
+            // measured lone-pair layouts (a pair of another bin than the one
+            // that set the plan's rows keeps its team size, within its passes)
+            for (int k = 0; k < p->n_fast; ++k) {
+                const KswJob &j = p->jobs[k];
+                const TuneEntry *e = tune_lookup(j.qlen, j.tlen);
+                if (!e) continue;
+                int t = e->team_for(pass_count(j.tlen, j.R));
+                if (t > kMaxTeam) t = std::min(kMaxWide, (t + kWideGroup - 1) / kWideGroup * kWideGroup);
+                team[k] = t, p->jobs[k].team = t, tuned[k] = 1;
+                spread[k] = e->spread && t > kWideGroup && t <= kMaxTeam;
+            }
+        }
         if (!wide_forced && ((!env && forced == 0) || (env && env->spread && forced >= 0))) {
             int64_t waves = 0;
             for (int k = 0; k < p->n_fast; ++k) waves += team[k];
             if (4 * waves <= stripe_resident_waves(kGroupLeft, R))
                 for (int k = 0; k < p->n_fast; ++k)
-                    if (team[k] > kWideGroup && team[k] <= kMaxTeam) spread[k] = 1;
+                    if (!tuned[k] && team[k] > kWideGroup && team[k] <= kMaxTeam) spread[k] = 1;
         }
         auto wide = [&](int k) { return wide_forced > 0 || team[k] > kMaxTeam || spread[k]; };
         p->optional_wide = true;

@@ -60,3 +60,20 @@ def test_mixed_groups_in_one_plan(monkeypatch):
     exp = [O.run_oracle(q, t, flag=f) for (q, t), f in zip(pairs, fl)]
     got = ksw2.extd2_batch([(q, t, f) for (q, t), f in zip(pairs, fl)])
     _check(got, exp, "mixed")
+
+
+def test_autotuned_lone_pair_layouts():
+    """Single calls on pairs whose bins the autotuner's table covers (short
+    swept, long striped sequence: eight rows, a few waves) give the
+    reference's results; the table is read by the library itself."""
+    rng = np.random.default_rng(41)
+    pairs = []
+    for qn, tn in ((1000, 9000), (1200, 20000), (2500, 16000), (2000, 30000), (4000, 30000)):
+        t = W.random_seq(rng, tn)
+        q = W.mutate(rng, t[tn // 3: tn // 3 + qn], 0.06, 0.015)
+        pairs.append((q, t))
+    for k, (q, t) in enumerate(pairs):
+        print(f"tuned lone pair {len(q)} x {len(t)}", flush=True)
+        g = ksw2.extd2(q, t)
+        r = O.run_oracle(q, t)
+        assert same(g, r), f"pair {k}: {diff(g, r)}"
