@@ -756,6 +756,27 @@ __device__ __forceinline__ void pair_epilogue(const StripeArgs &a, int job_id, c
                 __syncthreads();
             }
             const int rel = r - t_r_lo;
+            if (state == 0) {
+                // A run of diagonal moves at once: from state 0 the walk
+                // stays in state 0 and moves diagonally while each cell's
+                // argmax (its low three bits) is 0.  Lane k tests cell
+                // (r - k, c - k) of the staged tile (within the tile and the
+                // matrix); the run is the lanes before the first that fails.
+                const int lim = min(min(rel, c - t_c_lo), min(i, j));
+                const int lk = tid & (kWave - 1);
+                int fail = 1;
+                if (lk <= lim) {
+                    const int rr = rel - lk;
+                    fail = ((int)(tile[rr / R][c - lk - t_c_lo] >> (8 * (rr % R))) & 7) != 0;
+                }
+                const uint64_t fb = __ballot(fail);
+                const int run = fb ? (int)__builtin_ctzll(fb) : kWave;
+                if (run > 0) {
+                    push(0, run);
+                    i -= run, j -= run;
+                    continue;
+                }
+            }
             const int tmp = (int)(tile[rel / R][c - t_c_lo] >> (8 * (rel % R))) & 0xff;
             if (state == 0) state = tmp & 7;
             else if (!(tmp >> (state + 2) & 1)) state = 0;
