@@ -1308,13 +1308,25 @@ int run_batch_host(int n, const ksw2_pair_t *pairs, int8_t m, const int8_t *mat,
     for (int k : todo) total += pair_footprint(pairs[k], takes_general(pairs[k], m, cls));
     // (a batch of at most 1 GiB, a single call's usual size, skips the
     // device memory query)
-    if (todo.size() <= 1 || (total <= (1ull << 30) && !getenv("KSW2_HIP_MEM_BUDGET")) || total <= mem_budget(false)) {
+    // A plan larger than a single call's block is carved from the process's
+    // cached device block when that holds it (allocating and freeing
+    // hundreds of GB per call costs seconds: the driver clears freed memory
+    // before it hands it out again).
+    const bool small = todo.size() <= 1 || (total <= (1ull << 30) && !getenv("KSW2_HIP_MEM_BUDGET"));
+    std::unique_ptr<DevicePool> big;
+    if (!small && total > kCallBlockMax && total <= mem_budget(true)) big = cached_block_pool(mem_budget(true), total);
+    if (small || big || total <= mem_budget(false)) {
         // one plan holds the whole batch; a single pair up to kCallBlockMax
         // runs in the thread's cached context
         if (todo.empty()) return status;
         Plan *plan = nullptr;
         PlanEnv env;
         const PlanEnv *use = nullptr;
+        if (big) {
+            env.pool = big.get();
+            env.spread = true;  // a plan of its own
+            use = &env;
+        }
         if (todo.size() == 1 && total <= kCallBlockMax) {
             const ksw2_pair_t &pp = pairs[todo[0]];
             if (t_call.prepare((uint64_t)pp.qlen + pp.tlen + 64)) {

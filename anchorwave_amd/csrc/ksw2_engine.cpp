@@ -135,6 +135,13 @@ private:
 
 }  // namespace
 
+std::unique_ptr<DevicePool> cached_block_pool(uint64_t bytes, uint64_t need)
+{
+    auto pool = std::make_unique<FirstFitPool>();
+    if (pool->init(std::max(bytes, need), true) != KSW2_HIP_OK || pool->bytes < need) return nullptr;
+    return pool;
+}
+
 // Size classes keep a flight's members of similar duration, so short tasks
 // are not held back by long ones that share their plan.
 static int size_class(double cells) { return cells < 3e7 ? 0 : cells < 5e8 ? 1 : 2; }

@@ -534,6 +534,35 @@ __global__ void __launch_bounds__(64) ksw_general_backtrack_tiled_kernel(GenBack
                 __syncthreads();
             }
             const int L = r0 - r;
+            if (state == 0) {
+                // A run of diagonal moves at once: from state 0 the walk
+                // stays in state 0 and moves to (i-1, j-1), anti-diagonal
+                // r-2, while a cell is not forced and its argmax (low three
+                // bits) is 0.  Lane k tests cell (i-k, j-k) of the tile (row
+                // L+2k); the run is the lanes before the first that fails.
+                const int Lk = L + 2 * lane, ik = i - lane;
+                const int o_k = __shfl(ro, Lk & (kWave - 1)), e_k = __shfl(re, Lk & (kWave - 1));
+                const int b_k = __shfl(base, Lk & (kWave - 1));
+                int fail = 1;
+                if (Lk < kTileRows && ik >= 0 && lane <= j && ik >= ws && ik >= o_k && ik <= e_k) {
+                    const int byte = tb[Lk * kTileDw * 4 + (ik - b_k)];
+                    const int tmp = (uint32_t)(ik - o_k) < (uint32_t)ncolB ? byte : 0;
+                    fail = (tmp & 7) != 0;
+                }
+                const uint64_t fb = __ballot(fail);
+                const int run = fb ? (int)__builtin_ctzll(fb) : kWave;
+                if (run > 0) {
+                    i -= run, j -= run;
+                    if (run_op == 0) {
+                        run_len += run;
+                    } else {
+                        if (run_len > 0 && lane == 0) cig[n] = (uint32_t)run_len << 4 | (uint32_t)run_op;
+                        if (run_len > 0) ++n;
+                        run_op = 0, run_len = run;
+                    }
+                    continue;
+                }
+            }
             const int o_r = __builtin_amdgcn_readlane(ro, L), e_r = __builtin_amdgcn_readlane(re, L);
             const int b_r = __builtin_amdgcn_readlane(base, L);
             // (the row's byte is read whether or not it is used: the tile

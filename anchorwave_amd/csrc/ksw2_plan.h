@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <memory>
 #include <vector>
 
 #include "../../include/ksw2_hip.h"
@@ -82,6 +83,13 @@ int band_r_end(int qlen, int tlen, int w);
 // Device bytes an engine may hold: 85 % of the free device memory, with
 // the cached engine block counted as free when with_cache.
 uint64_t mem_budget(bool with_cache);
+
+struct DevicePool;
+// A pool over the process's cached device block (the batch engine's, kept
+// between calls), of `bytes` when it must be allocated; null when the block
+// is busy or holds less than `need` and cannot be replaced.  The block
+// returns to the cache when the pool is destroyed.
+std::unique_ptr<DevicePool> cached_block_pool(uint64_t bytes, uint64_t need);
 
 // Device memory pool of a serving engine: one hipMalloc carved first-fit.
 struct DevicePool {
