@@ -28,9 +28,14 @@ namespace {
 std::mutex g_init_mu;
 int g_device = -1;
 
-#define KSW_HIP_TRY(x)                                  \
-    do {                                                \
-        if ((x) != hipSuccess) return KSW2_HIP_EDEVICE; \
+// A failed HIP call names itself on stderr (the caller sees KSW2_HIP_EDEVICE)
+#define KSW_HIP_TRY(x)                                                                               \
+    do {                                                                                             \
+        const hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) {                                                                      \
+            fprintf(stderr, "[ksw2] %s:%d %s: %s\n", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
+            return KSW2_HIP_EDEVICE;                                                                 \
+        }                                                                                            \
     } while (0)
 
 uint64_t round_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
@@ -89,10 +94,15 @@ bool wait_slots_acquire(hipStream_t st, int q[2])
     for (int i = 0; i < 2; ++i) {
         hipEvent_t &ev = g_quarter[q[i]];
         if (!ev && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            fprintf(stderr, "[ksw2] wait slot %d: hipEventCreateWithFlags failed\n", q[i]);
             ev = nullptr;
             return false;
         }
-        if (hipStreamWaitEvent(st, ev, 0) != hipSuccess) return false;
+        const hipError_t e = hipStreamWaitEvent(st, ev, 0);
+        if (e != hipSuccess) {
+            fprintf(stderr, "[ksw2] wait slot %d: hipStreamWaitEvent: %s\n", q[i], hipGetErrorString(e));
+            return false;
+        }
         g_quarter_used[q[i]] = ++g_quarter_seq;
     }
     return true;
@@ -109,8 +119,13 @@ int wait_slots_free()
 
 bool wait_slots_release(hipStream_t st, const int q[2])
 {
-    for (int i = 0; i < 2; ++i)
-        if (hipEventRecord(g_quarter[q[i]], st) != hipSuccess) return false;
+    for (int i = 0; i < 2; ++i) {
+        const hipError_t e = hipEventRecord(g_quarter[q[i]], st);
+        if (e != hipSuccess) {
+            fprintf(stderr, "[ksw2] wait slot %d: hipEventRecord: %s\n", q[i], hipGetErrorString(e));
+            return false;
+        }
+    }
     return true;
 }
 }  // namespace
