@@ -166,7 +166,7 @@ struct Engine::Task {
     int status = 0;    // KSW2_HIP_OK, or why the device path did not run it
     bool general = false;
     bool resident = false;  // sequences in the engine's device arena at seq_off
-    uint64_t seq_off = 0;
+    uint64_t seq_off = 0, seq_pos = 0;  // and the end of their range in the submission stream
     double t_submit = 0, t_start = 0, t_done = 0;
 };
 
@@ -232,22 +232,20 @@ int Engine::start(bool cached, bool resident_seq)
     budget_ = pool->bytes;  // a cached block may be larger than asked
     if (st != KSW2_HIP_OK) return st;
     if (resident_seq) {
-        // a sixteenth of the pool, at most 4 GiB (about 400k tasks of 5 kbp);
+        // a sixteenth of the pool, at most 512 MiB (about 50k tasks of 5 kbp);
         // a pool under 16 MiB keeps none (flights then upload their own)
         SeqArena &A = arena_;
-        A.bytes = std::min<uint64_t>(budget_ / 16, 4ull << 30) / 256 * 256;
-        A.ring_bytes = std::min<uint64_t>(64ull << 20, A.bytes);
+        A.bytes = std::min<uint64_t>(budget_ / 16, 512ull << 20) / 256 * 256;
         if (A.bytes >= (1u << 20)) {
             if (!pool->alloc(A.bytes, &A.pool_off)) return KSW2_HIP_ENOMEM;
-            if (hipHostMalloc((void **)&A.ring, A.ring_bytes, hipHostMallocDefault) != hipSuccess) return KSW2_HIP_ENOMEM;
-            if (hipStreamCreateWithFlags(&A.up, hipStreamNonBlocking) != hipSuccess ||
-                hipEventCreateWithFlags(&A.ev_up, hipEventDisableTiming) != hipSuccess)
-                return KSW2_HIP_EDEVICE;
+            if (hipHostMalloc((void **)&A.ring, A.bytes, hipHostMallocDefault) != hipSuccess) return KSW2_HIP_ENOMEM;
+            if (hipStreamCreateWithFlags(&A.up, hipStreamNonBlocking) != hipSuccess) return KSW2_HIP_EDEVICE;
+            for (auto &ev : A.ev)
+                if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return KSW2_HIP_EDEVICE;
             A.dev = pool->base + A.pool_off;
-            A.carve.reset(A.bytes);
             budget_ -= A.bytes;
         } else {
-            A.bytes = 0, A.ring_bytes = 0;
+            A.bytes = 0;
         }
     }
     // the upload stream (created first) takes a hardware queue of its own:
@@ -283,7 +281,8 @@ Engine::~Engine()
         if (b) (void)hipHostFree(b);
     for (auto &m : mirror_) m.release();
     if (arena_.up) (void)hipStreamSynchronize(arena_.up), (void)hipStreamDestroy(arena_.up);
-    if (arena_.ev_up) (void)hipEventDestroy(arena_.ev_up);
+    for (auto &ev : arena_.ev)
+        if (ev) (void)hipEventDestroy(ev);
     if (arena_.ring) (void)hipHostFree(arena_.ring);
     if (h_stop_) (void)hipHostFree(h_stop_);
     for (auto &kv : tasks_) delete kv.second;
@@ -314,44 +313,69 @@ bool Engine::queues_empty() const
 // 16-byte boundary plus 16, then the target likewise.
 static uint64_t seq_span(int qlen, int tlen) { return (uint64_t)(qlen + 31) / 16 * 16 + (uint64_t)(tlen + 31) / 16 * 16; }
 
-bool Engine::upload_seq_locked(Task *t)
+bool Engine::stage_seq_locked(Task *t)
 {
     SeqArena &A = arena_;
     const int ql = t->pair.qlen, tl = t->pair.tlen;
     const uint64_t n = seq_span(ql, tl), tq = (uint64_t)(ql + 31) / 16 * 16;
-    if (n > A.ring_bytes) return false;
-    uint64_t off;
-    if (!A.carve.alloc(n, &off)) return false;
-    if (A.ring_w + n > A.ring_bytes) {
-        // the ring is reused from its start once the copies out of it are done
-        if (hipStreamSynchronize(A.up) != hipSuccess) {
-            A.carve.free(off);
-            return false;
-        }
-        A.ring_w = 0;
-    }
-    uint8_t *h = A.ring + A.ring_w;
+    uint64_t p = A.tail;
+    if (p % A.bytes + n > A.bytes) p += A.bytes - p % A.bytes;  // never straddle the end
+    if (p + n - A.head > A.bytes) return false;                  // the ring is full
+    uint8_t *h = A.ring + p % A.bytes;
     memcpy(h, t->pair.query, (size_t)ql);
     memset(h + ql, 0, (size_t)(tq - ql));
     memcpy(h + tq, t->pair.target, (size_t)tl);
     memset(h + tq + tl, 0, (size_t)(n - tq - tl));
-    if (hipMemcpyAsync(A.dev + off, h, n, hipMemcpyHostToDevice, A.up) != hipSuccess ||
-        hipEventRecord(A.ev_up, A.up) != hipSuccess) {
-        // the copy may be queued: keep the range (it returns with the engine)
-        return false;
-    }
-    A.ring_w += n;
-    A.uploads = true;
+    A.tail = p + n;
+    A.live.emplace_back(p, p + n);
     ++resident_tasks_;
-    t->seq_off = off, t->resident = true;
+    t->seq_off = p % A.bytes, t->seq_pos = p + n, t->resident = true;
     return true;
+}
+
+uint64_t Engine::upload_seq()
+{
+    std::lock_guard<std::mutex> lk(seq_mu_);
+    SeqArena &A = arena_;
+    if (!A.dev) return 0;
+    // completed uploads, oldest first
+    while (!A.pending.empty() && hipEventQuery(A.ev[A.pending.front().second]) == hipSuccess)
+        A.done = A.pending.front().first, A.pending.pop_front();
+    if (A.flushed < A.tail && (int)A.pending.size() < kUpEvents) {
+        // what was written since the last upload: one copy, two when it wraps
+        bool ok = true;
+        uint64_t from = A.flushed;
+        while (ok && from < A.tail) {
+            const uint64_t o = from % A.bytes, len = std::min(A.tail - from, A.bytes - o);
+            ok = hipMemcpyAsync(A.dev + o, A.ring + o, len, hipMemcpyHostToDevice, A.up) == hipSuccess;
+            from += len;
+        }
+        const int e = A.next_ev;
+        if (ok && hipEventRecord(A.ev[e], A.up) == hipSuccess) {
+            A.pending.emplace_back(A.tail, e);
+            A.next_ev = (e + 1) % kUpEvents;
+            A.flushed = A.tail;
+        }
+    }
+    return A.done;
 }
 
 void Engine::free_seq(Task *t)
 {
     if (!t->resident) return;
     std::lock_guard<std::mutex> lk(seq_mu_);
-    arena_.carve.free(t->seq_off);
+    SeqArena &A = arena_;
+    for (auto &r : A.live)
+        if (r.second == t->seq_pos) {
+            r.second = 0;
+            break;
+        }
+    // the head passes the freed ranges at the front; it never passes a
+    // range whose upload may still read the host ring
+    while (!A.live.empty() && A.live.front().second == 0) {
+        A.live.pop_front();
+        A.head = A.live.empty() ? std::min(A.tail, A.done) : A.live.front().first;
+    }
     t->resident = false;
 }
 
@@ -417,7 +441,7 @@ int Engine::submit(const ksw2_pair_t &p, ksw_extz_t *ez, bool copy, uint64_t *ti
             t->pair.query = t->qcopy.data(), t->pair.target = t->tcopy.data();
             if (arena_.dev) {
                 std::lock_guard<std::mutex> lk(seq_mu_);
-                (void)upload_seq_locked(t);  // else the flight uploads it
+                (void)stage_seq_locked(t);  // else the flight uploads it
             }
         }
         t->cells = (double)p.qlen * p.tlen;
@@ -842,21 +866,25 @@ bool Engine::admit_locked(std::unique_lock<std::mutex> &lk)
             env.mirror_cig = mr.cig, env.mirror_cig_dev = mr.cig_dev;
             env.mirror_flag = mr.flag, env.mirror_flag_dev = mr.flag_dev;
         }
-        // every task's sequences in the arena: the plan reads them there,
-        // after the uploads so far (the flight's stream waits for them)
+        // every task's sequences in the arena and uploaded: the plan reads
+        // them there (else it uploads its own)
         std::vector<uint64_t> seq_offs;
         bool resident = arena_.dev != nullptr;
-        for (const Task *tk : pick) resident = resident && tk->resident;
-        if (resident) {
+        uint64_t last = 0;
+        for (const Task *tk : pick) resident = resident && tk->resident, last = std::max(last, tk->seq_pos);
+        if (resident && last > upload_seq()) {
+            // the uploads normally land within microseconds: wait a little,
+            // never long (the upload stream may share a busy hardware queue)
+            for (int spin = 0; spin < 20 && last > upload_seq(); ++spin)
+                std::this_thread::sleep_for(std::chrono::microseconds(10));
+        }
+        if (resident && last <= upload_seq()) {
             seq_offs.resize(2 * pick.size());
             for (size_t k = 0; k < pick.size(); ++k) {
                 seq_offs[2 * k] = pick[k]->seq_off;
                 seq_offs[2 * k + 1] = pick[k]->seq_off + (uint64_t)(pick[k]->pair.qlen + 31) / 16 * 16;
             }
             env.seq_dev = arena_.dev, env.seq_offs = seq_offs.data();
-            std::lock_guard<std::mutex> sl(seq_mu_);
-            if (arena_.uploads && hipStreamWaitEvent(streams_[s], arena_.ev_up, 0) != hipSuccess)
-                env.seq_dev = nullptr, env.seq_offs = nullptr;  // upload them with the flight
         }
         const bool staged = !env.seq_dev;
         const double t_c0 = now_s();

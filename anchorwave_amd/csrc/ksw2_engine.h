@@ -131,25 +131,37 @@ private:
     uint8_t *staging_[kMaxStreams] = {};           // pinned upload buffers, per stream
     uint64_t staging_bytes_[kMaxStreams] = {};
     int32_t *h_stop_ = nullptr, *d_stop_ = nullptr;  // stop flag shared by all flights
-    // Device-resident sequences (serving): an arena carved from the pool,
-    // filled at submission from a pinned ring through one upload stream;
-    // ev_up_ is recorded after every upload, and a flight's stream waits on
-    // it before its kernels.  A task's range returns when it is delivered.
+    // Device-resident sequences (serving): a device arena carved from the
+    // pool and a pinned host ring of the same size, both used in submission
+    // order (a task's range never straddles the end).  A submission copies
+    // its pair into the host ring (no HIP call); the worker uploads what was
+    // written since its last pass with one or two copies on its upload
+    // stream and records an event; a flight whose tasks' uploads have all
+    // completed reads the arena with no device-side wait (which could stall
+    // behind other work sharing the upload stream's hardware queue),
+    // otherwise it uploads its own sequences.  Ranges are freed at delivery;
+    // the ring's head moves past the oldest freed ones.
+    static constexpr int kUpEvents = 64;
     struct SeqArena {
-        uint8_t *dev = nullptr;       // arena start (inside the pool)
+        uint8_t *dev = nullptr;       // device arena (inside the pool)
+        uint8_t *ring = nullptr;      // pinned host ring, same layout
         uint64_t pool_off = 0, bytes = 0;
-        RangeCarver carve;
-        uint8_t *ring = nullptr;      // pinned staging, filled linearly
-        uint64_t ring_bytes = 0, ring_w = 0;
+        // positions in the unbounded submission stream (ring offset =
+        // position % bytes): live ranges from head, written up to tail,
+        // uploaded up to flushed, known complete up to done
+        uint64_t head = 0, tail = 0, flushed = 0, done = 0;
+        std::deque<std::pair<uint64_t, uint64_t>> live;  // [start, end) in order; end 0: freed
         hipStream_t up = nullptr;
-        hipEvent_t ev_up = nullptr;
-        bool uploads = false;         // an upload was recorded since the last wait
+        hipEvent_t ev[kUpEvents] = {};
+        std::deque<std::pair<uint64_t, int>> pending;     // (end position, event) of uploads in flight
+        int next_ev = 0;
     };
     SeqArena arena_;
     std::mutex seq_mu_;
     int64_t resident_tasks_ = 0;   // under seq_mu_
     int64_t staged_flights_ = 0;   // under mu_
-    bool upload_seq_locked(Task *t);     // holds seq_mu_
+    bool stage_seq_locked(Task *t);  // copies into the host ring (holds seq_mu_)
+    uint64_t upload_seq();           // issues the pending uploads; returns the completed position
     void free_seq(Task *t);
     std::thread worker_;
     std::mutex mu_;

@@ -45,9 +45,10 @@ def test_server_results_alone_and_in_the_mix():
             assert lat >= 0
     ld = srv.load()
     srv.close()
-    # every pair's sequences went to the device arena at submission and no
-    # flight uploaded its own
-    assert ld["seq_arena_bytes"] > 0 and ld["resident_tasks"] == len(pairs) and ld["staged_flights"] == 0, ld
+    # every pair's sequences went to the device arena at submission; a flight
+    # uploads its own only when an upload it needs has not landed yet
+    assert ld["seq_arena_bytes"] > 0 and ld["resident_tasks"] == len(pairs), ld
+    assert ld["staged_flights"] <= len(pairs) // 4, ld
     for k, p in enumerate(pairs):
         alone = ksw2.extd2(p[0], p[1], flag=p[2], w=p[3], zdrop=p[4], end_bonus=p[5])
         exp = _oracle(p)
