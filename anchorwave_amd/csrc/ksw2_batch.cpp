@@ -869,6 +869,23 @@ static void stripe_args(const Plan *p, const int32_t *poll, StripeArgs &sa)
 // Captures an eligible plan's stripe launches on its stream (forking to the
 // lane streams the first run created, joining back) and instantiates them.
 // A failed capture leaves the plan on direct launches for good.
+// A launch of one-wave teams can run as persistent queue workers
+// (KSW2_HIP_QUEUE=1; =2: only when the launch has more pairs than the device
+// holds workers).  By default it is a grid of one workgroup per pair: the
+// hardware dispatcher hands out the same longest-first list with less work
+// per pair (graph replays of the fixed batch, 4,096 pairs: 12.83 against
+// 12.97 ms; 16,384 pairs of 300-800 bp: 6.93 against 6.99 ms).
+static int queue_mode()
+{
+    const char *qv = getenv("KSW2_HIP_QUEUE");
+    return qv ? std::max(0, std::min(2, atoi(qv))) : 0;
+}
+static bool use_queue(int mode, const Plan *p, const Plan::Launch &l)
+{
+    if (mode == 0 || l.team != 1) return false;
+    return mode == 1 || l.count > stripe_queue_workers(l.grp, p->R);
+}
+
 static void plan_capture(Plan *p)
 {
     hipStream_t s = p->stream;
@@ -883,6 +900,10 @@ static void plan_capture(Plan *p)
     stripe_args(p, p->poll_stop ? p->d_stop : nullptr, sa);
     ok = hipEventRecord(p->fork_ev, s) == hipSuccess;
     bool used[Plan::kLanes] = {false, false, false, false};
+    // the same launches as a direct run (plan_launch): one-wave teams as
+    // persistent queue workers, their heads cleared by a fill kernel node
+    const int qmode = queue_mode();
+    int nq = 0;
     for (const Plan::Launch &l : p->launches) {
         hipStream_t st = s;
         if (p->own_streams) {
@@ -891,7 +912,13 @@ static void plan_capture(Plan *p)
             if (ok && !used[l.lane]) ok = hipStreamWaitEvent(st, p->fork_ev, 0) == hipSuccess, used[l.lane] = true;
         }
         sa.order = p->d_order + l.off, sa.njobs = l.count;
-        if (ok) ok = launch_stripe(sa, l.grp, l.team, p->R, st) == hipSuccess;
+        if (ok && use_queue(qmode, p, l) && nq < kMaxQueueHeads) {
+            ok = clear_queue_heads(p->d_qhead + nq, 1, st) == hipSuccess &&
+                 launch_stripe_queue(sa, l.grp, p->R, p->d_qhead + nq, st) == hipSuccess;
+            ++nq;
+        } else if (ok) {
+            ok = launch_stripe(sa, l.grp, l.team, p->R, st) == hipSuccess;
+        }
     }
     for (int k = 0; k < Plan::kLanes; ++k)
         if (used[k] && ok)
@@ -1031,10 +1058,8 @@ int plan_launch(Plan *p, const Progress *prog)
         }
     }
     // one-wave teams run as persistent workers over the launch's job list (a
-    // device queue, longest first); KSW2_HIP_QUEUE=0 launches one workgroup
-    // per pair instead (tests)
-    const char *qv = getenv("KSW2_HIP_QUEUE");
-    const bool queue = !(qv && atoi(qv) == 0);
+    // device queue, longest first)
+    const int qmode = queue_mode();
     int nq = 0;
     for (size_t k = 0; k < p->launches.size(); ++k) {
         const Plan::Launch &l = p->launches[k];
@@ -1042,7 +1067,7 @@ int plan_launch(Plan *p, const Progress *prog)
         hipStream_t st = lane(l.lane);
         if (!st) return KSW2_HIP_EDEVICE;
         sa.order = p->d_order + l.off, sa.njobs = l.count;
-        if (queue && l.team == 1 && nq < kMaxQueueHeads) {
+        if (use_queue(qmode, p, l) && nq < kMaxQueueHeads) {
             // its head zeroed on the launch's own stream, before the workers
             KSW_HIP_TRY(clear_queue_heads(p->d_qhead + nq, 1, st));
             KSW_HIP_TRY(launch_stripe_queue(sa, l.grp, p->R, p->d_qhead + nq, st));

@@ -1,6 +1,7 @@
-"""Persistent one-wave workers over a device work queue: a plan's one-wave
-teams run as workers that claim pairs from the launch's longest-first list
-with a device-scope atomic add until it is exhausted.  More pairs than the
+"""Persistent one-wave workers over a device work queue (KSW2_HIP_QUEUE=1, or
+=2 for launches with more pairs than the device holds workers; the default
+is one workgroup per pair): workers claim pairs from the launch's
+longest-first list with a device-scope atomic add until it is exhausted.  More pairs than the
 device holds waves (each worker takes several), every flag group, repeated
 runs (graph replays), and the same pairs launched one workgroup per pair
 (KSW2_HIP_QUEUE=0) all give the reference's results."""
@@ -19,10 +20,11 @@ def _check(got, exp, tag):
     assert not bad, "\n".join(bad[:5]) + f"\n({len(bad)} of {len(exp)} differ)"
 
 
-def test_more_pairs_than_workers():
+def test_more_pairs_than_workers(monkeypatch):
     """9,000 short pairs: more than the 4,096 one-wave workers the device
     holds, so workers claim again after their first pair."""
     print("queue: 9000 short pairs", flush=True)
+    monkeypatch.setenv("KSW2_HIP_QUEUE", "2")
     pairs = random_pairs(3101, 9000, 20, 600)
     exp = [O.run_oracle(q, t) for q, t in pairs]
     plan = ksw2.Plan(pairs)
@@ -40,6 +42,7 @@ def test_queue_and_grid_agree(flag, monkeypatch):
     workgroup per pair."""
     print(f"queue vs grid flag {flag:#x}", flush=True)
     monkeypatch.setenv("KSW2_HIP_TEAM", "1")
+    monkeypatch.setenv("KSW2_HIP_QUEUE", "1")  # the queue even with one pair per worker
     pairs = W.fixed(seed=7, n=300) + random_pairs(3200 + flag, 500, 1, 700)
     exp = [O.run_oracle(q, t, flag=flag) for q, t in pairs]
     got_q = ksw2.extd2_batch(pairs, flag=flag)
@@ -53,6 +56,7 @@ def test_mixed_groups_in_one_plan(monkeypatch):
     """Four flag groups in one plan: four queue launches, each with its own
     head."""
     monkeypatch.setenv("KSW2_HIP_TEAM", "1")
+    monkeypatch.setenv("KSW2_HIP_QUEUE", "1")
     flags = [0, ksw2.KSW_EZ_RIGHT, ksw2.KSW_EZ_SCORE_ONLY, ksw2.KSW_EZ_SINGLETRACK]
     pairs = random_pairs(3300, 1600, 1, 900)
     rng = np.random.default_rng(5)
