@@ -74,8 +74,24 @@ namespace {
 constexpr int kQuarters = 4;
 std::mutex g_wait_mu;
 hipEvent_t g_quarter[kQuarters] = {};
+hipStream_t g_quarter_stream[kQuarters] = {};  // the stream each event was last recorded on
 uint64_t g_quarter_used[kQuarters] = {};
 uint64_t g_quarter_seq = 0;
+
+// Whether quarter k's last sequence has finished (caller holds g_wait_mu).
+// A query error other than "not ready" counts as busy and is cleared from
+// the thread's last error, which the launches that follow read.
+bool quarter_done(int k)
+{
+    if (!g_quarter[k]) return true;
+    const hipError_t e = hipEventQuery(g_quarter[k]);
+    if (e == hipSuccess) return true;
+    if (e != hipErrorNotReady) {
+        (void)hipGetLastError();
+        fprintf(stderr, "[ksw2] wait slot %d: hipEventQuery: %s\n", k, hipGetErrorString(e));
+    }
+    return false;
+}
 
 // Takes two quarters for a sequence about to be enqueued on `st` (caller
 // holds g_wait_mu); their indices go to q[0..1].
@@ -84,7 +100,7 @@ bool wait_slots_acquire(hipStream_t st, int q[2])
     int n = 0;
     bool taken[kQuarters] = {};
     for (int k = 0; k < kQuarters && n < 2; ++k)
-        if (!g_quarter[k] || hipEventQuery(g_quarter[k]) == hipSuccess) q[n++] = k, taken[k] = true;
+        if (quarter_done(k)) q[n++] = k, taken[k] = true;
     while (n < 2) {
         int best = -1;
         for (int k = 0; k < kQuarters; ++k)
@@ -113,7 +129,7 @@ int wait_slots_free()
 {
     int n = 0;
     for (int k = 0; k < kQuarters; ++k)
-        if (!g_quarter[k] || hipEventQuery(g_quarter[k]) == hipSuccess) ++n;
+        if (quarter_done(k)) ++n;
     return n;
 }
 
@@ -125,10 +141,27 @@ bool wait_slots_release(hipStream_t st, const int q[2])
             fprintf(stderr, "[ksw2] wait slot %d: hipEventRecord: %s\n", q[i], hipGetErrorString(e));
             return false;
         }
+        g_quarter_stream[q[i]] = st;
     }
     return true;
 }
 }  // namespace
+
+void wait_slots_forget(hipStream_t st)
+{
+    if (!st) return;
+    std::lock_guard<std::mutex> lk(g_wait_mu);
+    for (int k = 0; k < kQuarters; ++k)
+        if (g_quarter[k] && g_quarter_stream[k] == st) {
+            // a fresh event in its place (never recorded: the slot is free),
+            // so a caller that allocates nothing finds every slot's event
+            (void)hipEventSynchronize(g_quarter[k]);
+            hipEvent_t fresh = nullptr;
+            if (hipEventCreateWithFlags(&fresh, hipEventDisableTiming) != hipSuccess) fresh = nullptr;
+            (void)hipEventDestroy(g_quarter[k]);
+            g_quarter[k] = fresh, g_quarter_stream[k] = nullptr;
+        }
+}
 
 bool wait_slots_prepare()
 {
@@ -1221,9 +1254,9 @@ void plan_destroy(Plan *p)
     if (p->own_streams) {
         for (int l = 0; l < Plan::kLanes; ++l) {
             if (p->lane_done[l]) (void)hipEventDestroy(p->lane_done[l]);
-            if (p->lane_stream[l]) (void)hipStreamDestroy(p->lane_stream[l]);
+            if (p->lane_stream[l]) wait_slots_forget(p->lane_stream[l]), (void)hipStreamDestroy(p->lane_stream[l]);
         }
-        if (p->stream) (void)hipStreamDestroy(p->stream);
+        if (p->stream) wait_slots_forget(p->stream), (void)hipStreamDestroy(p->stream);
     }
     delete p;
 }
@@ -1303,7 +1336,7 @@ struct CallContext {
         if (h_stop) (void)hipHostFree(h_stop);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream) wait_slots_forget(stream), (void)hipStreamDestroy(stream);
     }
 };
 

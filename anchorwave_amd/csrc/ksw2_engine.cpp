@@ -276,7 +276,7 @@ Engine::~Engine()
         plan_destroy(f->plan);
     }
     for (auto &s : streams_)
-        if (s) (void)hipStreamDestroy(s);
+        if (s) wait_slots_forget(s), (void)hipStreamDestroy(s);
     for (auto *b : staging_)
         if (b) (void)hipHostFree(b);
     for (auto &m : mirror_) m.release();

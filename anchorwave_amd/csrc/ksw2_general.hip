@@ -474,18 +474,22 @@ __global__ void __launch_bounds__(64) ksw_general_backtrack_kernel(GenBacktrackA
 }
 
 // The same walk for long pairs, one wave per pair.  A step moves to
-// anti-diagonal r-1 or r-2 and to target position i or i-1, so the next 64
+// anti-diagonal r-1 or r-2 and to target position i or i-1, so the next 128
 // anti-diagonals stay within positions [i-127, i] of the current (r, i):
-// the wave stages those 64 rows (132 bytes each, from the 4-byte boundary
-// at or below i-127, zero outside the row) and their off/off_end in LDS,
-// and the walk reads LDS until it leaves the tile.  The walk itself is
-// uniform (scalar); the CIGAR runs go out one store per run, and the wave
-// reverses them at the end.
-constexpr int kTileRows = 64, kTileDw = 33;
+// the wave stages those 128 rows (132 bytes each, from the 4-byte boundary
+// at or below i-127, zero outside the row; lane L stages rows r-L and
+// r-64-L) and their off/off_end in LDS, and the walk reads LDS until it
+// leaves the tile.  The walk is uniform (scalar) but for runs of diagonal
+// moves and runs of gap moves, each taken at once by a ballot over the
+// lanes; the CIGAR runs go out one store per run, and the wave reverses
+// them at the end.
+constexpr int kTileRows = 128, kTileDw = 33;
 
 __global__ void __launch_bounds__(64) ksw_general_backtrack_tiled_kernel(GenBacktrackArgs a)
 {
     __shared__ uint32_t tile[kTileRows * kTileDw];
+    // per staged row: off[], off_end[] and the position of its first byte
+    __shared__ int t_ro[kTileRows], t_re[kTileRows], t_base[kTileRows];
     const int lane = threadIdx.x;
     const GenJob jb = a.jobs[blockIdx.x];
     const int flag = __builtin_amdgcn_readfirstlane(jb.flag);
@@ -504,31 +508,43 @@ __global__ void __launch_bounds__(64) ksw_general_backtrack_tiled_kernel(GenBack
         const uint8_t *tb = (const uint8_t *)tile;
         int i = __builtin_amdgcn_readfirstlane(i0), j = __builtin_amdgcn_readfirstlane(j0), state = 0;
         int run_op = -1, run_len = 0;
-        int r0 = -1, ws = 0;  // tile: anti-diagonals (r0 - 64, r0], positions from ws
-        // lane L: off[], off_end[] of row r0 - L and the position of its
-        // first staged byte
-        int ro = 0, re = -1, base = 0;
+        int r0 = -1, ws = 0;  // tile: anti-diagonals (r0 - 128, r0], positions from ws
+        // lane L: off[], off_end[] of rows r0 - L (ro[0] ...) and r0 - 64 - L
+        // (ro[1] ...) and the position of their first staged bytes
+        int ro[2] = {0, 0}, re[2] = {-1, -1}, base[2] = {0, 0};
+        const int nd = ncolB >> 2;
         while (i >= 0 && j >= 0) {
             const int r = i + j;
             if (r0 < 0 || r <= r0 - kTileRows || i < ws) {
                 r0 = r, ws = i - 127;
                 __syncthreads();  // the walk's last reads of the old tile are done
-                const int row = r0 - lane;
-                ro = 0, re = -1, base = 0;
-                if (row >= 0) {
-                    ro = off[row], re = off_end[row];
-                    const int d0 = (ws - ro) >> 2;  // floor
-                    base = ro + 4 * d0;
-                    const uint32_t *src = p32 + (uint64_t)row * (uint32_t)(ncolB >> 2);
-                    // all loads in flight at once (clamped in-row addresses),
-                    // then the bytes outside the row zeroed
-                    uint32_t v[kTileDw];
+                // all loads of both rows in flight at once (clamped in-row
+                // addresses), then the bytes outside the row zeroed
+                uint32_t v[2][kTileDw];
+                int d0[2];
 #pragma unroll
-                    for (int q = 0; q < kTileDw; ++q) v[q] = src[min(max(d0 + q, 0), (ncolB >> 2) - 1)];
+                for (int h = 0; h < 2; ++h) {
+                    const int row = r0 - kWave * h - lane;
+                    ro[h] = 0, re[h] = -1, base[h] = 0, d0[h] = 0;
+                    if (row >= 0) {
+                        ro[h] = off[row], re[h] = off_end[row];
+                        d0[h] = (ws - ro[h]) >> 2;  // floor
+                        base[h] = ro[h] + 4 * d0[h];
+                        const uint32_t *src = p32 + (uint64_t)row * (uint32_t)nd;
 #pragma unroll
-                    for (int q = 0; q < kTileDw; ++q) {
-                        const int d = d0 + q;
-                        tile[lane * kTileDw + q] = d >= 0 && d < (ncolB >> 2) ? v[q] : 0u;
+                        for (int q = 0; q < kTileDw; ++q) v[h][q] = src[min(max(d0[h] + q, 0), nd - 1)];
+                    }
+                }
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int Lr = kWave * h + lane;
+                    t_ro[Lr] = ro[h], t_re[Lr] = re[h], t_base[Lr] = base[h];
+                    if (r0 - Lr >= 0) {
+#pragma unroll
+                        for (int q = 0; q < kTileDw; ++q) {
+                            const int d = d0[h] + q;
+                            tile[Lr * kTileDw + q] = d >= 0 && d < nd ? v[h][q] : 0u;
+                        }
                     }
                 }
                 __syncthreads();
@@ -541,8 +557,8 @@ __global__ void __launch_bounds__(64) ksw_general_backtrack_tiled_kernel(GenBack
                 // bits) is 0.  Lane k tests cell (i-k, j-k) of the tile (row
                 // L+2k); the run is the lanes before the first that fails.
                 const int Lk = L + 2 * lane, ik = i - lane;
-                const int o_k = __shfl(ro, Lk & (kWave - 1)), e_k = __shfl(re, Lk & (kWave - 1));
-                const int b_k = __shfl(base, Lk & (kWave - 1));
+                const int Lc = min(Lk, kTileRows - 1);
+                const int o_k = t_ro[Lc], e_k = t_re[Lc], b_k = t_base[Lc];
                 int fail = 1;
                 if (Lk < kTileRows && ik >= 0 && lane <= j && ik >= ws && ik >= o_k && ik <= e_k) {
                     const int byte = tb[Lk * kTileDw * 4 + (ik - b_k)];
@@ -562,9 +578,45 @@ __global__ void __launch_bounds__(64) ksw_general_backtrack_tiled_kernel(GenBack
                     }
                     continue;
                 }
+            } else {
+                // A run of gap moves at once: in state s (1-4) the walk
+                // keeps s while a cell's continuation bit (s + 2) is set, or
+                // while the cell is forced into s itself, and moves to
+                // (i-1, j) for s = 1, 3 or (i, j-1) for s = 2, 4, anti-
+                // diagonal r-1.  Lane k tests the k-th cell of the run (row
+                // L+k).
+                const int s = state;
+                const bool di = s == 1 || s == 3;
+                const int Lk = L + lane, ik = di ? i - lane : i;
+                const int Lc = min(Lk, kTileRows - 1);
+                const int o_k = t_ro[Lc], e_k = t_re[Lc], b_k = t_base[Lc];
+                int fail = 1;
+                if (Lk < kTileRows && lane <= (di ? i : j)) {
+                    const int force = ik > e_k ? 1 : ik < o_k ? 2 : -1;
+                    const int byte = tb[Lk * kTileDw * 4 + (ik - b_k)];
+                    const int tmp = force < 0 && (uint32_t)(ik - o_k) < (uint32_t)ncolB ? byte : 0;
+                    fail = force >= 0 ? force != s : !(tmp >> (s + 2) & 1);
+                }
+                const uint64_t fb = __ballot(fail);
+                const int run = fb ? (int)__builtin_ctzll(fb) : kWave;
+                if (run > 0) {
+                    const int op = di ? 2 : 1;
+                    if (di) i -= run;
+                    else j -= run;
+                    if (run_op == op) {
+                        run_len += run;
+                    } else {
+                        if (run_len > 0 && lane == 0) cig[n] = (uint32_t)run_len << 4 | (uint32_t)run_op;
+                        if (run_len > 0) ++n;
+                        run_op = op, run_len = run;
+                    }
+                    continue;
+                }
             }
-            const int o_r = __builtin_amdgcn_readlane(ro, L), e_r = __builtin_amdgcn_readlane(re, L);
-            const int b_r = __builtin_amdgcn_readlane(base, L);
+            const int h = L >> 6, Lh = L & (kWave - 1);
+            const int o_r = __builtin_amdgcn_readlane(h ? ro[1] : ro[0], Lh);
+            const int e_r = __builtin_amdgcn_readlane(h ? re[1] : re[0], Lh);
+            const int b_r = __builtin_amdgcn_readlane(h ? base[1] : base[0], Lh);
             // (the row's byte is read whether or not it is used: the tile
             // holds zeros outside the row, and i - b_r is within the tile)
             const int byte = __builtin_amdgcn_readfirstlane((int)tb[L * kTileDw * 4 + (i - b_r)]);

@@ -244,6 +244,10 @@ int library_device();
 // Creates the process-wide events of the waiting-launch slots now, so that
 // later launches create none.  Returns false when a HIP call fails.
 bool wait_slots_prepare();
+// Called before a stream that may have run waiting launches is destroyed:
+// the slots whose events it recorded last wait for them and take fresh
+// events, so no slot's event refers to a destroyed stream.
+void wait_slots_forget(hipStream_t st);
 // Set-up of the no-allocation entry point on the library's device (address
 // of its scratch array, its stream and events); done by ksw2_hip_init and
 // otherwise by the entry point's first call.  Returns a status code.

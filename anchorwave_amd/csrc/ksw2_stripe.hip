@@ -776,6 +776,28 @@ __device__ __forceinline__ void pair_epilogue(const StripeArgs &a, int job_id, c
                     i -= run, j -= run;
                     continue;
                 }
+            } else {
+                // A run of gap moves at once: state s (1-4) stays while each
+                // cell's continuation bit (s + 2) is set, moving i (s = 1, 3)
+                // or j (s = 2, 4) back by one, which is one stored row or one
+                // column.  Lane k tests the k-th cell of the run.
+                const int s = state;
+                const bool di = s == 1 || s == 3, rows = di != tr;
+                const int lim = min(rows ? rel : c - t_c_lo, di ? i : j);
+                const int lk = tid & (kWave - 1);
+                int fail = 1;
+                if (lk <= lim) {
+                    const int rr = rows ? rel - lk : rel, cc = rows ? c : c - lk;
+                    fail = !((int)(tile[rr / R][cc - t_c_lo] >> (8 * (rr % R) + s + 2)) & 1);
+                }
+                const uint64_t fb = __ballot(fail);
+                const int run = fb ? (int)__builtin_ctzll(fb) : kWave;
+                if (run > 0) {
+                    push(di ? 2 : 1, run);
+                    if (di) i -= run;
+                    else j -= run;
+                    continue;
+                }
             }
             const int tmp = (int)(tile[rel / R][c - t_c_lo] >> (8 * (rel % R))) & 0xff;
             if (state == 0) state = tmp & 7;
