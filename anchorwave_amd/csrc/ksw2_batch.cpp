@@ -408,22 +408,52 @@ void make_gen_score(int8_t m, const int8_t *mat, const KswScore &sc, const Score
     for (int k = 0; k < m * m && k < kMaxM * kMaxM; ++k) g->mat[k] = mat[k];
 }
 
-// Threads of the general kernel's workgroup for one pair: about four cells
-// of the widest anti-diagonal per thread, a power of two in [64, 1024].
-// KSW2_HIP_GEN_THREADS forces a size (tuning and tests).
-bool takes_band(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls)
+// The band kernel's domain, whatever the band's width: the pair's only
+// departure from the stripe kernel's is a band narrower than the matrix, no
+// Singletrack (KSW2_HIP_BAND=0 or KSW2_HIP_FORCE_GENERAL: none).
+bool band_domain(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls)
 {
     const char *ev = getenv("KSW2_HIP_BAND");
     if (ev && atoi(ev) == 0) return false;
     const char *fg = getenv("KSW2_HIP_FORCE_GENERAL");
     if (fg && atoi(fg) > 0) return false;
     if (p.flag & KSW_EZ_SINGLETRACK) return false;
-    const char *mn = getenv("KSW2_HIP_BAND_MIN");
-    const int min_width = mn ? atoi(mn) : 512;
     // progress counts stripe * (qlen + tlen + 1) + anti-diagonal must fit int32
     const int64_t nstripes = ((int64_t)p.tlen + 63) / 64 * 64 / (kWave * kBandRowsPerLane) + 1;
     if ((nstripes + 1) * ((int64_t)p.qlen + p.tlen + 1) >= INT32_MAX) return false;
-    return fast_path_reject(p, m, cls) == kRejBand && norm_w(p) + 1 >= min_width;
+    return fast_path_reject(p, m, cls) == kRejBand;
+}
+
+bool takes_band(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls)
+{
+    const char *mn = getenv("KSW2_HIP_BAND_MIN");
+    const int min_width = mn ? atoi(mn) : kBandWide;
+    return band_domain(p, m, cls) && norm_w(p) + 1 >= min_width;
+}
+
+// Narrow bands (w + 1 < kBandWide) of one plan: the band kernel computes
+// whole 256-position stripes over 2w + 512 anti-diagonals each, the general
+// kernel only the band's cells, one workgroup per pair with a barrier per
+// anti-diagonal.  Measured on MI355X (2 kbp pairs, w = 256, and single
+// pairs of 2-50 kbp): the band kernel takes about 1.1 us per anti-diagonal
+// of a pair's chain and sustains 6.4e10 stripe positions per second, the
+// general kernel about 5.8 us per anti-diagonal and 4.4e10 in-band cells per
+// second.  So a few pairs finish several times sooner on the band kernel and
+// thousands sooner on the general kernel; the plan's narrow pairs take the
+// kernel whose predicted time is the shorter.
+static bool narrow_bands_to_band(const std::vector<const ksw2_pair_t *> &narrow)
+{
+    if (narrow.empty()) return false;
+    double nr_max = 0, stripe_work = 0, cells = 0;
+    for (const ksw2_pair_t *p : narrow) {
+        const double nr = (double)p->qlen + p->tlen - 1, w = norm_w(*p);
+        nr_max = std::max(nr_max, nr);
+        stripe_work += (double)((p->tlen + 255) / 256 * 256) * (2 * w + 512);
+        cells += std::min(2 * w + 1, (double)std::min(p->qlen, p->tlen)) * nr / 2;
+    }
+    const double t_band = std::max(1.1e-6 * nr_max, stripe_work / 6.4e10);
+    const double t_gen = std::max(5.8e-6 * nr_max, cells / 4.4e10);
+    return t_band < t_gen;
 }
 
 int band_r_end(int qlen, int tlen, int w)
@@ -451,6 +481,9 @@ int gen_ring(const ksw2_pair_t &p)
     return gen_ring_slots(std::min(std::min(p.qlen, p.tlen), norm_w(p) + 1) + 128);
 }
 
+// Threads of the general kernel's workgroup for one pair: about four cells
+// of the widest anti-diagonal per thread, a power of two in [64, 1024].
+// KSW2_HIP_GEN_THREADS forces a size (tuning and tests).
 int gen_threads(const ksw2_pair_t &p)
 {
     const char *ev = getenv("KSW2_HIP_GEN_THREADS");
@@ -532,7 +565,21 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
     }
     p->nstripes_total = stripes;
     std::vector<char> band(p->n_gen, 0);
-    for (int gk = 0; gk < p->n_gen; ++gk) band[gk] = takes_band(pairs[p->order[p->n_fast + gk]], m, cls);
+    {
+        // bands at least kBandWide wide always take the band kernel; the
+        // narrower ones together, when it is predicted to finish them sooner
+        // (KSW2_HIP_BAND_MIN: a fixed width rule instead)
+        const bool fixed_rule = getenv("KSW2_HIP_BAND_MIN") != nullptr;
+        std::vector<const ksw2_pair_t *> narrow;
+        for (int gk = 0; gk < p->n_gen; ++gk) {
+            const ksw2_pair_t &pp = pairs[p->order[p->n_fast + gk]];
+            band[gk] = takes_band(pp, m, cls);
+            if (!band[gk] && !fixed_rule && band_domain(pp, m, cls)) band[gk] = 2, narrow.push_back(&pp);
+        }
+        const bool to_band = narrow_bands_to_band(narrow);
+        for (char &b : band)
+            if (b == 2) b = to_band;
+    }
     for (int gk = 0; gk < p->n_gen; ++gk) {
         const int slot = p->n_fast + gk;
         const ksw2_pair_t &pp = pairs[p->order[slot]];

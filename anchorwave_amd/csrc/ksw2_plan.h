@@ -72,9 +72,16 @@ uint64_t pair_footprint(const ksw2_pair_t &p, bool general);
 int gen_threads(const ksw2_pair_t &p);
 // LDS ring slots of the general kernel for one pair (0: state in HBM).
 int gen_ring(const ksw2_pair_t &p);
-// True when a general-kernel pair runs on the band kernel instead: its only
+// True when a general-kernel pair may run on the band kernel: its only
 // departure from the stripe kernel's domain is a band narrower than the
-// matrix, no Singletrack, and a band at least KSW2_HIP_BAND_MIN (512) wide.
+// matrix, and no Singletrack.
+bool band_domain(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls);
+// Bands at least this wide in the band kernel's domain always take it;
+// narrower ones take it when a plan's narrow pairs are predicted to finish
+// sooner there than on the general kernel.
+constexpr int kBandWide = 512;
+// True when a pair in the band kernel's domain has a band at least
+// KSW2_HIP_BAND_MIN (default kBandWide) wide.
 bool takes_band(const ksw2_pair_t &p, int8_t m, const ScoreClass &cls);
 // Anti-diagonals the reference completes before the band leaves the
 // matrix (qlen + tlen - 1 when it never does).

@@ -118,16 +118,35 @@ def test_band_exit_long_pair():
     q = np.concatenate([t[:1200], t[2000:]])
     for flag in (0, ksw2.KSW_EZ_EXTZ_ONLY, ksw2.KSW_EZ_SEMIGLOBAL_END):
         for w in (100, 500, 999, 1001):
-            g = ksw2.extd2(q, t, w=w, flag=flag)
             r = O.run_oracle(q, t, w=w, flag=flag)
+            g = ksw2.extd2(q, t, w=w, flag=flag)
             assert same(g, r), f"w={w} flag={flag:#x}: {diff(g, r)}"
+            if w < 511:
+                # a lone narrow pair takes the band kernel; the general
+                # kernel's band exit too
+                with pytest.MonkeyPatch.context() as mp:
+                    mp.setenv("KSW2_HIP_BAND_MIN", "512")
+                    g = ksw2.extd2(q, t, w=w, flag=flag)
+                assert same(g, r), f"general kernel w={w} flag={flag:#x}: {diff(g, r)}"
 
 
 @pytest.mark.parametrize("name", ["band_medium", "zdrop_band", "wrapping", "positive_match"])
 def test_scenarios_state_in_hbm(name, monkeypatch):
     """The general kernel with its per-position state in HBM (the form for
-    bands wider than its LDS rings) on the same scenarios."""
+    bands wider than its LDS rings) on the same scenarios (narrow bands kept
+    on the general kernel)."""
     monkeypatch.setenv("KSW2_HIP_GEN_RING", "0")
+    monkeypatch.setenv("KSW2_HIP_BAND_MIN", "512")
+    test_scenarios_vs_oracle(name)
+
+
+@pytest.mark.parametrize("name", ["band_narrow", "band_medium"])
+def test_narrow_bands_on_either_kernel(name, monkeypatch):
+    """Narrow bands in the band kernel's domain: a small batch takes the
+    band kernel (the plan's prediction), a fixed width rule keeps them on
+    the general kernel; both equal the oracle."""
+    test_scenarios_vs_oracle(name)
+    monkeypatch.setenv("KSW2_HIP_BAND_MIN", "512")
     test_scenarios_vs_oracle(name)
 
 
@@ -138,6 +157,7 @@ def test_long_banded_pairs(ring, monkeypatch):
     rings and in HBM.  Includes a w = 4096 pair (the banded long-pair
     workload's band) and band exits."""
     monkeypatch.setenv("KSW2_HIP_GEN_RING", ring)
+    monkeypatch.setenv("KSW2_HIP_BAND_MIN", "512")  # narrow bands on the general kernel
     rng = np.random.default_rng(61)
     t1 = W.random_seq(rng, 20000)
     q1 = W.mutate(rng, t1, 0.08, 0.02)
