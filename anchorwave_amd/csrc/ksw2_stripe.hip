@@ -67,6 +67,7 @@ namespace {
 constexpr int kDppWaveShl1 = 0x130;
 constexpr int kDppWaveRor1 = 0x13C;
 constexpr int kLag = kWave;      // steps the high stripe trails the low one
+constexpr int kAhead = 16;       // steps before a block its boundary entries are loaded
 
 // What a pass stores per cell.
 enum Mode { kDirs = 0, kScoreOnly = 1, kTracks = 2 };
@@ -388,10 +389,8 @@ __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc,
         else known = wait_count(prog_prev, need, known);
     };
     if (!first) {
-        wait_prev(base_prev + min(2 * kWave, qlen));
-        const int c0 = lane, c1 = kWave + lane;
-        if (c0 < qlen) load_entry<WIDE>(bnd + 2 * c0, bin0, bin1);
-        if (c1 < qlen) load_entry<WIDE>(bnd + 2 * c1, bnx0, bnx1);
+        wait_prev(base_prev + min(kWave, qlen));
+        if (lane < qlen) load_entry<WIDE>(bnd + 2 * lane, bin0, bin1);
     }
     const int lim_hi = bits(kc.LIM) << 16;
     // LUT: per step, byte k of a half's table is the score candidate of
@@ -425,13 +424,18 @@ __device__ __forceinline__ void pass_body(const StripeArgs &a, const Consts &kc,
         v2s X = vec(carry(Xo, bxh, lane0)), X2 = vec(carry(X2o, bx2h, lane0)), V = vec(carry(Vo, bvh, lane0));
         const int Qw = carry(Qo, qh, lane0);
         if (!first) {
-            // rotate the queue so lane 0 sees column c+1 next step
+            // rotate the queue so lane 0 sees column c+1 next step; the
+            // next block's entries are waited for and loaded kAhead steps
+            // before the block starts (late enough that the producer need
+            // only be that far ahead, early enough to hide the load)
+            if ((c & (kWave - 1)) == kWave - 1 - kAhead) {
+                const int nb = c + 1 + kAhead;  // the next block's first column
+                wait_prev(base_prev + min(nb + kWave, qlen));
+                bnx0 = 0, bnx1 = 0;
+                if (nb + lane < qlen) load_entry<WIDE>(bnd + 2 * (nb + lane), bnx0, bnx1);
+            }
             if ((c & (kWave - 1)) == kWave - 1) {
                 bin0 = bnx0, bin1 = bnx1;
-                wait_prev(base_prev + min(c + 1 + 2 * kWave, qlen));
-                const int cn = c + 1 + kWave + lane;
-                bnx0 = 0, bnx1 = 0;
-                if (cn < qlen) load_entry<WIDE>(bnd + 2 * cn, bnx0, bnx1);
             } else {
                 bin0 = dpp_shl1(0, bin0), bin1 = dpp_shl1(0, bin1);
             }
