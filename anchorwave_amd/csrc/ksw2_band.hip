@@ -65,9 +65,12 @@ namespace {
 constexpr int R = kBandRowsPerLane;  // positions per lane (4: one step of a stripe is short)
 constexpr int SR = kWave * R;        // positions per stripe
 static_assert(R == 4, "a lane's direction bytes and query window are one dword");
-constexpr int kBlock = 64;       // anti-diagonals per block of hand-off loads
-constexpr int kPublish = 32;     // anti-diagonals between progress publications
-constexpr int kLead = 64;        // lead a stripe waits for on its predecessor
+// Hand-off granularity, measured on the banded long pairs (64 x 100-300
+// kbp, w = 4096): blocks, lead and publications of 64/64/32 anti-diagonals
+// 0.625 s per call, 32/32/16 0.596 s, 16/16/8 0.594 s, 16/16/16 0.582 s.
+constexpr int kBlock = 16;       // anti-diagonals per block of hand-off loads
+constexpr int kPublish = 16;     // anti-diagonals between progress publications
+constexpr int kLead = 16;        // lead a stripe waits for on its predecessor
 constexpr int kBufs = 3;         // hand-off buffers, by stripe mod 3
 static_assert(kBlock % kPublish == 0, "publications fall on block offsets");
 constexpr int kDppWaveShr1 = 0x138, kDppWaveShl1 = 0x130;
@@ -512,7 +515,7 @@ __global__ void __launch_bounds__(kWave * kBandGroup) ksw_band_kernel(BandArgs a
             const uint32_t c = j >= 0 && j < qlen ? (uint32_t)qs[j] : 0u;
             qwin[k >> 2] |= c << (8 * (k & 3));
         }
-        // per block of kBlock steps starting at rb, lane k holds in
+        // per block of kBlock steps starting at rb, lane k < kBlock holds in
         // registers what lane 0 needs at step rb + k: the query code
         // Q(rb + k - t0) and the predecessor's entry rb + k - 1 (x, v, x2
         // scaled, and H; initial values before its first anti-diagonal, its
@@ -542,7 +545,7 @@ __global__ void __launch_bounds__(kWave * kBandGroup) ksw_band_kernel(BandArgs a
                 bq = j >= 0 && j < qlen ? (int)qs[j] : 0;
                 const int e = min(rb + lane - 1, p_hi);
                 int pk = pack3(-G1, -G1, -G2), h = KSW_NEG_INF_I;
-                if (s > 0 && p_hi >= p_lo && e >= p_lo) {
+                if (lane < kBlock && s > 0 && p_hi >= p_lo && e >= p_lo) {
                     const uint64_t en = __hip_atomic_load((const uint64_t *)(in_buf + 2 * e), __ATOMIC_RELAXED,
                                                           __HIP_MEMORY_SCOPE_AGENT);
                     pk = (int)(uint32_t)en, h = (int)(uint32_t)(en >> 32);
