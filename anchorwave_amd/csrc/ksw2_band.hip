@@ -67,7 +67,8 @@ constexpr int SR = kWave * R;        // positions per stripe
 static_assert(R == 4, "a lane's direction bytes and query window are one dword");
 // Hand-off granularity, measured on the banded long pairs (64 x 100-300
 // kbp, w = 4096): blocks, lead and publications of 64/64/32 anti-diagonals
-// 0.625 s per call, 32/32/16 0.596 s, 16/16/8 0.594 s, 16/16/16 0.582 s.
+// 0.625 s per call, 32/32/16 0.596 s, 16/16/8 0.594 s, 16/16/16 0.582 s,
+// 8/8/8 0.620 s.
 constexpr int kBlock = 16;       // anti-diagonals per block of hand-off loads
 constexpr int kPublish = 16;     // anti-diagonals between progress publications
 constexpr int kLead = 16;        // lead a stripe waits for on its predecessor
