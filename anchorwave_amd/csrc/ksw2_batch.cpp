@@ -382,7 +382,8 @@ static int choose_rows(const ksw2_pair_t *pairs, const std::vector<int> &fast, c
 
 uint64_t pair_footprint(const ksw2_pair_t &p, bool general)
 {
-    const uint64_t cig = 4ull * ((uint64_t)p.qlen + p.tlen + 2) + 64, seq = (uint64_t)p.qlen + p.tlen + 32;
+    // CIGAR slot, twice: the arena and the dense region plan_fetch copies from
+    const uint64_t cig = 2 * (4ull * ((uint64_t)p.qlen + p.tlen + 2) + 64), seq = (uint64_t)p.qlen + p.tlen + 32;
     const bool cigar = !(p.flag & KSW_EZ_SCORE_ONLY);
     const uint64_t tracks = track_pair(p) ? 2 : 1;  // Singletrack: two bytes per cell
     if (!general) {
@@ -781,7 +782,7 @@ int plan_create(Plan **out, const ksw2_pair_t *pairs, const int *idx, int nidx, 
         return o;
     };
     const uint64_t o_seq = take(p->seq_bytes), o_dir = take(p->dir_bytes), o_bnd = take(p->bnd_dwords * 4);
-    const uint64_t o_cig = take(p->cig_words * 4), o_jobs = take(sizeof(KswJob) * std::max(p->n_fast, 1));
+    const uint64_t o_cig = take(p->cig_words * 8), o_jobs = take(sizeof(KswJob) * std::max(p->n_fast, 1));
     const uint64_t o_out = take(sizeof(KswOut) * std::max(p->n, 1));
     const uint64_t o_order = take(sizeof(int32_t) * std::max<size_t>(order.size(), 1));
     const uint64_t o_recs = take(sizeof(StripeRec) * std::max(stripes, 1));
@@ -1227,7 +1228,8 @@ bool plan_dir_span(const Plan *p, int k, uint64_t *off, uint64_t *len)
     return true;
 }
 
-int deliver_slot(const Plan *p, int k, const KswOut &o, const uint32_t *cig_arena, ksw_extz_t *z, void *km)
+// Delivers slot k's result with its CIGAR runs at `runs`.
+static int deliver_runs(const Plan *p, int k, const KswOut &o, const uint32_t *runs, ksw_extz_t *z, void *km)
 {
     z->max = (uint32_t)o.max, z->zdropped = o.zdropped;
     z->max_q = o.max_q, z->max_t = o.max_t;
@@ -1243,7 +1245,12 @@ int deliver_slot(const Plan *p, int k, const KswOut &o, const uint32_t *cig_aren
         return KSW2_HIP_OK;
     }
     if (o.n_cigar < 0) return drop_cigar(z, km);  // failed Singletrack walk
-    return deliver_cigar(z, cig_arena + p->cig_off[k], o.n_cigar, km);
+    return deliver_cigar(z, runs, o.n_cigar, km);
+}
+
+int deliver_slot(const Plan *p, int k, const KswOut &o, const uint32_t *cig_arena, ksw_extz_t *z, void *km)
+{
+    return deliver_runs(p, k, o, cig_arena + p->cig_off[k], z, km);
 }
 
 int plan_fetch(Plan *p, ksw_extz_t *ez, void *km, const std::vector<char> *skip)
@@ -1258,17 +1265,55 @@ int plan_fetch(Plan *p, ksw_extz_t *ez, void *km, const std::vector<char> *skip)
     for (int k = 0; k < p->n; ++k) todo += !skip || !(*skip)[k];
     if (todo == 0) return KSW2_HIP_OK;  // every pair taken from the host mirror
     std::vector<KswOut> outs(p->n);
-    std::unique_ptr<uint32_t[]> cig(new uint32_t[p->cig_words]);  // only [lo, hi) is filled and read
     KSW_HIP_TRY(hipMemcpyAsync(outs.data(), p->d_out, sizeof(KswOut) * p->n, hipMemcpyDeviceToHost, p->stream));
+    KSW_HIP_TRY(hipStreamSynchronize(p->stream));
+    // The runs of the slots to deliver, moved on the device into the dense
+    // region behind the arena (the move list at its head), so the host
+    // copies only the runs: a few hundred KB for a batch whose arena holds
+    // tens of MB.  When the runs and the list would not fit the region, the
+    // arena's range is copied as it is.
+    auto slot_words = [&](int k) -> uint64_t {
+        return (uint64_t)(k < p->n_fast ? p->jobs[k].qlen + p->jobs[k].tlen
+                                        : p->gjobs[k - p->n_fast].qlen + p->gjobs[k - p->n_fast].tlen) + 2;
+    };
+    std::vector<CigarMove> moves;
+    std::vector<uint64_t> at(p->n, 0);
+    uint64_t total = 0;
+    bool in_slots = true;  // every run count within its slot (the walks never write more)
+    for (int k = 0; k < p->n; ++k) {
+        if ((skip && (*skip)[k]) || outs[k].n_cigar <= 0) continue;
+        const uint64_t n = (uint64_t)outs[k].n_cigar;
+        if (n > slot_words(k) || p->cig_off[k] + n > p->cig_words) in_slots = false;
+        at[k] = total;
+        moves.push_back({p->cig_off[k], (uint32_t)total, (uint32_t)n});
+        total += n;
+    }
+    const uint64_t list_words = 4 * (uint64_t)moves.size();
+    if (in_slots && !moves.empty() && total < (1ull << 32) && list_words + total <= p->cig_words) {
+        uint32_t *region = p->d_cig + p->cig_words;
+        CigarMove *d_moves = (CigarMove *)region;
+        uint32_t *d_dense = region + list_words;
+        std::unique_ptr<uint32_t[]> dense(new uint32_t[total]);
+        KSW_HIP_TRY(hipMemcpyAsync(d_moves, moves.data(), sizeof(CigarMove) * moves.size(), hipMemcpyHostToDevice,
+                                   p->stream));
+        KSW_HIP_TRY(launch_cigar_compact(p->d_cig, d_dense, d_moves, (int)moves.size(), p->stream));
+        KSW_HIP_TRY(hipMemcpyAsync(dense.get(), d_dense, total * 4, hipMemcpyDeviceToHost, p->stream));
+        KSW_HIP_TRY(hipStreamSynchronize(p->stream));
+        for (int k = 0; k < p->n; ++k) {
+            if (skip && (*skip)[k]) continue;
+            const int st = deliver_runs(p, k, outs[k], dense.get() + at[k], ez + p->order[k], km);
+            if (st != KSW2_HIP_OK) return st;
+        }
+        return KSW2_HIP_OK;
+    }
+    std::unique_ptr<uint32_t[]> cig(new uint32_t[p->cig_words]);  // only [lo, hi) is filled and read
     // CIGAR words of the slots still to deliver only (pairs already taken
     // from the host mirror are skipped)
     uint64_t lo = p->cig_words, hi = 0;
     for (int k = 0; k < p->n; ++k)
-        if (!skip || !(*skip)[k]) {
-            const uint64_t len = (uint64_t)(k < p->n_fast ? p->jobs[k].qlen + p->jobs[k].tlen
-                                                         : p->gjobs[k - p->n_fast].qlen + p->gjobs[k - p->n_fast].tlen) + 2;
-            lo = std::min(lo, p->cig_off[k]), hi = std::max(hi, std::min<uint64_t>(p->cig_off[k] + len, p->cig_words));
-        }
+        if (!skip || !(*skip)[k])
+            lo = std::min(lo, p->cig_off[k]),
+            hi = std::max(hi, std::min<uint64_t>(p->cig_off[k] + slot_words(k), p->cig_words));
     if (hi > lo)
         KSW_HIP_TRY(hipMemcpyAsync(cig.get() + lo, p->d_cig + lo, (hi - lo) * 4, hipMemcpyDeviceToHost, p->stream));
     KSW_HIP_TRY(hipStreamSynchronize(p->stream));

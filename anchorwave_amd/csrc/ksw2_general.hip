@@ -744,6 +744,21 @@ hipError_t launch_general(const GenArgs &a, int threads, hipStream_t st)
     return hipGetLastError();
 }
 
+__global__ void __launch_bounds__(64) ksw_cigar_compact_kernel(const uint32_t *arena, uint32_t *dense,
+                                                               const CigarMove *moves)
+{
+    const CigarMove m = moves[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < m.n; i += 64) dense[m.dst + i] = arena[m.src + i];
+}
+
+hipError_t launch_cigar_compact(const uint32_t *arena, uint32_t *dense, const CigarMove *moves, int n_moves,
+                                hipStream_t st)
+{
+    if (n_moves <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ksw_cigar_compact_kernel, dim3(n_moves), dim3(64), 0, st, arena, dense, moves);
+    return hipGetLastError();
+}
+
 hipError_t launch_general_backtrack(const GenBacktrackArgs &a, hipStream_t st)
 {
     if (a.njobs <= 0) return hipSuccess;

@@ -115,6 +115,15 @@ struct GenWalkArgs {
 // General kernel: one workgroup of `threads` (64..1024) per job.
 hipError_t launch_general(const GenArgs &a, int threads, hipStream_t st);
 hipError_t launch_general_backtrack(const GenBacktrackArgs &a, hipStream_t st);
+// One move of a plan's CIGAR runs into a dense region: n words from word
+// offset src of the CIGAR arena to word offset dst of the region.
+struct CigarMove {
+    uint64_t src;
+    uint32_t dst, n;
+};
+// Runs the moves (one workgroup each), so the host copies only the runs.
+hipError_t launch_cigar_compact(const uint32_t *arena, uint32_t *dense, const CigarMove *moves, int n_moves,
+                                hipStream_t st);
 hipError_t launch_general_walk(const GenWalkArgs &a, hipStream_t st);
 
 // Stripe-kernel variants ("groups" of a plan): direction bytes with left or
