@@ -1228,6 +1228,11 @@ bool plan_dir_span(const Plan *p, int k, uint64_t *off, uint64_t *len)
     return true;
 }
 
+// A fetch whose arena range is larger than this copies only the CIGAR runs
+// (a compaction kernel, one more round trip); a smaller range, a single
+// call's for instance, is copied as it is.
+constexpr uint64_t kDenseFetchMin = 1ull << 20;
+
 // Delivers slot k's result with its CIGAR runs at `runs`.
 static int deliver_runs(const Plan *p, int k, const KswOut &o, const uint32_t *runs, ksw_extz_t *z, void *km)
 {
@@ -1264,56 +1269,61 @@ int plan_fetch(Plan *p, ksw_extz_t *ez, void *km, const std::vector<char> *skip)
     int todo = 0;
     for (int k = 0; k < p->n; ++k) todo += !skip || !(*skip)[k];
     if (todo == 0) return KSW2_HIP_OK;  // every pair taken from the host mirror
-    std::vector<KswOut> outs(p->n);
-    KSW_HIP_TRY(hipMemcpyAsync(outs.data(), p->d_out, sizeof(KswOut) * p->n, hipMemcpyDeviceToHost, p->stream));
-    KSW_HIP_TRY(hipStreamSynchronize(p->stream));
-    // The runs of the slots to deliver, moved on the device into the dense
-    // region behind the arena (the move list at its head), so the host
-    // copies only the runs: a few hundred KB for a batch whose arena holds
-    // tens of MB.  When the runs and the list would not fit the region, the
-    // arena's range is copied as it is.
     auto slot_words = [&](int k) -> uint64_t {
         return (uint64_t)(k < p->n_fast ? p->jobs[k].qlen + p->jobs[k].tlen
                                         : p->gjobs[k - p->n_fast].qlen + p->gjobs[k - p->n_fast].tlen) + 2;
     };
-    std::vector<CigarMove> moves;
-    std::vector<uint64_t> at(p->n, 0);
-    uint64_t total = 0;
-    bool in_slots = true;  // every run count within its slot (the walks never write more)
-    for (int k = 0; k < p->n; ++k) {
-        if ((skip && (*skip)[k]) || outs[k].n_cigar <= 0) continue;
-        const uint64_t n = (uint64_t)outs[k].n_cigar;
-        if (n > slot_words(k) || p->cig_off[k] + n > p->cig_words) in_slots = false;
-        at[k] = total;
-        moves.push_back({p->cig_off[k], (uint32_t)total, (uint32_t)n});
-        total += n;
-    }
-    const uint64_t list_words = 4 * (uint64_t)moves.size();
-    if (in_slots && !moves.empty() && total < (1ull << 32) && list_words + total <= p->cig_words) {
-        uint32_t *region = p->d_cig + p->cig_words;
-        CigarMove *d_moves = (CigarMove *)region;
-        uint32_t *d_dense = region + list_words;
-        std::unique_ptr<uint32_t[]> dense(new uint32_t[total]);
-        KSW_HIP_TRY(hipMemcpyAsync(d_moves, moves.data(), sizeof(CigarMove) * moves.size(), hipMemcpyHostToDevice,
-                                   p->stream));
-        KSW_HIP_TRY(launch_cigar_compact(p->d_cig, d_dense, d_moves, (int)moves.size(), p->stream));
-        KSW_HIP_TRY(hipMemcpyAsync(dense.get(), d_dense, total * 4, hipMemcpyDeviceToHost, p->stream));
-        KSW_HIP_TRY(hipStreamSynchronize(p->stream));
-        for (int k = 0; k < p->n; ++k) {
-            if (skip && (*skip)[k]) continue;
-            const int st = deliver_runs(p, k, outs[k], dense.get() + at[k], ez + p->order[k], km);
-            if (st != KSW2_HIP_OK) return st;
-        }
-        return KSW2_HIP_OK;
-    }
-    std::unique_ptr<uint32_t[]> cig(new uint32_t[p->cig_words]);  // only [lo, hi) is filled and read
-    // CIGAR words of the slots still to deliver only (pairs already taken
+    // the arena's words of the slots still to deliver (pairs already taken
     // from the host mirror are skipped)
     uint64_t lo = p->cig_words, hi = 0;
     for (int k = 0; k < p->n; ++k)
         if (!skip || !(*skip)[k])
             lo = std::min(lo, p->cig_off[k]),
             hi = std::max(hi, std::min<uint64_t>(p->cig_off[k] + slot_words(k), p->cig_words));
+    std::vector<KswOut> outs(p->n);
+    KSW_HIP_TRY(hipMemcpyAsync(outs.data(), p->d_out, sizeof(KswOut) * p->n, hipMemcpyDeviceToHost, p->stream));
+    const uint64_t dense_min = getenv("KSW2_HIP_DENSE_FETCH") ? 0 : kDenseFetchMin;  // tests: always dense
+    if (hi > lo && (hi - lo) * 4 > dense_min) {
+        // A large range: the runs of the slots to deliver are moved on the
+        // device into the dense region behind the arena (the move list at
+        // its head), so the host copies only the runs, a few hundred KB for
+        // a batch whose arena holds tens of MB.  When the runs and the list
+        // would not fit the region, the range is copied as it is.
+        KSW_HIP_TRY(hipStreamSynchronize(p->stream));
+        std::vector<CigarMove> moves;
+        std::vector<uint64_t> at(p->n, 0);
+        uint64_t total = 0;
+        bool in_slots = true;  // every run count within its slot (the walks never write more)
+        for (int k = 0; k < p->n; ++k) {
+            if ((skip && (*skip)[k]) || outs[k].n_cigar <= 0) continue;
+            const uint64_t n = (uint64_t)outs[k].n_cigar;
+            if (n > slot_words(k) || p->cig_off[k] + n > p->cig_words) in_slots = false;
+            at[k] = total;
+            moves.push_back({p->cig_off[k], (uint32_t)total, (uint32_t)n});
+            total += n;
+        }
+        const uint64_t list_words = 4 * (uint64_t)moves.size();
+        if (in_slots && total < (1ull << 32) && list_words + total <= p->cig_words) {
+            std::unique_ptr<uint32_t[]> dense(new uint32_t[std::max<uint64_t>(total, 1)]);
+            if (total > 0) {
+                uint32_t *region = p->d_cig + p->cig_words;
+                CigarMove *d_moves = (CigarMove *)region;
+                uint32_t *d_dense = region + list_words;
+                KSW_HIP_TRY(hipMemcpyAsync(d_moves, moves.data(), sizeof(CigarMove) * moves.size(),
+                                           hipMemcpyHostToDevice, p->stream));
+                KSW_HIP_TRY(launch_cigar_compact(p->d_cig, d_dense, d_moves, (int)moves.size(), p->stream));
+                KSW_HIP_TRY(hipMemcpyAsync(dense.get(), d_dense, total * 4, hipMemcpyDeviceToHost, p->stream));
+                KSW_HIP_TRY(hipStreamSynchronize(p->stream));
+            }
+            for (int k = 0; k < p->n; ++k) {
+                if (skip && (*skip)[k]) continue;
+                const int st = deliver_runs(p, k, outs[k], dense.get() + at[k], ez + p->order[k], km);
+                if (st != KSW2_HIP_OK) return st;
+            }
+            return KSW2_HIP_OK;
+        }
+    }
+    std::unique_ptr<uint32_t[]> cig(new uint32_t[p->cig_words]);  // only [lo, hi) is filled and read
     if (hi > lo)
         KSW_HIP_TRY(hipMemcpyAsync(cig.get() + lo, p->d_cig + lo, (hi - lo) * 4, hipMemcpyDeviceToHost, p->stream));
     KSW_HIP_TRY(hipStreamSynchronize(p->stream));
