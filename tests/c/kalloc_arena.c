@@ -91,8 +91,8 @@ int main(void)
             ksw_extz_t a, b;
             memset(&a, 0, sizeof(a));
             memset(&b, 0, sizeof(b));
-            for (int round = 0; round < 2; ++round) {
-                /* round 1 reuses each ez and its arena-grown CIGAR buffer */
+            for (int call = 0; call < 2; ++call) {
+                /* call 1 reuses each ez and its arena-grown CIGAR buffer */
                 if (st) {
                     ksw_extd2_singletrack_avx512(km, qlen, q, tlen, t, m, mat, 8, 2, 75, 1, cfg[c].w, cfg[c].zdrop, 0,
                                                  cfg[c].flag & ~KSW_EZ_SINGLETRACK, &a);
@@ -108,8 +108,8 @@ int main(void)
                 ++checked;
                 if (err != KSW2_HIP_OK || !same(&a, &b)) {
                     ++bad;
-                    fprintf(stderr, "differ: qlen %d tlen %d w %d zdrop %d flag 0x%x round %d status %d\n", qlen,
-                            tlen, cfg[c].w, cfg[c].zdrop, cfg[c].flag, round, err);
+                    fprintf(stderr, "differ: qlen %d tlen %d w %d zdrop %d flag 0x%x call %d status %d\n", qlen,
+                            tlen, cfg[c].w, cfg[c].zdrop, cfg[c].flag, call, err);
                     show("reference", &a);
                     show("hip", &b);
                 }
