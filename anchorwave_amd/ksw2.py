@@ -557,9 +557,16 @@ class Plan:
             raise KswError(f"ksw2_hip_plan_graph: {STATUS.get(st, st)}")
         return runs.value, rep.value
 
-    def fetch(self) -> List[Result]:
+    def fetch(self, timing: dict = None) -> List[Result]:
+        """``ksw2_hip_plan_fetch`` into ``ksw_extz_t`` structs, as Results.
+        When ``timing`` is a dict, its "seconds" receives the wall time of the
+        C call alone (scores and CIGARs delivered into the structs)."""
+        import time
         ezs = (KswExtz * max(self.n, 1))()
+        t0 = time.perf_counter()
         st = lib().ksw2_hip_plan_fetch(self.h, ezs)
+        if timing is not None:
+            timing["seconds"] = time.perf_counter() - t0
         out = []
         for k in range(self.n):
             out.append(Result.from_ez(ezs[k]))
